@@ -1,0 +1,177 @@
+"""WFS frames/s of the 30 m 6x60x60 LGS step on one MI355X.
+
+A step is one time step of all six LGS WFS: ray tracing of seven 4096^2
+screens, LLT uplink OTF, per-subaperture imaging (embed, FFT, |E|^2, FFT,
+x conj(uplink OTF) x ETF, inverse FFT, pixel sampling), matched-filter
+gradients and FP64 i0 accumulation.  Screens and tables are resident in HBM
+before the timed region.  One frame is one WFS x one step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+
+Prints one JSON line.  The CPU baseline is the FP64 oracle (a C restatement
+of MAOS's CPU path) timed on this host on a bounded sample of frames.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("WFS frames/s, 30 m 6×60×60 LGS (MFMA split-BF16 vs FP32 vs FP64), "
+          "% roofline")
+PEAK_FP32_TFLOPS = 157.3
+HBM_GBS = 8000.0
+
+
+def flops_per_subap(N, nxsa, nps, npix, llt=True, etf=True):
+    """Algorithmic FLOPs of one subaperture-wavelength (5 M log2 M per
+    complex transform of M = N^2 points)."""
+    M = N * N
+    f = 3 * 5 * M * math.log2(M)
+    f += 3 * M                       # |E|^2
+    f += 6 * M if llt else 0         # x conj(lotfc)
+    f += 6 * M if etf else 2 * M     # x ETF (complex) or nominal (real)
+    f += 8 * nps * nxsa * nxsa       # ray trace
+    f += 3 * nxsa * nxsa             # embed (2 mul + 1 sincos)
+    f += 8 * npix + 4 * npix         # sampling + matched filter
+    return f
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--i0-steps", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--order", type=int, default=60)
+    ap.add_argument("--nx", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the WFS path of this build is single-GPU "
+              "(see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    from maos_amd import configs, engine
+    from maos_amd.powfs import Powfs, genatm
+
+    t0 = time.time()
+    cfg = configs.lgs30m(nx=args.nx, order=args.order)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    log("powfs: nsa=%d nxsa=%d notf=%d npix=%d (%.1fs)" % (p.nsa, p.nxsa, p.notf, p.npix,
+                                                         time.time() - t0))
+    layers = genatm(cfg["atm"])
+    log("screens: %d x %d^2 (%.1fs)" % (len(layers), args.nx, time.time() - t0))
+    wfs = configs.wfs_specs(cfg, p)
+    nwfs = len(wfs)
+    dt = cfg["sim"].dt
+    engine.init(0)
+    engine.atm2gpu(layers)
+    eng = engine.WfsGrad(p, wfs)
+
+    # matched filter from the FP64 long-exposure i0 (mtch_cell on the host)
+    eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    eng.i0_control(enable=True, reset=True)
+    for isim in range(args.i0_steps):
+        eng.queue(isim, dt)
+    eng.wait()
+    for iwfs in range(nwfs):
+        i0, n = eng.i0_get(iwfs)
+        mtche, i0sum, _ = p.mtch(i0 / n)
+        eng.update_ref(mtche, i0sum, iwfs=iwfs)
+    eng.i0_control(enable=True, reset=True)
+    log("matched filter from %d-frame FP64 i0 (%.1fs)" % (args.i0_steps, time.time() - t0))
+
+    isim = args.i0_steps
+    for _ in range(args.warmup):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    t2 = time.perf_counter()
+    sec = t2 - t1
+    fps = nwfs * args.steps / sec
+    ms_step = sec / args.steps * 1e3
+    log("timed: %d steps in %.3fs -> %.1f frames/s" % (args.steps, sec, fps))
+
+    # per-launch kernel durations with HIP events on the engine's stream
+    nt = min(100, max(10, args.steps // 5))
+    eng.timing_enable(True)
+    for _ in range(nt):
+        eng.queue(isim, dt)
+        isim += 1
+    nl, ms_llt, ms_ints = eng.timing()
+    eng.timing_enable(False)
+    t_ints = ms_ints / nl * 1e-3
+    t_llt = ms_llt / nl * 1e-3
+    flop_launch = flops_per_subap(p.notf, p.nxsa, len(layers), p.npix) * p.nsa * nwfs
+    achieved = flop_launch / t_ints / 1e12
+    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "kernel": "k_shwfs_step<64,32,1>", "avg_launch_us": round(t_ints * 1e6, 2),
+            "llt_kernel_us": round(t_llt * 1e6, 2), "flop_per_launch": flop_launch,
+            "peak_kind": "fp32 (MI355X published); FP32 radix-FFT form runs on VALU"}
+
+    out = {"metric": METRIC, "value": round(fps, 2), "unit": "WFS frames/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (seeded von Karman screens, Gaussian sodium profile)",
+           "config": {"workload": "lgs30m: 30 m pupil, 6 LGS WFS x 60x60 (nsa=%d), 32x32 "
+                      "OPD pts -> 64x64 FFT, 6x6 px of 1\", 7 layers 4096^2 @1/64 m, LLT "
+                      "uplink, ETF, matched filter, 800 Hz" % p.nsa,
+                      "nwfs": nwfs, "nsa": p.nsa, "notf": p.notf, "npix": p.npix,
+                      "variant": "fp32-radix-fft"},
+           "roofline": roof}
+
+    # CPU baseline: FP64 oracle on the host cores, bounded sample of frames
+    if not args.no_cpu:
+        import oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+        o = oracle.Oracle(p)
+        o.atm(layers)
+        ref_isim = isim
+        eng.queue(ref_isim, dt)
+        g_gpu, ints_gpu = eng.sync(0)
+        mt = np.zeros(p.nsa * 2 * p.npix)
+        o.set_ref(mt, np.ones(p.nsa))
+        nf = 0
+        tc0 = time.perf_counter()
+        ri = None
+        while True:
+            iw = nf % nwfs
+            r = o.step(wfs[iw], ref_isim, dt, nthread=threads)
+            if nf == 0:
+                ri = r[0]
+            nf += 1
+            if time.perf_counter() - tc0 > args.cpu_seconds:
+                break
+        tc = time.perf_counter() - tc0
+        cpu_fps = nf / tc
+        out["cpu_baseline"] = {"value": round(cpu_fps, 3), "unit": "WFS frames/s", "cores": threads,
+                               "kind": "port",
+                               "sample": "%d full WFS frames (nsa=%d) of the same workload, FP64 "
+                                         "oracle with OpenMP over subapertures" % (nf, p.nsa)}
+        out["gpu_over_cpu"] = round(fps / cpu_fps, 1)
+        rel = float(np.linalg.norm(ints_gpu - ri) / np.linalg.norm(ri))
+        out["parity"] = {"ints_rel_l2_wfs0": rel}
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * maos_mi355x.h -- C ABI of the MI355X (gfx950) Shack-Hartmann
+ * physical-optics WFS library (libmaos_mi355x.so).
+ *
+ * The library is a drop-in for the WFS step that MAOS runs through
+ * wfsgrad_iwfs -> prop -> wfsints -> shwfs_grad (maos/wfsgrad.c:122-501,
+ * maos/wfsints.c:79-302, maos/utils.c:983-1122), with the lifecycle of the
+ * reference's GPU plugin (cuda/sim/gpu_sim.h:25-55): init, upload screens,
+ * queue a step, sync the gradients, update ETF / matched filter.
+ *
+ * Conventions (as MAOS): arrays are column-major with x fastest; complex
+ * arrays are interleaved (re, im) doubles; sparse matrices are MAOS dsp
+ * (CSC) arrays.  All host buffers are owned by the caller and only read or
+ * written during the call.  Every entry point returns 0 on success or a
+ * negative MX_E* code; mx_last_error() describes the last failure.
+ * Compute runs in FP32 on the GPU except the screen-coordinate arithmetic
+ * and the long-exposure i0 accumulation, which are FP64.
+ */
+#ifndef MAOS_MI355X_H
+#define MAOS_MI355X_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MX_OK 0
+#define MX_EINVAL -1   /* bad argument or unsupported configuration */
+#define MX_EHIP -2     /* HIP runtime error */
+#define MX_ESTATE -3   /* call out of order (e.g. no screens uploaded) */
+
+#define MX_PTYPE_MF 1  /* matched filter (PTYPE_MF) */
+#define MX_PTYPE_COG 2 /* thresholded centre of gravity (PTYPE_COG) */
+
+/* One turbulence layer: the fields of map_t that ray tracing reads
+ * (lib/accphi.h:31-49, math/type.h map_t). */
+typedef struct mx_atm_layer_t {
+    const double *p;     /* nx*ny screen values (m) */
+    long nx, ny;
+    double dx, dy;       /* sampling (m) */
+    double ox, oy;       /* coordinate of p[0] (m) */
+    double ht;           /* conjugate height (m) */
+    double vx, vy;       /* frozen-flow wind (m/s) */
+} mx_atm_layer_t;
+
+/* Static description of one powfs: the parms->powfs[ipowfs] keys and the
+ * powfs_t geometry that wfsints/shwfs_grad read (maos/types.h:76-134). */
+typedef struct mx_powfs_t {
+    long nsa;            /* powfs.saloc->nloc */
+    long nxsa;           /* powfs.pts->nxsa: OPD points per subaperture side */
+    long embfac;         /* parms->powfs.embfac (nwvf = nxsa*embfac) */
+    long notf;           /* powfs.notfx (== notfy) */
+    long pixpsax, pixpsay;
+    long nwvl;
+    double dx;           /* powfs.pts->dx */
+    const double *ptsx;  /* powfs.pts->origx [nsa] */
+    const double *ptsy;  /* powfs.pts->origy [nsa] */
+    const double *amp;   /* powfs.amp[0] [nsa*nxsa*nxsa] */
+    const double *saa;   /* powfs.saa[0] [nsa] */
+    double saat;         /* parms->powfs.saat */
+    double areascale;    /* powfs.areascale */
+    const double *wvl;   /* parms->powfs.wvl [nwvl] */
+    /* pixel processing */
+    int phytype;         /* MX_PTYPE_MF or MX_PTYPE_COG (phytype_sim) */
+    int sigmatch;        /* parms->powfs.sigmatch: 0, 1 or 2 */
+    double cogthres, cogoff;
+    double pixthetax;    /* radpixtheta */
+    double pixthetay;    /* pixtheta */
+    double siglev_dtrat; /* parms->powfs.siglev*dtrat (CoG sigmatch 0) */
+    double saasum;       /* powfs.saasum (CoG sigmatch 2) */
+    /* LLT uplink (powfs.llt); llt_nx == 0 means no LLT */
+    long llt_nx;         /* llt->pts->nxsa */
+    double llt_origx, llt_origy; /* llt->pts->origx/origy[0] */
+    double llt_ox, llt_oy;       /* parms->powfs.llt->ox/oy of this LLT */
+    const double *llt_amp;       /* llt->amp [llt_nx*llt_nx] */
+} mx_powfs_t;
+
+/* Per-WFS parameters (parms->wfs[iwfs]). */
+typedef struct mx_wfs_t {
+    double thetax, thetay; /* direction (rad) */
+    double hs;             /* source range (m), INFINITY for NGS */
+    double sigsim;         /* photons per subaperture per frame */
+    double misregx, misregy;
+    const double *wvlwts;  /* [nwvl] */
+} mx_wfs_t;
+
+typedef struct mx_wfsgrad_s mx_wfsgrad_t;
+
+/* Select the GPU (gpu_init, maos/parms.c:3796). */
+int mx_init(int device);
+const char *mx_last_error(void);
+
+/* Create the WFS engine for one powfs and its nwfs WFS
+ * (gpu_wfsgrad_init, cuda/sim/wfsinit.cu:159; called at maos/setup.c:131).
+ * Copies geometry; tables are supplied with the calls below. */
+int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *powfs,
+                    const mx_wfs_t *wfs, int nwfs);
+int mx_wfsgrad_free(mx_wfsgrad_t *h);
+
+/* Detector transfer tables of one wavelength: powfs.dtf[iwvl].nominal (real
+ * notf*notf, treated; ignored when the powfs has an LLT, whose ETF carries
+ * the nominal) and powfs.dtf[iwvl].si (dsp: si_nx = npix rows, si_ny =
+ * notf*notf columns, CSC pp/pi/px) (lib/mkdtf.c:89-184). */
+int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal,
+                       long si_nx, long si_ny, const long *si_pp,
+                       const long *si_pi, const double *si_px);
+
+/* Sodium elongation transfer function of one wavelength
+ * (gpu_wfsgrad_update_etf, cuda/sim/wfsinit.cu:32; maos/sim_utils.c:388):
+ * etf is powfs.etfsim[iwvl].etf, nsa*notf*notf complex with the nominal
+ * fused; etf2 (or NULL) is etfsim2 for profile interpolation with weight
+ * etf2wt = (isim%dtrat)/dtrat (maos/wfsints.c:215-223). */
+int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf,
+                          const double *etf2, double etf2wt);
+
+/* Matched filter and reference flux of WFS iwfs (gpu_wfsgrad_update_ref,
+ * cuda/sim/wfsinit.cu:107; maos/wfsgrad.c:1207): mtche is nsa matrices of
+ * 2 x npix (intstat->mtche column of this WFS), i0sum is nsa values
+ * (intstat->i0sum).  iwfs = -1 loads the same tables into every WFS.
+ * Either pointer may be NULL to leave that table unchanged. */
+int mx_wfsgrad_update_ref(mx_wfsgrad_t *h, int iwfs, const double *mtche,
+                          const double *i0sum);
+
+/* Upload the turbulence screens (gpu_atm2gpu, cuda/sim/atm.cu:106; called
+ * at maos/sim.c:79).  Values are stored as FP32; geometry stays FP64.
+ * atmscale multiplies every layer (simu->atmscale). */
+int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale);
+
+/* Enqueue step isim for every WFS of the engine (gpu_wfsgrad_queue,
+ * cuda/sim/wfsgrad.cu:483): ray trace, LLT uplink, imaging, gradients and
+ * (when enabled) FP64 i0 accumulation.  Asynchronous. */
+int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt);
+
+/* Wait for the queued step and copy WFS iwfs's gradients (2*nsa, x block
+ * then y block; entries of subapertures with saa<saat keep their previous
+ * value) and, if ints is not NULL, its pixel intensities (nsa*npix)
+ * (gpu_wfsgrad_sync, cuda/sim/wfsgrad.cu:747). */
+int mx_wfsgrad_sync(mx_wfsgrad_t *h, int iwfs, double *grad, double *ints);
+
+/* Long-exposure FP64 accumulation of ints (maos/wfsgrad.c:449-467).
+ * enable=1 accumulates every later step; reset=1 clears the sums. */
+int mx_wfsgrad_i0_control(mx_wfsgrad_t *h, int enable, int reset);
+/* Copy WFS iwfs's accumulated sum (nsa*npix) and the number of frames. */
+int mx_wfsgrad_i0_get(mx_wfsgrad_t *h, int iwfs, double *i0sum_out,
+                      long *nframe);
+
+/* Debug/parity outputs of the last step: the WFS OPD (nsa*nxsa*nxsa, m)
+ * and the uplink OTF lotfc of wavelength iwvl (notf*notf complex, as
+ * wfsints.c:149-199 leaves it before conjugation). */
+int mx_wfsgrad_debug_opd(mx_wfsgrad_t *h, int iwfs, double *opd);
+int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf);
+
+/* Time the imaging kernel of subsequent steps with HIP events on the
+ * engine's stream.  enable=1 starts recording (and clears the sums);
+ * mx_wfsgrad_timing reports launches and the summed milliseconds of the
+ * uplink kernel and of the fused imaging kernel. */
+int mx_wfsgrad_timing_enable(mx_wfsgrad_t *h, int enable);
+int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
+                      double *ms_ints);
+
+/* Wait for all queued work of the engine. */
+int mx_wfsgrad_wait(mx_wfsgrad_t *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

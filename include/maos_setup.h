@@ -1,0 +1,118 @@
+/*
+ * maos_setup.h -- host-side FP64 table builders for the Shack-Hartmann
+ * physical-optics WFS path (libmaos_setup.so).
+ *
+ * In a MAOS run these tables come from MAOS's own setup (maos/powfs.c,
+ * lib/mkdtf.c, lib/mtch.c, lib/turbulence.c) and are handed to the GPU
+ * library through maos_mi355x.h.  This library restates those builders so
+ * that the GPU path can be run and benchmarked without MAOS.  Every function
+ * cites the reference routine it follows.  Arrays are column-major with x
+ * fastest, as in MAOS's dmat/cmat (math/type.h:31-41).  Complex arrays are
+ * interleaved (re, im) doubles.
+ */
+#ifndef MAOS_SETUP_H
+#define MAOS_SETUP_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 2-D unnormalised FFT in place on nx*ny complex (x fastest). dir=-1 is the
+ * forward transform exp(-i...), dir=+1 the backward one.  Follows the FFTW
+ * convention used by cfft2 (math/fft.c:253-281). */
+void mxs_fft2(double *c, long nx, long ny, int dir);
+
+/* nextfftsize (math/mathmisc.c:109-146). */
+long mxs_nextfftsize(long n);
+
+/* Subaperture lower-left corners on the padded grid, before area pruning
+ * (maos/powfs.c:302-340).  Returns the count written (<= maxn). */
+long mxs_saloc(int order, double dsa, double din, double misregrmax,
+               double *locx, double *locy, long maxn);
+
+/* Anti-aliased annulus sampled at each point: loc_circle_add
+ * (math/loc.c:942-986) with val=1, added into amp. */
+void mxs_loc_circle_add(double *amp, const double *locx, const double *locy,
+                        long nloc, double dx, double dy, double cx, double cy,
+                        double r, double rin, double val);
+
+/* Normalised subaperture area from the amplitude map: wfsamp2saa
+ * (maos/powfs.c:86-99), then scaled by areascale (maos/powfs.c:412-414). */
+void mxs_saa(const double *amp, long nsa, long npsa, double areascale,
+             double *saa);
+
+/* Isolated-subaperture rule and area pruning (sa_reduce, maos/powfs.c:
+ * 141-237).  Compacts salocx/y, ptsx/y, amp and saa in place; returns the
+ * new nsa. */
+long mxs_sa_reduce(double *salocx, double *salocy, double *ptsx, double *ptsy,
+                   double *amp, double *saa, long nsa, long npsa, double dsa,
+                   double saat);
+
+/* Detector transfer function (dtf_otf, lib/mkdtf.c:51-80) with treat=1
+ * (dotf_treat, lib/mkdtf.c:26-39). nominal is notfx*notfy real. */
+void mxs_dtf_nominal(double *nominal, long notfx, long notfy, double pdthetax,
+                     double pdthetay, double pixrot, double pixblur);
+
+/* Bilinear PSF->pixel sampler (mkdtf's mkh call, lib/mkdtf.c:147-161 and
+ * lib/mkh.c:45-155).  Output is MAOS's dsp layout of H: CSC with nx=npix
+ * rows and ny=notfx*notfy columns.  pp has notfx*notfy+1 entries; pi/px
+ * need at most 4*npix entries.  Returns nnz. */
+long mxs_si(long notfx, long notfy, double dtheta, long pixpsax, long pixpsay,
+            double pixthetax, double pixthetay, double pixoffx,
+            double pixoffy, double pixrot, long *pp, long *pi, double *px);
+
+/* LGS subaperture polar coordinates relative to the LLT
+ * (maos/powfs.c:744-758). */
+void mxs_srot_srsa(const double *salocx, const double *salocy, long nsa,
+                   double dsa, double lltox, double lltoy, double *srot,
+                   double *srsa);
+
+/* Sodium profile resampling by linear binning (smooth, lib/mkdtf.c:
+ * 421-452).  prof is nh x 2 (column 0 height, column 1 strength).  Writes
+ * up to maxout rows to out (maxout x 2, column-major) and returns the row
+ * count; returns nh and copies when no smoothing is needed. */
+long mxs_smooth(const double *prof, long nh, double dxnew, double *out,
+                long maxout);
+
+/* Elongation transfer function, interpolation path of mketf
+ * (lib/mkdtf.c:281-378), fused with the nominal (lib/mkdtf.c:389-393).
+ * prof is nh x 2.  etf is nsa*notf*notf complex, peak in corner.
+ * If etf1d is not NULL it also receives the per-subaperture 1-D ETF
+ * (nsa * 4*notf complex, peak in centre) from which etf is interpolated. */
+int mxs_mketf(double *etf, double *etf1d, const double *nominal, long notf,
+              double dtheta, const double *prof, long nh, const double *srot,
+              const double *srsa, long nsa, double hs, double htel,
+              double za);
+
+/* LLT pupil (setup_powfs_llt, maos/powfs.c:1182-1242): Gaussian beam of
+ * waist widthp*d/2 clipped at d/2, normalised so sum(amp^2)=1 for the
+ * unclipped beam.  Returns nx; amp holds nx*nx values, *origin the pts
+ * origin (same in x and y). */
+long mxs_llt_amp(double lltd, double widthp, double dx, double dsa,
+                 double *amp, long maxn, double *origin);
+
+/* Seeded von Karman FFT screens (spect_screen_do, lib/turbulence.c:70-171;
+ * spatial_psd :436-480).  Two layers come from one complex FFT.  Output is
+ * nlayer*nx*ny; each layer is scaled by sqrt(wt[i]).  The random stream is
+ * this library's own (xoshiro256** + Box-Muller), not MAOS's MT19937. */
+void mxs_genscreen(double *screens, long nx, long ny, double dx, double r0,
+                   double L0, const double *wt, long nlayer,
+                   unsigned long long seed);
+
+/* Derivative by FFT (derive_by_fft, lib/mtch.c:39-57). */
+void mxs_derive_by_fft(double *out, const double *i0, long nx, long ny,
+                       double theta);
+
+/* Matched filter for all subapertures (mtch_cell, lib/mtch.c:223-335, with
+ * mtch :93-211).  i0/gx/gy are nsa*npix (gx/gy may be NULL: FFT derivative,
+ * as with powfs.mtchfft).  Outputs mtche nsa*(2*npix) (each a 2 x npix
+ * column-major matrix), sanea nsa*3 (optional), i0sum nsa, *i0sumsum. */
+int mxs_mtch(double *mtche, double *sanea, double *i0sum, double *i0sumsum,
+             const double *i0, const double *gx, const double *gy, long nsa,
+             long pixpsax, long pixpsay, double bkgrnd, double bkgrndc,
+             double rne, double pixthetax, double pixthetay, int mtchcr,
+             double sigratio);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,157 @@
+"""ctypes bindings of the native libraries.
+
+libmaos_mi355x.so is the GPU path (include/maos_mi355x.h); libmaos_setup.so
+holds the host FP64 table builders (include/maos_setup.h).  Loading fails
+loudly when a library has not been built: there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import HIP_LIB, SETUP_LIB
+
+c_double_p = C.POINTER(C.c_double)
+c_long_p = C.POINTER(C.c_long)
+
+
+class MxAtmLayer(C.Structure):
+    _fields_ = [("p", c_double_p), ("nx", C.c_long), ("ny", C.c_long),
+                ("dx", C.c_double), ("dy", C.c_double), ("ox", C.c_double),
+                ("oy", C.c_double), ("ht", C.c_double), ("vx", C.c_double),
+                ("vy", C.c_double)]
+
+
+class MxPowfs(C.Structure):
+    _fields_ = [("nsa", C.c_long), ("nxsa", C.c_long), ("embfac", C.c_long),
+                ("notf", C.c_long), ("pixpsax", C.c_long), ("pixpsay", C.c_long),
+                ("nwvl", C.c_long), ("dx", C.c_double), ("ptsx", c_double_p),
+                ("ptsy", c_double_p), ("amp", c_double_p), ("saa", c_double_p),
+                ("saat", C.c_double), ("areascale", C.c_double), ("wvl", c_double_p),
+                ("phytype", C.c_int), ("sigmatch", C.c_int), ("cogthres", C.c_double),
+                ("cogoff", C.c_double), ("pixthetax", C.c_double),
+                ("pixthetay", C.c_double), ("siglev_dtrat", C.c_double),
+                ("saasum", C.c_double), ("llt_nx", C.c_long), ("llt_origx", C.c_double),
+                ("llt_origy", C.c_double), ("llt_ox", C.c_double), ("llt_oy", C.c_double),
+                ("llt_amp", c_double_p)]
+
+
+class MxWfs(C.Structure):
+    _fields_ = [("thetax", C.c_double), ("thetay", C.c_double), ("hs", C.c_double),
+                ("sigsim", C.c_double), ("misregx", C.c_double), ("misregy", C.c_double),
+                ("wvlwts", c_double_p)]
+
+
+def dptr(a):
+    """Pointer to a C-contiguous float64 numpy array (or NULL for None)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, "need contiguous float64"
+    return a.ctypes.data_as(c_double_p)
+
+
+def lptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int64 and a.flags.c_contiguous, "need contiguous int64"
+    return a.ctypes.data_as(c_long_p)
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "native library %s is missing: build it with `python -m maos_amd.build` "
+            "(there is no fallback path)" % path)
+    return C.CDLL(path)
+
+
+_hip = None
+_setup = None
+
+# name -> (restype, argtypes) of every symbol in include/maos_mi355x.h
+HIP_API = {
+    "mx_init": (C.c_int, [C.c_int]),
+    "mx_last_error": (C.c_char_p, []),
+    "mx_wfsgrad_init": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(MxPowfs), C.POINTER(MxWfs), C.c_int]),
+    "mx_wfsgrad_free": (C.c_int, [C.c_void_p]),
+    "mx_wfsgrad_set_dtf": (C.c_int, [C.c_void_p, C.c_int, c_double_p, C.c_long, C.c_long,
+                                     c_long_p, c_long_p, c_double_p]),
+    "mx_wfsgrad_update_etf": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, C.c_double]),
+    "mx_wfsgrad_update_ref": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
+    "mx_atm2gpu": (C.c_int, [C.POINTER(MxAtmLayer), C.c_int, C.c_double]),
+    "mx_wfsgrad_queue": (C.c_int, [C.c_void_p, C.c_long, C.c_double]),
+    "mx_wfsgrad_sync": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
+    "mx_wfsgrad_i0_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mx_wfsgrad_i0_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_long_p]),
+    "mx_wfsgrad_debug_opd": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "mx_wfsgrad_debug_lotf": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p]),
+    "mx_wfsgrad_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "mx_wfsgrad_timing": (C.c_int, [C.c_void_p, c_long_p, c_double_p, c_double_p]),
+    "mx_wfsgrad_wait": (C.c_int, [C.c_void_p]),
+}
+
+# name -> (restype, argtypes) of every symbol in include/maos_setup.h
+SETUP_API = {
+    "mxs_fft2": (None, [c_double_p, C.c_long, C.c_long, C.c_int]),
+    "mxs_nextfftsize": (C.c_long, [C.c_long]),
+    "mxs_saloc": (C.c_long, [C.c_int, C.c_double, C.c_double, C.c_double, c_double_p,
+                             c_double_p, C.c_long]),
+    "mxs_loc_circle_add": (None, [c_double_p, c_double_p, c_double_p, C.c_long, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_double]),
+    "mxs_saa": (None, [c_double_p, C.c_long, C.c_long, C.c_double, c_double_p]),
+    "mxs_sa_reduce": (C.c_long, [c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                 c_double_p, C.c_long, C.c_long, C.c_double, C.c_double]),
+    "mxs_dtf_nominal": (None, [c_double_p, C.c_long, C.c_long, C.c_double, C.c_double,
+                               C.c_double, C.c_double]),
+    "mxs_si": (C.c_long, [C.c_long, C.c_long, C.c_double, C.c_long, C.c_long, C.c_double,
+                          C.c_double, C.c_double, C.c_double, C.c_double, c_long_p,
+                          c_long_p, c_double_p]),
+    "mxs_srot_srsa": (None, [c_double_p, c_double_p, C.c_long, C.c_double, C.c_double,
+                             C.c_double, c_double_p, c_double_p]),
+    "mxs_smooth": (C.c_long, [c_double_p, C.c_long, C.c_double, c_double_p, C.c_long]),
+    "mxs_mketf": (C.c_int, [c_double_p, c_double_p, c_double_p, C.c_long, C.c_double,
+                            c_double_p, C.c_long, c_double_p, c_double_p, C.c_long,
+                            C.c_double, C.c_double, C.c_double]),
+    "mxs_llt_amp": (C.c_long, [C.c_double, C.c_double, C.c_double, C.c_double, c_double_p,
+                               C.c_long, c_double_p]),
+    "mxs_genscreen": (None, [c_double_p, C.c_long, C.c_long, C.c_double, C.c_double,
+                             C.c_double, c_double_p, C.c_long, C.c_ulonglong]),
+    "mxs_derive_by_fft": (None, [c_double_p, c_double_p, C.c_long, C.c_long, C.c_double]),
+    "mxs_mtch": (C.c_int, [c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                           c_double_p, c_double_p, C.c_long, C.c_long, C.c_long,
+                           C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                           C.c_int, C.c_double]),
+}
+
+
+def _bind(lib, api):
+    for name, (res, args) in api.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def hip():
+    """libmaos_mi355x.so (loads it on first use)."""
+    global _hip
+    if _hip is None:
+        _hip = _bind(_load(HIP_LIB), HIP_API)
+    return _hip
+
+
+def setup():
+    """libmaos_setup.so (loads it on first use)."""
+    global _setup
+    if _setup is None:
+        _setup = _bind(_load(SETUP_LIB), SETUP_API)
+    return _setup
+
+
+def check(rc):
+    """Raise on a negative return code of libmaos_mi355x.so."""
+    if rc != 0:
+        msg = hip().mx_last_error()
+        raise RuntimeError("maos_mi355x error %d: %s" % (rc, msg.decode() if msg else ""))
+    return rc

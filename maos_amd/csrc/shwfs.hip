@@ -1,0 +1,1110 @@
+// MI355X (gfx950) Shack-Hartmann physical-optics WFS step.
+//
+// One 64-lane wave images 64/N subapertures of an N x N (N = notf) grid:
+// every lane owns one line (row or column) of its subaperture's field and
+// keeps it in registers, where a fully unrolled radix-2 FFT runs with the
+// twiddles folded into instruction literals.  Rows and columns are exchanged
+// through a 32 KiB XOR-swizzled LDS tile (N = 64), so the chain
+//   ray trace (FP64 coordinates) -> embed -> FFT -> |E|^2 -> FFT
+//   -> x conj(uplink OTF) x ETF/nominal -> inverse FFT -> pixel sampling
+//   -> matched filter / tCoG -> FP64 i0 accumulation
+// runs in one launch with no intermediate array in HBM
+// (maos/wfsgrad.c:122-501, maos/wfsints.c:79-302, maos/utils.c:983-1122).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/maos_mi355x.h"
+#include "twiddles.h"
+
+#define MX_MAXLAYER 16
+#define MX_MAXWVL 4
+
+namespace {
+
+// ---------------------------------------------------------------- errors
+thread_local char g_err[512] = "";
+
+int set_err(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return set_err(MX_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));   \
+    } while (0)
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+__host__ __device__ constexpr int bitrev(int x, int lg) {
+    int r = 0;
+    for (int i = 0; i < lg; i++) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
+}
+
+// a * exp(-/+ 2 pi i k / MX_TWN); k is a compile-time constant after unrolling
+template <bool INV>
+__device__ __forceinline__ float2 twmul(float2 a, int k) {
+    if (k == 0) return a;
+    if (k == MX_TWN / 4) return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+    const float c = kTwCos[k], s = kTwSin[k];
+    if (!INV) return make_float2(a.x * c + a.y * s, a.y * c - a.x * s);
+    return make_float2(a.x * c - a.y * s, a.y * c + a.x * s);
+}
+
+// Unnormalised N-point FFT of a register array; INV=false is exp(-i...)
+// (FFTW_FORWARD, cfft2(.,-1)), INV=true is exp(+i...) (cfft2(.,+1)).
+template <int N, bool INV>
+__device__ __forceinline__ void fft_reg(float2 (&v)[N]) {
+    constexpr int LG = ilog2c(N);
+    static_assert((1 << LG) == N && N <= MX_TWN, "power of two up to MX_TWN");
+    float2 t[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) t[i] = v[bitrev(i, LG)];
+#pragma unroll
+    for (int s = 1; s <= LG; s++) {
+        const int len = 1 << s, h = len >> 1;
+#pragma unroll
+        for (int j = 0; j < h; j++) {
+            const int k = j * (MX_TWN / len);
+#pragma unroll
+            for (int i0 = 0; i0 < N; i0 += len) {
+                float2 u = t[i0 + j];
+                float2 w = twmul<INV>(t[i0 + j + h], k);
+                t[i0 + j] = cadd(u, w);
+                t[i0 + j + h] = csub(u, w);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) v[i] = t[i];
+}
+
+// ---------------------------------------------------------------- params
+struct DevLayer {
+    const float *p;
+    int nx, ny;
+    double dx1, dy1;   // 1/dx, 1/dy
+    double ox, oy, ht, vx, vy;
+};
+
+struct DevWfs {
+    double thetax, thetay, hs, misregx, misregy;
+    float alpha[MX_MAXWVL]; // wvlwts[iwvl]*norm_ints
+};
+
+// Per-wavelength tables; transfer functions are stored transposed
+// ([ux][uy]) so that the lanes of a row pass read them coalesced.
+struct DevWvl {
+    float inv_wvl;
+    const float *nominal;
+    const float2 *etf, *etf2;
+    const int4 *si_idx;
+    const float4 *si_w;
+};
+
+struct KArgs {
+    int nsa, npix, pixpsax, pixpsay, nwvl, nwfs, nps;
+    double ptsdx;
+    const double *ptsx, *ptsy;
+    const float *amp;
+    const DevWvl *wv;   // [nwvl]
+    const DevLayer *layers;
+    double atmscale;
+    const DevWfs *wfs;
+    long isim;
+    double dt;
+    float etf1wt, etf2wt;
+    const float2 *lotf; // [nwfs][nwvl][N*N] conj(lotfc), or null
+    // pixel processing
+    int phytype, sigmatch;
+    float cogthres, cogoff, pixthetax, pixthetay, sigtot;
+    const float *mtche; // [nwfs][nsa][2*npix]
+    const float *i0sum; // [nwfs][nsa]
+    const float *saa;
+    float saat;
+    // outputs
+    float *ints;     // [nwfs][nsa][npix]
+    float *grad;     // [nwfs][2*nsa]
+    double *i0acc;   // [nwfs][nsa][npix] or null
+    float *opd_dbg;  // [nwfs][nsa][nxsa^2] or null
+};
+
+struct LltArgs {
+    int nwvl, nwfs, nps;
+    double ptsdx, origx, origy, ox, oy;
+    const float *amp; // nlx*nlx
+    const DevWvl *wv;
+    float norm;       // 1/nlwvf^2
+    const DevLayer *layers;
+    double atmscale;
+    const DevWfs *wfs;
+    long isim;
+    double dt;
+    float2 *lotf;     // [nwfs][nwvl][N*N] conj, transposed
+};
+
+// XOR-swizzled element (x, y) of an N x N complex tile
+template <int N>
+__device__ __forceinline__ int swz(int x, int y) { return y * N + (x ^ (y & (N - 1))); }
+
+template <int N>
+__device__ __forceinline__ void seg_sum(float &v) {
+#pragma unroll
+    for (int o = N / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+}
+
+// Ray trace every layer onto one NXSA x NXSA point grid whose first point is
+// at (origx, origy), accumulating into opd (LDS, x fastest).  Follows
+// prop_range + prop_grid_pts with wrap=1 (lib/accphi.c:223-244,
+// lib/accphi_grid_do.c:147-302): the point coordinate on each screen, its
+// periodic wrap, integer part and fraction are FP64; the four weights and
+// the screen values are FP32.  Coordinates of the NXSA columns and NXSA
+// rows are computed once per layer by 2*NXSA lanes and shared through LDS.
+template <int N, int NXSA>
+__device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, double hs,
+                                          double thetax, double thetay, double misregx,
+                                          double misregy, double shift0x, double shift0y,
+                                          long isim, double dt, double atmscale, double ptsdx,
+                                          double origx, double origy, int line, float *opd,
+                                          int *cidx, float *cfrac) {
+    static_assert(2 * NXSA <= N, "coordinate lanes");
+    if (line < 2 * NXSA) {
+        const bool isx = line < NXSA;
+        const int i = isx ? line : line - NXSA;
+        for (int l = 0; l < nps; l++) {
+            const DevLayer L = layers[l];
+            int ii = 0;
+            float ff = 0.f;
+            bool skip = hs != 0. && L.ht > hs;
+            if (!skip) {
+                const double scale = hs != 0. ? 1. - L.ht / hs : 1.;
+                double disp, o, d1, nn;
+                if (isx) {
+                    disp = L.ht * thetax + scale * misregx + shift0x - L.vx * dt * (double)isim;
+                    d1 = L.dx1;
+                    o = origx;
+                    nn = (double)L.nx;
+                } else {
+                    disp = L.ht * thetay + scale * misregy + shift0y - L.vy * dt * (double)isim;
+                    d1 = L.dy1;
+                    o = origy;
+                    nn = (double)L.ny;
+                }
+                const double o0 = isx ? L.ox : L.oy;
+                const double dout = ptsdx * d1 * scale;
+                const double oout = o * d1 * scale + (disp - o0) * d1;
+                double X = fma((double)i, dout, oout);
+                X -= floor(X / nn) * nn;
+                double fl = floor(X);
+                ii = (int)fl;
+                double fr = X - fl;
+                const int n = isx ? L.nx : L.ny;
+                if (ii >= n) { ii -= n; }
+                if (ii < 0) { ii += n; fr = 0.; }
+                ff = (float)fr;
+            }
+            cidx[l * 2 * NXSA + line] = skip ? -1 : ii;
+            cfrac[l * 2 * NXSA + line] = ff;
+        }
+    }
+    __syncthreads();
+    for (int pt = line; pt < NXSA * NXSA; pt += N) {
+        const int ix = pt % NXSA, iy = pt / NXSA;
+        float acc = 0.f;
+        for (int l = 0; l < nps; l++) {
+            const int x0 = cidx[l * 2 * NXSA + ix];
+            if (x0 < 0) continue;
+            const DevLayer L = layers[l];
+            const int y0 = cidx[l * 2 * NXSA + NXSA + iy];
+            const float fx = cfrac[l * 2 * NXSA + ix];
+            const float fy = cfrac[l * 2 * NXSA + NXSA + iy];
+            const int x1 = x0 == L.nx - 1 ? 0 : x0 + 1;
+            const int y1 = y0 == L.ny - 1 ? 0 : y0 + 1;
+            const float *r0 = L.p + (long)y0 * L.nx;
+            const float *r1 = L.p + (long)y1 * L.nx;
+            const float v = (1.f - fy) * ((1.f - fx) * r0[x0] + fx * r0[x1]) +
+                            fy * ((1.f - fx) * r1[x0] + fx * r1[x1]);
+            acc += (float)atmscale * v;
+        }
+        opd[pt] += acc;
+    }
+}
+
+// XCD-aware unit index: consecutive units (the WFS of one subaperture
+// group, which share the ETF) land on the same XCD and L2.
+__device__ __forceinline__ int xcd_unit(int b, int total) {
+    const int nx = 8;
+    const int xcd = b % nx, k = b / nx;
+    const int q = total / nx, r = total % nx;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// ------------------------------------------------------------ main kernel
+template <int N, int NXSA, int PPL>
+__global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
+    constexpr int SPW = 64 / N;        // subapertures per wave
+    constexpr int SKIP = (N - NXSA) / 2;
+    __shared__ float2 tile[SPW][N * N];
+    const int lane = threadIdx.x;
+    const int sub = lane / N, line = lane % N;
+    const int ngroup = (a.nsa + SPW - 1) / SPW;
+    const int unit = xcd_unit(blockIdx.x, ngroup * a.nwfs);
+    const int iwfs = unit % a.nwfs;
+    const int isa_raw = (unit / a.nwfs) * SPW + sub;
+    const bool valid = isa_raw < a.nsa;
+    const int isa = valid ? isa_raw : a.nsa - 1;
+    const DevWfs &W = a.wfs[iwfs];
+    float2 *T = tile[sub];
+    float *opd = reinterpret_cast<float *>(T);
+    float *amp = opd + NXSA * NXSA;
+    int *cidx = reinterpret_cast<int *>(amp + NXSA * NXSA);
+    float *cfrac = reinterpret_cast<float *>(cidx + MX_MAXLAYER * 2 * NXSA);
+
+    float pv[PPL];
+#pragma unroll
+    for (int j = 0; j < PPL; j++) pv[j] = 0.f;
+    float2 v[N];
+    for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
+        // stage: OPD by ray tracing, amplitude staged into LDS
+        for (int pt = line; pt < NXSA * NXSA; pt += N) {
+            opd[pt] = 0.f;
+            amp[pt] = a.amp[(long)isa * NXSA * NXSA + pt];
+        }
+        ray_trace<N, NXSA>(a.layers, a.nps, W.hs, W.thetax, W.thetay, W.misregx, W.misregy, 0., 0.,
+                           a.isim, a.dt, a.atmscale, a.ptsdx, a.ptsx[isa], a.ptsy[isa], line, opd,
+                           cidx, cfrac);
+        __syncthreads();
+        // piston (unweighted mean) is removed before exp(i k opd); |E|^2 is
+        // invariant to it and the FP32 phase stays small
+        float psum = 0.f;
+        for (int pt = line; pt < NXSA * NXSA; pt += N) psum += opd[pt];
+        seg_sum<N>(psum);
+        const float mean = psum * (1.f / (NXSA * NXSA));
+        if (a.opd_dbg && valid && iwvl == 0)
+            for (int pt = line; pt < NXSA * NXSA; pt += N)
+                a.opd_dbg[((long)iwfs * a.nsa + isa) * NXSA * NXSA + pt] = opd[pt];
+        const DevWvl WV = a.wv[iwvl];
+        const float inv_wvl = WV.inv_wvl;
+        // pass 0 (rows y): embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf,
+        // math/matcomp.c:56-82) and FFT along x
+        const int y = line;
+        const bool inband = y >= SKIP && y < SKIP + NXSA;
+        if (inband) {
+            const int iy = y - SKIP;
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int ix = 0; ix < NXSA; ix++) {
+                const float o = opd[iy * NXSA + ix] - mean;
+                const float am = amp[iy * NXSA + ix];
+                float sn, cs;
+                sincospif(2.f * o * inv_wvl, &sn, &cs);
+                v[SKIP + ix] = make_float2(am * cs, am * sn);
+            }
+            fft_reg<N, false>(v);
+        }
+        __syncthreads(); // opd/amp reads done before the tile is overwritten
+#pragma unroll
+        for (int k = 0; k < N; k++) T[swz<N>(k, y)] = inband ? v[k] : make_float2(0.f, 0.f);
+        __syncthreads();
+        // pass 1 (columns x): FFT along y, |E|^2, first half of PSF->OTF FFT
+        {
+            const int x = line;
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = T[swz<N>(x, k)];
+            fft_reg<N, false>(v);
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = make_float2(v[k].x * v[k].x + v[k].y * v[k].y, 0.f);
+            fft_reg<N, false>(v);
+#pragma unroll
+            for (int k = 0; k < N; k++) T[swz<N>(x, k)] = v[k];
+        }
+        __syncthreads();
+        // pass 2 (rows uy): finish OTF, x conj(lotfc) (ccwmc), x ETF or
+        // nominal (apply_dtf_etf, maos/wfsints.c:38-64), inverse FFT along x
+        {
+            const int uy = line;
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = T[swz<N>(k, uy)];
+            fft_reg<N, false>(v);
+            if (a.lotf) {
+                const float2 *lt = a.lotf + ((long)iwfs * a.nwvl + iwvl) * N * N + uy;
+#pragma unroll
+                for (int k = 0; k < N; k++) v[k] = cmul(v[k], lt[k * N]);
+            }
+            if (WV.etf) {
+                const float2 *e1 = WV.etf + (long)isa * N * N + uy;
+                if (WV.etf2) {
+                    const float2 *e2 = WV.etf2 + (long)isa * N * N + uy;
+                    const float w1 = a.etf1wt, w2 = a.etf2wt;
+#pragma unroll
+                    for (int k = 0; k < N; k++) {
+                        const float2 e = e1[k * N], f = e2[k * N];
+                        v[k] = cmul(v[k], make_float2(e.x * w1 + f.x * w2, e.y * w1 + f.y * w2));
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < N; k++) v[k] = cmul(v[k], e1[k * N]);
+                }
+            } else if (WV.nominal) {
+                const float *nm = WV.nominal + uy;
+#pragma unroll
+                for (int k = 0; k < N; k++) {
+                    const float s = nm[k * N];
+                    v[k] = make_float2(v[k].x * s, v[k].y * s);
+                }
+            }
+            fft_reg<N, true>(v);
+#pragma unroll
+            for (int k = 0; k < N; k++) T[swz<N>(k, uy)] = v[k];
+        }
+        __syncthreads();
+        // pass 3 (columns x): inverse FFT along y -> image, real part to LDS
+        {
+            const int x = line;
+#pragma unroll
+            for (int k = 0; k < N; k++) v[k] = T[swz<N>(x, k)];
+            fft_reg<N, true>(v);
+            __syncthreads();
+            float *img = reinterpret_cast<float *>(T);
+#pragma unroll
+            for (int k = 0; k < N; k++) img[k * N + x] = v[k].x;
+        }
+        __syncthreads();
+        // bilinear pixel sampler (dspmulcreal with si, math/spmm.c:77-86)
+        {
+            const float *img = reinterpret_cast<const float *>(T);
+            const int4 *sidx = WV.si_idx;
+            const float4 *sw = WV.si_w;
+            const float alpha = W.alpha[iwvl];
+#pragma unroll
+            for (int j = 0; j < PPL; j++) {
+                const int p = line + j * N;
+                if (p < a.npix) {
+                    const int4 id = sidx[p];
+                    const float4 w = sw[p];
+                    pv[j] += alpha * (w.x * img[id.x] + w.y * img[id.y] + w.z * img[id.z] + w.w * img[id.w]);
+                }
+            }
+        }
+        __syncthreads(); // image reads done before the next wavelength
+    }
+
+    // outputs and pixel processing (shwfs_grad, maos/utils.c:983-1122)
+    const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int j = 0; j < PPL; j++) {
+        const int p = line + j * N;
+        if (p < a.npix) {
+            if (valid) {
+                a.ints[ibase + p] = pv[j];
+                if (a.i0acc) a.i0acc[ibase + p] += (double)pv[j];
+            }
+            if (a.phytype == MX_PTYPE_MF) {
+                const float *m = a.mtche + ibase * 2 + 2 * p;
+                s0 += m[0] * pv[j];
+                s1 += m[1] * pv[j];
+                s2 += pv[j];
+            } else {
+                const float iI = pv[j] - a.cogoff;
+                if (iI > a.cogthres) {
+                    s0 += iI * (float)(p % a.pixpsax);
+                    s1 += iI * (float)(p / a.pixpsax);
+                    s2 += iI;
+                }
+            }
+        }
+    }
+    seg_sum<N>(s0);
+    seg_sum<N>(s1);
+    seg_sum<N>(s2);
+    if (line == 0 && valid && !(a.saa[isa] < a.saat)) {
+        float gx = 0.f, gy = 0.f;
+        if (a.phytype == MX_PTYPE_MF) {
+            float scale = 1.f;
+            if (a.sigmatch == 1) {
+                scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
+                if (scale <= 0.f || scale > 10.f || !isfinite(scale)) scale = 1.f;
+            }
+            gx = s0 * scale;
+            gy = s1 * scale;
+        } else {
+            float sum = s2;
+            if (a.sigmatch == 0)
+                sum = a.i0sum ? a.i0sum[(long)iwfs * a.nsa + isa] : a.sigtot * a.saa[isa];
+            if (sum > a.cogthres) {
+                gx = (s0 / sum - (float)(a.pixpsax - 1) * 0.5f) * a.pixthetax;
+                gy = (s1 / sum - (float)(a.pixpsay - 1) * 0.5f) * a.pixthetay;
+            }
+        }
+        a.grad[(long)iwfs * 2 * a.nsa + isa] = gx;
+        a.grad[(long)iwfs * 2 * a.nsa + a.nsa + isa] = gy;
+    }
+}
+
+// ------------------------------------------------------------ uplink OTF
+// lotfc of wfsints (maos/wfsints.c:149-199) for nlwvf == nwvf == notf: LLT
+// OPD by ray tracing (maos/wfsgrad.c:329-386), embed, FFT, |.|^2/nlwvf^2,
+// FFT.  Stored conjugated and transposed for the imaging kernel.
+template <int N, int NLX>
+__global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
+    constexpr int SKIP = (N - NLX) / 2;
+    __shared__ float2 T[N * N];
+    const int lane = threadIdx.x;
+    const int line = lane % N;
+    const bool active = lane < N;
+    const int iwfs = blockIdx.x / a.nwvl, iwvl = blockIdx.x % a.nwvl;
+    const DevWfs &W = a.wfs[iwfs];
+    float *opd = reinterpret_cast<float *>(T);
+    float *amp = opd + NLX * NLX;
+    int *cidx = reinterpret_cast<int *>(amp + NLX * NLX);
+    float *cfrac = reinterpret_cast<float *>(cidx + MX_MAXLAYER * 2 * NLX);
+    if (active)
+        for (int pt = line; pt < NLX * NLX; pt += N) {
+            opd[pt] = 0.f;
+            amp[pt] = a.amp[pt];
+        }
+    // theta of the uplink: theta - o/hs; shift o - v*dt*isim
+    ray_trace<N, NLX>(a.layers, a.nps, W.hs, W.thetax - a.ox / W.hs, W.thetay - a.oy / W.hs, 0.,
+                      0., a.ox, a.oy, a.isim, a.dt, a.atmscale, a.ptsdx, a.origx, a.origy,
+                      active ? line : (1 << 20), opd, cidx, cfrac);
+    __syncthreads();
+    float psum = 0.f;
+    if (active)
+        for (int pt = line; pt < NLX * NLX; pt += N) psum += opd[pt];
+    seg_sum<N>(psum);
+    const float mean = psum * (1.f / (NLX * NLX));
+    float2 v[N];
+    const float inv_wvl = a.wv[iwvl].inv_wvl;
+    const int y = line;
+    const bool inband = active && y >= SKIP && y < SKIP + NLX;
+    if (inband) {
+        const int iy = y - SKIP;
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int ix = 0; ix < NLX; ix++) {
+            const float o = opd[iy * NLX + ix] - mean;
+            const float am = amp[iy * NLX + ix];
+            float sn, cs;
+            sincospif(2.f * o * inv_wvl, &sn, &cs);
+            v[SKIP + ix] = make_float2(am * cs, am * sn);
+        }
+        fft_reg<N, false>(v);
+    }
+    __syncthreads();
+    if (active)
+#pragma unroll
+        for (int k = 0; k < N; k++) T[swz<N>(k, y)] = inband ? v[k] : make_float2(0.f, 0.f);
+    __syncthreads();
+    if (active) {
+        const int x = line;
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = T[swz<N>(x, k)];
+        fft_reg<N, false>(v);
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = make_float2((v[k].x * v[k].x + v[k].y * v[k].y) * a.norm, 0.f);
+        fft_reg<N, false>(v);
+#pragma unroll
+        for (int k = 0; k < N; k++) T[swz<N>(x, k)] = v[k];
+    }
+    __syncthreads();
+    if (active) {
+        const int uy = line;
+#pragma unroll
+        for (int k = 0; k < N; k++) v[k] = T[swz<N>(k, uy)];
+        fft_reg<N, false>(v);
+        float2 *out = a.lotf + ((long)iwfs * a.nwvl + iwvl) * N * N + uy;
+#pragma unroll
+        for (int k = 0; k < N; k++) out[k * N] = make_float2(v[k].x, -v[k].y);
+    }
+}
+
+} // namespace
+
+// ======================================================================
+// C ABI
+// ======================================================================
+struct mx_wfsgrad_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    long nsa = 0, nxsa = 0, embfac = 0, nwvf = 0, notf = 0, npix = 0, pixpsax = 0, pixpsay = 0,
+         nwvl = 0;
+    int nwfs = 0;
+    mx_powfs_t cfg{};
+    std::vector<double> wvl;
+    std::vector<mx_wfs_t> wfs_cfg;
+    std::vector<double> wvlwts; // nwfs*nwvl
+    // device
+    double *d_ptsx = nullptr, *d_ptsy = nullptr;
+    float *d_amp = nullptr, *d_saa = nullptr;
+    DevWfs *d_wfs = nullptr;
+    float *d_nominal[MX_MAXWVL] = {};
+    float2 *d_etf[MX_MAXWVL] = {}, *d_etf2[MX_MAXWVL] = {};
+    float etf2wt = 0;
+    int4 *d_si_idx[MX_MAXWVL] = {};
+    float4 *d_si_w[MX_MAXWVL] = {};
+    bool dtf_set[MX_MAXWVL] = {};
+    float *d_llt_amp = nullptr;
+    float2 *d_lotf = nullptr;
+    DevWvl *d_wv = nullptr;
+    DevWvl h_wv[MX_MAXWVL] = {};
+    float *d_mtche = nullptr, *d_i0sum = nullptr;
+    bool have_mtche = false, have_i0sum = false;
+    float *d_ints = nullptr, *d_grad = nullptr, *d_opd = nullptr;
+    double *d_i0acc = nullptr;
+    bool i0_enable = false;
+    long i0_nframe = 0;
+    bool queued = false;
+    // timing
+    bool timing = false;
+    long t_n = 0;
+    double t_llt = 0, t_ints = 0;
+    std::vector<hipEvent_t> ev;
+    bool pending_timing = false;
+    // staging
+    std::vector<float> hbuf;
+};
+
+namespace {
+struct AtmState {
+    int nps = 0;
+    double atmscale = 1;
+    DevLayer h_layers[MX_MAXLAYER];
+    DevLayer *d_layers = nullptr;
+    float *d_data = nullptr;
+    size_t bytes = 0;
+};
+AtmState g_atm;
+int g_device = 0;
+
+template <typename T>
+int dev_alloc(T **p, size_t n) {
+    if (*p) return 0;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T) + 16);
+    if (e != hipSuccess) return set_err(MX_EHIP, "hipMalloc(%zu) failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+// transposed FP32 copy of an N x N table: dst[ux*N+uy] = src[ux+uy*N]
+void transpose_c(float2 *dst, const double *src, long N, bool conj) {
+    for (long uy = 0; uy < N; uy++)
+        for (long ux = 0; ux < N; ux++) {
+            const double *s = src + 2 * (ux + uy * N);
+            dst[ux * N + uy] = make_float2((float)s[0], (float)(conj ? -s[1] : s[1]));
+        }
+}
+
+bool supported(long notf, long nxsa, long embfac) {
+    return (notf == 64 && nxsa == 32 && embfac == 2) || (notf == 32 && nxsa == 16 && embfac == 2);
+}
+} // namespace
+
+extern "C" {
+
+const char *mx_last_error(void) { return g_err; }
+
+int mx_init(int device) {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(MX_EINVAL, "device %d out of range (%d GPUs)", device, n);
+    HIPCHK(hipSetDevice(device));
+    g_device = device;
+    return MX_OK;
+}
+
+int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs, int nwfs) {
+    if (!out || !p || !wfs || nwfs <= 0) return set_err(MX_EINVAL, "mx_wfsgrad_init: null argument");
+    *out = nullptr;
+    if (!supported(p->notf, p->nxsa, p->embfac))
+        return set_err(MX_EINVAL, "unsupported size: notf=%ld nxsa=%ld embfac=%ld (supported: 64/32/2, 32/16/2)",
+                       p->notf, p->nxsa, p->embfac);
+    if (p->nwvl < 1 || p->nwvl > MX_MAXWVL) return set_err(MX_EINVAL, "nwvl=%ld out of range", p->nwvl);
+    const long npix = p->pixpsax * p->pixpsay;
+    const long ppl = (npix + p->notf - 1) / p->notf;
+    if (npix < 1 || ppl > 2) return set_err(MX_EINVAL, "npix=%ld too large for notf=%ld", npix, p->notf);
+    if (p->phytype != MX_PTYPE_MF && p->phytype != MX_PTYPE_COG)
+        return set_err(MX_EINVAL, "phytype %d unsupported", p->phytype);
+    if (p->sigmatch < 0 || p->sigmatch > 1)
+        return set_err(MX_EINVAL, "sigmatch %d unsupported (0 or 1)", p->sigmatch);
+    if (p->llt_nx > 0 && !(p->llt_nx * p->embfac == p->notf && p->llt_nx == p->nxsa))
+        return set_err(MX_EINVAL, "LLT grid %ld must equal nxsa with nlwvf == notf", p->llt_nx);
+    if (p->nsa < 1) return set_err(MX_EINVAL, "nsa=%ld", p->nsa);
+    HIPCHK(hipSetDevice(g_device));
+    mx_wfsgrad_t *h = new mx_wfsgrad_t();
+    h->device = g_device;
+    h->cfg = *p;
+    h->nsa = p->nsa;
+    h->nxsa = p->nxsa;
+    h->embfac = p->embfac;
+    h->nwvf = p->nxsa * p->embfac;
+    h->notf = p->notf;
+    h->pixpsax = p->pixpsax;
+    h->pixpsay = p->pixpsay;
+    h->npix = npix;
+    h->nwvl = p->nwvl;
+    h->nwfs = nwfs;
+    h->wvl.assign(p->wvl, p->wvl + p->nwvl);
+    h->wfs_cfg.assign(wfs, wfs + nwfs);
+    int rc = 0;
+#define ALLOC(ptr, n)                                   \
+    if ((rc = dev_alloc(&(ptr), (n))) != 0) {           \
+        mx_wfsgrad_free(h);                             \
+        return rc;                                      \
+    }
+    const long nsa = h->nsa, N = h->notf, npsa = h->nxsa * h->nxsa;
+    ALLOC(h->d_ptsx, nsa);
+    ALLOC(h->d_ptsy, nsa);
+    ALLOC(h->d_amp, nsa * npsa);
+    ALLOC(h->d_saa, nsa);
+    ALLOC(h->d_wfs, nwfs);
+    ALLOC(h->d_wv, MX_MAXWVL);
+    ALLOC(h->d_ints, nwfs * nsa * npix);
+    ALLOC(h->d_grad, nwfs * 2 * nsa);
+    ALLOC(h->d_opd, nwfs * nsa * npsa);
+    ALLOC(h->d_i0acc, nwfs * nsa * npix);
+    ALLOC(h->d_mtche, nwfs * nsa * 2 * npix);
+    ALLOC(h->d_i0sum, nwfs * nsa);
+    if (p->llt_nx > 0) {
+        ALLOC(h->d_llt_amp, p->llt_nx * p->llt_nx);
+        ALLOC(h->d_lotf, nwfs * h->nwvl * N * N);
+    }
+#undef ALLOC
+    auto fail = [&](int code) {
+        mx_wfsgrad_free(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(MX_EHIP, "hipStreamCreate failed"));
+    std::vector<float> tmp(nsa * npsa);
+    for (long i = 0; i < nsa * npsa; i++) tmp[i] = (float)p->amp[i];
+    if (hipMemcpy(h->d_amp, tmp.data(), sizeof(float) * nsa * npsa, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_ptsx, p->ptsx, sizeof(double) * nsa, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_ptsy, p->ptsy, sizeof(double) * nsa, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(MX_EHIP, "upload of geometry failed"));
+    for (long i = 0; i < nsa; i++) tmp[i] = (float)p->saa[i];
+    if (hipMemcpy(h->d_saa, tmp.data(), sizeof(float) * nsa, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(MX_EHIP, "upload of saa failed"));
+    if (p->llt_nx > 0) {
+        std::vector<float> la(p->llt_nx * p->llt_nx);
+        for (size_t i = 0; i < la.size(); i++) la[i] = (float)p->llt_amp[i];
+        if (hipMemcpy(h->d_llt_amp, la.data(), sizeof(float) * la.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(MX_EHIP, "upload of LLT amplitude failed"));
+    }
+    // per-WFS constants; norm_ints of wfsints.c:111-115
+    const double norm_psf = std::sqrt(p->areascale) / (double)(h->nxsa * h->nwvf);
+    std::vector<DevWfs> dw(nwfs);
+    for (int i = 0; i < nwfs; i++) {
+        dw[i].thetax = wfs[i].thetax;
+        dw[i].thetay = wfs[i].thetay;
+        dw[i].hs = std::isinf(wfs[i].hs) ? 0. : wfs[i].hs;
+        dw[i].misregx = wfs[i].misregx;
+        dw[i].misregy = wfs[i].misregy;
+        const double norm_ints = wfs[i].sigsim * norm_psf * norm_psf / ((double)N * N);
+        for (int w = 0; w < MX_MAXWVL; w++)
+            dw[i].alpha[w] = w < h->nwvl ? (float)(wfs[i].wvlwts[w] * norm_ints) : 0.f;
+    }
+    if (hipMemcpy(h->d_wfs, dw.data(), sizeof(DevWfs) * nwfs, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(MX_EHIP, "upload of WFS parameters failed"));
+    if (hipMemset(h->d_grad, 0, sizeof(float) * nwfs * 2 * nsa) != hipSuccess ||
+        hipMemset(h->d_i0acc, 0, sizeof(double) * nwfs * nsa * npix) != hipSuccess ||
+        hipMemset(h->d_mtche, 0, sizeof(float) * nwfs * nsa * 2 * npix) != hipSuccess ||
+        hipMemset(h->d_i0sum, 0, sizeof(float) * nwfs * nsa) != hipSuccess)
+        return fail(set_err(MX_EHIP, "clearing buffers failed"));
+    *out = h;
+    return MX_OK;
+}
+
+int mx_wfsgrad_free(mx_wfsgrad_t *h) {
+    if (!h) return MX_OK;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_lotf,
+                    h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
+    for (void *q : ptrs)
+        if (q) hipFree(q);
+    for (int w = 0; w < MX_MAXWVL; w++) {
+        if (h->d_nominal[w]) hipFree(h->d_nominal[w]);
+        if (h->d_etf[w]) hipFree(h->d_etf[w]);
+        if (h->d_etf2[w]) hipFree(h->d_etf2[w]);
+        if (h->d_si_idx[w]) hipFree(h->d_si_idx[w]);
+        if (h->d_si_w[w]) hipFree(h->d_si_w[w]);
+    }
+    for (hipEvent_t e : h->ev) hipEventDestroy(e);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si_nx, long si_ny,
+                       const long *si_pp, const long *si_pi, const double *si_px) {
+    if (!h || iwvl < 0 || iwvl >= h->nwvl) return set_err(MX_EINVAL, "mx_wfsgrad_set_dtf: bad handle or iwvl");
+    const long N = h->notf;
+    if (si_nx != h->npix || si_ny != N * N || !si_pp || !si_pi || !si_px)
+        return set_err(MX_EINVAL, "si must be %ld x %ld (got %ld x %ld)", h->npix, N * N, si_nx, si_ny);
+    HIPCHK(hipSetDevice(h->device));
+    // per-pixel list of up to 4 (index, weight) pairs
+    std::vector<int4> idx(h->npix);
+    std::vector<float4> wt(h->npix);
+    std::vector<int> cnt(h->npix, 0);
+    for (long i = 0; i < h->npix; i++) {
+        idx[i] = make_int4(0, 0, 0, 0);
+        wt[i] = make_float4(0, 0, 0, 0);
+    }
+    for (long col = 0; col < N * N; col++) {
+        for (long k = si_pp[col]; k < si_pp[col + 1]; k++) {
+            long r = si_pi[k];
+            if (r < 0 || r >= h->npix) return set_err(MX_EINVAL, "si row index %ld out of range", r);
+            int c = cnt[r]++;
+            if (c >= 4) return set_err(MX_EINVAL, "si pixel %ld has more than 4 entries", r);
+            int *ip = &idx[r].x;
+            float *wp = &wt[r].x;
+            ip[c] = (int)col;
+            wp[c] = (float)si_px[k];
+        }
+    }
+    int rc;
+    if ((rc = dev_alloc(&h->d_si_idx[iwvl], h->npix)) || (rc = dev_alloc(&h->d_si_w[iwvl], h->npix))) return rc;
+    HIPCHK(hipMemcpy(h->d_si_idx[iwvl], idx.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_si_w[iwvl], wt.data(), sizeof(float4) * h->npix, hipMemcpyHostToDevice));
+    if (nominal && h->cfg.llt_nx == 0) {
+        if ((rc = dev_alloc(&h->d_nominal[iwvl], N * N))) return rc;
+        std::vector<float> nt(N * N);
+        for (long uy = 0; uy < N; uy++)
+            for (long ux = 0; ux < N; ux++) nt[ux * N + uy] = (float)nominal[ux + uy * N];
+        HIPCHK(hipMemcpy(h->d_nominal[iwvl], nt.data(), sizeof(float) * N * N, hipMemcpyHostToDevice));
+    }
+    h->dtf_set[iwvl] = true;
+    return MX_OK;
+}
+
+int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const double *etf2, double etf2wt) {
+    if (!h || iwvl < 0 || iwvl >= h->nwvl || !etf) return set_err(MX_EINVAL, "mx_wfsgrad_update_etf: bad argument");
+    if (h->cfg.llt_nx == 0) return set_err(MX_EINVAL, "ETF given for a powfs without LLT");
+    HIPCHK(hipSetDevice(h->device));
+    const long N = h->notf, nsa = h->nsa;
+    std::vector<float2> t(nsa * N * N);
+    int rc;
+    for (int which = 0; which < 2; which++) {
+        const double *src = which ? etf2 : etf;
+        if (!src) {
+            if (which && h->d_etf2[iwvl]) {
+                hipFree(h->d_etf2[iwvl]);
+                h->d_etf2[iwvl] = nullptr;
+            }
+            continue;
+        }
+        for (long isa = 0; isa < nsa; isa++) transpose_c(t.data() + isa * N * N, src + 2 * isa * N * N, N, false);
+        float2 **dst = which ? &h->d_etf2[iwvl] : &h->d_etf[iwvl];
+        if ((rc = dev_alloc(dst, nsa * N * N))) return rc;
+        HIPCHK(hipMemcpy(*dst, t.data(), sizeof(float2) * nsa * N * N, hipMemcpyHostToDevice));
+    }
+    h->etf2wt = etf2 ? (float)etf2wt : 0.f;
+    return MX_OK;
+}
+
+int mx_wfsgrad_update_ref(mx_wfsgrad_t *h, int iwfs, const double *mtche, const double *i0sum) {
+    if (!h || iwfs < -1 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_update_ref: bad handle or iwfs");
+    HIPCHK(hipSetDevice(h->device));
+    const long n1 = h->nsa * 2 * h->npix;
+    const int w0 = iwfs < 0 ? 0 : iwfs, w1 = iwfs < 0 ? h->nwfs : iwfs + 1;
+    std::vector<float> t(n1);
+    for (int w = w0; w < w1; w++) {
+        if (mtche) {
+            for (long i = 0; i < n1; i++) t[i] = (float)mtche[i];
+            HIPCHK(hipMemcpy(h->d_mtche + w * n1, t.data(), sizeof(float) * n1, hipMemcpyHostToDevice));
+        }
+        if (i0sum) {
+            for (long i = 0; i < h->nsa; i++) t[i] = (float)i0sum[i];
+            HIPCHK(hipMemcpy(h->d_i0sum + w * h->nsa, t.data(), sizeof(float) * h->nsa, hipMemcpyHostToDevice));
+        }
+    }
+    if (mtche) h->have_mtche = true;
+    if (i0sum) h->have_i0sum = true;
+    return MX_OK;
+}
+
+int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
+    if (nps < 0 || nps > MX_MAXLAYER || (nps && !layers)) return set_err(MX_EINVAL, "mx_atm2gpu: nps=%d", nps);
+    HIPCHK(hipSetDevice(g_device));
+    size_t total = 0;
+    for (int i = 0; i < nps; i++) {
+        if (layers[i].nx < 2 || layers[i].ny < 2 || !layers[i].p || layers[i].dx <= 0 || layers[i].dy <= 0)
+            return set_err(MX_EINVAL, "layer %d is invalid", i);
+        total += (size_t)layers[i].nx * layers[i].ny;
+    }
+    if (g_atm.d_data && g_atm.bytes < total * sizeof(float)) {
+        hipFree(g_atm.d_data);
+        g_atm.d_data = nullptr;
+    }
+    if (!g_atm.d_data && total) {
+        HIPCHK(hipMalloc(&g_atm.d_data, total * sizeof(float)));
+        g_atm.bytes = total * sizeof(float);
+    }
+    if (!g_atm.d_layers) HIPCHK(hipMalloc(&g_atm.d_layers, sizeof(DevLayer) * MX_MAXLAYER));
+    size_t off = 0;
+    std::vector<float> tmp;
+    for (int i = 0; i < nps; i++) {
+        const mx_atm_layer_t &L = layers[i];
+        size_t n = (size_t)L.nx * L.ny;
+        tmp.resize(n);
+        for (size_t k = 0; k < n; k++) tmp[k] = (float)L.p[k];
+        HIPCHK(hipMemcpy(g_atm.d_data + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        DevLayer &D = g_atm.h_layers[i];
+        D.p = g_atm.d_data + off;
+        D.nx = (int)L.nx;
+        D.ny = (int)L.ny;
+        D.dx1 = 1. / L.dx;
+        D.dy1 = 1. / L.dy;
+        D.ox = L.ox;
+        D.oy = L.oy;
+        D.ht = L.ht;
+        D.vx = L.vx;
+        D.vy = L.vy;
+        off += n;
+    }
+    HIPCHK(hipMemcpy(g_atm.d_layers, g_atm.h_layers, sizeof(DevLayer) * MX_MAXLAYER, hipMemcpyHostToDevice));
+    g_atm.nps = nps;
+    g_atm.atmscale = atmscale;
+    return MX_OK;
+}
+
+int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
+    if (!h) return set_err(MX_EINVAL, "mx_wfsgrad_queue: null handle");
+    if (!g_atm.d_layers) return set_err(MX_ESTATE, "no screens uploaded (mx_atm2gpu)");
+    for (int w = 0; w < h->nwvl; w++)
+        if (!h->dtf_set[w]) return set_err(MX_ESTATE, "dtf of wavelength %d not set", w);
+    if (h->cfg.llt_nx > 0)
+        for (int w = 0; w < h->nwvl; w++)
+            if (!h->d_etf[w]) return set_err(MX_ESTATE, "etf of wavelength %d not set", w);
+    if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche) return set_err(MX_ESTATE, "matched filter not set");
+    if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum)
+        return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=1)");
+    HIPCHK(hipSetDevice(h->device));
+    const long N = h->notf;
+    {
+        DevWvl hw[MX_MAXWVL] = {};
+        for (int w = 0; w < h->nwvl; w++) {
+            hw[w].inv_wvl = (float)(1. / h->wvl[w]);
+            hw[w].nominal = h->d_nominal[w];
+            hw[w].etf = h->d_etf[w];
+            hw[w].etf2 = h->d_etf2[w];
+            hw[w].si_idx = h->d_si_idx[w];
+            hw[w].si_w = h->d_si_w[w];
+        }
+        if (std::memcmp(hw, h->h_wv, sizeof(hw)) != 0) {
+            HIPCHK(hipMemcpyAsync(h->d_wv, hw, sizeof(hw), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            std::memcpy(h->h_wv, hw, sizeof(hw));
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    if (h->timing) {
+        if (h->ev.size() < 3) {
+            h->ev.resize(3);
+            for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
+        }
+        e0 = h->ev[0];
+        e1 = h->ev[1];
+        e2 = h->ev[2];
+        HIPCHK(hipEventRecord(e0, h->stream));
+    }
+    if (h->cfg.llt_nx > 0) {
+        LltArgs la{};
+        la.nwvl = (int)h->nwvl;
+        la.nwfs = h->nwfs;
+        la.nps = g_atm.nps;
+        la.ptsdx = h->cfg.dx;
+        la.origx = h->cfg.llt_origx;
+        la.origy = h->cfg.llt_origy;
+        la.ox = h->cfg.llt_ox;
+        la.oy = h->cfg.llt_oy;
+        la.amp = h->d_llt_amp;
+        la.wv = h->d_wv;
+        const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
+        la.norm = (float)(1. / (nlwvf * nlwvf));
+        la.layers = g_atm.d_layers;
+        la.atmscale = g_atm.atmscale;
+        la.wfs = h->d_wfs;
+        la.isim = isim;
+        la.dt = dt;
+        la.lotf = h->d_lotf;
+        dim3 grid((unsigned)(h->nwfs * h->nwvl));
+        if (N == 64) hipLaunchKernelGGL((k_llt_otf<64, 32>), grid, dim3(64), 0, h->stream, la);
+        else hipLaunchKernelGGL((k_llt_otf<32, 16>), grid, dim3(64), 0, h->stream, la);
+        HIPCHK(hipGetLastError());
+    }
+    if (h->timing) HIPCHK(hipEventRecord(e1, h->stream));
+    KArgs a{};
+    a.nsa = (int)h->nsa;
+    a.npix = (int)h->npix;
+    a.pixpsax = (int)h->pixpsax;
+    a.pixpsay = (int)h->pixpsay;
+    a.nwvl = (int)h->nwvl;
+    a.nwfs = h->nwfs;
+    a.nps = g_atm.nps;
+    a.ptsdx = h->cfg.dx;
+    a.ptsx = h->d_ptsx;
+    a.ptsy = h->d_ptsy;
+    a.amp = h->d_amp;
+    a.wv = h->d_wv;
+    a.etf2wt = h->etf2wt;
+    a.etf1wt = 1.f - h->etf2wt;
+    a.layers = g_atm.d_layers;
+    a.atmscale = g_atm.atmscale;
+    a.wfs = h->d_wfs;
+    a.isim = isim;
+    a.dt = dt;
+    a.lotf = h->cfg.llt_nx > 0 ? h->d_lotf : nullptr;
+    a.phytype = h->cfg.phytype;
+    a.sigmatch = h->cfg.sigmatch;
+    a.cogthres = (float)h->cfg.cogthres;
+    a.cogoff = (float)h->cfg.cogoff;
+    a.pixthetax = (float)h->cfg.pixthetax;
+    a.pixthetay = (float)h->cfg.pixthetay;
+    a.sigtot = (float)h->cfg.siglev_dtrat;
+    a.mtche = h->d_mtche;
+    a.i0sum = h->have_i0sum ? h->d_i0sum : nullptr;
+    a.saa = h->d_saa;
+    a.saat = (float)h->cfg.saat;
+    a.ints = h->d_ints;
+    a.grad = h->d_grad;
+    a.i0acc = h->i0_enable ? h->d_i0acc : nullptr;
+    a.opd_dbg = h->d_opd;
+    const int spw = (int)(64 / N);
+    const long ngroup = (h->nsa + spw - 1) / spw;
+    dim3 grid((unsigned)(ngroup * h->nwfs));
+    if (N == 64) hipLaunchKernelGGL((k_shwfs_step<64, 32, 1>), grid, dim3(64), 0, h->stream, a);
+    else hipLaunchKernelGGL((k_shwfs_step<32, 16, 2>), grid, dim3(64), 0, h->stream, a);
+    HIPCHK(hipGetLastError());
+    if (h->timing) {
+        HIPCHK(hipEventRecord(e2, h->stream));
+        HIPCHK(hipEventSynchronize(e2));
+        float m1 = 0, m2 = 0;
+        HIPCHK(hipEventElapsedTime(&m1, e0, e1));
+        HIPCHK(hipEventElapsedTime(&m2, e1, e2));
+        h->t_llt += m1;
+        h->t_ints += m2;
+        h->t_n++;
+    }
+    if (h->i0_enable) h->i0_nframe++;
+    h->queued = true;
+    return MX_OK;
+}
+
+int mx_wfsgrad_wait(mx_wfsgrad_t *h) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return MX_OK;
+}
+
+int mx_wfsgrad_sync(mx_wfsgrad_t *h, int iwfs, double *grad, double *ints) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_sync: bad handle or iwfs");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long nsa = h->nsa;
+    if (grad) {
+        h->hbuf.resize(2 * nsa);
+        HIPCHK(hipMemcpy(h->hbuf.data(), h->d_grad + iwfs * 2 * nsa, sizeof(float) * 2 * nsa, hipMemcpyDeviceToHost));
+        for (long i = 0; i < 2 * nsa; i++) grad[i] = h->hbuf[i];
+    }
+    if (ints) {
+        const long n = nsa * h->npix;
+        h->hbuf.resize(n);
+        HIPCHK(hipMemcpy(h->hbuf.data(), h->d_ints + iwfs * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (long i = 0; i < n; i++) ints[i] = h->hbuf[i];
+    }
+    return MX_OK;
+}
+
+int mx_wfsgrad_i0_control(mx_wfsgrad_t *h, int enable, int reset) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    if (reset) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemset(h->d_i0acc, 0, sizeof(double) * h->nwfs * h->nsa * h->npix));
+        h->i0_nframe = 0;
+    }
+    h->i0_enable = enable != 0;
+    return MX_OK;
+}
+
+int mx_wfsgrad_i0_get(mx_wfsgrad_t *h, int iwfs, double *out, long *nframe) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_i0_get: bad handle or iwfs");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long n = h->nsa * h->npix;
+    if (out) HIPCHK(hipMemcpy(out, h->d_i0acc + iwfs * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (nframe) *nframe = h->i0_nframe;
+    return MX_OK;
+}
+
+int mx_wfsgrad_debug_opd(mx_wfsgrad_t *h, int iwfs, double *opd) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs || !opd) return set_err(MX_EINVAL, "mx_wfsgrad_debug_opd: bad argument");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long n = h->nsa * h->nxsa * h->nxsa;
+    h->hbuf.resize(n);
+    HIPCHK(hipMemcpy(h->hbuf.data(), h->d_opd + iwfs * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (long i = 0; i < n; i++) opd[i] = h->hbuf[i];
+    return MX_OK;
+}
+
+int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs || iwvl < 0 || iwvl >= h->nwvl || !lotf)
+        return set_err(MX_EINVAL, "mx_wfsgrad_debug_lotf: bad argument");
+    if (!h->d_lotf) return set_err(MX_ESTATE, "powfs has no LLT");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long N = h->notf;
+    std::vector<float2> t(N * N);
+    HIPCHK(hipMemcpy(t.data(), h->d_lotf + ((long)iwfs * h->nwvl + iwvl) * N * N, sizeof(float2) * N * N,
+                     hipMemcpyDeviceToHost));
+    for (long ux = 0; ux < N; ux++)
+        for (long uy = 0; uy < N; uy++) {
+            const float2 v = t[ux * N + uy];
+            lotf[2 * (ux + uy * N)] = v.x;
+            lotf[2 * (ux + uy * N) + 1] = -v.y; // stored conjugated
+        }
+    return MX_OK;
+}
+
+int mx_wfsgrad_timing_enable(mx_wfsgrad_t *h, int enable) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    h->timing = enable != 0;
+    h->t_n = 0;
+    h->t_llt = h->t_ints = 0;
+    return MX_OK;
+}
+
+int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *n, double *ms_llt, double *ms_ints) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    if (n) *n = h->t_n;
+    if (ms_llt) *ms_llt = h->t_llt;
+    if (ms_ints) *ms_ints = h->t_ints;
+    return MX_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,143 @@
+"""Python handle on the GPU WFS engine (libmaos_mi355x.so).
+
+WfsGrad follows the lifecycle of MAOS's GPU plugin (cuda/sim/gpu_sim.h:
+25-55): build it from a Powfs and its WFS list (gpu_wfsgrad_init), give it
+screens (gpu_atm2gpu), then queue steps and sync gradients
+(gpu_wfsgrad_queue / gpu_wfsgrad_sync).  Every call goes through the C ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import MxAtmLayer, MxPowfs, MxWfs, check, dptr, lptr
+
+
+class WfsSpec:
+    """parms->wfs[iwfs]: direction (rad), range, photons/subap/frame."""
+
+    def __init__(self, thetax=0.0, thetay=0.0, hs=np.inf, sigsim=1.0, wvlwts=(1.0,),
+                 misregx=0.0, misregy=0.0):
+        self.thetax, self.thetay, self.hs, self.sigsim = thetax, thetay, hs, sigsim
+        self.wvlwts = np.ascontiguousarray(wvlwts, dtype=np.float64)
+        self.misregx, self.misregy = misregx, misregy
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    check(_lib.hip().mx_init(device))
+    _initialised = True
+
+
+def atm2gpu(layers, atmscale=1.0):
+    """Upload screens (list of powfs.Layer)."""
+    if not _initialised:
+        init(0)
+    arr = (MxAtmLayer * max(1, len(layers)))()
+    keep = []
+    for i, L in enumerate(layers):
+        p = np.ascontiguousarray(L.p, dtype=np.float64)
+        keep.append(p)
+        arr[i] = MxAtmLayer(dptr(p), L.nx, L.ny, L.dx, L.dx, L.ox, L.oy, L.ht, L.vx, L.vy)
+    check(_lib.hip().mx_atm2gpu(arr, len(layers), atmscale))
+
+
+class WfsGrad:
+    def __init__(self, powfs, wfs_list):
+        if not _initialised:
+            init(0)
+        self.powfs = powfs
+        self.nwfs = len(wfs_list)
+        self.nsa, self.npix = powfs.nsa, powfs.npix
+        p = powfs
+        self._keep = [p.ptsx, p.ptsy, p.amp, p.saa, p.wvl]
+        llt_amp = getattr(p, "llt_amp", None)
+        cfg = MxPowfs(
+            nsa=p.nsa, nxsa=p.nxsa, embfac=p.cfg.embfac, notf=p.notf, pixpsax=p.pixpsax,
+            pixpsay=p.pixpsay, nwvl=p.nwvl, dx=p.dx, ptsx=dptr(p.ptsx), ptsy=dptr(p.ptsy),
+            amp=dptr(p.amp), saa=dptr(p.saa), saat=p.cfg.saat, areascale=p.areascale,
+            wvl=dptr(p.wvl), phytype=p.cfg.phytype, sigmatch=p.cfg.sigmatch,
+            cogthres=p.cfg.cogthres, cogoff=p.cfg.cogoff, pixthetax=p.pixthetax,
+            pixthetay=p.pixthetay, siglev_dtrat=p.cfg.siglev * p.cfg.dtrat, saasum=p.saasum,
+            llt_nx=p.llt_nx, llt_origx=getattr(p, "llt_origin", 0.0),
+            llt_origy=getattr(p, "llt_origin", 0.0),
+            llt_ox=p.llt.ox if p.llt else 0.0, llt_oy=p.llt.oy if p.llt else 0.0,
+            llt_amp=dptr(llt_amp))
+        wa = (MxWfs * self.nwfs)()
+        for i, w in enumerate(wfs_list):
+            self._keep.append(w.wvlwts)
+            wa[i] = MxWfs(w.thetax, w.thetay, w.hs, w.sigsim, w.misregx, w.misregy, dptr(w.wvlwts))
+        h = C.c_void_p()
+        check(_lib.hip().mx_wfsgrad_init(C.byref(h), C.byref(cfg), wa, self.nwfs))
+        self.h = h
+        for iw in range(p.nwvl):
+            check(_lib.hip().mx_wfsgrad_set_dtf(
+                h, iw, dptr(p.nominal[iw]), p.npix, p.notf * p.notf, lptr(p.si_pp[iw]),
+                lptr(p.si_pi[iw]), dptr(p.si_px[iw])))
+            if p.etf is not None:
+                check(_lib.hip().mx_wfsgrad_update_etf(h, iw, dptr(p.etf[iw]), None, 0.0))
+
+    def update_ref(self, mtche=None, i0sum=None, iwfs=-1):
+        m = None if mtche is None else np.ascontiguousarray(mtche, dtype=np.float64)
+        s = None if i0sum is None else np.ascontiguousarray(i0sum, dtype=np.float64)
+        check(_lib.hip().mx_wfsgrad_update_ref(self.h, iwfs, dptr(m), dptr(s)))
+
+    def update_etf(self, iwvl, etf, etf2=None, etf2wt=0.0):
+        check(_lib.hip().mx_wfsgrad_update_etf(self.h, iwvl, dptr(etf), dptr(etf2), etf2wt))
+
+    def queue(self, isim, dt):
+        check(_lib.hip().mx_wfsgrad_queue(self.h, isim, dt))
+
+    def wait(self):
+        check(_lib.hip().mx_wfsgrad_wait(self.h))
+
+    def sync(self, iwfs, want_ints=True):
+        grad = np.zeros(2 * self.nsa)
+        ints = np.zeros(self.nsa * self.npix) if want_ints else None
+        check(_lib.hip().mx_wfsgrad_sync(self.h, iwfs, dptr(grad), dptr(ints)))
+        return grad, (ints.reshape(self.nsa, self.npix) if want_ints else None)
+
+    def i0_control(self, enable=True, reset=False):
+        check(_lib.hip().mx_wfsgrad_i0_control(self.h, int(enable), int(reset)))
+
+    def i0_get(self, iwfs):
+        out = np.zeros(self.nsa * self.npix)
+        n = C.c_long(0)
+        check(_lib.hip().mx_wfsgrad_i0_get(self.h, iwfs, dptr(out), C.cast(C.byref(n), _lib.c_long_p)))
+        return out.reshape(self.nsa, self.npix), n.value
+
+    def debug_opd(self, iwfs):
+        out = np.zeros(self.nsa * self.powfs.nxsa ** 2)
+        check(_lib.hip().mx_wfsgrad_debug_opd(self.h, iwfs, dptr(out)))
+        return out
+
+    def debug_lotf(self, iwfs, iwvl=0):
+        out = np.zeros(2 * self.powfs.notf ** 2)
+        check(_lib.hip().mx_wfsgrad_debug_lotf(self.h, iwfs, iwvl, dptr(out)))
+        return out
+
+    def timing_enable(self, on=True):
+        check(_lib.hip().mx_wfsgrad_timing_enable(self.h, int(on)))
+
+    def timing(self):
+        n = C.c_long(0)
+        a = C.c_double(0)
+        b = C.c_double(0)
+        check(_lib.hip().mx_wfsgrad_timing(self.h, C.cast(C.byref(n), _lib.c_long_p),
+                                           C.cast(C.byref(a), _lib.c_double_p),
+                                           C.cast(C.byref(b), _lib.c_double_p)))
+        return n.value, a.value, b.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.hip().mx_wfsgrad_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

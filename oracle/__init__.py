@@ -1,0 +1,183 @@
+"""ctypes front end of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py's cpu_baseline leg.  The product never uses it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+dp = C.POINTER(C.c_double)
+lp = C.POINTER(C.c_long)
+
+
+class OrcPowfs(C.Structure):
+    _fields_ = [("nsa", C.c_long), ("nxsa", C.c_long), ("embfac", C.c_long), ("notf", C.c_long),
+                ("pixpsax", C.c_long), ("pixpsay", C.c_long), ("nwvl", C.c_long),
+                ("dx", C.c_double), ("areascale", C.c_double), ("saat", C.c_double),
+                ("ptsx", dp), ("ptsy", dp), ("amp", dp), ("saa", dp), ("wvl", dp),
+                ("nominal", dp), ("si_pp", lp), ("si_pi", lp), ("si_px", dp),
+                ("si_nzmax", C.c_long), ("etf", dp), ("etf2", dp), ("etf2wt", C.c_double),
+                ("llt_nx", C.c_long), ("llt_origx", C.c_double), ("llt_origy", C.c_double),
+                ("llt_ox", C.c_double), ("llt_oy", C.c_double), ("llt_amp", dp),
+                ("phytype", C.c_int), ("sigmatch", C.c_int), ("cogthres", C.c_double),
+                ("cogoff", C.c_double), ("pixthetax", C.c_double), ("pixthetay", C.c_double),
+                ("siglev_dtrat", C.c_double), ("saasum", C.c_double), ("mtche", dp),
+                ("i0sum", dp), ("i0sumsum", C.c_double)]
+
+
+class OrcWfs(C.Structure):
+    _fields_ = [("thetax", C.c_double), ("thetay", C.c_double), ("hs", C.c_double),
+                ("sigsim", C.c_double), ("misregx", C.c_double), ("misregy", C.c_double),
+                ("wvlwts", dp)]
+
+
+class OrcAtm(C.Structure):
+    _fields_ = [("nps", C.c_long), ("nx", C.c_long), ("ny", C.c_long), ("dx", C.c_double),
+                ("ox", C.c_double), ("oy", C.c_double), ("screens", C.POINTER(dp)),
+                ("ht", dp), ("vx", dp), ("vy", dp)]
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(dp)
+
+
+def _l(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int64 and a.flags.c_contiguous
+    return a.ctypes.data_as(lp)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError("oracle not built: make -C %s" % HERE)
+        L = C.CDLL(LIB)
+        L.orc_fft2.argtypes = [dp, C.c_long, C.c_long, C.c_int]
+        L.orc_prop_grid_pts.restype = C.c_long
+        L.orc_prop_grid_pts.argtypes = [dp, C.c_long, C.c_long, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, dp, dp, C.c_long, C.c_long, C.c_double, dp,
+                                        C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.c_long, C.c_long]
+        L.orc_prop.restype = C.c_long
+        L.orc_prop.argtypes = [dp, C.c_long, C.c_long, C.c_double, C.c_double, C.c_double,
+                               C.c_double, dp, dp, C.c_long, C.c_long, C.c_double, dp, C.c_double,
+                               C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_double, C.c_int]
+        L.orc_embed_wvf.argtypes = [dp, C.c_long, dp, dp, C.c_long, C.c_double]
+        L.orc_lotf.argtypes = [dp, dp, dp, C.c_long, C.c_long, C.c_long, C.c_long, C.c_double]
+        L.orc_wfsints.argtypes = [dp, dp, dp, C.POINTER(OrcPowfs), C.POINTER(OrcWfs), C.c_long,
+                                  C.c_long]
+        L.orc_dcog.argtypes = [dp, dp, C.c_long, C.c_long, C.c_double, C.c_double, C.c_double,
+                               C.c_double, C.c_double]
+        L.orc_shwfs_grad.argtypes = [dp, dp, C.POINTER(OrcPowfs), C.c_double]
+        L.orc_wfsgrad_iwfs.argtypes = [dp, dp, C.POINTER(OrcPowfs), C.POINTER(OrcWfs),
+                                       C.POINTER(OrcAtm), C.c_long, C.c_double, C.c_double,
+                                       C.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+def fft2(a, dir):
+    """In-place 2-D FFT of a complex128 array [ny, nx] (x fastest)."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    ny, nx = a.shape
+    lib().orc_fft2(a.view(np.float64).ctypes.data_as(dp), nx, ny, dir)
+    return a
+
+
+def dcog(im, thres=0.0, bkgrnd=0.0, flux=0.0, offx=0.0, offy=0.0):
+    im = np.ascontiguousarray(im, dtype=np.float64)
+    ny, nx = im.shape
+    g = np.zeros(2)
+    lib().orc_dcog(_d(g), _d(im), nx, ny, offx, offy, thres, bkgrnd, flux)
+    return g
+
+
+class Oracle:
+    """FP64 oracle bound to one Powfs (maos_amd.powfs.Powfs tables)."""
+
+    def __init__(self, powfs, mtche=None, i0sum=None):
+        p = powfs
+        self.p = p
+        self.keep = []
+
+        def k(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a)
+            self.keep.append(a)
+            return a
+
+        self.s = OrcPowfs()
+        s = self.s
+        s.nsa, s.nxsa, s.embfac, s.notf = p.nsa, p.nxsa, p.cfg.embfac, p.notf
+        s.pixpsax, s.pixpsay, s.nwvl = p.pixpsax, p.pixpsay, p.nwvl
+        s.dx, s.areascale, s.saat = p.dx, p.areascale, p.cfg.saat
+        s.ptsx, s.ptsy = _d(k(p.ptsx)), _d(k(p.ptsy))
+        s.amp, s.saa, s.wvl = _d(k(p.amp)), _d(k(p.saa)), _d(k(p.wvl))
+        s.nominal = _d(k(p.nominal.reshape(-1)))
+        s.si_pp, s.si_pi = _l(k(p.si_pp.reshape(-1))), _l(k(p.si_pi.reshape(-1)))
+        s.si_px = _d(k(p.si_px.reshape(-1)))
+        s.si_nzmax = p.si_nzmax
+        s.etf = _d(k(p.etf.reshape(-1))) if p.etf is not None else None
+        s.etf2 = None
+        s.etf2wt = 0.0
+        s.llt_nx = p.llt_nx
+        if p.llt_nx:
+            s.llt_origx = s.llt_origy = p.llt_origin
+            s.llt_ox, s.llt_oy = p.llt.ox, p.llt.oy
+            s.llt_amp = _d(k(p.llt_amp))
+        s.phytype, s.sigmatch = p.cfg.phytype, p.cfg.sigmatch
+        s.cogthres, s.cogoff = p.cfg.cogthres, p.cfg.cogoff
+        s.pixthetax, s.pixthetay = p.pixthetax, p.pixthetay
+        s.siglev_dtrat = p.cfg.siglev * p.cfg.dtrat
+        s.saasum = p.saasum
+        self.set_ref(mtche, i0sum)
+
+    def set_ref(self, mtche=None, i0sum=None):
+        if mtche is not None:
+            self._mtche = np.ascontiguousarray(mtche, dtype=np.float64)
+            self.s.mtche = _d(self._mtche)
+        if i0sum is not None:
+            self._i0sum = np.ascontiguousarray(i0sum, dtype=np.float64)
+            self.s.i0sum = _d(self._i0sum)
+            self.s.i0sumsum = float(self._i0sum.sum())
+
+    def atm(self, layers):
+        self._layers = [np.ascontiguousarray(L.p, dtype=np.float64) for L in layers]
+        arr = (dp * len(layers))(*[_d(x) for x in self._layers])
+        self._ht = np.array([L.ht for L in layers])
+        self._vx = np.array([L.vx for L in layers])
+        self._vy = np.array([L.vy for L in layers])
+        L0 = layers[0]
+        self._atm = OrcAtm(len(layers), L0.nx, L0.ny, L0.dx, L0.ox, L0.oy, arr, _d(self._ht),
+                           _d(self._vx), _d(self._vy))
+        self._arr = arr
+
+    def step(self, wfs, isim, dt, atmscale=1.0, nthread=0, want_opd=False, want_lotf=False):
+        """One wfsgrad_iwfs step for WfsSpec wfs; returns (ints, grad, opd, lotf)."""
+        p = self.p
+        w = OrcWfs(wfs.thetax, wfs.thetay, 0.0 if np.isinf(wfs.hs) else wfs.hs, wfs.sigsim,
+                   wfs.misregx, wfs.misregy, _d(wfs.wvlwts))
+        if np.isinf(wfs.hs):
+            w.hs = float("inf")
+        ints = np.zeros(p.nsa * p.npix)
+        grad = np.zeros(2 * p.nsa)
+        opd = np.zeros(p.nsa * p.nxsa ** 2) if want_opd else None
+        lotf = np.zeros(2 * p.notf ** 2 * p.nwvl) if (want_lotf and p.llt_nx) else None
+        lib().orc_wfsgrad_iwfs(_d(ints), _d(grad), C.byref(self.s), C.byref(w), C.byref(self._atm),
+                               isim, dt, atmscale, nthread, _d(opd), _d(lotf))
+        return ints.reshape(p.nsa, p.npix), grad, opd, lotf

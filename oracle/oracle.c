@@ -1,0 +1,530 @@
+/*
+ * FP64 CPU oracle for the MAOS Shack-Hartmann physical-optics WFS step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): a restatement of the reference
+ * algorithm used as the checker and as the timed CPU baseline.
+ */
+#include "oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex cplx;
+#define ORC_EPS 1.e-15 /* sys/common.h:58-63, double build */
+
+/* ---------------------------------------------------------------------
+ * FFT: cached radix-2 plans (bit reversal + twiddles); other sizes by a
+ * direct DFT.  Sign and scaling follow FFTW as used by cfft2
+ * (math/fft.c:253-281): dir=-1 is exp(-i 2 pi jk/n), no normalisation.
+ * ------------------------------------------------------------------- */
+typedef struct {
+    long n;
+    long *rev;
+    cplx *tw; /* exp(-i 2 pi k/n), k<n/2 */
+} plan_t;
+#define NPLAN 24
+static plan_t plans[NPLAN];
+static int nplans = 0;
+
+static const plan_t *get_plan(long n) {
+    for (int i = 0; i < nplans; i++)
+        if (plans[i].n == n) return &plans[i];
+    const plan_t *out = NULL;
+#pragma omp critical(orc_plan)
+    {
+        for (int i = 0; i < nplans; i++)
+            if (plans[i].n == n) out = &plans[i];
+        if (!out && nplans < NPLAN) {
+            plan_t p;
+            p.n = n;
+            p.rev = (long *)malloc(sizeof(long) * n);
+            p.tw = (cplx *)malloc(sizeof(cplx) * (n / 2 + 1));
+            int lg = 0;
+            while ((1L << lg) < n) lg++;
+            for (long i = 0; i < n; i++) {
+                long r = 0;
+                for (int b = 0; b < lg; b++)
+                    if (i & (1L << b)) r |= 1L << (lg - 1 - b);
+                p.rev[i] = r;
+            }
+            for (long k = 0; k <= n / 2; k++) {
+                double a = -2. * M_PI * (double)k / (double)n;
+                p.tw[k] = cos(a) + I * sin(a);
+            }
+            plans[nplans] = p;
+            __atomic_thread_fence(__ATOMIC_SEQ_CST);
+            out = &plans[nplans];
+            nplans++;
+        }
+    }
+    return out;
+}
+
+static void fft_pow2(cplx *x, long n, int dir, const plan_t *p) {
+    for (long i = 0; i < n; i++) {
+        long j = p->rev[i];
+        if (i < j) {
+            cplx t = x[i];
+            x[i] = x[j];
+            x[j] = t;
+        }
+    }
+    for (long len = 2; len <= n; len <<= 1) {
+        long h = len >> 1, step = n / len;
+        for (long i0 = 0; i0 < n; i0 += len) {
+            for (long j = 0; j < h; j++) {
+                cplx w = p->tw[j * step];
+                if (dir > 0) w = conj(w);
+                cplx u = x[i0 + j], v = x[i0 + j + h] * w;
+                x[i0 + j] = u + v;
+                x[i0 + j + h] = u - v;
+            }
+        }
+    }
+}
+
+static void fft_any(cplx *x, long n, int dir, cplx *work) {
+    if (n <= 1) return;
+    if ((n & (n - 1)) == 0) {
+        fft_pow2(x, n, dir, get_plan(n));
+        return;
+    }
+    for (long k = 0; k < n; k++) {
+        cplx s = 0;
+        for (long j = 0; j < n; j++) {
+            double a = dir * 2. * M_PI * (double)((j * k) % n) / (double)n;
+            s += x[j] * (cos(a) + I * sin(a));
+        }
+        work[k] = s;
+    }
+    memcpy(x, work, sizeof(cplx) * n);
+}
+
+void orc_fft2(double *c, long nx, long ny, int dir) {
+    cplx *a = (cplx *)c;
+    long nmax = nx > ny ? nx : ny;
+    cplx stackbuf[2 * 1024];
+    cplx *work = nmax <= 1024 ? stackbuf : (cplx *)malloc(sizeof(cplx) * 2 * nmax);
+    cplx *col = work + nmax;
+    for (long iy = 0; iy < ny; iy++) fft_any(a + iy * nx, nx, dir, work);
+    if (ny > 1) {
+        for (long ix = 0; ix < nx; ix++) {
+            for (long iy = 0; iy < ny; iy++) col[iy] = a[ix + iy * nx];
+            fft_any(col, ny, dir, work);
+            for (long iy = 0; iy < ny; iy++) a[ix + iy * nx] = col[iy];
+        }
+    }
+    if (work != stackbuf) free(work);
+}
+
+/* ---------------------------------------------------------------------
+ * Ray tracing: lib/accphi_grid_do.c
+ * ------------------------------------------------------------------- */
+static inline long ifloor_(double x) { return (long)floor(x); }
+static inline long iceil_(double x) { return (long)ceil(x); }
+
+/* prop_grid_block for wrap=1, TRANSPOSE=0 (lib/accphi_grid_do.c:49-302) */
+static long prop_block_wrap(const double *phiin, long nxin, long nyin, double *phiout,
+                            long nxout, long nyout, double dxout, double dyout,
+                            double oxout, double oyout, double alpha) {
+    const long wrapx = nxin - 1, wrapy = nyin - 1;
+    if (fabs(dxout - 1.) < ORC_EPS && fabs(dyout - 1.) < ORC_EPS) {
+        /* equal sampling: one set of weights for the whole block (:59-146) */
+        double dplocx = oxout, dplocy = oyout;
+        dplocx -= floor(dplocx / (double)nxin) * nxin;
+        long nplocx = ifloor_(dplocx);
+        dplocx -= nplocx;
+        long rowdiv0 = wrapx - nplocx;
+        if (rowdiv0 > nxout) rowdiv0 = nxout;
+        if (rowdiv0 < 0) rowdiv0 = 0;
+        dplocy -= floor(dplocy / (double)nyin) * nyin;
+        long nplocy0 = ifloor_(dplocy);
+        dplocy -= nplocy0;
+        const double bl = dplocx * dplocy;
+        const double br = (1. - dplocx) * dplocy;
+        const double tl = dplocx * (1 - dplocy);
+        const double tr = (1. - dplocx) * (1 - dplocy);
+        const double *pin = phiin + nplocx;
+        for (long icol = 0; icol < nyout; icol++) {
+            double *phiout2 = phiout + icol * nxout;
+            long nplocy = nplocy0 + icol;
+            while (nplocy >= nyin) nplocy -= nyin;
+            const double *phicol = pin + nplocy * nxin;
+            const double *phicol2 = pin + (nplocy == wrapy ? 0 : (nplocy + 1)) * nxin;
+            long irow = 0, rowdiv = rowdiv0;
+            do {
+                for (; irow < rowdiv; irow++)
+                    phiout2[irow] += alpha * (bl * phicol2[irow + 1] + br * phicol2[irow] +
+                                              tl * phicol[irow + 1] + tr * phicol[irow]);
+                if (irow < nxout) {
+                    phiout2[irow] += alpha * (bl * phicol2[irow - wrapx] + br * phicol2[irow] +
+                                              tl * phicol[irow - wrapx] + tr * phicol[irow]);
+                    irow++;
+                }
+                rowdiv = irow + wrapx;
+                phicol2 -= nxin;
+                phicol -= nxin;
+                if (rowdiv > nxout) rowdiv = nxout;
+            } while (irow < nxout);
+        }
+        return 0;
+    }
+    /* unequal sampling with cached x interpolation (:147-302) */
+    const double dxout1 = 1. / dxout;
+    oyout -= floor(oyout / (double)nyin) * nyin;
+    oxout -= floor(oxout / (double)nxin) * nxin;
+    long rowdiv0 = iceil_((wrapx - oxout) * dxout1);
+    if (rowdiv0 <= 0) rowdiv0 = 0;
+    else if (rowdiv0 > nxout) rowdiv0 = nxout;
+    const double dplocx00 = oxout;
+    const double xover = 1;
+    double *cd = (double *)malloc(sizeof(double) * nxout);
+    long *cn = (long *)malloc(sizeof(long) * nxout * 2);
+    long *co = cn + nxout;
+    {
+        long rowdiv = rowdiv0, irow = 0;
+        double dplocx = dplocx00;
+        do {
+            for (; irow < rowdiv; irow++, dplocx += dxout) {
+                long n0 = ifloor_(dplocx);
+                cd[irow] = dplocx - n0;
+                cn[irow] = n0;
+                co[irow] = 1;
+            }
+            for (; irow < nxout && dplocx < wrapx + xover; irow++, dplocx += dxout) {
+                long n0 = ifloor_(dplocx);
+                cd[irow] = dplocx - n0;
+                cn[irow] = n0;
+                co[irow] = -wrapx;
+            }
+            dplocx -= nxin;
+            rowdiv = irow + iceil_((wrapx - dplocx) * dxout1);
+            if (rowdiv > nxout) rowdiv = nxout;
+        } while (irow < nxout);
+    }
+    for (long icol = 0; icol < nyout; icol++) {
+        double *phiout2 = phiout + icol * nxout;
+        double dplocy = fma((double)icol, dyout, oyout);
+        while (dplocy >= nyin) {
+            dplocy -= nyin;
+            oyout -= nyin;
+        }
+        long nplocy = ifloor_(dplocy);
+        double dplocy0 = dplocy - nplocy;
+        const double *phicol = phiin + nplocy * nxin;
+        const double *phicol2 = phiin + (nplocy == wrapy ? 0 : (nplocy + 1)) * nxin;
+        const double dplocy1 = 1. - dplocy0;
+        for (long irow = 0; irow < nxout; irow++) {
+            double dplocx0 = cd[irow];
+            long n0 = cn[irow], off = co[irow];
+            phiout2[irow] += alpha * (dplocy1 * ((1. - dplocx0) * phicol[n0] + dplocx0 * phicol[n0 + off]) +
+                                      dplocy0 * ((1. - dplocx0) * phicol2[n0] + dplocx0 * phicol2[n0 + off]));
+        }
+    }
+    free(cd);
+    free(cn);
+    return 0;
+}
+
+long orc_prop_grid_pts(const double *phiin, long nxin, long nyin, double dxin,
+                       double dyin, double oxin, double oyin, const double *ptsx,
+                       const double *ptsy, long nsa, long nxsa, double ptsdx,
+                       double *phiout, double alpha, double displacex,
+                       double displacey, double scale, int wrap, long start, long end) {
+    if (!wrap) return -1; /* the WFS path always wraps (maos/sim_utils.c:1017) */
+    const double dx_in1 = 1. / dxin, dy_in1 = 1. / dyin;
+    const double dxout = ptsdx * dx_in1 * scale;
+    const double dyout = ptsdx * dy_in1 * scale;
+    displacex = (displacex - oxin) * dx_in1;
+    displacey = (displacey - oyin) * dy_in1;
+    if (!end) end = nsa;
+    for (long isa = start; isa < end; isa++) {
+        const double oxout = ptsx[isa] * dx_in1 * scale + displacex;
+        const double oyout = ptsy[isa] * dy_in1 * scale + displacey;
+        prop_block_wrap(phiin, nxin, nyin, phiout + isa * nxsa * nxsa, nxsa, nxsa, dxout, dyout,
+                        oxout, oyout, alpha);
+    }
+    return 0;
+}
+
+long orc_prop(const double *phiin, long nxin, long nyin, double dxin, double oxin,
+              double oyin, double ht, const double *ptsx, const double *ptsy, long nsa,
+              long nxsa, double ptsdx, double *phiout, double alpha, double hs,
+              double thetax, double thetay, double misregx, double misregy, double shiftx,
+              double shifty, int wrap) {
+    /* prop_range, lib/accphi.c:223-244 (hc=0 for pts output, dratio=0) */
+    if (!alpha) alpha = 1;
+    if (hs && ht > hs) return -1;
+    const double scale = hs ? (1. - ht / hs) : 1.;
+    const double displacex = ht * thetax + scale * misregx + shiftx;
+    const double displacey = ht * thetay + scale * misregy + shifty;
+    return orc_prop_grid_pts(phiin, nxin, nyin, dxin, dxin, oxin, oyin, ptsx, ptsy, nsa, nxsa,
+                             ptsdx, phiout, alpha, displacex, displacey, scale, wrap, 0, nsa);
+}
+
+/* ---------------------------------------------------------------------
+ * Imaging: math/matcomp.c, maos/wfsints.c
+ * ------------------------------------------------------------------- */
+void orc_embed_wvf(double *wvf_, long nwvf, const double *opd, const double *amp, long nopd,
+                   double wvl) {
+    cplx *psf = (cplx *)wvf_;
+    const double wvk = 2. * M_PI / wvl;
+    memset(psf, 0, sizeof(cplx) * nwvf * nwvf);
+    const long skip = (nwvf - nopd) / 2;
+    cplx *psf0 = psf + skip * nwvf + skip;
+    for (long iy = 0; iy < nopd; iy++) {
+        for (long ix = 0; ix < nopd; ix++) {
+            double ph = wvk * opd[iy * nopd + ix];
+            psf0[iy * nwvf + ix] = amp[iy * nopd + ix] * (cos(ph) + I * sin(ph));
+        }
+    }
+}
+
+/* ccpcorner, math/matcomp.c:176-216 (C_FULL) */
+static void ccpcorner(cplx *out, long nx, long ny, const cplx *in, long ninx, long niny) {
+    memset(out, 0, sizeof(cplx) * nx * ny);
+    const long ny2 = (ny < niny) ? ny / 2 : niny / 2;
+    const long nx2 = (nx < ninx) ? nx / 2 : ninx / 2;
+    for (long i = 0; i < ny2; i++) {
+        memcpy(out + (ny - ny2 + i) * nx + (nx - nx2), in + (niny - ny2 + i) * ninx + (ninx - nx2), sizeof(cplx) * nx2);
+        memcpy(out + (ny - ny2 + i) * nx, in + (niny - ny2 + i) * ninx, sizeof(cplx) * nx2);
+        memcpy(out + i * nx + (nx - nx2), in + i * ninx + (ninx - nx2), sizeof(cplx) * nx2);
+        memcpy(out + i * nx, in + i * ninx, sizeof(cplx) * nx2);
+    }
+}
+
+void orc_lotf(double *lotfc_, const double *lltopd, const double *lltamp, long nlx, long embfac,
+              long nwvf, long notf, double wvl) {
+    const long nlwvf = nlx * embfac;
+    cplx *lwvf = (cplx *)malloc(sizeof(cplx) * nlwvf * nlwvf);
+    cplx *lotfc = (cplx *)lotfc_;
+    orc_embed_wvf((double *)lwvf, nlwvf, lltopd, lltamp, nlx, wvl);
+    orc_fft2((double *)lwvf, nlwvf, nlwvf, -1);
+    for (long i = 0; i < nlwvf * nlwvf; i++) {
+        double a = creal(lwvf[i]), b = cimag(lwvf[i]);
+        lwvf[i] = (a * a + b * b) * (1. / ((double)nlwvf * nlwvf));
+    }
+    int islotf = 0;
+    cplx *lotfc2;
+    if (nlwvf != nwvf) {
+        islotf = 1;
+        orc_fft2((double *)lwvf, nlwvf, nlwvf, -1);
+        lotfc2 = (cplx *)malloc(sizeof(cplx) * nwvf * nwvf);
+        ccpcorner(lotfc2, nwvf, nwvf, lwvf, nlwvf, nlwvf);
+    } else {
+        lotfc2 = lwvf;
+    }
+    if (nwvf != notf) {
+        if (islotf) {
+            orc_fft2((double *)lotfc2, nwvf, nwvf, 1);
+            for (long i = 0; i < nwvf * nwvf; i++) lotfc2[i] *= 1. / ((double)nwvf * nwvf);
+            islotf = 0;
+        }
+        ccpcorner(lotfc, notf, notf, lotfc2, nwvf, nwvf);
+    } else {
+        memcpy(lotfc, lotfc2, sizeof(cplx) * notf * notf);
+    }
+    if (!islotf) orc_fft2((double *)lotfc, notf, notf, -1);
+    if (lotfc2 != lwvf) free(lotfc2);
+    free(lwvf);
+}
+
+void orc_wfsints(double *ints, const double *opd, const double *lotfc_, const orc_powfs_t *p,
+                 const orc_wfs_t *w, long isa_start, long isa_end) {
+    const long nopd = p->nxsa, nwvf = nopd * p->embfac, notf = p->notf;
+    const long npix = p->pixpsax * p->pixpsay, notf2 = notf * notf;
+    const long saoffset = nopd * nopd;
+    const int hasllt = p->llt_nx > 0 && p->etf;
+    const double norm_psf = sqrt(p->areascale) / (double)(nopd * nwvf);
+    const double norm_ints = w->sigsim * norm_psf * norm_psf / ((double)notf * notf);
+    cplx *wvf = (cplx *)malloc(sizeof(cplx) * nwvf * nwvf);
+    cplx *psf = nwvf != notf ? (cplx *)malloc(sizeof(cplx) * notf2) : wvf;
+    for (long iwvl = 0; iwvl < p->nwvl; iwvl++) {
+        const double wvl = p->wvl[iwvl];
+        const cplx *lotfc = lotfc_ ? (const cplx *)lotfc_ + iwvl * notf2 : NULL;
+        const double *nominal = (!hasllt && p->nominal) ? p->nominal + iwvl * notf2 : NULL;
+        const long *pp = p->si_pp + iwvl * (notf2 + 1);
+        const long *pi = p->si_pi + iwvl * p->si_nzmax;
+        const double *px = p->si_px + iwvl * p->si_nzmax;
+        const cplx *etf1 = p->etf ? (const cplx *)p->etf + iwvl * p->nsa * notf2 : NULL;
+        const cplx *etf2 = p->etf2 ? (const cplx *)p->etf2 + iwvl * p->nsa * notf2 : NULL;
+        double etf2wt = etf2 ? p->etf2wt : 0, etf1wt = etf2 ? 1. - p->etf2wt : 1;
+        const double alpha = w->wvlwts[iwvl] * norm_ints;
+        for (long isa = isa_start; isa < isa_end; isa++) {
+            orc_embed_wvf((double *)wvf, nwvf, opd + isa * saoffset, p->amp + isa * saoffset, nopd, wvl);
+            orc_fft2((double *)wvf, nwvf, nwvf, -1);
+            if (psf != wvf) ccpcorner(psf, notf, notf, wvf, nwvf, nwvf);
+            for (long i = 0; i < notf2; i++) {
+                double a = creal(psf[i]), b = cimag(psf[i]);
+                psf[i] = a * a + b * b;
+            }
+            orc_fft2((double *)psf, notf, notf, -1);
+            if (lotfc)
+                for (long i = 0; i < notf2; i++) psf[i] *= conj(lotfc[i]);
+            /* apply_dtf_etf, maos/wfsints.c:38-64 */
+            const cplx *B = (hasllt && etf1) ? etf1 + isa * notf2 : NULL;
+            const cplx *C = (hasllt && etf2) ? etf2 + isa * notf2 : NULL;
+            if (C) {
+                for (long i = 0; i < notf2; i++)
+                    psf[i] *= (nominal ? nominal[i] : 1.) * (B[i] * etf1wt + C[i] * etf2wt);
+            } else if (B) {
+                for (long i = 0; i < notf2; i++) psf[i] *= (nominal ? nominal[i] : 1.) * B[i];
+            } else if (nominal) {
+                for (long i = 0; i < notf2; i++) psf[i] *= nominal[i];
+            }
+            orc_fft2((double *)psf, notf, notf, 1);
+            /* dspmulcreal, math/spmm.c:77-86 */
+            double *out = ints + isa * npix;
+            for (long icol = 0; icol < notf2; icol++)
+                for (long k = pp[icol]; k < pp[icol + 1]; k++)
+                    out[pi[k]] += alpha * px[k] * creal(psf[icol]);
+        }
+    }
+    if (psf != wvf) free(psf);
+    free(wvf);
+}
+
+/* ---------------------------------------------------------------------
+ * Pixel processing: math/matmath.c, maos/utils.c
+ * ------------------------------------------------------------------- */
+void orc_dcog(double *grad, const double *im, long nx, long ny, double offsetx,
+              double offsety, double thres, double bkgrnd, double flux) {
+    double sum = 0, sumx = 0, sumy = 0;
+    for (long iy = 0; iy < ny; iy++) {
+        for (long ix = 0; ix < nx; ix++) {
+            double iI = im[ix + iy * nx] - bkgrnd;
+            if (iI > thres) {
+                sum += iI;
+                sumx += iI * ix;
+                sumy += iI * iy;
+            }
+        }
+    }
+    if (flux) sum = flux;
+    if (sum > thres) {
+        grad[0] = sumx / sum - ((double)(nx - 1) * 0.5 + offsetx);
+        grad[1] = sumy / sum - ((double)(ny - 1) * 0.5 + offsety);
+    } else {
+        grad[0] = 0;
+        grad[1] = 0;
+    }
+}
+
+void orc_shwfs_grad(double *grad, const double *ints, const orc_powfs_t *p, double siglev) {
+    (void)siglev;
+    const long nsa = p->nsa, npix = p->pixpsax * p->pixpsay;
+    if (p->phytype == 1 && (!p->mtche || (p->sigmatch == 1 && !p->i0sum))) return; /* no MF yet */
+    double *gx = grad, *gy = grad + nsa;
+    double i1sum = 0;
+    if (p->sigmatch == 2)
+        for (long isa = 0; isa < nsa; isa++)
+            for (long i = 0; i < npix; i++) i1sum += ints[isa * npix + i];
+    double sigtot = NAN;
+    if (p->sigmatch == 0) sigtot = p->siglev_dtrat;
+    else if (p->sigmatch == 2) sigtot = i1sum / p->saasum;
+    for (long isa = 0; isa < nsa; isa++) {
+        double geach[3] = {0, 0, 1};
+        if (p->saa[isa] < p->saat) continue;
+        const double *im = ints + isa * npix;
+        if (p->phytype == 1) {
+            double scale = 1.;
+            if (p->sigmatch == 1) {
+                double s = 0;
+                for (long i = 0; i < npix; i++) s += im[i];
+                scale = p->i0sum[isa] / s;
+                if (scale <= 0 || scale > 10 || !isfinite(scale)) scale = 1;
+            } else if (p->sigmatch == 2) {
+                scale = p->i0sumsum / i1sum;
+            }
+            /* dmulvec, math/matmath.c:384-396: y += A*x*alpha, A is 2 x npix */
+            const double *m = p->mtche + isa * 2 * npix;
+            for (long iy = 0; iy < npix; iy++) {
+                geach[0] += (im[iy] * scale) * m[0 + 2 * iy];
+                geach[1] += (im[iy] * scale) * m[1 + 2 * iy];
+            }
+        } else {
+            double sumi = 0;
+            if (p->sigmatch == 0) sumi = p->i0sum ? p->i0sum[isa] : sigtot * p->saa[isa];
+            else if (p->sigmatch == 2) sumi = sigtot * p->saa[isa];
+            orc_dcog(geach, im, p->pixpsax, p->pixpsay, 0., 0., p->cogthres, p->cogoff, sumi);
+            geach[0] *= p->pixthetax;
+            geach[1] *= p->pixthetay;
+        }
+        gx[isa] = geach[0];
+        gy[isa] = geach[1];
+    }
+}
+
+/* ---------------------------------------------------------------------
+ * One WFS step: maos/wfsgrad.c:122-501 (open loop, noise free)
+ * ------------------------------------------------------------------- */
+void orc_wfsgrad_iwfs(double *ints, double *grad, const orc_powfs_t *p, const orc_wfs_t *w,
+                      const orc_atm_t *atm, long isim, double dt, double atmscale, int nthread,
+                      double *opd_out, double *lotf_out) {
+    const long nsa = p->nsa, nxsa = p->nxsa, npsa = nxsa * nxsa;
+    const long npix = p->pixpsax * p->pixpsay;
+    const long notf2 = p->notf * p->notf;
+#ifdef _OPENMP
+    if (nthread <= 0) nthread = omp_get_max_threads();
+#else
+    nthread = 1;
+#endif
+    double *opd = opd_out ? opd_out : (double *)malloc(sizeof(double) * nsa * npsa);
+    memset(opd, 0, sizeof(double) * nsa * npsa);
+    /* atmosphere ray trace, one layer after another, subapertures split
+       over threads (maos/wfsgrad.c:175-194, maos/sim_utils.c:1004-1033) */
+    for (long ips = 0; ips < atm->nps; ips++) {
+        const double ht = atm->ht[ips];
+        if (w->hs && ht > w->hs) continue;
+        const double scale = w->hs ? (1. - ht / w->hs) : 1.;
+        const double shiftx = -atm->vx[ips] * dt * isim;
+        const double shifty = -atm->vy[ips] * dt * isim;
+        const double dispx = ht * w->thetax + scale * w->misregx + shiftx;
+        const double dispy = ht * w->thetay + scale * w->misregy + shifty;
+#pragma omp parallel for num_threads(nthread) schedule(static)
+        for (long isa = 0; isa < nsa; isa++) {
+            orc_prop_grid_pts(atm->screens[ips], atm->nx, atm->ny, atm->dx, atm->dx, atm->ox,
+                              atm->oy, p->ptsx, p->ptsy, nsa, nxsa, p->dx, opd, atmscale, dispx,
+                              dispy, scale, 1, isa, isa + 1);
+        }
+    }
+    /* LLT uplink (maos/wfsgrad.c:329-386) */
+    double *lotfc = NULL;
+    if (p->llt_nx > 0) {
+        const long nlx = p->llt_nx;
+        double *lltopd = (double *)calloc(nlx * nlx, sizeof(double));
+        double lx = p->llt_origx, ly = p->llt_origy;
+        for (long ips = 0; ips < atm->nps; ips++) {
+            const double hl = atm->ht[ips];
+            const double scale = 1. - hl / w->hs;
+            if (scale < 0) continue;
+            const double thetax = w->thetax - p->llt_ox / w->hs;
+            const double thetay = w->thetay - p->llt_oy / w->hs;
+            orc_prop(atm->screens[ips], atm->nx, atm->ny, atm->dx, atm->ox, atm->oy, hl, &lx, &ly, 1,
+                     nlx, p->dx, lltopd, atmscale, w->hs, thetax, thetay, 0, 0,
+                     p->llt_ox - atm->vx[ips] * isim * dt, p->llt_oy - atm->vy[ips] * isim * dt, 1);
+        }
+        lotfc = lotf_out ? lotf_out : (double *)malloc(sizeof(double) * 2 * notf2 * p->nwvl);
+        for (long iwvl = 0; iwvl < p->nwvl; iwvl++)
+            orc_lotf(lotfc + iwvl * 2 * notf2, lltopd, p->llt_amp, nlx, p->embfac, nxsa * p->embfac,
+                     p->notf, p->wvl[iwvl]);
+        free(lltopd);
+    }
+    memset(ints, 0, sizeof(double) * nsa * npix);
+    /* wfsints split over subapertures (maos/sim_utils.c:1069-1079) */
+    long chunk = (nsa + nthread - 1) / nthread;
+#pragma omp parallel for num_threads(nthread) schedule(static, 1)
+    for (long it = 0; it < nthread; it++) {
+        long s0 = it * chunk, s1 = s0 + chunk;
+        if (s1 > nsa) s1 = nsa;
+        if (s0 < s1) orc_wfsints(ints, opd, lotfc, p, w, s0, s1);
+    }
+    if (grad) orc_shwfs_grad(grad, ints, p, 0);
+    if (lotfc && lotfc != lotf_out) free(lotfc);
+    if (opd != opd_out) free(opd);
+}

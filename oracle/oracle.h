@@ -1,0 +1,122 @@
+/*
+ * FP64 CPU oracle for the MAOS Shack-Hartmann physical-optics WFS step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
+ * algorithm, used by tests/ to check the GPU path, by smoke() as the checker
+ * and by bench.py as the timed CPU baseline.  The product library never
+ * links, loads or calls it.
+ *
+ * Each function cites the reference routine it follows.  Arrays are
+ * column-major with x fastest (math/type.h:31-41); complex arrays are
+ * interleaved (re, im) doubles.
+ */
+#ifndef MAOS_ORACLE_H
+#define MAOS_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 2-D unnormalised FFT, dir=-1 forward, +1 backward (cfft2, math/fft.c:253-281) */
+void orc_fft2(double *c, long nx, long ny, int dir);
+
+/* prop_grid_pts + prop_grid_block (lib/accphi_grid_do.c:49-352, 408-446),
+ * non-transposed, with both the equal-sampling and the unequal-sampling
+ * code paths.  phiout (nsa*nxsa*nxsa) is accumulated.  Returns the number of
+ * points that fell outside the screen (0 with wrap). */
+long orc_prop_grid_pts(const double *phiin, long nxin, long nyin, double dxin,
+                       double dyin, double oxin, double oyin, const double *ptsx,
+                       const double *ptsy, long nsa, long nxsa, double ptsdx,
+                       double *phiout, double alpha, double displacex,
+                       double displacey, double scale, int wrap, long start,
+                       long end);
+
+/* prop_range (lib/accphi.c:223-244) for a screen (ht, dratio=0) onto pts:
+ * derives scale, displacement and alpha from the ray parameters, then calls
+ * orc_prop_grid_pts.  Returns -1 if the layer is above the source. */
+long orc_prop(const double *phiin, long nxin, long nyin, double dxin, double oxin,
+              double oyin, double ht, const double *ptsx, const double *ptsy,
+              long nsa, long nxsa, double ptsdx, double *phiout, double alpha,
+              double hs, double thetax, double thetay, double misregx,
+              double misregy, double shiftx, double shifty, int wrap);
+
+/* Static per-powfs description (powfs_t / parms->powfs fields the step uses). */
+typedef struct orc_powfs_t {
+    long nsa, nxsa, embfac, notf, pixpsax, pixpsay, nwvl;
+    double dx, areascale, saat;
+    const double *ptsx, *ptsy; /* pts->origx/origy [nsa] */
+    const double *amp;         /* [nsa*nxsa*nxsa] */
+    const double *saa;         /* [nsa] */
+    const double *wvl;         /* [nwvl] */
+    const double *nominal;     /* [nwvl][notf*notf] real, or NULL */
+    const long *si_pp;         /* [nwvl][notf*notf+1] CSC of H (npix x notf^2) */
+    const long *si_pi;         /* [nwvl][si_nzmax] */
+    const double *si_px;       /* [nwvl][si_nzmax] */
+    long si_nzmax;
+    const double *etf;         /* [nwvl][nsa][notf*notf] complex, or NULL */
+    const double *etf2;        /* second ETF for interpolation, or NULL */
+    double etf2wt;             /* weight of etf2; etf gets 1-etf2wt */
+    /* LLT (uplink); llt_nx==0 disables it */
+    long llt_nx;
+    double llt_origx, llt_origy, llt_ox, llt_oy;
+    const double *llt_amp;     /* [llt_nx*llt_nx] */
+    /* pixel processing */
+    int phytype;               /* 1: matched filter, 2: CoG */
+    int sigmatch;
+    double cogthres, cogoff, pixthetax, pixthetay;
+    double siglev_dtrat;       /* siglev*dtrat (sigmatch 0 CoG) */
+    double saasum;             /* sum(saa) (sigmatch 2) */
+    const double *mtche;       /* [nsa][2*npix] for this WFS */
+    const double *i0sum;       /* [nsa] */
+    double i0sumsum;
+} orc_powfs_t;
+
+typedef struct orc_wfs_t {
+    double thetax, thetay, hs, sigsim;
+    double misregx, misregy;
+    const double *wvlwts; /* [nwvl] */
+} orc_wfs_t;
+
+typedef struct orc_atm_t {
+    long nps, nx, ny;
+    double dx, ox, oy;
+    const double *const *screens; /* [nps] each nx*ny */
+    const double *ht, *vx, *vy;   /* [nps] */
+} orc_atm_t;
+
+/* cembed_wvf without rotation (math/matcomp.c:56-82) */
+void orc_embed_wvf(double *wvf, long nwvf, const double *opd, const double *amp,
+                   long nopd, double wvl);
+
+/* The uplink OTF lotfc of wfsints (maos/wfsints.c:149-199). */
+void orc_lotf(double *lotfc, const double *lltopd, const double *lltamp, long nlx,
+              long embfac, long nwvf, long notf, double wvl);
+
+/* wfsints over [isa_start, isa_end) (maos/wfsints.c:79-302), accumulating
+ * into ints [nsa][npix].  lotfc is [nwvl][notf*notf] complex or NULL. */
+void orc_wfsints(double *ints, const double *opd, const double *lotfc,
+                 const orc_powfs_t *p, const orc_wfs_t *w, long isa_start,
+                 long isa_end);
+
+/* dcog (math/matmath.c:717-761) on an nx*ny image */
+void orc_dcog(double *grad, const double *im, long nx, long ny, double offsetx,
+              double offsety, double thres, double bkgrnd, double flux);
+
+/* shwfs_grad for MF and CoG (maos/utils.c:983-1122). grad is 2*nsa
+ * (x block then y block); entries of vignetted subapertures are left as is. */
+void orc_shwfs_grad(double *grad, const double *ints, const orc_powfs_t *p,
+                    double siglev);
+
+/* One physical-optics WFS step of wfsgrad_iwfs (maos/wfsgrad.c:122-501,
+ * noise-free, open loop): ray trace every layer (FP64), LLT uplink, wfsints
+ * over all subapertures with OpenMP (nthread<=0: all), then shwfs_grad.
+ * ints [nsa*npix] is zeroed first (dtrat=1).  opd_out/lotf_out are optional
+ * debug outputs. */
+void orc_wfsgrad_iwfs(double *ints, double *grad, const orc_powfs_t *p,
+                      const orc_wfs_t *w, const orc_atm_t *atm, long isim,
+                      double dt, double atmscale, int nthread, double *opd_out,
+                      double *lotf_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

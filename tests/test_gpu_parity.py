@@ -1,0 +1,133 @@
+"""Parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances: pixel intensities relative L2 <= 2e-5 per
+frame, gradients <= 0.1 mas absolute.  Inputs are identical seeded screens
+and identical setup tables for both paths.
+"""
+import numpy as np
+import pytest
+
+from conftest import RAD2MAS, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+INTS_TOL = 2e-5
+GRAD_TOL_MAS = 0.1
+
+
+def _engine(cfg, p, layers):
+    from maos_amd import configs, engine
+    engine.init(0)
+    engine.atm2gpu(layers)
+    wfs = configs.wfs_specs(cfg, p)
+    return engine.WfsGrad(p, wfs), wfs
+
+
+def _oracle(p, layers, mtche=None, i0sum=None):
+    import oracle
+    o = oracle.Oracle(p, mtche, i0sum)
+    o.atm(layers)
+    return o
+
+
+def test_ngs8m_step_parity(ngs_small):
+    cfg, p, layers = ngs_small
+    print("case: ngs8m nsa=%d notf=%d 512^2 screen" % (p.nsa, p.notf), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    for isim in (0, 57, 99):
+        eng.queue(isim, dt)
+        g, ints = eng.sync(0)
+        opd = eng.debug_opd(0)
+        ri, rg, ropd, _ = o.step(wfs[0], isim, dt, want_opd=True)
+        e_opd = np.abs(opd - ropd).max() / np.abs(ropd).max()
+        e_int = rel_l2(ints, ri)
+        e_g = np.abs(g - rg).max() * RAD2MAS
+        print("isim=%d opd maxrel=%.2e ints relL2=%.2e grad maxabs=%.3e mas" % (isim, e_opd, e_int, e_g))
+        assert e_opd < 1e-5
+        assert e_int <= INTS_TOL
+        assert e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
+def _lgs_ref(p, o, wfs, dt, isim=3):
+    """Matched filter built from the oracle's images of one frame."""
+    ri, _, _, _ = o.step(wfs[0], isim, dt)
+    mtche, i0sum, _ = p.mtch(ri)
+    return mtche, i0sum
+
+
+def test_lgs_step_parity(lgs_small):
+    cfg, p, layers = lgs_small
+    print("case: lgs order20 nsa=%d notf=%d nwfs=%d 1024^2 screens" % (p.nsa, p.notf, len(cfg["wfs"])),
+          flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+    o.set_ref(mtche, i0sum)
+    eng.update_ref(mtche, i0sum)
+    for isim in (0, 123):
+        eng.queue(isim, dt)
+        for iwfs in range(len(wfs)):
+            g, ints = eng.sync(iwfs)
+            ri, rg, ropd, rlotf = o.step(wfs[iwfs], isim, dt, want_opd=True, want_lotf=True)
+            opd = eng.debug_opd(iwfs)
+            lotf = eng.debug_lotf(iwfs)
+            e_opd = np.abs(opd - ropd).max() / np.abs(ropd).max()
+            e_lotf = rel_l2(lotf, rlotf)
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(g - rg).max() * RAD2MAS
+            print("isim=%d wfs=%d opd=%.2e lotf=%.2e ints=%.2e grad=%.3e mas"
+                  % (isim, iwfs, e_opd, e_lotf, e_int, e_g))
+            assert e_opd < 1e-5
+            assert e_lotf < 1e-5
+            assert e_int <= INTS_TOL
+            assert e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
+def test_i0_accumulation_fp64(lgs_small):
+    """Long-exposure i0 is accumulated in FP64 on the GPU: it equals the sum
+    of the per-frame images to double rounding, and the oracle's sum to the
+    intensity tolerance (maos/wfsgrad.c:449-467)."""
+    cfg, p, layers = lgs_small
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+    eng.update_ref(mtche, i0sum)
+    eng.i0_control(enable=True, reset=True)
+    nstep = 5
+    acc = np.zeros((p.nsa, p.npix))
+    ref = np.zeros((p.nsa, p.npix))
+    for isim in range(nstep):
+        eng.queue(isim, dt)
+        _, ints = eng.sync(2)
+        acc += ints
+        ref += o.step(wfs[2], isim, dt)[0]
+    i0, nframe = eng.i0_get(2)
+    assert nframe == nstep
+    assert np.abs(i0 - acc).max() <= 1e-12 * np.abs(acc).max()
+    assert rel_l2(i0, ref) <= INTS_TOL
+    eng.i0_control(enable=False, reset=True)
+    i0b, n0 = eng.i0_get(2)
+    assert n0 == 0 and np.all(i0b == 0)
+    eng.close()
+
+
+def test_errors_reported():
+    """Invalid calls return error codes with a message and launch nothing."""
+    import ctypes as C
+    from maos_amd import _lib, engine
+    engine.init(0)
+    L = _lib.hip()
+    h = C.c_void_p()
+    assert L.mx_wfsgrad_init(C.byref(h), None, None, 0) == -1
+    assert b"null" in L.mx_last_error()
+    cfgp = _lib.MxPowfs(nsa=10, nxsa=24, embfac=2, notf=48, pixpsax=6, pixpsay=6, nwvl=1)
+    w = (_lib.MxWfs * 1)()
+    assert L.mx_wfsgrad_init(C.byref(h), C.byref(cfgp), w, 1) == -1
+    assert b"unsupported" in L.mx_last_error()
+    assert L.mx_atm2gpu(None, 17, 1.0) == -1
