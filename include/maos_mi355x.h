@@ -148,13 +148,14 @@ int mx_wfsgrad_i0_get(mx_wfsgrad_t *h, int iwfs, double *i0sum_out,
 int mx_wfsgrad_debug_opd(mx_wfsgrad_t *h, int iwfs, double *opd);
 int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf);
 
-/* Time the imaging kernel of subsequent steps with HIP events on the
- * engine's stream.  enable=1 starts recording (and clears the sums);
- * mx_wfsgrad_timing reports launches and the summed milliseconds of the
- * uplink kernel and of the fused imaging kernel. */
+/* Time the kernels of subsequent steps with HIP events on the engine's
+ * stream (each timed step waits for its kernels).  enable=1 starts
+ * recording and clears the sums; mx_wfsgrad_timing reports the number of
+ * timed steps and the summed milliseconds of the uplink-OTF kernel, the
+ * ray-trace kernel and the fused imaging kernel. */
 int mx_wfsgrad_timing_enable(mx_wfsgrad_t *h, int enable);
 int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
-                      double *ms_ints);
+                      double *ms_opd, double *ms_ints);
 
 /* Wait for all queued work of the engine. */
 int mx_wfsgrad_wait(mx_wfsgrad_t *h);

@@ -86,7 +86,7 @@ HIP_API = {
     "mx_wfsgrad_debug_opd": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "mx_wfsgrad_debug_lotf": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p]),
     "mx_wfsgrad_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
-    "mx_wfsgrad_timing": (C.c_int, [C.c_void_p, c_long_p, c_double_p, c_double_p]),
+    "mx_wfsgrad_timing": (C.c_int, [C.c_void_p, c_long_p, c_double_p, c_double_p, c_double_p]),
     "mx_wfsgrad_wait": (C.c_int, [C.c_void_p]),
 }
 

@@ -71,32 +71,48 @@ __device__ __forceinline__ float2 twmul(float2 a, int k) {
     return make_float2(a.x * c - a.y * s, a.y * c + a.x * s);
 }
 
-// Unnormalised N-point FFT of a register array; INV=false is exp(-i...)
+// Unnormalised N-point FFTs of a register array.  INV=false is exp(-i...)
 // (FFTW_FORWARD, cfft2(.,-1)), INV=true is exp(+i...) (cfft2(.,+1)).
+// fft_dif: natural-order input, bit-reversed output (v[k] = X[bitrev(k)]).
+// fft_dit: bit-reversed input, natural-order output.  Alternating the two
+// keeps every permutation in compile-time indices.
 template <int N, bool INV>
-__device__ __forceinline__ void fft_reg(float2 (&v)[N]) {
+__device__ __forceinline__ void fft_dif(float2 (&v)[N]) {
     constexpr int LG = ilog2c(N);
     static_assert((1 << LG) == N && N <= MX_TWN, "power of two up to MX_TWN");
-    float2 t[N];
 #pragma unroll
-    for (int i = 0; i < N; i++) t[i] = v[bitrev(i, LG)];
+    for (int s = LG; s >= 1; s--) {
+        const int len = 1 << s, h = len >> 1;
+#pragma unroll
+        for (int i0 = 0; i0 < N; i0 += len) {
+#pragma unroll
+            for (int j = 0; j < h; j++) {
+                const float2 u = v[i0 + j], w = v[i0 + j + h];
+                v[i0 + j] = cadd(u, w);
+                v[i0 + j + h] = twmul<INV>(csub(u, w), j * (MX_TWN / len));
+            }
+        }
+    }
+}
+
+template <int N, bool INV>
+__device__ __forceinline__ void fft_dit(float2 (&v)[N]) {
+    constexpr int LG = ilog2c(N);
+    static_assert((1 << LG) == N && N <= MX_TWN, "power of two up to MX_TWN");
 #pragma unroll
     for (int s = 1; s <= LG; s++) {
         const int len = 1 << s, h = len >> 1;
 #pragma unroll
-        for (int j = 0; j < h; j++) {
-            const int k = j * (MX_TWN / len);
+        for (int i0 = 0; i0 < N; i0 += len) {
 #pragma unroll
-            for (int i0 = 0; i0 < N; i0 += len) {
-                float2 u = t[i0 + j];
-                float2 w = twmul<INV>(t[i0 + j + h], k);
-                t[i0 + j] = cadd(u, w);
-                t[i0 + j + h] = csub(u, w);
+            for (int j = 0; j < h; j++) {
+                const float2 u = v[i0 + j];
+                const float2 w = twmul<INV>(v[i0 + j + h], j * (MX_TWN / len));
+                v[i0 + j] = cadd(u, w);
+                v[i0 + j + h] = csub(u, w);
             }
         }
     }
-#pragma unroll
-    for (int i = 0; i < N; i++) v[i] = t[i];
 }
 
 // ---------------------------------------------------------------- params
@@ -142,11 +158,23 @@ struct KArgs {
     const float *i0sum; // [nwfs][nsa]
     const float *saa;
     float saat;
+    const float *opd; // [nwfs][nsa][nxsa^2] from k_wfs_opd
     // outputs
     float *ints;     // [nwfs][nsa][npix]
     float *grad;     // [nwfs][2*nsa]
     double *i0acc;   // [nwfs][nsa][npix] or null
-    float *opd_dbg;  // [nwfs][nsa][nxsa^2] or null
+};
+
+struct OpdArgs {
+    int nsa, nwfs, nps;
+    double ptsdx;
+    const double *ptsx, *ptsy;
+    const DevLayer *layers;
+    double atmscale;
+    const DevWfs *wfs;
+    long isim;
+    double dt;
+    float *opd; // [nwfs][nsa][nxsa^2]
 };
 
 struct LltArgs {
@@ -174,8 +202,8 @@ __device__ __forceinline__ void seg_sum(float &v) {
 }
 
 // Ray trace every layer onto one NXSA x NXSA point grid whose first point is
-// at (origx, origy), accumulating into opd (LDS, x fastest).  Follows
-// prop_range + prop_grid_pts with wrap=1 (lib/accphi.c:223-244,
+// at (origx, origy), writing the sum over layers to opd (LDS, x fastest).
+// Follows prop_range + prop_grid_pts with wrap=1 (lib/accphi.c:223-244,
 // lib/accphi_grid_do.c:147-302): the point coordinate on each screen, its
 // periodic wrap, integer part and fraction are FP64; the four weights and
 // the screen values are FP32.  Coordinates of the NXSA columns and NXSA
@@ -228,26 +256,58 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
         }
     }
     __syncthreads();
-    for (int pt = line; pt < NXSA * NXSA; pt += N) {
-        const int ix = pt % NXSA, iy = pt / NXSA;
-        float acc = 0.f;
-        for (int l = 0; l < nps; l++) {
-            const int x0 = cidx[l * 2 * NXSA + ix];
-            if (x0 < 0) continue;
-            const DevLayer L = layers[l];
-            const int y0 = cidx[l * 2 * NXSA + NXSA + iy];
-            const float fx = cfrac[l * 2 * NXSA + ix];
-            const float fy = cfrac[l * 2 * NXSA + NXSA + iy];
-            const int x1 = x0 == L.nx - 1 ? 0 : x0 + 1;
-            const int y1 = y0 == L.ny - 1 ? 0 : y0 + 1;
-            const float *r0 = L.p + (long)y0 * L.nx;
-            const float *r1 = L.p + (long)y1 * L.nx;
-            const float v = (1.f - fy) * ((1.f - fx) * r0[x0] + fx * r0[x1]) +
-                            fy * ((1.f - fx) * r1[x0] + fx * r1[x1]);
-            acc += (float)atmscale * v;
+    // Per layer, the (NXSA+2)^2 screen tile that the subaperture's points
+    // touch (the grid step on the screen is <= 1 pixel) is staged in LDS with
+    // coalesced loads, wrapping periodically, then interpolated from LDS.
+    constexpr int PPT = NXSA * NXSA / N;
+    constexpr int TW = NXSA + 2;
+    constexpr int NFILL = (TW * TW + N - 1) / N;
+    static_assert(PPT * N == NXSA * NXSA, "points per lane");
+    float *tile = cfrac + MX_MAXLAYER * 2 * NXSA;
+    const bool act = line < N;
+    float acc[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; j++) acc[j] = 0.f;
+    for (int l = 0; l < nps; l++) {
+        const int *ci = cidx + l * 2 * NXSA;
+        const float *cf = cfrac + l * 2 * NXSA;
+        const int tx0 = ci[0], ty0 = ci[NXSA];
+        if (tx0 < 0) continue; // layer above the source (uniform)
+        const float *p = layers[l].p;
+        const int nx = layers[l].nx, ny = layers[l].ny;
+        __syncthreads(); // previous tile fully read
+        if (act) {
+#pragma unroll
+            for (int k = 0; k < NFILL; k++) {
+                const int e = line + k * N;
+                if (e < TW * TW) {
+                    const int r = e / TW, c = e - r * TW;
+                    int gx = tx0 + c, gy = ty0 + r;
+                    gx -= gx >= nx ? nx : 0;
+                    gy -= gy >= ny ? ny : 0;
+                    tile[e] = p[(long)gy * nx + gx];
+                }
+            }
         }
-        opd[pt] += acc;
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < PPT; j++) {
+                const int pt = line + j * N;
+                const int ix = pt % NXSA, iy = pt / NXSA;
+                int lc = ci[ix] - tx0, lr = ci[NXSA + iy] - ty0;
+                lc += lc < 0 ? nx : 0;
+                lr += lr < 0 ? ny : 0;
+                const float fx = cf[ix], fy = cf[NXSA + iy];
+                const float *t0 = tile + lr * TW + lc;
+                acc[j] += (1.f - fy) * ((1.f - fx) * t0[0] + fx * t0[1]) +
+                          fy * ((1.f - fx) * t0[TW] + fx * t0[TW + 1]);
+            }
+        }
     }
+    if (act)
+#pragma unroll
+        for (int j = 0; j < PPT; j++) opd[line + j * N] = (float)atmscale * acc[j];
 }
 
 // XCD-aware unit index: consecutive units (the WFS of one subaperture
@@ -264,6 +324,7 @@ template <int N, int NXSA, int PPL>
 __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
     constexpr int SPW = 64 / N;        // subapertures per wave
     constexpr int SKIP = (N - NXSA) / 2;
+    constexpr int LG = ilog2c(N);
     __shared__ float2 tile[SPW][N * N];
     const int lane = threadIdx.x;
     const int sub = lane / N, line = lane % N;
@@ -277,22 +338,21 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
     float2 *T = tile[sub];
     float *opd = reinterpret_cast<float *>(T);
     float *amp = opd + NXSA * NXSA;
-    int *cidx = reinterpret_cast<int *>(amp + NXSA * NXSA);
-    float *cfrac = reinterpret_cast<float *>(cidx + MX_MAXLAYER * 2 * NXSA);
 
     float pv[PPL];
 #pragma unroll
     for (int j = 0; j < PPL; j++) pv[j] = 0.f;
     float2 v[N];
     for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
-        // stage: OPD by ray tracing, amplitude staged into LDS
-        for (int pt = line; pt < NXSA * NXSA; pt += N) {
-            opd[pt] = 0.f;
-            amp[pt] = a.amp[(long)isa * NXSA * NXSA + pt];
+        // stage: OPD (from k_wfs_opd) and amplitude into LDS
+        {
+            const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA;
+            const float *ag = a.amp + (long)isa * NXSA * NXSA;
+            for (int pt = line; pt < NXSA * NXSA; pt += N) {
+                opd[pt] = og[pt];
+                amp[pt] = ag[pt];
+            }
         }
-        ray_trace<N, NXSA>(a.layers, a.nps, W.hs, W.thetax, W.thetay, W.misregx, W.misregy, 0., 0.,
-                           a.isim, a.dt, a.atmscale, a.ptsdx, a.ptsx[isa], a.ptsy[isa], line, opd,
-                           cidx, cfrac);
         __syncthreads();
         // piston (unweighted mean) is removed before exp(i k opd); |E|^2 is
         // invariant to it and the FP32 phase stays small
@@ -300,9 +360,6 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
         for (int pt = line; pt < NXSA * NXSA; pt += N) psum += opd[pt];
         seg_sum<N>(psum);
         const float mean = psum * (1.f / (NXSA * NXSA));
-        if (a.opd_dbg && valid && iwvl == 0)
-            for (int pt = line; pt < NXSA * NXSA; pt += N)
-                a.opd_dbg[((long)iwfs * a.nsa + isa) * NXSA * NXSA + pt] = opd[pt];
         const DevWvl WV = a.wv[iwvl];
         const float inv_wvl = WV.inv_wvl;
         // pass 0 (rows y): embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf,
@@ -321,21 +378,21 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
                 sincospif(2.f * o * inv_wvl, &sn, &cs);
                 v[SKIP + ix] = make_float2(am * cs, am * sn);
             }
-            fft_reg<N, false>(v);
+            fft_dif<N, false>(v);
         }
         __syncthreads(); // opd/amp reads done before the tile is overwritten
 #pragma unroll
-        for (int k = 0; k < N; k++) T[swz<N>(k, y)] = inband ? v[k] : make_float2(0.f, 0.f);
+        for (int k = 0; k < N; k++) T[swz<N>(bitrev(k, LG), y)] = inband ? v[k] : make_float2(0.f, 0.f);
         __syncthreads();
         // pass 1 (columns x): FFT along y, |E|^2, first half of PSF->OTF FFT
         {
             const int x = line;
 #pragma unroll
             for (int k = 0; k < N; k++) v[k] = T[swz<N>(x, k)];
-            fft_reg<N, false>(v);
+            fft_dif<N, false>(v);
 #pragma unroll
             for (int k = 0; k < N; k++) v[k] = make_float2(v[k].x * v[k].x + v[k].y * v[k].y, 0.f);
-            fft_reg<N, false>(v);
+            fft_dit<N, false>(v);
 #pragma unroll
             for (int k = 0; k < N; k++) T[swz<N>(x, k)] = v[k];
         }
@@ -346,11 +403,11 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
             const int uy = line;
 #pragma unroll
             for (int k = 0; k < N; k++) v[k] = T[swz<N>(k, uy)];
-            fft_reg<N, false>(v);
+            fft_dif<N, false>(v); // v[k] = OTF(ux = bitrev(k), uy)
             if (a.lotf) {
                 const float2 *lt = a.lotf + ((long)iwfs * a.nwvl + iwvl) * N * N + uy;
 #pragma unroll
-                for (int k = 0; k < N; k++) v[k] = cmul(v[k], lt[k * N]);
+                for (int k = 0; k < N; k++) v[k] = cmul(v[k], lt[bitrev(k, LG) * N]);
             }
             if (WV.etf) {
                 const float2 *e1 = WV.etf + (long)isa * N * N + uy;
@@ -359,22 +416,22 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
                     const float w1 = a.etf1wt, w2 = a.etf2wt;
 #pragma unroll
                     for (int k = 0; k < N; k++) {
-                        const float2 e = e1[k * N], f = e2[k * N];
+                        const float2 e = e1[bitrev(k, LG) * N], f = e2[bitrev(k, LG) * N];
                         v[k] = cmul(v[k], make_float2(e.x * w1 + f.x * w2, e.y * w1 + f.y * w2));
                     }
                 } else {
 #pragma unroll
-                    for (int k = 0; k < N; k++) v[k] = cmul(v[k], e1[k * N]);
+                    for (int k = 0; k < N; k++) v[k] = cmul(v[k], e1[bitrev(k, LG) * N]);
                 }
             } else if (WV.nominal) {
                 const float *nm = WV.nominal + uy;
 #pragma unroll
                 for (int k = 0; k < N; k++) {
-                    const float s = nm[k * N];
+                    const float s = nm[bitrev(k, LG) * N];
                     v[k] = make_float2(v[k].x * s, v[k].y * s);
                 }
             }
-            fft_reg<N, true>(v);
+            fft_dit<N, true>(v);
 #pragma unroll
             for (int k = 0; k < N; k++) T[swz<N>(k, uy)] = v[k];
         }
@@ -384,11 +441,11 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
             const int x = line;
 #pragma unroll
             for (int k = 0; k < N; k++) v[k] = T[swz<N>(x, k)];
-            fft_reg<N, true>(v);
+            fft_dif<N, true>(v);
             __syncthreads();
             float *img = reinterpret_cast<float *>(T);
 #pragma unroll
-            for (int k = 0; k < N; k++) img[k * N + x] = v[k].x;
+            for (int k = 0; k < N; k++) img[bitrev(k, LG) * N + x] = v[k].x;
         }
         __syncthreads();
         // bilinear pixel sampler (dspmulcreal with si, math/spmm.c:77-86)
@@ -463,6 +520,23 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
     }
 }
 
+// ------------------------------------------------------------ ray trace
+// OPD of every (WFS, subaperture) at step isim (the atmosphere loop of
+// wfsgrad_iwfs, maos/wfsgrad.c:175-194).  Small register and LDS footprint,
+// so many waves per SIMD hide the screen-fetch latency; the FP32 OPD goes to
+// HBM for the imaging kernel.
+template <int NXSA>
+__global__ __launch_bounds__(64) void k_wfs_opd(OpdArgs a) {
+    __shared__ int cidx[MX_MAXLAYER * 2 * NXSA];
+    __shared__ float cfrac[MX_MAXLAYER * 2 * NXSA + (NXSA + 2) * (NXSA + 2)];
+    const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
+    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    const DevWfs &W = a.wfs[iwfs];
+    ray_trace<64, NXSA>(a.layers, a.nps, W.hs, W.thetax, W.thetay, W.misregx, W.misregy, 0., 0.,
+                        a.isim, a.dt, a.atmscale, a.ptsdx, a.ptsx[isa], a.ptsy[isa], threadIdx.x,
+                        a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac);
+}
+
 // ------------------------------------------------------------ uplink OTF
 // lotfc of wfsints (maos/wfsints.c:149-199) for nlwvf == nwvf == notf: LLT
 // OPD by ray tracing (maos/wfsgrad.c:329-386), embed, FFT, |.|^2/nlwvf^2,
@@ -470,6 +544,7 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
 template <int N, int NLX>
 __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
     constexpr int SKIP = (N - NLX) / 2;
+    constexpr int LG = ilog2c(N);
     __shared__ float2 T[N * N];
     const int lane = threadIdx.x;
     const int line = lane % N;
@@ -481,10 +556,7 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
     int *cidx = reinterpret_cast<int *>(amp + NLX * NLX);
     float *cfrac = reinterpret_cast<float *>(cidx + MX_MAXLAYER * 2 * NLX);
     if (active)
-        for (int pt = line; pt < NLX * NLX; pt += N) {
-            opd[pt] = 0.f;
-            amp[pt] = a.amp[pt];
-        }
+        for (int pt = line; pt < NLX * NLX; pt += N) amp[pt] = a.amp[pt];
     // theta of the uplink: theta - o/hs; shift o - v*dt*isim
     ray_trace<N, NLX>(a.layers, a.nps, W.hs, W.thetax - a.ox / W.hs, W.thetay - a.oy / W.hs, 0.,
                       0., a.ox, a.oy, a.isim, a.dt, a.atmscale, a.ptsdx, a.origx, a.origy,
@@ -511,21 +583,21 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
             sincospif(2.f * o * inv_wvl, &sn, &cs);
             v[SKIP + ix] = make_float2(am * cs, am * sn);
         }
-        fft_reg<N, false>(v);
+        fft_dif<N, false>(v);
     }
     __syncthreads();
     if (active)
 #pragma unroll
-        for (int k = 0; k < N; k++) T[swz<N>(k, y)] = inband ? v[k] : make_float2(0.f, 0.f);
+        for (int k = 0; k < N; k++) T[swz<N>(bitrev(k, LG), y)] = inband ? v[k] : make_float2(0.f, 0.f);
     __syncthreads();
     if (active) {
         const int x = line;
 #pragma unroll
         for (int k = 0; k < N; k++) v[k] = T[swz<N>(x, k)];
-        fft_reg<N, false>(v);
+        fft_dif<N, false>(v);
 #pragma unroll
         for (int k = 0; k < N; k++) v[k] = make_float2((v[k].x * v[k].x + v[k].y * v[k].y) * a.norm, 0.f);
-        fft_reg<N, false>(v);
+        fft_dit<N, false>(v);
 #pragma unroll
         for (int k = 0; k < N; k++) T[swz<N>(x, k)] = v[k];
     }
@@ -534,10 +606,10 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
         const int uy = line;
 #pragma unroll
         for (int k = 0; k < N; k++) v[k] = T[swz<N>(k, uy)];
-        fft_reg<N, false>(v);
+        fft_dif<N, false>(v);
         float2 *out = a.lotf + ((long)iwfs * a.nwvl + iwvl) * N * N + uy;
 #pragma unroll
-        for (int k = 0; k < N; k++) out[k * N] = make_float2(v[k].x, -v[k].y);
+        for (int k = 0; k < N; k++) out[bitrev(k, LG) * N] = make_float2(v[k].x, -v[k].y);
     }
 }
 
@@ -580,7 +652,7 @@ struct mx_wfsgrad_s {
     // timing
     bool timing = false;
     long t_n = 0;
-    double t_llt = 0, t_ints = 0;
+    double t_llt = 0, t_opd = 0, t_ints = 0;
     std::vector<hipEvent_t> ev;
     bool pending_timing = false;
     // staging
@@ -898,6 +970,9 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     if (h->cfg.llt_nx > 0)
         for (int w = 0; w < h->nwvl; w++)
             if (!h->d_etf[w]) return set_err(MX_ESTATE, "etf of wavelength %d not set", w);
+    for (int l = 0; l < g_atm.nps; l++) // the staged screen tile assumes a grid step <= 1 pixel
+        if (h->cfg.dx * g_atm.h_layers[l].dx1 > 1. + 1e-9 || h->cfg.dx * g_atm.h_layers[l].dy1 > 1. + 1e-9)
+            return set_err(MX_EINVAL, "layer %d is sampled more coarsely than the OPD grid (dx=%g)", l, h->cfg.dx);
     if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche) return set_err(MX_ESTATE, "matched filter not set");
     if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum)
         return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=1)");
@@ -919,15 +994,16 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
             std::memcpy(h->h_wv, hw, sizeof(hw));
         }
     }
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
     if (h->timing) {
-        if (h->ev.size() < 3) {
-            h->ev.resize(3);
+        if (h->ev.size() < 4) {
+            h->ev.resize(4);
             for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
         }
         e0 = h->ev[0];
         e1 = h->ev[1];
         e2 = h->ev[2];
+        e3 = h->ev[3];
         HIPCHK(hipEventRecord(e0, h->stream));
     }
     if (h->cfg.llt_nx > 0) {
@@ -956,6 +1032,26 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         HIPCHK(hipGetLastError());
     }
     if (h->timing) HIPCHK(hipEventRecord(e1, h->stream));
+    {
+        OpdArgs o{};
+        o.nsa = (int)h->nsa;
+        o.nwfs = h->nwfs;
+        o.nps = g_atm.nps;
+        o.ptsdx = h->cfg.dx;
+        o.ptsx = h->d_ptsx;
+        o.ptsy = h->d_ptsy;
+        o.layers = g_atm.d_layers;
+        o.atmscale = g_atm.atmscale;
+        o.wfs = h->d_wfs;
+        o.isim = isim;
+        o.dt = dt;
+        o.opd = h->d_opd;
+        dim3 grid((unsigned)(h->nsa * h->nwfs));
+        if (h->nxsa == 32) hipLaunchKernelGGL((k_wfs_opd<32>), grid, dim3(64), 0, h->stream, o);
+        else hipLaunchKernelGGL((k_wfs_opd<16>), grid, dim3(64), 0, h->stream, o);
+        HIPCHK(hipGetLastError());
+    }
+    if (h->timing) HIPCHK(hipEventRecord(e2, h->stream));
     KArgs a{};
     a.nsa = (int)h->nsa;
     a.npix = (int)h->npix;
@@ -991,7 +1087,7 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     a.ints = h->d_ints;
     a.grad = h->d_grad;
     a.i0acc = h->i0_enable ? h->d_i0acc : nullptr;
-    a.opd_dbg = h->d_opd;
+    a.opd = h->d_opd;
     const int spw = (int)(64 / N);
     const long ngroup = (h->nsa + spw - 1) / spw;
     dim3 grid((unsigned)(ngroup * h->nwfs));
@@ -999,13 +1095,15 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     else hipLaunchKernelGGL((k_shwfs_step<32, 16, 2>), grid, dim3(64), 0, h->stream, a);
     HIPCHK(hipGetLastError());
     if (h->timing) {
-        HIPCHK(hipEventRecord(e2, h->stream));
-        HIPCHK(hipEventSynchronize(e2));
-        float m1 = 0, m2 = 0;
+        HIPCHK(hipEventRecord(e3, h->stream));
+        HIPCHK(hipEventSynchronize(e3));
+        float m1 = 0, m2 = 0, m3 = 0;
         HIPCHK(hipEventElapsedTime(&m1, e0, e1));
         HIPCHK(hipEventElapsedTime(&m2, e1, e2));
+        HIPCHK(hipEventElapsedTime(&m3, e2, e3));
         h->t_llt += m1;
-        h->t_ints += m2;
+        h->t_opd += m2;
+        h->t_ints += m3;
         h->t_n++;
     }
     if (h->i0_enable) h->i0_nframe++;
@@ -1095,14 +1193,15 @@ int mx_wfsgrad_timing_enable(mx_wfsgrad_t *h, int enable) {
     if (!h) return set_err(MX_EINVAL, "null handle");
     h->timing = enable != 0;
     h->t_n = 0;
-    h->t_llt = h->t_ints = 0;
+    h->t_llt = h->t_opd = h->t_ints = 0;
     return MX_OK;
 }
 
-int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *n, double *ms_llt, double *ms_ints) {
+int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *n, double *ms_llt, double *ms_opd, double *ms_ints) {
     if (!h) return set_err(MX_EINVAL, "null handle");
     if (n) *n = h->t_n;
     if (ms_llt) *ms_llt = h->t_llt;
+    if (ms_opd) *ms_opd = h->t_opd;
     if (ms_ints) *ms_ints = h->t_ints;
     return MX_OK;
 }

@@ -123,13 +123,14 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_timing_enable(self.h, int(on)))
 
     def timing(self):
+        """(timed steps, ms uplink OTF, ms ray trace, ms imaging), summed."""
         n = C.c_long(0)
-        a = C.c_double(0)
-        b = C.c_double(0)
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
         check(_lib.hip().mx_wfsgrad_timing(self.h, C.cast(C.byref(n), _lib.c_long_p),
                                            C.cast(C.byref(a), _lib.c_double_p),
-                                           C.cast(C.byref(b), _lib.c_double_p)))
-        return n.value, a.value, b.value
+                                           C.cast(C.byref(b), _lib.c_double_p),
+                                           C.cast(C.byref(c), _lib.c_double_p)))
+        return n.value, a.value, b.value, c.value
 
     def close(self):
         if getattr(self, "h", None):
