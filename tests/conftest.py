@@ -18,8 +18,8 @@ RAD2MAS = 180.0 / np.pi * 3600.0 * 1000.0
 
 def rel_l2(a, b):
     """||a-b||_2 / ||b||_2"""
-    a = np.asarray(a, dtype=np.float64).ravel()
-    b = np.asarray(b, dtype=np.float64).ravel()
+    a = np.asarray(a).ravel()
+    b = np.asarray(b).ravel()
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 

@@ -1,0 +1,276 @@
+"""Pins of the FP64 oracle against known answers and mathematical identities.
+
+Each test names the oracle function it pins and what fixes the expected
+value: a library routine (numpy's FFT), a brute-force sum, a closed form, an
+invariant, or the reference's own unit test values.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_l2
+
+RNG = np.random.RandomState(7)
+
+
+# ---------------------------------------------------------------- orc_fft2
+@pytest.mark.parametrize("nx,ny", [(4, 4), (8, 2), (32, 32), (64, 64), (6, 6), (6, 4)])
+def test_fft2_matches_numpy_and_brute_force(nx, ny):
+    a = RNG.randn(ny, nx) + 1j * RNG.randn(ny, nx)
+    fwd = oracle.fft2(a.copy(), -1)
+    bwd = oracle.fft2(a.copy(), +1)
+    # FFTW_FORWARD = exp(-i), unnormalised; numpy.fft.fft2 has the same convention
+    assert np.allclose(fwd, np.fft.fft2(a), atol=1e-12 * np.abs(a).sum())
+    assert np.allclose(bwd, np.fft.ifft2(a) * nx * ny, atol=1e-12 * np.abs(a).sum())
+    if nx * ny <= 64:
+        ky, kx = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+        brute = np.zeros_like(a)
+        for y in range(ny):
+            for x in range(nx):
+                brute += a[y, x] * np.exp(-2j * np.pi * (kx * x / nx + ky * y / ny))
+        assert np.allclose(fwd, brute, atol=1e-10)
+
+
+# ------------------------------------------------------- orc_prop_grid_pts
+def _plane_screen(n, a, b, c):
+    y, x = np.mgrid[0:n, 0:n].astype(float)
+    return a + b * x + c * y
+
+
+def _pts(nsa, nxsa, dx, ox, oy, step):
+    px = ox + step * np.arange(nsa)
+    py = oy + 0.5 * step * np.arange(nsa)
+    return np.ascontiguousarray(px), np.ascontiguousarray(py)
+
+
+@pytest.mark.parametrize("scale", [1.0, 0.95, 0.8222])
+def test_prop_exact_on_planes(scale):
+    """Bilinear interpolation reproduces a plane exactly (away from the
+    periodic seam), for both the equal-sampling and the cone (scale<1) code
+    paths of prop_grid_block."""
+    n, dxin = 256, 1.0 / 32
+    a, b, c = 0.3, 1e-3, -2e-3
+    scr = np.ascontiguousarray(_plane_screen(n, a, b, c))
+    ox = oy = -(n // 2) * dxin
+    nsa, nxsa = 5, 16
+    px, py = _pts(nsa, nxsa, dxin, -1.5, -1.0, 0.5)
+    out = np.zeros(nsa * nxsa * nxsa)
+    dispx, dispy = 0.123, -0.077
+    oracle.lib().orc_prop_grid_pts(scr.ctypes.data_as(oracle.dp), n, n, dxin, dxin, ox, oy,
+                                   px.ctypes.data_as(oracle.dp), py.ctypes.data_as(oracle.dp),
+                                   nsa, nxsa, dxin, out.ctypes.data_as(oracle.dp), 1.0, dispx,
+                                   dispy, scale, 1, 0, nsa)
+    out = out.reshape(nsa, nxsa, nxsa)
+    g = np.arange(nxsa) * dxin
+    for isa in range(nsa):
+        X = (px[isa] + g[None, :]) * scale + dispx
+        Y = (py[isa] + g[:, None]) * scale + dispy
+        ix = (X - ox) / dxin
+        iy = (Y - oy) / dxin
+        assert np.allclose(out[isa], a + b * ix + c * iy, atol=1e-12)
+
+
+def test_prop_periodic_wrap_matches_modular_bilinear():
+    """Points beyond the screen edge wrap periodically: compare with a
+    textbook bilinear interpolation using modular indices."""
+    n, dxin = 64, 0.5
+    scr = np.ascontiguousarray(RNG.randn(n, n))
+    ox = oy = -(n // 2) * dxin
+    nsa, nxsa = 3, 8
+    px = np.array([14.0, -17.0, 15.6])
+    py = np.array([15.1, 12.0, -16.9])
+    for scale in (1.0, 0.9):
+        out = np.zeros(nsa * nxsa * nxsa)
+        oracle.lib().orc_prop_grid_pts(scr.ctypes.data_as(oracle.dp), n, n, dxin, dxin, ox, oy,
+                                       px.ctypes.data_as(oracle.dp), py.ctypes.data_as(oracle.dp),
+                                       nsa, nxsa, dxin, out.ctypes.data_as(oracle.dp), 2.0, 0.31,
+                                       -0.44, scale, 1, 0, nsa)
+        out = out.reshape(nsa, nxsa, nxsa)
+        g = np.arange(nxsa) * dxin
+        for isa in range(nsa):
+            X = ((px[isa] + g[None, :]) * scale + 0.31 - ox) / dxin + 0 * g[:, None]
+            Y = ((py[isa] + g[:, None]) * scale - 0.44 - oy) / dxin + 0 * g[None, :]
+            x0, y0 = np.floor(X).astype(int), np.floor(Y).astype(int)
+            fx, fy = X - x0, Y - y0
+            v = lambda yy, xx: scr[yy % n, xx % n]
+            ref = ((1 - fy) * ((1 - fx) * v(y0, x0) + fx * v(y0, x0 + 1)) +
+                   fy * ((1 - fx) * v(y0 + 1, x0) + fx * v(y0 + 1, x0 + 1)))
+            assert np.allclose(out[isa], 2.0 * ref, atol=1e-12)
+
+
+def test_prop_range_cone_and_skip():
+    """prop_range: layer above the source is skipped; the cone scale is
+    1-ht/hs and the displacement ht*theta + shift."""
+    n, dxin = 128, 1.0 / 16
+    scr = np.ascontiguousarray(_plane_screen(n, 0.0, 1.0, 0.0))
+    ox = oy = -(n // 2) * dxin
+    px = np.array([0.0])
+    py = np.array([0.0])
+    out = np.zeros(4)
+    L = oracle.lib()
+    rc = L.orc_prop(scr.ctypes.data_as(oracle.dp), n, n, dxin, ox, oy, 100e3,
+                    px.ctypes.data_as(oracle.dp), py.ctypes.data_as(oracle.dp), 1, 2, dxin,
+                    out.ctypes.data_as(oracle.dp), 1.0, 90e3, 0, 0, 0, 0, 0, 0, 1)
+    assert rc == -1 and np.all(out == 0)
+    ht, hs, th = 10e3, 90e3, 5e-6
+    L.orc_prop(scr.ctypes.data_as(oracle.dp), n, n, dxin, ox, oy, ht, px.ctypes.data_as(oracle.dp),
+               py.ctypes.data_as(oracle.dp), 1, 2, dxin, out.ctypes.data_as(oracle.dp), 1.0, hs, th,
+               0, 0, 0, 0.25, 0, 1)
+    scale = 1 - ht / hs
+    x = np.array([0, dxin, 0, dxin]) * scale + ht * th + 0.25
+    assert np.allclose(out, (x - ox) / dxin, atol=1e-12)
+
+
+# ------------------------------------------------------------ orc_embed_wvf
+def test_embed_wvf_closed_form():
+    nopd, nwvf, wvl = 4, 8, 0.5e-6
+    opd = RNG.randn(nopd * nopd) * 1e-7
+    amp = RNG.rand(nopd * nopd)
+    w = np.zeros(2 * nwvf * nwvf)
+    oracle.lib().orc_embed_wvf(w.ctypes.data_as(oracle.dp), nwvf, opd.ctypes.data_as(oracle.dp),
+                               amp.ctypes.data_as(oracle.dp), nopd, wvl)
+    w = (w[0::2] + 1j * w[1::2]).reshape(nwvf, nwvf)
+    ref = np.zeros((nwvf, nwvf), complex)
+    s = (nwvf - nopd) // 2
+    ref[s:s + nopd, s:s + nopd] = (amp * np.exp(+2j * np.pi / wvl * opd)).reshape(nopd, nopd)
+    assert np.allclose(w, ref, atol=1e-15)
+
+
+# ----------------------------------------------------------------- orc_dcog
+def test_dcog_reference_dcircle_values():
+    """test_dcircle (test/cmocka_matmath.c:80-113): a disc of radius 3 at
+    (3,5) in a 10x10 image has CoG (3-4.5, 5-4.5) within 0.1 pixel."""
+    nn, r, cx, cy = 10, 3.0, 3.0, 5.0
+    A = np.zeros((nn, nn))
+    for iy in range(nn):
+        for ix in range(nn):
+            A[iy, ix] = 2.0 if (ix - cx) ** 2 + (iy - cy) ** 2 < r * r else 0.0
+    g = oracle.dcog(A)
+    assert abs(g[0] - (cx - (nn - 1) * 0.5)) < 0.1
+    assert abs(g[1] - (cy - (nn - 1) * 0.5)) < 0.1
+
+
+def test_dcog_threshold_flux_and_background():
+    im = np.zeros((6, 6))
+    im[2, 3] = 10.0
+    im[4, 1] = 2.0
+    # centre of the 6x6 grid is 2.5; weighted mean x=(3*10+1*2)/12, y=(2*10+4*2)/12
+    g = oracle.dcog(im)
+    assert np.allclose(g, [32 / 12 - 2.5, 28 / 12 - 2.5])
+    g = oracle.dcog(im, thres=3.0)            # drops the faint pixel
+    assert np.allclose(g, [0.5, -0.5])
+    g = oracle.dcog(im + 1.0, bkgrnd=1.0)    # background removed before weighting
+    assert np.allclose(g, [32 / 12 - 2.5, 28 / 12 - 2.5])
+    g = oracle.dcog(im, flux=24.0)            # linearised: divide by the given flux
+    assert np.allclose(g, [32 / 24 - 2.5, 28 / 24 - 2.5])
+    assert np.all(oracle.dcog(np.zeros((4, 4))) == 0)  # sum <= threshold -> 0
+
+
+# ------------------------------------------- orc_wfsints / orc_lotf / steps
+def test_wfsints_flat_wavefront_closed_form(ngs_small):
+    """Flat wavefront, nominal and si from the tables: the detector image is
+    real, four-fold symmetric and, summed over a full subaperture, equals
+    sigsim times the fraction of the diffraction-limited PSF (a Dirichlet
+    kernel, computed here in closed form) that the detector samples."""
+    cfg, p, layers = ngs_small
+    from maos_amd.powfs import Layer
+    zero = [Layer(np.zeros_like(layers[0].p), layers[0].dx, 0.0, 0.0, 0.0)]
+    o = oracle.Oracle(p)
+    o.atm(zero)
+    from maos_amd import configs
+    w = configs.wfs_specs(cfg, p)[0]
+    ints, grad, _, _ = o.step(w, 0, cfg["sim"].dt)
+    full = np.argmax(p.saa)
+    img = ints[full].reshape(p.pixpsay, p.pixpsax)
+    assert np.allclose(img, img[::-1, :], rtol=1e-9) and np.allclose(img, img[:, ::-1], rtol=1e-9)
+    assert np.allclose(img, img.T, rtol=1e-9)
+    assert np.abs(grad).max() < 1e-15
+    # closed-form PSF of a full square subaperture: |D(k)|^2 with
+    # D(k) = sum_{n<nxsa} exp(-2 pi i k n / nwvf)
+    nxsa, nwvf = p.nxsa, p.nxsa * 2
+    k = np.arange(nwvf)
+    D = np.array([np.exp(-2j * np.pi * kk * np.arange(nxsa) / nwvf).sum() for kk in k])
+    psf1 = np.abs(D) ** 2 / (nxsa * nwvf)  # normalised: sum over k of psf1 = 1
+    assert math.isclose(psf1.sum(), 1.0, rel_tol=1e-12)
+    # flux seen by the detector is below sigsim (PSF wings leave the FoV)
+    tot = ints[full].sum()
+    assert 0.9 * p.sigsim < tot < p.sigsim
+
+
+def test_lotf_hermitian_and_dc(lgs_small):
+    """Uplink OTF of a real PSF is Hermitian, and its DC term is the PSF sum
+    sum|FFT(amp)|^2/nlwvf^2 = sum(amp^2) (Parseval)."""
+    cfg, p, layers = lgs_small
+    nl = p.llt_nx
+    opd = np.zeros(nl * nl)
+    N = p.notf
+    out = np.zeros(2 * N * N)
+    oracle.lib().orc_lotf(out.ctypes.data_as(oracle.dp), opd.ctypes.data_as(oracle.dp),
+                          p.llt_amp.ctypes.data_as(oracle.dp), nl, 2, nl * 2, N, p.wvl[0])
+    L = (out[0::2] + 1j * out[1::2]).reshape(N, N)
+    assert math.isclose(L[0, 0].real, (p.llt_amp ** 2).sum(), rel_tol=1e-12)
+    neg = np.roll(np.roll(L[::-1, ::-1], 1, 0), 1, 1)
+    assert np.allclose(L, np.conj(neg), atol=1e-12)
+    # numpy restatement of the same chain
+    lw = np.zeros((N, N), complex)
+    s = (N - nl) // 2
+    lw[s:s + nl, s:s + nl] = p.llt_amp.reshape(nl, nl)
+    ref = np.fft.fft2(np.abs(np.fft.fft2(lw)) ** 2 / (N * N))
+    assert rel_l2(L, ref) < 1e-12
+
+
+def test_step_tilt_moves_cog_linearly(ngs_small):
+    """A pure tilt on a flat screen moves the tCoG gradient in proportion
+    (small tilts, well inside the linear range)."""
+    cfg, p, layers = ngs_small
+    from maos_amd import configs
+    from maos_amd.powfs import Layer
+    n = layers[0].nx
+    dx = layers[0].dx
+    y, x = np.mgrid[0:n, 0:n] * dx
+    w = configs.wfs_specs(cfg, p)[0]
+    g = []
+    for t in (0.0, 0.02e-6, 0.04e-6):  # 0, 20 and 40 nm/m of x-tilt
+        scr = t * x
+        o = oracle.Oracle(p)
+        o.atm([Layer(scr, dx, 0.0, 0.0, 0.0)])
+        g.append(o.step(w, 0, cfg["sim"].dt)[1])
+    g = np.array(g)
+    full = np.argmax(p.saa)
+    gx = g[:, full]
+    gy = g[:, p.nsa + full]
+    assert abs(gx[0]) < 1e-15 and np.abs(gy).max() < 1e-12
+    assert gx[1] > 0 and math.isclose(gx[2], 2 * gx[1], rel_tol=2e-2)
+
+
+def test_mf_identities(lgs_small):
+    """The matched filter is a weighted pseudo-inverse of [gx gy i0 ...]:
+    applied through shwfs_grad it returns 0 for i0 itself and the shift for
+    i0 + dx*gx (lib/mtch.c:93-211, maos/utils.c:1043-1059)."""
+    cfg, p, layers = lgs_small
+    from maos_amd import configs
+    wfs = configs.wfs_specs(cfg, p)
+    o = oracle.Oracle(p)
+    o.atm(layers)
+    i0 = o.step(wfs[0], 2, cfg["sim"].dt)[0]
+    mtche, i0sum, _ = p.mtch(i0)
+    from maos_amd import _lib
+    gx = np.zeros_like(i0)
+    for isa in range(p.nsa):
+        d = np.zeros(p.npix)
+        _lib.setup().mxs_derive_by_fft(_lib.dptr(d), _lib.dptr(np.ascontiguousarray(i0[isa])),
+                                       p.pixpsax, p.pixpsay, 0.0)
+        gx[isa] = d / p.pixthetax
+    o.set_ref(mtche, i0sum)
+    grad = np.zeros(2 * p.nsa)
+    s = o.s
+    live = mtche.reshape(p.nsa, -1).any(axis=1)
+    for ints, expect in ((i0, 0.0), (i0 + 1e-8 * gx, 1e-8)):
+        arr = np.ascontiguousarray(ints.reshape(-1))
+        oracle.lib().orc_shwfs_grad(grad.ctypes.data_as(oracle.dp), arr.ctypes.data_as(oracle.dp),
+                                    oracle.C.byref(s), 0.0)
+        # sigmatch=1 rescales by i0sum/sum(ints) which is 1 to first order
+        assert np.abs(grad[:p.nsa][live] - expect).max() < 2e-3 * 1e-8 + 1e-14
+        assert np.abs(grad[p.nsa:][live]).max() < 2e-3 * 1e-8 + 1e-14
