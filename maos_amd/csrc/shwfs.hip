@@ -14,6 +14,8 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -189,6 +191,7 @@ struct LltArgs {
     long isim;
     double dt;
     float2 *lotf;     // [nwfs][nwvl][N*N] conj, transposed
+    float2 *lotf_t;   // [nwfs][nwvl][4][16][64] conj, MFMA C-tile layout (N = 64), or null
 };
 
 // XOR-swizzled element (x, y) of an N x N complex tile
@@ -610,9 +613,367 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
         float2 *out = a.lotf + ((long)iwfs * a.nwvl + iwvl) * N * N + uy;
 #pragma unroll
         for (int k = 0; k < N; k++) out[bitrev(k, LG) * N] = make_float2(v[k].x, -v[k].y);
+        if (N == 64 && a.lotf_t) {
+            // element (ux, uy) of the 32x32 C tile (ux/32, uy/32): v and lane
+            // follow v_mfma_f32_32x32x16 (row 8(v/4) + 4(l/32) + v%4, column l%32)
+            float2 *ot = a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * 4 * 16 * 64;
+#pragma unroll
+            for (int k = 0; k < N; k++) {
+                const int ux = bitrev(k, LG), r = ux % 32;
+                const int vv = 4 * (r / 8) + (r % 4), ll = 32 * ((r % 8) / 4) + uy % 32;
+                ot[((ux / 32) * 2 + uy / 32) * 16 * 64 + vv * 64 + ll] = make_float2(v[k].x, -v[k].y);
+            }
+        }
     }
 }
 
+
+// ------------------------------------------------------------ MFMA form
+// The same imaging chain with every 1-D DFT stage computed as a complex
+// matrix product on the matrix cores (v_mfma_f32_32x32x16_bf16 and
+// v_mfma_f32_16x16x32_bf16).  Operands are split into BF16 hi + lo pairs and
+// each real product is hi*hi + hi*lo + lo*hi, accumulated in FP32.  One
+// workgroup of four waves images one subaperture (N = 64, nxsa = 32); the
+// field moves between stages through one LDS tile:
+//   1a Z  = F[:,band] W[band,band]          (only the 32x32 nonzero block)
+//   1b Y' = F[:,band] Z'  -> P' = |Y'|^2    (P' = P transposed)
+//   2a Q  = F P                             (P real)
+//   2b O' = F Q' -> X' = O' conj(lotf') ETF'
+//   3a R' = Gx X'   (Gx: conj-DFT rows of the sampled image columns, <= 16)
+//   3b I  = Re(Gy R)                        (sampled image rows, <= 16)
+// then the si sampler, pixel processing and FP64 i0 as in k_shwfs_step.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MLD = 68;                // LDS row stride (floats) of the 64x64 planes
+constexpr int MX_MFMA_NS = 16;         // sampled image rows/columns kept by stage 3
+
+// A-operand tables: [.. tile ..][part][lane] bf16x8 with parts re_hi, re_lo,
+// im_hi, im_lo, -im_hi, -im_lo.
+struct MfmaWvl {
+    const float2 *etf_t;     // [nsa][4 tiles][16][64] ETF (nominal fused), or null
+    const float *nominal_t;  // [4][16][64] nominal when there is no ETF, or null
+    const bf16x8 *gx, *gy;   // [2 ks][6][64] rows of conj(F) at the sampled x / y
+    const int4 *si_c;        // [npix] compact indices (iy*16 + ix) into the 16x16 image
+    const float4 *si_w;      // [npix]
+    float inv_wvl;
+};
+
+struct MfmaArgs {
+    int nsa, npix, pixpsax, pixpsay, nwvl, nwfs;
+    const float *amp;
+    const float *opd;
+    const bf16x8 *F;          // [2 mt][4 ks][6][64]
+    const MfmaWvl *wv;
+    const float2 *lotf_t;     // [nwfs][nwvl][4][16][64] conj(lotfc), or null
+    const DevWfs *wfs;
+    int phytype, sigmatch;
+    float cogthres, cogoff, pixthetax, pixthetay, sigtot;
+    const float *mtche, *i0sum, *saa;
+    float saat;
+    float *ints, *grad;
+    double *i0acc;
+};
+
+__device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const __bf16 h = (__bf16)v[j];
+        hi[j] = h;
+        lo[j] = (__bf16)(v[j] - (float)h);
+    }
+}
+
+#define MF32(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0)
+#define MF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0)
+
+// c += A * B for one K step, split products; A = {hi, lo}, B = {hi, lo}
+template <typename ACC, int SZ>
+__device__ __forceinline__ void mac3(ACC &c, bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl) {
+    if constexpr (SZ == 32) {
+        c = MF32(ah, bh, c);
+        c = MF32(ah, bl, c);
+        c = MF32(al, bh, c);
+    } else {
+        c = MF16(ah, bh, c);
+        c = MF16(ah, bl, c);
+        c = MF16(al, bh, c);
+    }
+}
+
+// complex (cr, ci) += A * B; A from a 6-part table entry, B = re/im hi/lo
+template <typename ACC, int SZ>
+__device__ __forceinline__ void cmac(ACC &cr, ACC &ci, const bf16x8 *A, const bf16x8 (&br)[2],
+                                     const bf16x8 (&bi)[2], bool breal) {
+    const int L = 64;
+    const bf16x8 arh = A[0 * L], arl = A[1 * L], aih = A[2 * L], ail = A[3 * L];
+    mac3<ACC, SZ>(cr, arh, arl, br[0], br[1]);
+    mac3<ACC, SZ>(ci, aih, ail, br[0], br[1]);
+    if (!breal) {
+        const bf16x8 anh = A[4 * L], anl = A[5 * L];
+        mac3<ACC, SZ>(cr, anh, anl, bi[0], bi[1]);
+        mac3<ACC, SZ>(ci, arh, arl, bi[0], bi[1]);
+    }
+}
+
+// B operands from the LDS planes.  NAT: k is the row index; otherwise k is
+// the column index.  SZ=32: k0 + 8*(lane/32) + j, n0 + lane%32.
+// SZ=16: k0 + 8*(lane/16) + j, n0 + lane%16.
+template <int SZ, bool NAT>
+__device__ __forceinline__ void ldsB(const float *P, int k0, int n0, int lane, float (&v)[8]) {
+    const int h = lane / SZ, n = n0 + lane % SZ;
+    if (NAT) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = P[(k0 + 8 * h + j) * MLD + n];
+    } else {
+        const float4 *q = reinterpret_cast<const float4 *>(P + n * MLD + k0 + 8 * h);
+        const float4 x = q[0], y = q[1];
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    }
+}
+
+template <int SZ, bool NAT>
+__device__ __forceinline__ void loadB(const float *Pr, const float *Pi, int k0, int n0, int lane,
+                                      bf16x8 (&br)[2], bf16x8 (&bi)[2], bool breal) {
+    float v[8];
+    ldsB<SZ, NAT>(Pr, k0, n0, lane, v);
+    split8(v, br[0], br[1]);
+    if (!breal) {
+        ldsB<SZ, NAT>(Pi, k0, n0, lane, v);
+        split8(v, bi[0], bi[1]);
+    }
+}
+
+// C layout of a 32x32 tile: element v of lane l is (m0 + 8(v/4) + 4(l/32) + v%4, n0 + l%32)
+__device__ __forceinline__ int c32row(int v, int lane) { return 8 * (v / 4) + 4 * (lane / 32) + (v % 4); }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_shwfs_mfma(MfmaArgs a) {
+    __shared__ __attribute__((aligned(16))) float Lre[64 * MLD];
+    __shared__ __attribute__((aligned(16))) float Lim[64 * MLD];
+    __shared__ float red[8];
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
+    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    const DevWfs &W = a.wfs[iwfs];
+    constexpr int NX = 32, B0 = 16; // nonzero block [16, 48)
+    // OPD and amplitude of this subaperture: 4 points per thread
+    float op[4], am[4];
+    {
+        const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
+        const float *ag = a.amp + (long)isa * NX * NX;
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            op[j] = og[tid + 256 * j];
+            am[j] = ag[tid + 256 * j];
+            s += op[j];
+        }
+        seg_sum<64>(s);
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        const float mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
+#pragma unroll
+        for (int j = 0; j < 4; j++) op[j] -= mean;
+    }
+    float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
+    const int mt = wave / 2, nt = wave % 2;
+    for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
+        const MfmaWvl &WV = a.wv[iwvl];
+        // embed amp*exp(+i 2 pi opd/wvl) into the band (cembed_wvf)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int pt = tid + 256 * j, ix = pt % NX, iy = pt / NX;
+            float sn, cs;
+            sincospif(2.f * op[j] * WV.inv_wvl, &sn, &cs);
+            Lre[(B0 + iy) * MLD + B0 + ix] = am[j] * cs;
+            Lim[(B0 + iy) * MLD + B0 + ix] = am[j] * sn;
+        }
+        __syncthreads();
+        bf16x8 br[2], bi[2];
+        // 1a: Z[ky][x] = sum_y F[ky][y] W[y][x], y and x in the band (two waves)
+        f32x16 zr = {}, zi = {};
+        if (wave < 2) {
+#pragma unroll
+            for (int ks = 1; ks <= 2; ks++) {
+                loadB<32, true>(Lre, Lim, 16 * ks, B0, lane, br, bi, false);
+                cmac<f32x16, 32>(zr, zi, a.F + ((wave * 4 + ks) * 6) * 64 + lane, br, bi, false);
+            }
+        }
+        __syncthreads();
+        if (wave < 2) {
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int r = 32 * wave + c32row(v, lane), c = B0 + lane % 32;
+                Lre[r * MLD + c] = zr[v];
+                Lim[r * MLD + c] = zi[v];
+            }
+        }
+        __syncthreads();
+        // 1b: Y'[kx][ky] = sum_x F[kx][x] Z[ky][x]; P' = |Y'|^2
+        f32x16 cr = {}, ci = {};
+#pragma unroll
+        for (int ks = 1; ks <= 2; ks++) {
+            loadB<32, false>(Lre, Lim, 16 * ks, 32 * nt, lane, br, bi, false);
+            cmac<f32x16, 32>(cr, ci, a.F + ((mt * 4 + ks) * 6) * 64 + lane, br, bi, false);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int r = 32 * mt + c32row(v, lane), c = 32 * nt + lane % 32;
+            Lre[r * MLD + c] = cr[v] * cr[v] + ci[v] * ci[v];
+        }
+        __syncthreads();
+        // 2a: Q[uy][kx] = sum_ky F[uy][ky] P[ky][kx] = sum_ky F[uy][ky] P'[kx][ky]
+        cr = f32x16{};
+        ci = f32x16{};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            loadB<32, false>(Lre, Lim, 16 * ks, 32 * nt, lane, br, bi, true);
+            cmac<f32x16, 32>(cr, ci, a.F + ((mt * 4 + ks) * 6) * 64 + lane, br, bi, true);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int r = 32 * mt + c32row(v, lane), c = 32 * nt + lane % 32;
+            Lre[r * MLD + c] = cr[v];
+            Lim[r * MLD + c] = ci[v];
+        }
+        __syncthreads();
+        // 2b: O'[ux][uy] = sum_kx F[ux][kx] Q[uy][kx]; X' = O' conj(lotf') ETF'
+        cr = f32x16{};
+        ci = f32x16{};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            loadB<32, false>(Lre, Lim, 16 * ks, 32 * nt, lane, br, bi, false);
+            cmac<f32x16, 32>(cr, ci, a.F + ((mt * 4 + ks) * 6) * 64 + lane, br, bi, false);
+        }
+        {
+            const int tile = mt * 2 + nt;
+            if (a.lotf_t) {
+                const float2 *lt = a.lotf_t + (((long)iwfs * a.nwvl + iwvl) * 4 + tile) * 16 * 64 + lane;
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const float2 e = lt[v * 64];
+                    const float re = cr[v] * e.x - ci[v] * e.y, im = cr[v] * e.y + ci[v] * e.x;
+                    cr[v] = re;
+                    ci[v] = im;
+                }
+            }
+            if (WV.etf_t) {
+                const float2 *et = WV.etf_t + ((long)isa * 4 + tile) * 16 * 64 + lane;
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const float2 e = et[v * 64];
+                    const float re = cr[v] * e.x - ci[v] * e.y, im = cr[v] * e.y + ci[v] * e.x;
+                    cr[v] = re;
+                    ci[v] = im;
+                }
+            } else if (WV.nominal_t) {
+                const float *nm = WV.nominal_t + tile * 16 * 64 + lane;
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    cr[v] *= nm[v * 64];
+                    ci[v] *= nm[v * 64];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int r = 32 * mt + c32row(v, lane), c = 32 * nt + lane % 32;
+            Lre[r * MLD + c] = cr[v];
+            Lim[r * MLD + c] = ci[v];
+        }
+        __syncthreads();
+        // 3a: R'[xs][uy] = sum_ux Gx[xs][ux] X'[ux][uy]; wave w: uy in [16w, 16w+16)
+        f32x4 rr = {}, ri = {};
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            loadB<16, true>(Lre, Lim, 32 * ks, 16 * wave, lane, br, bi, false);
+            cmac<f32x4, 16>(rr, ri, WV.gx + (ks * 6) * 64 + lane, br, bi, false);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int r = 4 * (lane / 16) + v, c = 16 * wave + lane % 16;
+            Lre[r * MLD + c] = rr[v];
+            Lim[r * MLD + c] = ri[v];
+        }
+        __syncthreads();
+        // 3b: I[ys][xs] = Re sum_uy Gy[ys][uy] R[uy][xs] = Re sum Gy[ys][uy] R'[xs][uy]
+        float *img = Lre + 32 * MLD; // rows >= 16 are free now
+        if (wave == 0) {
+            f32x4 ir = {}, dummy = {};
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                loadB<16, false>(Lre, Lim, 32 * ks, 0, lane, br, bi, false);
+                const bf16x8 *A = WV.gy + (ks * 6) * 64 + lane;
+                mac3<f32x4, 16>(ir, A[0], A[64], br[0], br[1]);
+                mac3<f32x4, 16>(ir, A[4 * 64], A[5 * 64], bi[0], bi[1]);
+            }
+            (void)dummy;
+#pragma unroll
+            for (int v = 0; v < 4; v++) img[(4 * (lane / 16) + v) * 16 + lane % 16] = ir[v];
+        }
+        __syncthreads();
+        if (tid < a.npix) {
+            const int4 id = WV.si_c[tid];
+            const float4 w = WV.si_w[tid];
+            pv += W.alpha[iwvl] * (w.x * img[id.x] + w.y * img[id.y] + w.z * img[id.z] + w.w * img[id.w]);
+        }
+        __syncthreads();
+    }
+    // outputs and pixel processing (shwfs_grad, maos/utils.c:983-1122), wave 0
+    if (wave == 0) {
+        const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+        const int p = lane;
+        if (p < a.npix) {
+            a.ints[ibase + p] = pv;
+            if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
+            if (a.phytype == MX_PTYPE_MF) {
+                const float *m = a.mtche + ibase * 2 + 2 * p;
+                s0 = m[0] * pv;
+                s1 = m[1] * pv;
+                s2 = pv;
+            } else {
+                const float iI = pv - a.cogoff;
+                if (iI > a.cogthres) {
+                    s0 = iI * (float)(p % a.pixpsax);
+                    s1 = iI * (float)(p / a.pixpsax);
+                    s2 = iI;
+                }
+            }
+        }
+        seg_sum<64>(s0);
+        seg_sum<64>(s1);
+        seg_sum<64>(s2);
+        if (lane == 0 && !(a.saa[isa] < a.saat)) {
+            float gx = 0.f, gy = 0.f;
+            if (a.phytype == MX_PTYPE_MF) {
+                float scale = 1.f;
+                if (a.sigmatch == 1) {
+                    scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
+                    if (scale <= 0.f || scale > 10.f || !isfinite(scale)) scale = 1.f;
+                }
+                gx = s0 * scale;
+                gy = s1 * scale;
+            } else {
+                float sum = s2;
+                if (a.sigmatch == 0)
+                    sum = a.i0sum ? a.i0sum[(long)iwfs * a.nsa + isa] : a.sigtot * a.saa[isa];
+                if (sum > a.cogthres) {
+                    gx = (s0 / sum - (float)(a.pixpsax - 1) * 0.5f) * a.pixthetax;
+                    gy = (s1 / sum - (float)(a.pixpsay - 1) * 0.5f) * a.pixthetay;
+                }
+            }
+            a.grad[(long)iwfs * 2 * a.nsa + isa] = gx;
+            a.grad[(long)iwfs * 2 * a.nsa + a.nsa + isa] = gy;
+        }
+    }
+}
 } // namespace
 
 // ======================================================================
@@ -642,6 +1003,18 @@ struct mx_wfsgrad_s {
     float2 *d_lotf = nullptr;
     DevWvl *d_wv = nullptr;
     DevWvl h_wv[MX_MAXWVL] = {};
+    // MFMA form
+    int variant = MX_VARIANT_AUTO;     // requested form
+    bool mfma_size = false;            // N = 64, nxsa = 32
+    bool mfma_wvl_ok[MX_MAXWVL] = {};  // sampled rows/columns fit in 16
+    uint16_t *d_F = nullptr;           // bf16x8 table [2][4][6][64]
+    uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};
+    int4 *d_sic[MX_MAXWVL] = {};
+    float2 *d_etf_t[MX_MAXWVL] = {};
+    float *d_nominal_t[MX_MAXWVL] = {};
+    float2 *d_lotf_t = nullptr;
+    MfmaWvl *d_mwv = nullptr;
+    MfmaWvl h_mwv[MX_MAXWVL] = {};
     float *d_mtche = nullptr, *d_i0sum = nullptr;
     bool have_mtche = false, have_i0sum = false;
     float *d_ints = nullptr, *d_grad = nullptr, *d_opd = nullptr;
@@ -690,6 +1063,98 @@ void transpose_c(float2 *dst, const double *src, long N, bool conj) {
 
 bool supported(long notf, long nxsa, long embfac) {
     return (notf == 64 && nxsa == 32 && embfac == 2) || (notf == 32 && nxsa == 16 && embfac == 2);
+}
+
+// FP32 -> BF16 with round-to-nearest-even (finite inputs)
+uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float bf2f(uint16_t b) {
+    uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+// hi/lo BF16 pair of an FP64 value: hi = bf16(x), lo = bf16(x - hi)
+void split_bf(double x, uint16_t &hi, uint16_t &lo) {
+    hi = f2bf((float)x);
+    lo = f2bf((float)(x - (double)bf2f(hi)));
+}
+// one table entry (6 parts x 64 lanes x 8) of complex values c[lane][j]
+void put_entry(uint16_t *dst, const double (*re)[8], const double (*im)[8]) {
+    for (int lane = 0; lane < 64; lane++)
+        for (int j = 0; j < 8; j++) {
+            uint16_t h, l;
+            split_bf(re[lane][j], h, l);
+            dst[(0 * 64 + lane) * 8 + j] = h;
+            dst[(1 * 64 + lane) * 8 + j] = l;
+            split_bf(im[lane][j], h, l);
+            dst[(2 * 64 + lane) * 8 + j] = h;
+            dst[(3 * 64 + lane) * 8 + j] = l;
+            split_bf(-im[lane][j], h, l);
+            dst[(4 * 64 + lane) * 8 + j] = h;
+            dst[(5 * 64 + lane) * 8 + j] = l;
+        }
+}
+// forward DFT matrix F[m][k] = exp(-2 pi i m k / 64) as A operands of
+// v_mfma_f32_32x32x16_bf16: entry (mt, ks), lane l, j -> m = 32 mt + l%32,
+// k = 16 ks + 8 (l/32) + j
+std::vector<uint16_t> build_F64() {
+    std::vector<uint16_t> t(2 * 4 * 6 * 64 * 8);
+    double re[64][8], im[64][8];
+    for (int mt = 0; mt < 2; mt++)
+        for (int ks = 0; ks < 4; ks++) {
+            for (int lane = 0; lane < 64; lane++)
+                for (int j = 0; j < 8; j++) {
+                    const int m = 32 * mt + lane % 32, k = 16 * ks + 8 * (lane / 32) + j;
+                    const double ang = -2. * M_PI * (double)((m * k) % 64) / 64.;
+                    re[lane][j] = std::cos(ang);
+                    im[lane][j] = std::sin(ang);
+                }
+            put_entry(t.data() + (size_t)(mt * 4 + ks) * 6 * 64 * 8, re, im);
+        }
+    return t;
+}
+// rows of the inverse DFT conj(F) at the sampled coordinates rows[0..n) as A
+// operands of v_mfma_f32_16x16x32_bf16: entry ks, lane l, j -> row l%16,
+// k = 32 ks + 8 (l/16) + j; rows beyond n are zero
+std::vector<uint16_t> build_G(const std::vector<int> &rows) {
+    std::vector<uint16_t> t(2 * 6 * 64 * 8);
+    double re[64][8], im[64][8];
+    for (int ks = 0; ks < 2; ks++) {
+        for (int lane = 0; lane < 64; lane++)
+            for (int j = 0; j < 8; j++) {
+                const int r = lane % 16, k = 32 * ks + 8 * (lane / 16) + j;
+                if (r < (int)rows.size()) {
+                    const double ang = 2. * M_PI * (double)((rows[r] * k) % 64) / 64.;
+                    re[lane][j] = std::cos(ang);
+                    im[lane][j] = std::sin(ang);
+                } else {
+                    re[lane][j] = im[lane][j] = 0.;
+                }
+            }
+        put_entry(t.data() + (size_t)ks * 6 * 64 * 8, re, im);
+    }
+    return t;
+}
+// element (ux, uy) of an N=64 table -> its slot in the C-tile layout
+// [tile = 2(ux/32) + uy/32][v][lane] of v_mfma_f32_32x32x16
+inline long tile_slot(int ux, int uy) {
+    const int r = ux % 32;
+    const int v = 4 * (r / 8) + (r % 4), l = 32 * ((r % 8) / 4) + uy % 32;
+    return ((long)((ux / 32) * 2 + uy / 32) * 16 + v) * 64 + l;
+}
+// form used by the next step: the requested one, or for AUTO the MFMA form
+// whenever its size and sampling limits hold; -1 if the request cannot run
+int effective_variant(const mx_wfsgrad_t *h) {
+    bool ok = h->mfma_size;
+    for (int w = 0; w < h->nwvl; w++) ok = ok && h->mfma_wvl_ok[w] && !h->d_etf2[w];
+    if (h->variant == MX_VARIANT_FFT32) return MX_VARIANT_FFT32;
+    if (h->variant == MX_VARIANT_MFMA_BF16) return ok ? MX_VARIANT_MFMA_BF16 : -1;
+    return ok ? MX_VARIANT_MFMA_BF16 : MX_VARIANT_FFT32;
 }
 } // namespace
 
@@ -758,9 +1223,15 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
     ALLOC(h->d_i0acc, nwfs * nsa * npix);
     ALLOC(h->d_mtche, nwfs * nsa * 2 * npix);
     ALLOC(h->d_i0sum, nwfs * nsa);
+    h->mfma_size = (N == 64 && h->nxsa == 32);
     if (p->llt_nx > 0) {
         ALLOC(h->d_llt_amp, p->llt_nx * p->llt_nx);
         ALLOC(h->d_lotf, nwfs * h->nwvl * N * N);
+        if (h->mfma_size) ALLOC(h->d_lotf_t, nwfs * h->nwvl * N * N);
+    }
+    if (h->mfma_size) {
+        ALLOC(h->d_F, 2 * 4 * 6 * 64 * 8);
+        ALLOC(h->d_mwv, MX_MAXWVL);
     }
 #undef ALLOC
     auto fail = [&](int code) {
@@ -799,6 +1270,11 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
     }
     if (hipMemcpy(h->d_wfs, dw.data(), sizeof(DevWfs) * nwfs, hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_err(MX_EHIP, "upload of WFS parameters failed"));
+    if (h->mfma_size) {
+        std::vector<uint16_t> F = build_F64();
+        if (hipMemcpy(h->d_F, F.data(), F.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(MX_EHIP, "upload of the DFT table failed"));
+    }
     if (hipMemset(h->d_grad, 0, sizeof(float) * nwfs * 2 * nsa) != hipSuccess ||
         hipMemset(h->d_i0acc, 0, sizeof(double) * nwfs * nsa * npix) != hipSuccess ||
         hipMemset(h->d_mtche, 0, sizeof(float) * nwfs * nsa * 2 * npix) != hipSuccess ||
@@ -813,11 +1289,17 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_lotf,
+                    h->d_lotf_t, h->d_F, h->d_mwv,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
     for (void *q : ptrs)
         if (q) hipFree(q);
     for (int w = 0; w < MX_MAXWVL; w++) {
         if (h->d_nominal[w]) hipFree(h->d_nominal[w]);
+        if (h->d_gx[w]) hipFree(h->d_gx[w]);
+        if (h->d_gy[w]) hipFree(h->d_gy[w]);
+        if (h->d_sic[w]) hipFree(h->d_sic[w]);
+        if (h->d_etf_t[w]) hipFree(h->d_etf_t[w]);
+        if (h->d_nominal_t[w]) hipFree(h->d_nominal_t[w]);
         if (h->d_etf[w]) hipFree(h->d_etf[w]);
         if (h->d_etf2[w]) hipFree(h->d_etf2[w]);
         if (h->d_si_idx[w]) hipFree(h->d_si_idx[w]);
@@ -867,6 +1349,50 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
             for (long ux = 0; ux < N; ux++) nt[ux * N + uy] = (float)nominal[ux + uy * N];
         HIPCHK(hipMemcpy(h->d_nominal[iwvl], nt.data(), sizeof(float) * N * N, hipMemcpyHostToDevice));
     }
+    if (h->mfma_size) {
+        // image columns / rows the sampler reads, and si in compact 16x16 indices
+        std::vector<int> xs, ys;
+        for (long i = 0; i < h->npix; i++) {
+            const int *ip = &idx[i].x;
+            const float *wp = &wt[i].x;
+            for (int c = 0; c < 4; c++)
+                if (wp[c] != 0.f) {
+                    xs.push_back(ip[c] % (int)N);
+                    ys.push_back(ip[c] / (int)N);
+                }
+        }
+        std::sort(xs.begin(), xs.end());
+        xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+        std::sort(ys.begin(), ys.end());
+        ys.erase(std::unique(ys.begin(), ys.end()), ys.end());
+        h->mfma_wvl_ok[iwvl] = xs.size() <= MX_MFMA_NS && ys.size() <= MX_MFMA_NS;
+        if (h->mfma_wvl_ok[iwvl]) {
+            std::vector<int4> sic(h->npix);
+            for (long i = 0; i < h->npix; i++) {
+                int *op = &sic[i].x;
+                const int *ip = &idx[i].x;
+                for (int c = 0; c < 4; c++) {
+                    const int cx = (int)(std::lower_bound(xs.begin(), xs.end(), ip[c] % (int)N) - xs.begin());
+                    const int cy = (int)(std::lower_bound(ys.begin(), ys.end(), ip[c] / (int)N) - ys.begin());
+                    op[c] = (&wt[i].x)[c] != 0.f ? cy * MX_MFMA_NS + cx : 0;
+                }
+            }
+            std::vector<uint16_t> gx = build_G(xs), gy = build_G(ys);
+            if ((rc = dev_alloc(&h->d_gx[iwvl], gx.size())) || (rc = dev_alloc(&h->d_gy[iwvl], gy.size())) ||
+                (rc = dev_alloc(&h->d_sic[iwvl], h->npix)))
+                return rc;
+            HIPCHK(hipMemcpy(h->d_gx[iwvl], gx.data(), gx.size() * 2, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->d_gy[iwvl], gy.data(), gy.size() * 2, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->d_sic[iwvl], sic.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
+            if (nominal && h->cfg.llt_nx == 0) {
+                std::vector<float> nt(N * N);
+                for (long uy = 0; uy < N; uy++)
+                    for (long ux = 0; ux < N; ux++) nt[tile_slot((int)ux, (int)uy)] = (float)nominal[ux + uy * N];
+                if ((rc = dev_alloc(&h->d_nominal_t[iwvl], N * N))) return rc;
+                HIPCHK(hipMemcpy(h->d_nominal_t[iwvl], nt.data(), sizeof(float) * N * N, hipMemcpyHostToDevice));
+            }
+        }
+    }
     h->dtf_set[iwvl] = true;
     return MX_OK;
 }
@@ -891,6 +1417,16 @@ int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const do
         float2 **dst = which ? &h->d_etf2[iwvl] : &h->d_etf[iwvl];
         if ((rc = dev_alloc(dst, nsa * N * N))) return rc;
         HIPCHK(hipMemcpy(*dst, t.data(), sizeof(float2) * nsa * N * N, hipMemcpyHostToDevice));
+        if (h->mfma_size && !which) {
+            for (long isa = 0; isa < nsa; isa++)
+                for (long uy = 0; uy < N; uy++)
+                    for (long ux = 0; ux < N; ux++) {
+                        const double *e = src + 2 * (isa * N * N + ux + uy * N);
+                        t[isa * N * N + tile_slot((int)ux, (int)uy)] = make_float2((float)e[0], (float)e[1]);
+                    }
+            if ((rc = dev_alloc(&h->d_etf_t[iwvl], nsa * N * N))) return rc;
+            HIPCHK(hipMemcpy(h->d_etf_t[iwvl], t.data(), sizeof(float2) * nsa * N * N, hipMemcpyHostToDevice));
+        }
     }
     h->etf2wt = etf2 ? (float)etf2wt : 0.f;
     return MX_OK;
@@ -976,6 +1512,10 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche) return set_err(MX_ESTATE, "matched filter not set");
     if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum)
         return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=1)");
+    const int variant = effective_variant(h);
+    if (variant < 0)
+        return set_err(MX_EINVAL, "the MFMA form needs notf=64, nxsa=32, <= %d sampled image rows and "
+                       "columns and no second ETF", MX_MFMA_NS);
     HIPCHK(hipSetDevice(h->device));
     const long N = h->notf;
     {
@@ -1026,6 +1566,7 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         la.isim = isim;
         la.dt = dt;
         la.lotf = h->d_lotf;
+        la.lotf_t = h->d_lotf_t;
         dim3 grid((unsigned)(h->nwfs * h->nwvl));
         if (N == 64) hipLaunchKernelGGL((k_llt_otf<64, 32>), grid, dim3(64), 0, h->stream, la);
         else hipLaunchKernelGGL((k_llt_otf<32, 16>), grid, dim3(64), 0, h->stream, la);
@@ -1088,11 +1629,57 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     a.grad = h->d_grad;
     a.i0acc = h->i0_enable ? h->d_i0acc : nullptr;
     a.opd = h->d_opd;
-    const int spw = (int)(64 / N);
-    const long ngroup = (h->nsa + spw - 1) / spw;
-    dim3 grid((unsigned)(ngroup * h->nwfs));
-    if (N == 64) hipLaunchKernelGGL((k_shwfs_step<64, 32, 1>), grid, dim3(64), 0, h->stream, a);
-    else hipLaunchKernelGGL((k_shwfs_step<32, 16, 2>), grid, dim3(64), 0, h->stream, a);
+    if (variant == MX_VARIANT_MFMA_BF16) {
+        MfmaWvl hw[MX_MAXWVL] = {};
+        for (int w = 0; w < h->nwvl; w++) {
+            hw[w].etf_t = h->d_etf_t[w];
+            hw[w].nominal_t = h->d_nominal_t[w];
+            hw[w].gx = reinterpret_cast<const bf16x8 *>(h->d_gx[w]);
+            hw[w].gy = reinterpret_cast<const bf16x8 *>(h->d_gy[w]);
+            hw[w].si_c = h->d_sic[w];
+            hw[w].si_w = h->d_si_w[w];
+            hw[w].inv_wvl = (float)(1. / h->wvl[w]);
+        }
+        if (std::memcmp(hw, h->h_mwv, sizeof(hw)) != 0) {
+            HIPCHK(hipMemcpyAsync(h->d_mwv, hw, sizeof(hw), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            std::memcpy(h->h_mwv, hw, sizeof(hw));
+        }
+        MfmaArgs m{};
+        m.nsa = a.nsa;
+        m.npix = a.npix;
+        m.pixpsax = a.pixpsax;
+        m.pixpsay = a.pixpsay;
+        m.nwvl = a.nwvl;
+        m.nwfs = a.nwfs;
+        m.amp = a.amp;
+        m.opd = a.opd;
+        m.F = reinterpret_cast<const bf16x8 *>(h->d_F);
+        m.wv = h->d_mwv;
+        m.lotf_t = h->cfg.llt_nx > 0 ? h->d_lotf_t : nullptr;
+        m.wfs = a.wfs;
+        m.phytype = a.phytype;
+        m.sigmatch = a.sigmatch;
+        m.cogthres = a.cogthres;
+        m.cogoff = a.cogoff;
+        m.pixthetax = a.pixthetax;
+        m.pixthetay = a.pixthetay;
+        m.sigtot = a.sigtot;
+        m.mtche = a.mtche;
+        m.i0sum = a.i0sum;
+        m.saa = a.saa;
+        m.saat = a.saat;
+        m.ints = a.ints;
+        m.grad = a.grad;
+        m.i0acc = a.i0acc;
+        hipLaunchKernelGGL(k_shwfs_mfma, dim3((unsigned)(h->nsa * h->nwfs)), dim3(256), 0, h->stream, m);
+    } else {
+        const int spw = (int)(64 / N);
+        const long ngroup = (h->nsa + spw - 1) / spw;
+        dim3 grid((unsigned)(ngroup * h->nwfs));
+        if (N == 64) hipLaunchKernelGGL((k_shwfs_step<64, 32, 1>), grid, dim3(64), 0, h->stream, a);
+        else hipLaunchKernelGGL((k_shwfs_step<32, 16, 2>), grid, dim3(64), 0, h->stream, a);
+    }
     HIPCHK(hipGetLastError());
     if (h->timing) {
         HIPCHK(hipEventRecord(e3, h->stream));
@@ -1186,6 +1773,22 @@ int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf) {
             lotf[2 * (ux + uy * N)] = v.x;
             lotf[2 * (ux + uy * N) + 1] = -v.y; // stored conjugated
         }
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    if (variant != MX_VARIANT_AUTO && variant != MX_VARIANT_FFT32 && variant != MX_VARIANT_MFMA_BF16)
+        return set_err(MX_EINVAL, "unknown variant %d", variant);
+    if (variant == MX_VARIANT_MFMA_BF16 && !h->mfma_size)
+        return set_err(MX_EINVAL, "the MFMA form needs notf=64 and nxsa=32");
+    h->variant = variant;
+    return MX_OK;
+}
+
+int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant) {
+    if (!h || !variant) return set_err(MX_EINVAL, "null argument");
+    *variant = effective_variant(h);
     return MX_OK;
 }
 

@@ -119,6 +119,15 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_debug_lotf(self.h, iwfs, iwvl, dptr(out)))
         return out
 
+    def set_variant(self, variant):
+        """_lib.VARIANT_AUTO, VARIANT_FFT32 or VARIANT_MFMA_BF16."""
+        check(_lib.hip().mx_wfsgrad_set_variant(self.h, variant))
+
+    def variant(self):
+        v = C.c_int(0)
+        check(_lib.hip().mx_wfsgrad_get_variant(self.h, C.byref(v)))
+        return v.value
+
     def timing_enable(self, on=True):
         check(_lib.hip().mx_wfsgrad_timing_enable(self.h, int(on)))
 

@@ -31,9 +31,13 @@ def _oracle(p, layers, mtche=None, i0sum=None):
 
 
 def test_ngs8m_step_parity(ngs_small):
+    from maos_amd import _lib
     cfg, p, layers = ngs_small
     print("case: ngs8m nsa=%d notf=%d 512^2 screen" % (p.nsa, p.notf), flush=True)
     eng, wfs = _engine(cfg, p, layers)
+    assert eng.variant() == _lib.VARIANT_FFT32  # notf=32: the MFMA form does not apply
+    with pytest.raises(RuntimeError):
+        eng.set_variant(_lib.VARIANT_MFMA_BF16)
     o = _oracle(p, layers)
     dt = cfg["sim"].dt
     for isim in (0, 57, 99):
@@ -58,11 +62,16 @@ def _lgs_ref(p, o, wfs, dt, isim=3):
     return mtche, i0sum
 
 
-def test_lgs_step_parity(lgs_small):
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16"])
+def test_lgs_step_parity(lgs_small, variant):
+    from maos_amd import _lib
     cfg, p, layers = lgs_small
-    print("case: lgs order20 nsa=%d notf=%d nwfs=%d 1024^2 screens" % (p.nsa, p.notf, len(cfg["wfs"])),
-          flush=True)
+    print("case: lgs order20 nsa=%d notf=%d nwfs=%d 1024^2 screens, %s"
+          % (p.nsa, p.notf, len(cfg["wfs"]), variant), flush=True)
     eng, wfs = _engine(cfg, p, layers)
+    want = {"fft32": _lib.VARIANT_FFT32, "mfma_bf16": _lib.VARIANT_MFMA_BF16}[variant]
+    eng.set_variant(want)
+    assert eng.variant() == want
     o = _oracle(p, layers)
     dt = cfg["sim"].dt
     mtche, i0sum = _lgs_ref(p, o, wfs, dt)
