@@ -614,14 +614,16 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
 #pragma unroll
         for (int k = 0; k < N; k++) out[bitrev(k, LG) * N] = make_float2(v[k].x, -v[k].y);
         if (N == 64 && a.lotf_t) {
-            // element (ux, uy) of the 32x32 C tile (ux/32, uy/32): v and lane
-            // follow v_mfma_f32_32x32x16 (row 8(v/4) + 4(l/32) + v%4, column l%32)
+            // element (ux, uy) -> the C tile of the MFMA imaging kernel's
+            // stage 2b: tile 2 (uy/32) + ux/32, uy%32 = 8 (v/4) + 4 (l/32) + v%4,
+            // ux%32 = l%32
             float2 *ot = a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * 4 * 16 * 64;
+            const int r = uy % 32, vv = 4 * (r / 8) + (r % 4), hh = (r % 8) / 4;
 #pragma unroll
             for (int k = 0; k < N; k++) {
-                const int ux = bitrev(k, LG), r = ux % 32;
-                const int vv = 4 * (r / 8) + (r % 4), ll = 32 * ((r % 8) / 4) + uy % 32;
-                ot[((ux / 32) * 2 + uy / 32) * 16 * 64 + vv * 64 + ll] = make_float2(v[k].x, -v[k].y);
+                const int ux = bitrev(k, LG);
+                ot[((uy / 32) * 2 + ux / 32) * 16 * 64 + vv * 64 + 32 * hh + ux % 32] =
+                    make_float2(v[k].x, -v[k].y);
             }
         }
     }
@@ -630,31 +632,42 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
 
 // ------------------------------------------------------------ MFMA form
 // The same imaging chain with every 1-D DFT stage computed as a complex
-// matrix product on the matrix cores (v_mfma_f32_32x32x16_bf16 and
-// v_mfma_f32_16x16x32_bf16).  Operands are split into BF16 hi + lo pairs and
-// each real product is hi*hi + hi*lo + lo*hi, accumulated in FP32.  One
-// workgroup of four waves images one subaperture (N = 64, nxsa = 32); the
-// field moves between stages through one LDS tile:
-//   1a Z  = F[:,band] W[band,band]          (only the 32x32 nonzero block)
-//   1b Y' = F[:,band] Z'  -> P' = |Y'|^2    (P' = P transposed)
-//   2a Q  = F P                             (P real)
-//   2b O' = F Q' -> X' = O' conj(lotf') ETF'
-//   3a R' = Gx X'   (Gx: conj-DFT rows of the sampled image columns, <= 16)
-//   3b I  = Re(Gy R)                        (sampled image rows, <= 16)
-// then the si sampler, pixel processing and FP64 i0 as in k_shwfs_step.
+// matrix product on the matrix cores (v_mfma_f32_32x32x16_bf16, and
+// v_mfma_f32_16x16x32_bf16 for the sampled inverse).  Operands are split
+// into BF16 hi + lo pairs and each real product is hi*hi + hi*lo + lo*hi,
+// accumulated in FP32.  One workgroup of four waves images one subaperture
+// (N = 64, nxsa = 32).
+//
+// In every stage the field is the A operand and the (symmetric) DFT matrix
+// the B operand, so a stage reads its input as rows that are contiguous
+// along the summed index and writes its output transposed, contiguous along
+// the index the next stage sums over.  The field therefore lives in LDS as
+// four BF16 planes (re hi, re lo, im hi, im lo), split once by the stage
+// that produces it; operands are plain 16-byte LDS reads.  Each wave keeps
+// its 32 columns of the DFT matrix in registers.
+//   1a Z[ky][x]   = sum_y W[x][y] F[y][ky]         (only the 32x32 block)
+//   1b Y[kx][ky]  = sum_x Z[ky][x] F[x][kx]  -> P = |Y|^2
+//   2a Q[uy][kx]  = sum_ky P[kx][ky] F[ky][uy]
+//   2b O[ux][uy]  = sum_kx Q[uy][kx] F[kx][ux] -> X = O conj(lotf) ETF
+//   3a R[ys][ux]  = sum_uy X[ux][uy] G[uy][ys]      (sampled image rows)
+//   3b I[ys][xs]  = Re sum_ux R[ys][ux] G[ux][xs]   (sampled image columns)
+// (G = conj F restricted to <= 16 sampled coordinates), then the si sampler,
+// pixel processing and FP64 i0 as in k_shwfs_step.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int MLD = 68;                // LDS row stride (floats) of the 64x64 planes
-constexpr int MX_MFMA_NS = 16;         // sampled image rows/columns kept by stage 3
+constexpr int MLDB = 72;        // bf16 row stride of the LDS planes
+constexpr int MPLANE = 64 * MLDB;
+constexpr int MX_MFMA_NS = 16;  // sampled image rows/columns kept by stage 3
 
-// A-operand tables: [.. tile ..][part][lane] bf16x8 with parts re_hi, re_lo,
-// im_hi, im_lo, -im_hi, -im_lo.
+// Table entries: [.. entry ..][part][lane] bf16x8, parts re hi, re lo, im hi,
+// im lo; a lane holds 8 consecutive k of one n (B-operand layout).
 struct MfmaWvl {
     const float2 *etf_t;     // [nsa][4 tiles][16][64] ETF (nominal fused), or null
     const float *nominal_t;  // [4][16][64] nominal when there is no ETF, or null
-    const bf16x8 *gx, *gy;   // [2 ks][6][64] rows of conj(F) at the sampled x / y
+    const bf16x8 *gx, *gy;   // [2 ks][4][64] conj(F) at the sampled columns / rows
     const int4 *si_c;        // [npix] compact indices (iy*16 + ix) into the 16x16 image
     const float4 *si_w;      // [npix]
     float inv_wvl;
@@ -664,7 +677,7 @@ struct MfmaArgs {
     int nsa, npix, pixpsax, pixpsay, nwvl, nwfs;
     const float *amp;
     const float *opd;
-    const bf16x8 *F;          // [2 mt][4 ks][6][64]
+    const bf16x8 *F;          // [2 blocks][4 ks][4][64]
     const MfmaWvl *wv;
     const float2 *lotf_t;     // [nwfs][nwvl][4][16][64] conj(lotfc), or null
     const DevWfs *wfs;
@@ -676,19 +689,10 @@ struct MfmaArgs {
     double *i0acc;
 };
 
-__device__ __forceinline__ void split8(const float (&v)[8], bf16x8 &hi, bf16x8 &lo) {
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-        const __bf16 h = (__bf16)v[j];
-        hi[j] = h;
-        lo[j] = (__bf16)(v[j] - (float)h);
-    }
-}
-
 #define MF32(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0)
 #define MF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0)
 
-// c += A * B for one K step, split products; A = {hi, lo}, B = {hi, lo}
+// c += A * B for one K step with split operands: hi*hi + hi*lo + lo*hi
 template <typename ACC, int SZ>
 __device__ __forceinline__ void mac3(ACC &c, bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl) {
     if constexpr (SZ == 32) {
@@ -702,63 +706,77 @@ __device__ __forceinline__ void mac3(ACC &c, bf16x8 ah, bf16x8 al, bf16x8 bh, bf
     }
 }
 
-// complex (cr, ci) += A * B; A from a 6-part table entry, B = re/im hi/lo
-template <typename ACC, int SZ>
-__device__ __forceinline__ void cmac(ACC &cr, ACC &ci, const bf16x8 *A, const bf16x8 (&br)[2],
-                                     const bf16x8 (&bi)[2], bool breal) {
-    const int L = 64;
-    const bf16x8 arh = A[0 * L], arl = A[1 * L], aih = A[2 * L], ail = A[3 * L];
-    mac3<ACC, SZ>(cr, arh, arl, br[0], br[1]);
-    mac3<ACC, SZ>(ci, aih, ail, br[0], br[1]);
-    if (!breal) {
-        const bf16x8 anh = A[4 * L], anl = A[5 * L];
-        mac3<ACC, SZ>(cr, anh, anl, bi[0], bi[1]);
-        mac3<ACC, SZ>(ci, arh, arl, bi[0], bi[1]);
+// (cr, ci) += D * B with D the field (A operand, from LDS) and B = (br, bi)
+template <typename ACC, int SZ, bool DREAL>
+__device__ __forceinline__ void cmacd(ACC &cr, ACC &ci, const __bf16 *const (&L)[4], int row, int k,
+                                      const bf16x8 (&br)[2], const bf16x8 (&bi)[2]) {
+    const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(L[0] + row * MLDB + k);
+    const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(L[1] + row * MLDB + k);
+    mac3<ACC, SZ>(cr, dh, dl, br[0], br[1]);
+    mac3<ACC, SZ>(ci, dh, dl, bi[0], bi[1]);
+    if (!DREAL) {
+        const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(L[2] + row * MLDB + k);
+        const bf16x8 el = *reinterpret_cast<const bf16x8 *>(L[3] + row * MLDB + k);
+        mac3<ACC, SZ>(cr, -eh, -el, bi[0], bi[1]);
+        mac3<ACC, SZ>(ci, eh, el, br[0], br[1]);
     }
 }
 
-// B operands from the LDS planes.  NAT: k is the row index; otherwise k is
-// the column index.  SZ=32: k0 + 8*(lane/32) + j, n0 + lane%32.
-// SZ=16: k0 + 8*(lane/16) + j, n0 + lane%16.
-template <int SZ, bool NAT>
-__device__ __forceinline__ void ldsB(const float *P, int k0, int n0, int lane, float (&v)[8]) {
-    const int h = lane / SZ, n = n0 + lane % SZ;
-    if (NAT) {
+__device__ __forceinline__ void loadT(const bf16x8 *T, int lane, bf16x8 (&br)[2], bf16x8 (&bi)[2]) {
+    br[0] = T[0 * 64 + lane];
+    br[1] = T[1 * 64 + lane];
+    bi[0] = T[2 * 64 + lane];
+    bi[1] = T[3 * 64 + lane];
+}
+
+// split 4 values into hi/lo and store them at plane row, columns k..k+3
+__device__ __forceinline__ void st4(__bf16 *ph, __bf16 *pl, int row, int k, float a0, float a1, float a2,
+                                    float a3) {
+    bf16x4 h, l;
+    h[0] = (__bf16)a0;
+    h[1] = (__bf16)a1;
+    h[2] = (__bf16)a2;
+    h[3] = (__bf16)a3;
+    l[0] = (__bf16)(a0 - (float)h[0]);
+    l[1] = (__bf16)(a1 - (float)h[1]);
+    l[2] = (__bf16)(a2 - (float)h[2]);
+    l[3] = (__bf16)(a3 - (float)h[3]);
+    *reinterpret_cast<bf16x4 *>(ph + row * MLDB + k) = h;
+    *reinterpret_cast<bf16x4 *>(pl + row * MLDB + k) = l;
+}
+
+// write a 32x32 C tile transposed: element (m0 + 8g + 4h + i, n0 + l%32) goes
+// to plane row n0 + l%32, column m0 + 8g + 4h + i
+__device__ __forceinline__ void st_tile32(__bf16 *const (&L)[4], const f32x16 &cr, const f32x16 &ci,
+                                          int m0, int n0, int lane, bool real) {
+    const int row = n0 + lane % 32, h = lane / 32;
 #pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = P[(k0 + 8 * h + j) * MLD + n];
-    } else {
-        const float4 *q = reinterpret_cast<const float4 *>(P + n * MLD + k0 + 8 * h);
-        const float4 x = q[0], y = q[1];
-        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-        v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    for (int g = 0; g < 4; g++) {
+        const int k = m0 + 8 * g + 4 * h;
+        st4(L[0], L[1], row, k, cr[4 * g], cr[4 * g + 1], cr[4 * g + 2], cr[4 * g + 3]);
+        if (!real) st4(L[2], L[3], row, k, ci[4 * g], ci[4 * g + 1], ci[4 * g + 2], ci[4 * g + 3]);
     }
 }
-
-template <int SZ, bool NAT>
-__device__ __forceinline__ void loadB(const float *Pr, const float *Pi, int k0, int n0, int lane,
-                                      bf16x8 (&br)[2], bf16x8 (&bi)[2], bool breal) {
-    float v[8];
-    ldsB<SZ, NAT>(Pr, k0, n0, lane, v);
-    split8(v, br[0], br[1]);
-    if (!breal) {
-        ldsB<SZ, NAT>(Pi, k0, n0, lane, v);
-        split8(v, bi[0], bi[1]);
-    }
-}
-
-// C layout of a 32x32 tile: element v of lane l is (m0 + 8(v/4) + 4(l/32) + v%4, n0 + l%32)
-__device__ __forceinline__ int c32row(int v, int lane) { return 8 * (v / 4) + 4 * (lane / 32) + (v % 4); }
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_shwfs_mfma(MfmaArgs a) {
-    __shared__ __attribute__((aligned(16))) float Lre[64 * MLD];
-    __shared__ __attribute__((aligned(16))) float Lim[64 * MLD];
-    __shared__ float red[8];
+    __shared__ __attribute__((aligned(16))) __bf16 Lb[4 * MPLANE];
+    __shared__ float img[MX_MFMA_NS * MX_MFMA_NS];
+    __shared__ float red[4];
+    __bf16 *const L[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
+    const __bf16 *const Lc[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
     const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
     const DevWfs &W = a.wfs[iwfs];
     constexpr int NX = 32, B0 = 16; // nonzero block [16, 48)
-    // OPD and amplitude of this subaperture: 4 points per thread
+    const int mb = wave % 2, nb = wave / 2; // m block (rows of the field), n block (frequencies)
+    const int h32 = lane / 32, l32 = lane % 32;
+    // this wave's 32 DFT columns n = 32 nb + l%32, k = 16 ks + 8 (l/32) + j
+    bf16x8 fr[4][2], fi[4][2];
+#pragma unroll
+    for (int ks = 0; ks < 4; ks++) loadT(a.F + (nb * 4 + ks) * 4 * 64, lane, fr[ks], fi[ks]);
+    // OPD and amplitude: thread (ix = tid%32, iy = 4 (tid/32) + j)
+    const int ix = tid % 32, iy0 = 4 * (tid / 32);
     float op[4], am[4];
     {
         const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
@@ -766,8 +784,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         float s = 0.f;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            op[j] = og[tid + 256 * j];
-            am[j] = ag[tid + 256 * j];
+            op[j] = og[(iy0 + j) * NX + ix];
+            am[j] = ag[(iy0 + j) * NX + ix];
             s += op[j];
         }
         seg_sum<64>(s);
@@ -778,79 +796,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int j = 0; j < 4; j++) op[j] -= mean;
     }
     float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
-    const int mt = wave / 2, nt = wave % 2;
     for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
         const MfmaWvl &WV = a.wv[iwvl];
-        // embed amp*exp(+i 2 pi opd/wvl) into the band (cembed_wvf)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int pt = tid + 256 * j, ix = pt % NX, iy = pt / NX;
-            float sn, cs;
-            sincospif(2.f * op[j] * WV.inv_wvl, &sn, &cs);
-            Lre[(B0 + iy) * MLD + B0 + ix] = am[j] * cs;
-            Lim[(B0 + iy) * MLD + B0 + ix] = am[j] * sn;
-        }
-        __syncthreads();
-        bf16x8 br[2], bi[2];
-        // 1a: Z[ky][x] = sum_y F[ky][y] W[y][x], y and x in the band (two waves)
-        f32x16 zr = {}, zi = {};
-        if (wave < 2) {
-#pragma unroll
-            for (int ks = 1; ks <= 2; ks++) {
-                loadB<32, true>(Lre, Lim, 16 * ks, B0, lane, br, bi, false);
-                cmac<f32x16, 32>(zr, zi, a.F + ((wave * 4 + ks) * 6) * 64 + lane, br, bi, false);
-            }
-        }
-        __syncthreads();
-        if (wave < 2) {
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int r = 32 * wave + c32row(v, lane), c = B0 + lane % 32;
-                Lre[r * MLD + c] = zr[v];
-                Lim[r * MLD + c] = zi[v];
-            }
-        }
-        __syncthreads();
-        // 1b: Y'[kx][ky] = sum_x F[kx][x] Z[ky][x]; P' = |Y'|^2
-        f32x16 cr = {}, ci = {};
-#pragma unroll
-        for (int ks = 1; ks <= 2; ks++) {
-            loadB<32, false>(Lre, Lim, 16 * ks, 32 * nt, lane, br, bi, false);
-            cmac<f32x16, 32>(cr, ci, a.F + ((mt * 4 + ks) * 6) * 64 + lane, br, bi, false);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int r = 32 * mt + c32row(v, lane), c = 32 * nt + lane % 32;
-            Lre[r * MLD + c] = cr[v] * cr[v] + ci[v] * ci[v];
-        }
-        __syncthreads();
-        // 2a: Q[uy][kx] = sum_ky F[uy][ky] P[ky][kx] = sum_ky F[uy][ky] P'[kx][ky]
-        cr = f32x16{};
-        ci = f32x16{};
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++) {
-            loadB<32, false>(Lre, Lim, 16 * ks, 32 * nt, lane, br, bi, true);
-            cmac<f32x16, 32>(cr, ci, a.F + ((mt * 4 + ks) * 6) * 64 + lane, br, bi, true);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int r = 32 * mt + c32row(v, lane), c = 32 * nt + lane % 32;
-            Lre[r * MLD + c] = cr[v];
-            Lim[r * MLD + c] = ci[v];
-        }
-        __syncthreads();
-        // 2b: O'[ux][uy] = sum_kx F[ux][kx] Q[uy][kx]; X' = O' conj(lotf') ETF'
-        cr = f32x16{};
-        ci = f32x16{};
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++) {
-            loadB<32, false>(Lre, Lim, 16 * ks, 32 * nt, lane, br, bi, false);
-            cmac<f32x16, 32>(cr, ci, a.F + ((mt * 4 + ks) * 6) * 64 + lane, br, bi, false);
-        }
+        // embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf) as W[x][y], y contiguous
         {
-            const int tile = mt * 2 + nt;
+            float re[4], im[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float sn, cs;
+                sincospif(2.f * op[j] * WV.inv_wvl, &sn, &cs);
+                re[j] = am[j] * cs;
+                im[j] = am[j] * sn;
+            }
+            st4(L[0], L[1], B0 + ix, B0 + iy0, re[0], re[1], re[2], re[3]);
+            st4(L[2], L[3], B0 + ix, B0 + iy0, im[0], im[1], im[2], im[3]);
+        }
+        __syncthreads();
+        // 1a: Z[ky][x] = sum_{y in band} W[x][y] F[y][ky]  (waves with mb = 0)
+        f32x16 cr = {}, ci = {};
+        if (mb == 0) {
+#pragma unroll
+            for (int ks = 1; ks <= 2; ks++)
+                cmacd<f32x16, 32, false>(cr, ci, Lc, B0 + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
+        }
+        __syncthreads();
+        if (mb == 0) st_tile32(L, cr, ci, B0, 32 * nb, lane, false); // Z row ky, column x
+        __syncthreads();
+        // 1b: Y[kx][ky] = sum_{x in band} Z[ky][x] F[x][kx]; P = |Y|^2 stored [kx][ky]
+        cr = f32x16{};
+        ci = f32x16{};
+#pragma unroll
+        for (int ks = 1; ks <= 2; ks++)
+            cmacd<f32x16, 32, false>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < 16; v++) cr[v] = cr[v] * cr[v] + ci[v] * ci[v];
+        st_tile32(L, cr, ci, 32 * mb, 32 * nb, lane, true);
+        __syncthreads();
+        // 2a: Q[uy][kx] = sum_ky P[kx][ky] F[ky][uy]  (P real)
+        cr = f32x16{};
+        ci = f32x16{};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+            cmacd<f32x16, 32, true>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
+        __syncthreads();
+        st_tile32(L, cr, ci, 32 * mb, 32 * nb, lane, false); // row uy, column kx
+        __syncthreads();
+        // 2b: O[ux][uy] = sum_kx Q[uy][kx] F[kx][ux]; X = O conj(lotf) ETF
+        cr = f32x16{};
+        ci = f32x16{};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+            cmacd<f32x16, 32, false>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
+        {
+            // element v of this lane: uy = 32 mb + 8 (v/4) + 4 h + v%4, ux = 32 nb + l%32
+            const int tile = mb * 2 + nb;
             if (a.lotf_t) {
                 const float2 *lt = a.lotf_t + (((long)iwfs * a.nwvl + iwvl) * 4 + tile) * 16 * 64 + lane;
 #pragma unroll
@@ -880,42 +880,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
         __syncthreads();
-#pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int r = 32 * mt + c32row(v, lane), c = 32 * nt + lane % 32;
-            Lre[r * MLD + c] = cr[v];
-            Lim[r * MLD + c] = ci[v];
-        }
+        st_tile32(L, cr, ci, 32 * mb, 32 * nb, lane, false); // row ux, column uy
         __syncthreads();
-        // 3a: R'[xs][uy] = sum_ux Gx[xs][ux] X'[ux][uy]; wave w: uy in [16w, 16w+16)
-        f32x4 rr = {}, ri = {};
-#pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-            loadB<16, true>(Lre, Lim, 32 * ks, 16 * wave, lane, br, bi, false);
-            cmac<f32x4, 16>(rr, ri, WV.gx + (ks * 6) * 64 + lane, br, bi, false);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int r = 4 * (lane / 16) + v, c = 16 * wave + lane % 16;
-            Lre[r * MLD + c] = rr[v];
-            Lim[r * MLD + c] = ri[v];
-        }
-        __syncthreads();
-        // 3b: I[ys][xs] = Re sum_uy Gy[ys][uy] R[uy][xs] = Re sum Gy[ys][uy] R'[xs][uy]
-        float *img = Lre + 32 * MLD; // rows >= 16 are free now
-        if (wave == 0) {
-            f32x4 ir = {}, dummy = {};
+        // 3a: R[ys][ux] = sum_uy X[ux][uy] Gy[uy][ys]; wave w: ux in [16w, 16w+16)
+        {
+            f32x4 rr = {}, ri = {};
+            const int l16 = lane % 16, h16 = lane / 16;
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) {
-                loadB<16, false>(Lre, Lim, 32 * ks, 0, lane, br, bi, false);
-                const bf16x8 *A = WV.gy + (ks * 6) * 64 + lane;
-                mac3<f32x4, 16>(ir, A[0], A[64], br[0], br[1]);
-                mac3<f32x4, 16>(ir, A[4 * 64], A[5 * 64], bi[0], bi[1]);
+                bf16x8 gr[2], gi[2];
+                loadT(WV.gy + ks * 4 * 64, lane, gr, gi);
+                cmacd<f32x4, 16, false>(rr, ri, Lc, 16 * wave + l16, 32 * ks + 8 * h16, gr, gi);
             }
-            (void)dummy;
+            __syncthreads();
+            // element v: ux = 16 w + 4 (l/16) + v, ys = l%16 -> row ys, columns ux
+            st4(L[0], L[1], l16, 16 * wave + 4 * h16, rr[0], rr[1], rr[2], rr[3]);
+            st4(L[2], L[3], l16, 16 * wave + 4 * h16, ri[0], ri[1], ri[2], ri[3]);
+        }
+        __syncthreads();
+        // 3b: I[ys][xs] = Re sum_ux R[ys][ux] Gx[ux][xs]  (wave 0)
+        if (wave == 0) {
+            f32x4 ir = {};
+            const int l16 = lane % 16, h16 = lane / 16;
 #pragma unroll
-            for (int v = 0; v < 4; v++) img[(4 * (lane / 16) + v) * 16 + lane % 16] = ir[v];
+            for (int ks = 0; ks < 2; ks++) {
+                bf16x8 gr[2], gi[2];
+                loadT(WV.gx + ks * 4 * 64, lane, gr, gi);
+                const int k = 32 * ks + 8 * h16;
+                const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(Lc[0] + l16 * MLDB + k);
+                const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(Lc[1] + l16 * MLDB + k);
+                const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(Lc[2] + l16 * MLDB + k);
+                const bf16x8 el = *reinterpret_cast<const bf16x8 *>(Lc[3] + l16 * MLDB + k);
+                mac3<f32x4, 16>(ir, dh, dl, gr[0], gr[1]);   // Re R * Re G
+                mac3<f32x4, 16>(ir, -eh, -el, gi[0], gi[1]); // - Im R * Im G
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) img[(4 * (lane / 16) + v) * MX_MFMA_NS + lane % 16] = ir[v];
         }
         __syncthreads();
         if (tid < a.npix) {
@@ -1007,7 +1007,7 @@ struct mx_wfsgrad_s {
     int variant = MX_VARIANT_AUTO;     // requested form
     bool mfma_size = false;            // N = 64, nxsa = 32
     bool mfma_wvl_ok[MX_MAXWVL] = {};  // sampled rows/columns fit in 16
-    uint16_t *d_F = nullptr;           // bf16x8 table [2][4][6][64]
+    uint16_t *d_F = nullptr;           // bf16x8 table [2][4][4][64]
     uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};
     int4 *d_sic[MX_MAXWVL] = {};
     float2 *d_etf_t[MX_MAXWVL] = {};
@@ -1083,7 +1083,7 @@ void split_bf(double x, uint16_t &hi, uint16_t &lo) {
     hi = f2bf((float)x);
     lo = f2bf((float)(x - (double)bf2f(hi)));
 }
-// one table entry (6 parts x 64 lanes x 8) of complex values c[lane][j]
+// one table entry (4 parts x 64 lanes x 8) of complex values
 void put_entry(uint16_t *dst, const double (*re)[8], const double (*im)[8]) {
     for (int lane = 0; lane < 64; lane++)
         for (int j = 0; j < 8; j++) {
@@ -1094,16 +1094,13 @@ void put_entry(uint16_t *dst, const double (*re)[8], const double (*im)[8]) {
             split_bf(im[lane][j], h, l);
             dst[(2 * 64 + lane) * 8 + j] = h;
             dst[(3 * 64 + lane) * 8 + j] = l;
-            split_bf(-im[lane][j], h, l);
-            dst[(4 * 64 + lane) * 8 + j] = h;
-            dst[(5 * 64 + lane) * 8 + j] = l;
         }
 }
-// forward DFT matrix F[m][k] = exp(-2 pi i m k / 64) as A operands of
-// v_mfma_f32_32x32x16_bf16: entry (mt, ks), lane l, j -> m = 32 mt + l%32,
-// k = 16 ks + 8 (l/32) + j
+// forward DFT matrix F[k][n] = exp(-2 pi i k n / 64) (symmetric) as B
+// operands of v_mfma_f32_32x32x16_bf16: entry (nb, ks), lane l, j ->
+// n = 32 nb + l%32, k = 16 ks + 8 (l/32) + j
 std::vector<uint16_t> build_F64() {
-    std::vector<uint16_t> t(2 * 4 * 6 * 64 * 8);
+    std::vector<uint16_t> t(2 * 4 * 4 * 64 * 8);
     double re[64][8], im[64][8];
     for (int mt = 0; mt < 2; mt++)
         for (int ks = 0; ks < 4; ks++) {
@@ -1114,15 +1111,16 @@ std::vector<uint16_t> build_F64() {
                     re[lane][j] = std::cos(ang);
                     im[lane][j] = std::sin(ang);
                 }
-            put_entry(t.data() + (size_t)(mt * 4 + ks) * 6 * 64 * 8, re, im);
+            put_entry(t.data() + (size_t)(mt * 4 + ks) * 4 * 64 * 8, re, im);
         }
     return t;
 }
-// rows of the inverse DFT conj(F) at the sampled coordinates rows[0..n) as A
-// operands of v_mfma_f32_16x16x32_bf16: entry ks, lane l, j -> row l%16,
-// k = 32 ks + 8 (l/16) + j; rows beyond n are zero
+// the inverse DFT conj(F) at the sampled coordinates rows[0..n) as B
+// operands of v_mfma_f32_16x16x32_bf16: entry ks, lane l, j -> column l%16
+// (sampled coordinate rows[l%16]), k = 32 ks + 8 (l/16) + j; columns beyond
+// n are zero
 std::vector<uint16_t> build_G(const std::vector<int> &rows) {
-    std::vector<uint16_t> t(2 * 6 * 64 * 8);
+    std::vector<uint16_t> t(2 * 4 * 64 * 8);
     double re[64][8], im[64][8];
     for (int ks = 0; ks < 2; ks++) {
         for (int lane = 0; lane < 64; lane++)
@@ -1136,16 +1134,17 @@ std::vector<uint16_t> build_G(const std::vector<int> &rows) {
                     re[lane][j] = im[lane][j] = 0.;
                 }
             }
-        put_entry(t.data() + (size_t)ks * 6 * 64 * 8, re, im);
+        put_entry(t.data() + (size_t)ks * 4 * 64 * 8, re, im);
     }
     return t;
 }
-// element (ux, uy) of an N=64 table -> its slot in the C-tile layout
-// [tile = 2(ux/32) + uy/32][v][lane] of v_mfma_f32_32x32x16
+// element (ux, uy) of an N=64 table -> its slot [tile][v][lane] in the C tile
+// of stage 2b (rows uy, columns ux of v_mfma_f32_32x32x16): tile = 2 (uy/32)
+// + ux/32, uy%32 = 8 (v/4) + 4 (lane/32) + v%4, ux%32 = lane%32
 inline long tile_slot(int ux, int uy) {
-    const int r = ux % 32;
-    const int v = 4 * (r / 8) + (r % 4), l = 32 * ((r % 8) / 4) + uy % 32;
-    return ((long)((ux / 32) * 2 + uy / 32) * 16 + v) * 64 + l;
+    const int r = uy % 32;
+    const int v = 4 * (r / 8) + (r % 4), l = 32 * ((r % 8) / 4) + ux % 32;
+    return ((long)((uy / 32) * 2 + ux / 32) * 16 + v) * 64 + l;
 }
 // form used by the next step: the requested one, or for AUTO the MFMA form
 // whenever its size and sampling limits hold; -1 if the request cannot run
@@ -1230,7 +1229,7 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
         if (h->mfma_size) ALLOC(h->d_lotf_t, nwfs * h->nwvl * N * N);
     }
     if (h->mfma_size) {
-        ALLOC(h->d_F, 2 * 4 * 6 * 64 * 8);
+        ALLOC(h->d_F, 2 * 4 * 4 * 64 * 8);
         ALLOC(h->d_mwv, MX_MAXWVL);
     }
 #undef ALLOC
