@@ -7,9 +7,17 @@ gradients and FP64 i0 accumulation.  Screens and tables are resident in HBM
 before the timed region.  One frame is one WFS x one step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+                    [--variant auto|fft32|mfma] [--copy-calib BYTES]
 
 Prints one JSON line.  The CPU baseline is the FP64 oracle (a C restatement
 of MAOS's CPU path) timed on this host on a bounded sample of frames.
+
+The imaging kernel has two forms: the FP32 radix-FFT form (VALU) and the
+split-BF16 MFMA form (the default where it applies).  The line reports the
+selected form as `value` and the other form's frames/s beside it.
+`--copy-calib BYTES` adds a plain device copy of that size after the timed
+region, so that a hardware-counter run of this script can scale the
+kernel's FETCH_SIZE / WRITE_SIZE against a copy of known traffic.
 """
 import argparse
 import json
@@ -26,7 +34,10 @@ sys.path.insert(0, ROOT)
 METRIC = ("WFS frames/s, 30 m 6×60×60 LGS (MFMA split-BF16 vs FP32 vs FP64), "
           "% roofline")
 PEAK_FP32_TFLOPS = 157.3
+PEAK_BF16_TFLOPS = 2500.0
 HBM_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_shwfs_mfma_lgs30m_traffic.txt")
+KERNELS = {0: "k_shwfs_step<64,32,1>", 1: "k_shwfs_mfma"}
 
 
 def flops_per_subap(N, nxsa, nps, npix, llt=True, etf=True):
@@ -43,6 +54,45 @@ def flops_per_subap(N, nxsa, nps, npix, llt=True, etf=True):
     return f
 
 
+def mfma_flops_per_subap():
+    """BF16 MFMA FLOPs the MFMA imaging kernel executes per subaperture-
+    wavelength at N=64, nxsa=32, three products per split multiply:
+    forward 2-D DFT over the 32-wide band (K=32): 48 + 96 32x32x16 MFMAs;
+    half-plane inverse transform of the real |E|^2 (Hermitian): 48 + 96;
+    detector-row and -column sampling: 48 + 12 16x16x32 MFMAs."""
+    n32 = 48 + 96 + 48 + 96
+    n16 = 48 + 12
+    return n32 * 32 * 32 * 16 * 2 + n16 * 16 * 16 * 32 * 2
+
+
+def read_traffic():
+    """HBM bytes per launch of the MFMA imaging kernel recorded by the
+    counter runs (FETCH_SIZE + WRITE_SIZE scaled by a plain copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def time_variant(eng, variant, isim, dt, nsteps):
+    """frames/s of one imaging form, and the next isim"""
+    eng.set_variant(variant)
+    for _ in range(5):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    t1 = time.perf_counter()
+    for _ in range(nsteps):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    return nsteps / (time.perf_counter() - t1), isim
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -57,13 +107,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", type=int, default=60)
     ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--variant", choices=("auto", "fft32", "mfma"), default="auto")
+    ap.add_argument("--copy-calib", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the WFS path of this build is single-GPU "
               "(see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
 
-    from maos_amd import configs, engine
+    from maos_amd import _lib, configs, engine
     from maos_amd.powfs import Powfs, genatm
 
     t0 = time.time()
@@ -79,6 +131,10 @@ def main():
     engine.init(0)
     engine.atm2gpu(layers)
     eng = engine.WfsGrad(p, wfs)
+    if args.variant != "auto":
+        eng.set_variant(_lib.VARIANT_FFT32 if args.variant == "fft32" else _lib.VARIANT_MFMA_BF16)
+    variant = eng.variant()
+    log("imaging form: %s" % KERNELS[variant])
 
     # matched filter from the FP64 long-exposure i0 (mtch_cell on the host)
     eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
@@ -120,25 +176,49 @@ def main():
     t_ints = ms_ints / nl * 1e-3
     t_llt = ms_llt / nl * 1e-3
     t_opd = ms_opd / nl * 1e-3
-    flop_launch = flops_per_subap(p.notf, p.nxsa, len(layers), p.npix) * p.nsa * nwfs
+    # imaging kernel only: the ray trace runs in its own kernel
+    flop_launch = flops_per_subap(p.notf, p.nxsa, 0, p.npix) * p.nsa * nwfs
     achieved = flop_launch / t_ints / 1e12
+    traffic = read_traffic() if variant == _lib.VARIANT_MFMA_BF16 else None
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
-            "kernel": "k_shwfs_step<64,32,1>", "avg_launch_us": round(t_ints * 1e6, 2),
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+            "kernel": KERNELS[variant], "avg_launch_us": round(t_ints * 1e6, 2),
             "llt_kernel_us": round(t_llt * 1e6, 2), "raytrace_kernel_us": round(t_opd * 1e6, 2),
             "flop_per_launch": flop_launch,
-            "peak_kind": "fp32 (MI355X published); FP32 radix-FFT form runs on VALU"}
+            "peak_kind": "fp32 dense (MI355X published): the path's results are FP32-accurate"}
+    if variant == _lib.VARIANT_MFMA_BF16:
+        ex = mfma_flops_per_subap() * p.nsa * nwfs
+        roof["mfma_executed_tflops"] = round(ex / t_ints / 1e12, 2)
+        roof["mfma_executed_frac_of_bf16_peak"] = round(ex / t_ints / 1e12 / PEAK_BF16_TFLOPS, 4)
+        if traffic is not None:
+            roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
+    if args.copy_calib > 0:
+        import torch
+        n = args.copy_calib // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 5 x %d bytes" % (4 * n))
+    other = None
+    if eng.variant_ok(1 - variant):
+        fps_other, isim = time_variant(eng, 1 - variant, isim, dt, 100)
+        other = {"kernel": KERNELS[1 - variant], "frames_per_s": round(fps_other * nwfs, 2)}
+        eng.set_variant(variant)
 
     out = {"metric": METRIC, "value": round(fps, 2), "unit": "WFS frames/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3" if variant == _lib.VARIANT_MFMA_BF16 else "fp32",
            "data": "synthetic (seeded von Karman screens, Gaussian sodium profile)",
            "config": {"workload": "lgs30m: 30 m pupil, 6 LGS WFS x 60x60 (nsa=%d), 32x32 "
                       "OPD pts -> 64x64 FFT, 6x6 px of 1\", 7 layers 4096^2 @1/64 m, LLT "
                       "uplink, ETF, matched filter, 800 Hz" % p.nsa,
                       "nwfs": nwfs, "nsa": p.nsa, "notf": p.notf, "npix": p.npix,
-                      "variant": "fp32-radix-fft"},
+                      "variant": {0: "fp32-radix-fft", 1: "mfma-split-bf16"}[variant]},
            "roofline": roof}
+    if other is not None:
+        out["other_form"] = other
 
     # CPU baseline: FP64 oracle on the host cores, bounded sample of frames
     if not args.no_cpu:

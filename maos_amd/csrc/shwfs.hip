@@ -655,6 +655,8 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
 // pixel processing and FP64 i0 as in k_shwfs_step.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -663,11 +665,15 @@ constexpr int MPLANE = 64 * MLDB;
 constexpr int MX_MFMA_NS = 16;  // sampled image rows/columns kept by stage 3
 
 // Table entries: [.. entry ..][part][lane] bf16x8, parts re hi, re lo, im hi,
-// im lo; a lane holds 8 consecutive k of one n (B-operand layout).
+// im lo, -im hi, -im lo; a lane holds 8 consecutive k of one n (B-operand
+// layout).  The negated imaginary part lets every complex product accumulate
+// with additions only.
+constexpr int MX_TPARTS = 6;
 struct MfmaWvl {
     const float2 *etf_t;     // [nsa][4 tiles][16][64] ETF (nominal fused), or null
     const float *nominal_t;  // [4][16][64] nominal when there is no ETF, or null
-    const bf16x8 *gx, *gy;   // [2 ks][4][64] conj(F) at the sampled columns / rows
+    const bf16x8 *gx;        // [2 ks][6][64] conj(F) at the sampled columns
+    const bf16x8 *gy;        // [6][64] w(uy) conj(F) at the sampled rows, uy < 32
     const int4 *si_c;        // [npix] compact indices (iy*16 + ix) into the 16x16 image
     const float4 *si_w;      // [npix]
     float inv_wvl;
@@ -677,7 +683,7 @@ struct MfmaArgs {
     int nsa, npix, pixpsax, pixpsay, nwvl, nwfs;
     const float *amp;
     const float *opd;
-    const bf16x8 *F;          // [2 blocks][4 ks][4][64]
+    const bf16x8 *F;          // [2 blocks][4 ks][6][64]
     const MfmaWvl *wv;
     const float2 *lotf_t;     // [nwfs][nwvl][4][16][64] conj(lotfc), or null
     const DevWfs *wfs;
@@ -706,43 +712,50 @@ __device__ __forceinline__ void mac3(ACC &c, bf16x8 ah, bf16x8 al, bf16x8 bh, bf
     }
 }
 
-// (cr, ci) += D * B with D the field (A operand, from LDS) and B = (br, bi)
+// (cr, ci) += D * B with D the field (A operand, from LDS) and B = b[0] + i
+// b[1]; b[2] = -b[1]
 template <typename ACC, int SZ, bool DREAL>
 __device__ __forceinline__ void cmacd(ACC &cr, ACC &ci, const __bf16 *const (&L)[4], int row, int k,
-                                      const bf16x8 (&br)[2], const bf16x8 (&bi)[2]) {
+                                      const bf16x8 (&b)[3][2]) {
     const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(L[0] + row * MLDB + k);
     const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(L[1] + row * MLDB + k);
-    mac3<ACC, SZ>(cr, dh, dl, br[0], br[1]);
-    mac3<ACC, SZ>(ci, dh, dl, bi[0], bi[1]);
+    mac3<ACC, SZ>(cr, dh, dl, b[0][0], b[0][1]);
+    mac3<ACC, SZ>(ci, dh, dl, b[1][0], b[1][1]);
     if (!DREAL) {
         const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(L[2] + row * MLDB + k);
         const bf16x8 el = *reinterpret_cast<const bf16x8 *>(L[3] + row * MLDB + k);
-        mac3<ACC, SZ>(cr, -eh, -el, bi[0], bi[1]);
-        mac3<ACC, SZ>(ci, eh, el, br[0], br[1]);
+        mac3<ACC, SZ>(cr, eh, el, b[2][0], b[2][1]);
+        mac3<ACC, SZ>(ci, eh, el, b[0][0], b[0][1]);
     }
 }
 
-__device__ __forceinline__ void loadT(const bf16x8 *T, int lane, bf16x8 (&br)[2], bf16x8 (&bi)[2]) {
-    br[0] = T[0 * 64 + lane];
-    br[1] = T[1 * 64 + lane];
-    bi[0] = T[2 * 64 + lane];
-    bi[1] = T[3 * 64 + lane];
+__device__ __forceinline__ void loadT(const bf16x8 *T, int lane, bf16x8 (&b)[3][2]) {
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        b[p][0] = T[(2 * p) * 64 + lane];
+        b[p][1] = T[(2 * p + 1) * 64 + lane];
+    }
+}
+
+// hi = bf16(a) (round to nearest even), lo = bf16(a - hi) for a pair, packed
+// two per dword: one packed convert, two unpacks, two subtractions, one
+// packed convert
+__device__ __forceinline__ void split2(float a0, float a1, unsigned &h, unsigned &l) {
+    const bf16x2 hb = __builtin_convertvector((float2v){a0, a1}, bf16x2);
+    h = __builtin_bit_cast(unsigned, hb);
+    const float h0 = __uint_as_float(h << 16), h1 = __uint_as_float(h & 0xffff0000u);
+    const bf16x2 lb = __builtin_convertvector((float2v){a0 - h0, a1 - h1}, bf16x2);
+    l = __builtin_bit_cast(unsigned, lb);
 }
 
 // split 4 values into hi/lo and store them at plane row, columns k..k+3
 __device__ __forceinline__ void st4(__bf16 *ph, __bf16 *pl, int row, int k, float a0, float a1, float a2,
                                     float a3) {
-    bf16x4 h, l;
-    h[0] = (__bf16)a0;
-    h[1] = (__bf16)a1;
-    h[2] = (__bf16)a2;
-    h[3] = (__bf16)a3;
-    l[0] = (__bf16)(a0 - (float)h[0]);
-    l[1] = (__bf16)(a1 - (float)h[1]);
-    l[2] = (__bf16)(a2 - (float)h[2]);
-    l[3] = (__bf16)(a3 - (float)h[3]);
-    *reinterpret_cast<bf16x4 *>(ph + row * MLDB + k) = h;
-    *reinterpret_cast<bf16x4 *>(pl + row * MLDB + k) = l;
+    uint2 h, l;
+    split2(a0, a1, h.x, l.x);
+    split2(a2, a3, h.y, l.y);
+    *reinterpret_cast<uint2 *>(ph + row * MLDB + k) = h;
+    *reinterpret_cast<uint2 *>(pl + row * MLDB + k) = l;
 }
 
 // write a 32x32 C tile transposed: element (m0 + 8g + 4h + i, n0 + l%32) goes
@@ -769,12 +782,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
     const DevWfs &W = a.wfs[iwfs];
     constexpr int NX = 32, B0 = 16; // nonzero block [16, 48)
-    const int mb = wave % 2, nb = wave / 2; // m block (rows of the field), n block (frequencies)
+    // The stages give the four waves unequal work; the role of a wave (lw)
+    // rotates with the block so that the SIMDs of a CU share it evenly.
+    const int lw = (wave + (int)blockIdx.x) & 3;
+    const int mb = lw % 2, nb = lw / 2; // m block (rows of the field), n block (frequencies)
     const int h32 = lane / 32, l32 = lane % 32;
     // this wave's 32 DFT columns n = 32 nb + l%32, k = 16 ks + 8 (l/32) + j
-    bf16x8 fr[4][2], fi[4][2];
+    bf16x8 f[4][3][2];
 #pragma unroll
-    for (int ks = 0; ks < 4; ks++) loadT(a.F + (nb * 4 + ks) * 4 * 64, lane, fr[ks], fi[ks]);
+    for (int ks = 0; ks < 4; ks++) loadT(a.F + (nb * 4 + ks) * MX_TPARTS * 64, lane, f[ks]);
     // OPD and amplitude: thread (ix = tid%32, iy = 4 (tid/32) + j)
     const int ix = tid % 32, iy0 = 4 * (tid / 32);
     float op[4], am[4];
@@ -812,45 +828,55 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             st4(L[2], L[3], B0 + ix, B0 + iy0, im[0], im[1], im[2], im[3]);
         }
         __syncthreads();
-        // 1a: Z[ky][x] = sum_{y in band} W[x][y] F[y][ky]  (waves with mb = 0)
+        // 1a: Z[ky][x] = sum_{y in band} W[x][y] F[y][ky]  (roles 1 and 3, one ky block each)
         f32x16 cr = {}, ci = {};
-        if (mb == 0) {
+        const bool r1a = (lw & 1) != 0;
+        if (r1a) {
 #pragma unroll
             for (int ks = 1; ks <= 2; ks++)
-                cmacd<f32x16, 32, false>(cr, ci, Lc, B0 + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
+                cmacd<f32x16, 32, false>(cr, ci, Lc, B0 + l32, 16 * ks + 8 * h32, f[ks]);
         }
         __syncthreads();
-        if (mb == 0) st_tile32(L, cr, ci, B0, 32 * nb, lane, false); // Z row ky, column x
+        if (r1a) st_tile32(L, cr, ci, B0, 32 * nb, lane, false); // Z row ky, column x
         __syncthreads();
         // 1b: Y[kx][ky] = sum_{x in band} Z[ky][x] F[x][kx]; P = |Y|^2 stored [kx][ky]
         cr = f32x16{};
         ci = f32x16{};
 #pragma unroll
         for (int ks = 1; ks <= 2; ks++)
-            cmacd<f32x16, 32, false>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
+            cmacd<f32x16, 32, false>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, f[ks]);
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < 16; v++) cr[v] = cr[v] * cr[v] + ci[v] * ci[v];
         st_tile32(L, cr, ci, 32 * mb, 32 * nb, lane, true);
         __syncthreads();
-        // 2a: Q[uy][kx] = sum_ky P[kx][ky] F[ky][uy]  (P real)
+        // P is real, so its transform O is Hermitian, and so is X = O conj(lotf)
+        // ETF (every factor is the transform of a real function).  The field
+        // fills 32 of the 64 samples per axis, so O (its autocorrelation) is
+        // zero at lag 32: the half plane uy in [0, 32) carries all of X.
+        // 2a: Q[uy][kx] = sum_ky P[kx][ky] F[ky][uy], uy < 32 (roles 0 and 1, one kx block each)
+        const bool r2a = lw < 2;
         cr = f32x16{};
         ci = f32x16{};
+        if (r2a) {
 #pragma unroll
-        for (int ks = 0; ks < 4; ks++)
-            cmacd<f32x16, 32, true>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
+            for (int ks = 0; ks < 4; ks++)
+                cmacd<f32x16, 32, true>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, f[ks]);
+        }
         __syncthreads();
-        st_tile32(L, cr, ci, 32 * mb, 32 * nb, lane, false); // row uy, column kx
+        if (r2a) st_tile32(L, cr, ci, 32 * mb, 0, lane, false); // row uy, column kx
         __syncthreads();
-        // 2b: O[ux][uy] = sum_kx Q[uy][kx] F[kx][ux]; X = O conj(lotf) ETF
+        // 2b: O[ux][uy] = sum_kx Q[uy][kx] F[kx][ux], uy < 32; X = O conj(lotf) ETF
+        // (roles 0 and 2, one ux block each)
+        const bool r2b = (lw & 1) == 0;
         cr = f32x16{};
         ci = f32x16{};
+        if (r2b) {
 #pragma unroll
-        for (int ks = 0; ks < 4; ks++)
-            cmacd<f32x16, 32, false>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, fr[ks], fi[ks]);
-        {
-            // element v of this lane: uy = 32 mb + 8 (v/4) + 4 h + v%4, ux = 32 nb + l%32
-            const int tile = mb * 2 + nb;
+            for (int ks = 0; ks < 4; ks++)
+                cmacd<f32x16, 32, false>(cr, ci, Lc, l32, 16 * ks + 8 * h32, f[ks]);
+            // element v of this lane: uy = 8 (v/4) + 4 h + v%4, ux = 32 nb + l%32
+            const int tile = nb;
             if (a.lotf_t) {
                 const float2 *lt = a.lotf_t + (((long)iwfs * a.nwvl + iwvl) * 4 + tile) * 16 * 64 + lane;
 #pragma unroll
@@ -880,39 +906,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
             }
         }
         __syncthreads();
-        st_tile32(L, cr, ci, 32 * mb, 32 * nb, lane, false); // row ux, column uy
+        if (r2b) st_tile32(L, cr, ci, 0, 32 * nb, lane, false); // row ux, column uy
         __syncthreads();
-        // 3a: R[ys][ux] = sum_uy X[ux][uy] Gy[uy][ys]; wave w: ux in [16w, 16w+16)
+        // 3a: R[ys][ux] = sum_{uy < 32} X[ux][uy] w(uy) Gy[uy][ys], w = 1 at uy = 0
+        // and 2 elsewhere (folded into the table); role r: ux in [16r, 16r+16)
         {
             f32x4 rr = {}, ri = {};
             const int l16 = lane % 16, h16 = lane / 16;
-#pragma unroll
-            for (int ks = 0; ks < 2; ks++) {
-                bf16x8 gr[2], gi[2];
-                loadT(WV.gy + ks * 4 * 64, lane, gr, gi);
-                cmacd<f32x4, 16, false>(rr, ri, Lc, 16 * wave + l16, 32 * ks + 8 * h16, gr, gi);
-            }
+            bf16x8 g[3][2];
+            loadT(WV.gy, lane, g);
+            cmacd<f32x4, 16, false>(rr, ri, Lc, 16 * lw + l16, 8 * h16, g);
             __syncthreads();
-            // element v: ux = 16 w + 4 (l/16) + v, ys = l%16 -> row ys, columns ux
-            st4(L[0], L[1], l16, 16 * wave + 4 * h16, rr[0], rr[1], rr[2], rr[3]);
-            st4(L[2], L[3], l16, 16 * wave + 4 * h16, ri[0], ri[1], ri[2], ri[3]);
+            // element v: ux = 16 lw + 4 (l/16) + v, ys = l%16 -> row ys, columns ux
+            st4(L[0], L[1], l16, 16 * lw + 4 * h16, rr[0], rr[1], rr[2], rr[3]);
+            st4(L[2], L[3], l16, 16 * lw + 4 * h16, ri[0], ri[1], ri[2], ri[3]);
         }
         __syncthreads();
-        // 3b: I[ys][xs] = Re sum_ux R[ys][ux] Gx[ux][xs]  (wave 0)
-        if (wave == 0) {
+        // 3b: I[ys][xs] = Re sum_ux R[ys][ux] Gx[ux][xs]  (role 3)
+        if (lw == 3) {
             f32x4 ir = {};
             const int l16 = lane % 16, h16 = lane / 16;
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) {
-                bf16x8 gr[2], gi[2];
-                loadT(WV.gx + ks * 4 * 64, lane, gr, gi);
+                bf16x8 g[3][2];
+                loadT(WV.gx + ks * MX_TPARTS * 64, lane, g);
                 const int k = 32 * ks + 8 * h16;
                 const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(Lc[0] + l16 * MLDB + k);
                 const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(Lc[1] + l16 * MLDB + k);
                 const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(Lc[2] + l16 * MLDB + k);
                 const bf16x8 el = *reinterpret_cast<const bf16x8 *>(Lc[3] + l16 * MLDB + k);
-                mac3<f32x4, 16>(ir, dh, dl, gr[0], gr[1]);   // Re R * Re G
-                mac3<f32x4, 16>(ir, -eh, -el, gi[0], gi[1]); // - Im R * Im G
+                mac3<f32x4, 16>(ir, dh, dl, g[0][0], g[0][1]); // Re R * Re G
+                mac3<f32x4, 16>(ir, eh, el, g[2][0], g[2][1]); // Im R * (-Im G)
             }
 #pragma unroll
             for (int v = 0; v < 4; v++) img[(4 * (lane / 16) + v) * MX_MFMA_NS + lane % 16] = ir[v];
@@ -1007,7 +1031,7 @@ struct mx_wfsgrad_s {
     int variant = MX_VARIANT_AUTO;     // requested form
     bool mfma_size = false;            // N = 64, nxsa = 32
     bool mfma_wvl_ok[MX_MAXWVL] = {};  // sampled rows/columns fit in 16
-    uint16_t *d_F = nullptr;           // bf16x8 table [2][4][4][64]
+    uint16_t *d_F = nullptr;           // bf16x8 table [2][4][6][64]
     uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};
     int4 *d_sic[MX_MAXWVL] = {};
     float2 *d_etf_t[MX_MAXWVL] = {};
@@ -1083,24 +1107,24 @@ void split_bf(double x, uint16_t &hi, uint16_t &lo) {
     hi = f2bf((float)x);
     lo = f2bf((float)(x - (double)bf2f(hi)));
 }
-// one table entry (4 parts x 64 lanes x 8) of complex values
+// one table entry (6 parts x 64 lanes x 8) of complex values: re, im, -im
 void put_entry(uint16_t *dst, const double (*re)[8], const double (*im)[8]) {
     for (int lane = 0; lane < 64; lane++)
         for (int j = 0; j < 8; j++) {
-            uint16_t h, l;
-            split_bf(re[lane][j], h, l);
-            dst[(0 * 64 + lane) * 8 + j] = h;
-            dst[(1 * 64 + lane) * 8 + j] = l;
-            split_bf(im[lane][j], h, l);
-            dst[(2 * 64 + lane) * 8 + j] = h;
-            dst[(3 * 64 + lane) * 8 + j] = l;
+            const double v[3] = {re[lane][j], im[lane][j], -im[lane][j]};
+            for (int p = 0; p < 3; p++) {
+                uint16_t h, l;
+                split_bf(v[p], h, l);
+                dst[((2 * p) * 64 + lane) * 8 + j] = h;
+                dst[((2 * p + 1) * 64 + lane) * 8 + j] = l;
+            }
         }
 }
 // forward DFT matrix F[k][n] = exp(-2 pi i k n / 64) (symmetric) as B
 // operands of v_mfma_f32_32x32x16_bf16: entry (nb, ks), lane l, j ->
 // n = 32 nb + l%32, k = 16 ks + 8 (l/32) + j
 std::vector<uint16_t> build_F64() {
-    std::vector<uint16_t> t(2 * 4 * 4 * 64 * 8);
+    std::vector<uint16_t> t(2 * 4 * MX_TPARTS * 64 * 8);
     double re[64][8], im[64][8];
     for (int mt = 0; mt < 2; mt++)
         for (int ks = 0; ks < 4; ks++) {
@@ -1111,7 +1135,7 @@ std::vector<uint16_t> build_F64() {
                     re[lane][j] = std::cos(ang);
                     im[lane][j] = std::sin(ang);
                 }
-            put_entry(t.data() + (size_t)(mt * 4 + ks) * 4 * 64 * 8, re, im);
+            put_entry(t.data() + (size_t)(mt * 4 + ks) * MX_TPARTS * 64 * 8, re, im);
         }
     return t;
 }
@@ -1120,7 +1144,7 @@ std::vector<uint16_t> build_F64() {
 // (sampled coordinate rows[l%16]), k = 32 ks + 8 (l/16) + j; columns beyond
 // n are zero
 std::vector<uint16_t> build_G(const std::vector<int> &rows) {
-    std::vector<uint16_t> t(2 * 4 * 64 * 8);
+    std::vector<uint16_t> t(2 * MX_TPARTS * 64 * 8);
     double re[64][8], im[64][8];
     for (int ks = 0; ks < 2; ks++) {
         for (int lane = 0; lane < 64; lane++)
@@ -1134,8 +1158,30 @@ std::vector<uint16_t> build_G(const std::vector<int> &rows) {
                     re[lane][j] = im[lane][j] = 0.;
                 }
             }
-        put_entry(t.data() + (size_t)ks * 4 * 64 * 8, re, im);
+        put_entry(t.data() + (size_t)ks * MX_TPARTS * 64 * 8, re, im);
     }
+    return t;
+}
+// the inverse DFT over the half plane uy in [0, 32) with the Hermitian weight
+// w(uy) = 1 at uy = 0 and 2 elsewhere, at the sampled rows rows[0..n), as the B
+// operand of one v_mfma_f32_16x16x32_bf16 (entry lane l, j -> column l%16,
+// k = 8 (l/16) + j)
+std::vector<uint16_t> build_Gh(const std::vector<int> &rows) {
+    std::vector<uint16_t> t(MX_TPARTS * 64 * 8);
+    double re[64][8], im[64][8];
+    for (int lane = 0; lane < 64; lane++)
+        for (int j = 0; j < 8; j++) {
+            const int r = lane % 16, k = 8 * (lane / 16) + j;
+            const double w = k == 0 ? 1. : 2.;
+            if (r < (int)rows.size()) {
+                const double ang = 2. * M_PI * (double)((rows[r] * k) % 64) / 64.;
+                re[lane][j] = w * std::cos(ang);
+                im[lane][j] = w * std::sin(ang);
+            } else {
+                re[lane][j] = im[lane][j] = 0.;
+            }
+        }
+    put_entry(t.data(), re, im);
     return t;
 }
 // element (ux, uy) of an N=64 table -> its slot [tile][v][lane] in the C tile
@@ -1229,7 +1275,7 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
         if (h->mfma_size) ALLOC(h->d_lotf_t, nwfs * h->nwvl * N * N);
     }
     if (h->mfma_size) {
-        ALLOC(h->d_F, 2 * 4 * 4 * 64 * 8);
+        ALLOC(h->d_F, 2 * 4 * MX_TPARTS * 64 * 8);
         ALLOC(h->d_mwv, MX_MAXWVL);
     }
 #undef ALLOC
@@ -1376,7 +1422,7 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
                     op[c] = (&wt[i].x)[c] != 0.f ? cy * MX_MFMA_NS + cx : 0;
                 }
             }
-            std::vector<uint16_t> gx = build_G(xs), gy = build_G(ys);
+            std::vector<uint16_t> gx = build_G(xs), gy = build_Gh(ys);
             if ((rc = dev_alloc(&h->d_gx[iwvl], gx.size())) || (rc = dev_alloc(&h->d_gy[iwvl], gy.size())) ||
                 (rc = dev_alloc(&h->d_sic[iwvl], h->npix)))
                 return rc;

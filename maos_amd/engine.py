@@ -128,6 +128,14 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_get_variant(self.h, C.byref(v)))
         return v.value
 
+    def variant_ok(self, variant):
+        """True if this configuration runs `variant`; the current variant is kept."""
+        cur = self.variant()
+        if _lib.hip().mx_wfsgrad_set_variant(self.h, variant) != 0:
+            return False
+        self.set_variant(cur)
+        return True
+
     def timing_enable(self, on=True):
         check(_lib.hip().mx_wfsgrad_timing_enable(self.h, int(on)))
 
