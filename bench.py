@@ -56,12 +56,13 @@ def flops_per_subap(N, nxsa, nps, npix, llt=True, etf=True):
 
 def mfma_flops_per_subap():
     """BF16 MFMA FLOPs the MFMA imaging kernel executes per subaperture-
-    wavelength at N=64, nxsa=32, three products per split multiply:
-    forward 2-D DFT over the 32-wide band (K=32): 48 + 96 32x32x16 MFMAs;
-    half-plane inverse transform of the real |E|^2 (Hermitian): 48 + 96;
-    detector-row and -column sampling: 48 + 12 16x16x32 MFMAs."""
-    n32 = 48 + 96 + 48 + 96
-    n16 = 48 + 12
+    wavelength at N=64, nxsa=32, three products per split multiply (four
+    waves): forward transform over the 32-wide band, 96 16x16x32 (first
+    axis) + 96 32x32x16 (second axis, |E|^2); half-plane inverse transform
+    with radix-2 halved sums, 48 + 96 16x16x32; detector rows and columns,
+    48 + 12 16x16x32."""
+    n32 = 96
+    n16 = 96 + 48 + 96 + 48 + 12
     return n32 * 32 * 32 * 16 * 2 + n16 * 16 * 16 * 32 * 2
 
 
@@ -91,6 +92,26 @@ def time_variant(eng, variant, isim, dt, nsteps):
         isim += 1
     eng.wait()
     return nsteps / (time.perf_counter() - t1), isim
+
+
+def copy_calibration(nbytes, reps=5):
+    """A plain device-to-device copy of known size through the HIP runtime
+    (its blit kernel), for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    ok = hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    ok = ok and hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        ok = ok and hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # device to device
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    for p in (src, dst):
+        if p.value:
+            hip.hipFree(p)
+    if not ok:
+        raise RuntimeError("copy calibration failed")
+    log("copy calibration: %d x %d bytes" % (reps, nbytes))
 
 
 def log(msg):
@@ -193,14 +214,7 @@ def main():
         if traffic is not None:
             roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     if args.copy_calib > 0:
-        import torch
-        n = args.copy_calib // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy calibration: 5 x %d bytes" % (4 * n))
+        copy_calibration(args.copy_calib)
     other = None
     if eng.variant_ok(1 - variant):
         fps_other, isim = time_variant(eng, 1 - variant, isim, dt, 100)

@@ -13,8 +13,10 @@
  * (CSC) arrays.  All host buffers are owned by the caller and only read or
  * written during the call.  Every entry point returns 0 on success or a
  * negative MX_E* code; mx_last_error() describes the last failure.
- * Compute runs in FP32 on the GPU except the screen-coordinate arithmetic
- * and the long-exposure i0 accumulation, which are FP64.
+ * Compute runs in FP32 on the GPU (the MFMA imaging form: BF16 hi/lo
+ * operand pairs, three products, FP32 accumulation) except the
+ * screen-coordinate arithmetic and the long-exposure i0 accumulation, which
+ * are FP64.
  */
 #ifndef MAOS_MI355X_H
 #define MAOS_MI355X_H
@@ -156,7 +158,7 @@ int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf);
  * stream (each timed step waits for its kernels).  enable=1 starts
  * recording and clears the sums; mx_wfsgrad_timing reports the number of
  * timed steps and the summed milliseconds of the uplink-OTF kernel, the
- * ray-trace kernel and the fused imaging kernel. */
+ * ray-trace kernel and the imaging kernel. */
 int mx_wfsgrad_timing_enable(mx_wfsgrad_t *h, int enable);
 int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
                       double *ms_opd, double *ms_ints);

@@ -177,21 +177,33 @@ struct OpdArgs {
     long isim;
     double dt;
     float *opd; // [nwfs][nsa][nxsa^2]
+    // LLT uplink grids (nllt = nwfs, or 0 without LLT): origin and launch
+    // offset; the grid is nxsa x nxsa like a subaperture's
+    int nllt;
+    double llt_origx, llt_origy, llt_ox, llt_oy;
+    float *llt_opd; // [nwfs][nxsa^2]
 };
 
+// Transfer-function tables of the MFMA imaging kernel (N = 64) cover the half
+// plane uy < 32 in the C-tile order of its stage 2b: element (ux, uy) sits in
+// 16x16 tile t = 2 (2 e + hf) + uy/16 (e = ux%2, hf = (ux/2)/16), element
+// v = uy%4, lane 16 ((uy%16)/4) + (ux/2)%16.
+constexpr int MX_HALF = 2048;
+__host__ __device__ inline int half_slot(int ux, int uy) {
+    const int e = ux & 1, c = ux >> 1, hf = c >> 4, col = c & 15;
+    const int mt = uy >> 4, r = uy & 15;
+    const int t = (2 * e + hf) * 2 + mt;
+    return (t * 4 + (r & 3)) * 64 + 16 * (r >> 2) + col;
+}
+
 struct LltArgs {
-    int nwvl, nwfs, nps;
-    double ptsdx, origx, origy, ox, oy;
+    int nwvl, nwfs;
     const float *amp; // nlx*nlx
+    const float *opd; // [nwfs][nlx*nlx] uplink OPD from k_wfs_opd
     const DevWvl *wv;
     float norm;       // 1/nlwvf^2
-    const DevLayer *layers;
-    double atmscale;
-    const DevWfs *wfs;
-    long isim;
-    double dt;
     float2 *lotf;     // [nwfs][nwvl][N*N] conj, transposed
-    float2 *lotf_t;   // [nwfs][nwvl][4][16][64] conj, MFMA C-tile layout (N = 64), or null
+    float2 *lotf_t;   // [nwfs][nwvl][MX_HALF] conj, half plane in half_slot order (N = 64), or null
 };
 
 // XOR-swizzled element (x, y) of an N x N complex tile
@@ -211,74 +223,74 @@ __device__ __forceinline__ void seg_sum(float &v) {
 // periodic wrap, integer part and fraction are FP64; the four weights and
 // the screen values are FP32.  Coordinates of the NXSA columns and NXSA
 // rows are computed once per layer by 2*NXSA lanes and shared through LDS.
+// Screen column (isx) or row index and fraction of OPD grid point i on layer
+// L, FP64 (prop_grid_pts, lib/accphi_grid_do.c:408-446, with the frozen-flow
+// displacement of maos/wfsgrad.c:175-194): disp = ht theta + scale misreg +
+// shift0 - v dt isim, scale = 1 - ht/hs (cone), periodic wrap.
+__device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i, double hs, double theta,
+                                             double misreg, double shift0, long isim, double dt,
+                                             double ptsdx, double orig, int &ii, float &ff) {
+    const double scale = hs != 0. ? 1. - L.ht / hs : 1.;
+    const double disp = L.ht * theta + scale * misreg + shift0 - (isx ? L.vx : L.vy) * dt * (double)isim;
+    const double d1 = isx ? L.dx1 : L.dy1;
+    const double nn = (double)(isx ? L.nx : L.ny);
+    const double o0 = isx ? L.ox : L.oy;
+    const double dout = ptsdx * d1 * scale;
+    const double oout = orig * d1 * scale + (disp - o0) * d1;
+    double X = fma((double)i, dout, oout);
+    X -= floor(X / nn) * nn;
+    const double fl = floor(X);
+    ii = (int)fl;
+    double fr = X - fl;
+    const int n = isx ? L.nx : L.ny;
+    if (ii >= n) ii -= n;
+    if (ii < 0) {
+        ii += n;
+        fr = 0.;
+    }
+    ff = (float)fr;
+}
+
+// OPD of one NXSA x NXSA grid (origin origx, origy, step ptsdx) summed over
+// the layers, by N lanes.  Per layer, lanes [0, 2 NXSA) find the column and
+// row coordinates (cidx, cfrac: 2 NXSA entries), every lane finds the tile
+// origin itself, the (NXSA+2)^2 screen tile that the grid touches (its step
+// on the screen is <= 1 pixel) is staged in LDS with coalesced loads,
+// wrapping periodically, and the bilinear interpolation runs from LDS.
 template <int N, int NXSA>
 __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, double hs,
                                           double thetax, double thetay, double misregx,
                                           double misregy, double shift0x, double shift0y,
                                           long isim, double dt, double atmscale, double ptsdx,
                                           double origx, double origy, int line, float *opd,
-                                          int *cidx, float *cfrac) {
+                                          int *cidx, float *cfrac, float *tile) {
     static_assert(2 * NXSA <= N, "coordinate lanes");
-    if (line < 2 * NXSA) {
-        const bool isx = line < NXSA;
-        const int i = isx ? line : line - NXSA;
-        for (int l = 0; l < nps; l++) {
-            const DevLayer L = layers[l];
-            int ii = 0;
-            float ff = 0.f;
-            bool skip = hs != 0. && L.ht > hs;
-            if (!skip) {
-                const double scale = hs != 0. ? 1. - L.ht / hs : 1.;
-                double disp, o, d1, nn;
-                if (isx) {
-                    disp = L.ht * thetax + scale * misregx + shift0x - L.vx * dt * (double)isim;
-                    d1 = L.dx1;
-                    o = origx;
-                    nn = (double)L.nx;
-                } else {
-                    disp = L.ht * thetay + scale * misregy + shift0y - L.vy * dt * (double)isim;
-                    d1 = L.dy1;
-                    o = origy;
-                    nn = (double)L.ny;
-                }
-                const double o0 = isx ? L.ox : L.oy;
-                const double dout = ptsdx * d1 * scale;
-                const double oout = o * d1 * scale + (disp - o0) * d1;
-                double X = fma((double)i, dout, oout);
-                X -= floor(X / nn) * nn;
-                double fl = floor(X);
-                ii = (int)fl;
-                double fr = X - fl;
-                const int n = isx ? L.nx : L.ny;
-                if (ii >= n) { ii -= n; }
-                if (ii < 0) { ii += n; fr = 0.; }
-                ff = (float)fr;
-            }
-            cidx[l * 2 * NXSA + line] = skip ? -1 : ii;
-            cfrac[l * 2 * NXSA + line] = ff;
-        }
-    }
-    __syncthreads();
-    // Per layer, the (NXSA+2)^2 screen tile that the subaperture's points
-    // touch (the grid step on the screen is <= 1 pixel) is staged in LDS with
-    // coalesced loads, wrapping periodically, then interpolated from LDS.
     constexpr int PPT = NXSA * NXSA / N;
     constexpr int TW = NXSA + 2;
     constexpr int NFILL = (TW * TW + N - 1) / N;
     static_assert(PPT * N == NXSA * NXSA, "points per lane");
-    float *tile = cfrac + MX_MAXLAYER * 2 * NXSA;
     const bool act = line < N;
     float acc[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) acc[j] = 0.f;
     for (int l = 0; l < nps; l++) {
-        const int *ci = cidx + l * 2 * NXSA;
-        const float *cf = cfrac + l * 2 * NXSA;
-        const int tx0 = ci[0], ty0 = ci[NXSA];
-        if (tx0 < 0) continue; // layer above the source (uniform)
-        const float *p = layers[l].p;
-        const int nx = layers[l].nx, ny = layers[l].ny;
-        __syncthreads(); // previous tile fully read
+        const DevLayer L = layers[l];
+        if (hs != 0. && L.ht > hs) continue; // layer above the source (uniform)
+        int tx0, ty0;
+        float f0;
+        screen_coord(L, true, 0, hs, thetax, misregx, shift0x, isim, dt, ptsdx, origx, tx0, f0);
+        screen_coord(L, false, 0, hs, thetay, misregy, shift0y, isim, dt, ptsdx, origy, ty0, f0);
+        __syncthreads(); // the previous layer's coordinates and tile fully read
+        if (line < 2 * NXSA) {
+            const bool isx = line < NXSA;
+            int ii;
+            float ff;
+            screen_coord(L, isx, isx ? line : line - NXSA, hs, isx ? thetax : thetay, isx ? misregx : misregy,
+                         isx ? shift0x : shift0y, isim, dt, ptsdx, isx ? origx : origy, ii, ff);
+            cidx[line] = ii;
+            cfrac[line] = ff;
+        }
+        const int nx = L.nx, ny = L.ny;
         if (act) {
 #pragma unroll
             for (int k = 0; k < NFILL; k++) {
@@ -288,7 +300,7 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
                     int gx = tx0 + c, gy = ty0 + r;
                     gx -= gx >= nx ? nx : 0;
                     gy -= gy >= ny ? ny : 0;
-                    tile[e] = p[(long)gy * nx + gx];
+                    tile[e] = L.p[(long)gy * nx + gx];
                 }
             }
         }
@@ -298,10 +310,10 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
             for (int j = 0; j < PPT; j++) {
                 const int pt = line + j * N;
                 const int ix = pt % NXSA, iy = pt / NXSA;
-                int lc = ci[ix] - tx0, lr = ci[NXSA + iy] - ty0;
+                int lc = cidx[ix] - tx0, lr = cidx[NXSA + iy] - ty0;
                 lc += lc < 0 ? nx : 0;
                 lr += lr < 0 ? ny : 0;
-                const float fx = cf[ix], fy = cf[NXSA + iy];
+                const float fx = cfrac[ix], fy = cfrac[NXSA + iy];
                 const float *t0 = tile + lr * TW + lc;
                 acc[j] += (1.f - fy) * ((1.f - fx) * t0[0] + fx * t0[1]) +
                           fy * ((1.f - fx) * t0[TW] + fx * t0[TW + 1]);
@@ -528,22 +540,37 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
 // wfsgrad_iwfs, maos/wfsgrad.c:175-194).  Small register and LDS footprint,
 // so many waves per SIMD hide the screen-fetch latency; the FP32 OPD goes to
 // HBM for the imaging kernel.
+#ifndef MX_OPD_WAVES
+#define MX_OPD_WAVES 4
+#endif
 template <int NXSA>
-__global__ __launch_bounds__(64) void k_wfs_opd(OpdArgs a) {
-    __shared__ int cidx[MX_MAXLAYER * 2 * NXSA];
-    __shared__ float cfrac[MX_MAXLAYER * 2 * NXSA + (NXSA + 2) * (NXSA + 2)];
-    const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MX_OPD_WAVES))) void k_wfs_opd(OpdArgs a) {
+    __shared__ int cidx[2 * NXSA];
+    __shared__ float cfrac[2 * NXSA];
+    __shared__ float tile[(NXSA + 2) * (NXSA + 2)];
+    const int b = (int)blockIdx.x;
+    if (b < a.nllt) {
+        // LLT uplink grid of WFS b (maos/wfsgrad.c:329-386): theta - o/hs,
+        // shift o - v dt isim.  These blocks come first so that they start
+        // early.
+        const DevWfs &W = a.wfs[b];
+        ray_trace<64, NXSA>(a.layers, a.nps, W.hs, W.thetax - a.llt_ox / W.hs, W.thetay - a.llt_oy / W.hs, 0.,
+                            0., a.llt_ox, a.llt_oy, a.isim, a.dt, a.atmscale, a.ptsdx, a.llt_origx, a.llt_origy,
+                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile);
+        return;
+    }
+    const int unit = xcd_unit(b - a.nllt, a.nsa * a.nwfs);
     const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
     const DevWfs &W = a.wfs[iwfs];
     ray_trace<64, NXSA>(a.layers, a.nps, W.hs, W.thetax, W.thetay, W.misregx, W.misregy, 0., 0.,
                         a.isim, a.dt, a.atmscale, a.ptsdx, a.ptsx[isa], a.ptsy[isa], threadIdx.x,
-                        a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac);
+                        a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac, tile);
 }
 
 // ------------------------------------------------------------ uplink OTF
-// lotfc of wfsints (maos/wfsints.c:149-199) for nlwvf == nwvf == notf: LLT
-// OPD by ray tracing (maos/wfsgrad.c:329-386), embed, FFT, |.|^2/nlwvf^2,
-// FFT.  Stored conjugated and transposed for the imaging kernel.
+// lotfc of wfsints (maos/wfsints.c:149-199) for nlwvf == nwvf == notf: from
+// the LLT OPD (traced by k_wfs_opd), embed, FFT, |.|^2/nlwvf^2, FFT.
+// Stored conjugated and transposed for the imaging kernels.
 template <int N, int NLX>
 __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
     constexpr int SKIP = (N - NLX) / 2;
@@ -553,17 +580,13 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
     const int line = lane % N;
     const bool active = lane < N;
     const int iwfs = blockIdx.x / a.nwvl, iwvl = blockIdx.x % a.nwvl;
-    const DevWfs &W = a.wfs[iwfs];
     float *opd = reinterpret_cast<float *>(T);
     float *amp = opd + NLX * NLX;
-    int *cidx = reinterpret_cast<int *>(amp + NLX * NLX);
-    float *cfrac = reinterpret_cast<float *>(cidx + MX_MAXLAYER * 2 * NLX);
     if (active)
-        for (int pt = line; pt < NLX * NLX; pt += N) amp[pt] = a.amp[pt];
-    // theta of the uplink: theta - o/hs; shift o - v*dt*isim
-    ray_trace<N, NLX>(a.layers, a.nps, W.hs, W.thetax - a.ox / W.hs, W.thetay - a.oy / W.hs, 0.,
-                      0., a.ox, a.oy, a.isim, a.dt, a.atmscale, a.ptsdx, a.origx, a.origy,
-                      active ? line : (1 << 20), opd, cidx, cfrac);
+        for (int pt = line; pt < NLX * NLX; pt += N) {
+            amp[pt] = a.amp[pt];
+            opd[pt] = a.opd[(long)iwfs * NLX * NLX + pt];
+        }
     __syncthreads();
     float psum = 0.f;
     if (active)
@@ -613,18 +636,12 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
         float2 *out = a.lotf + ((long)iwfs * a.nwvl + iwvl) * N * N + uy;
 #pragma unroll
         for (int k = 0; k < N; k++) out[bitrev(k, LG) * N] = make_float2(v[k].x, -v[k].y);
-        if (N == 64 && a.lotf_t) {
-            // element (ux, uy) -> the C tile of the MFMA imaging kernel's
-            // stage 2b: tile 2 (uy/32) + ux/32, uy%32 = 8 (v/4) + 4 (l/32) + v%4,
-            // ux%32 = l%32
-            float2 *ot = a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * 4 * 16 * 64;
-            const int r = uy % 32, vv = 4 * (r / 8) + (r % 4), hh = (r % 8) / 4;
+        if (N == 64 && a.lotf_t && uy < 32) {
+            // the half plane uy < 32 in the C-tile order of the MFMA imaging
+            // kernel's stage 2b
+            float2 *ot = a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * MX_HALF;
 #pragma unroll
-            for (int k = 0; k < N; k++) {
-                const int ux = bitrev(k, LG);
-                ot[((uy / 32) * 2 + ux / 32) * 16 * 64 + vv * 64 + 32 * hh + ux % 32] =
-                    make_float2(v[k].x, -v[k].y);
-            }
+            for (int k = 0; k < N; k++) ot[half_slot(bitrev(k, LG), uy)] = make_float2(v[k].x, -v[k].y);
         }
     }
 }
@@ -632,30 +649,38 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
 
 // ------------------------------------------------------------ MFMA form
 // The same imaging chain with every 1-D DFT stage computed as a complex
-// matrix product on the matrix cores (v_mfma_f32_32x32x16_bf16, and
-// v_mfma_f32_16x16x32_bf16 for the sampled inverse).  Operands are split
-// into BF16 hi + lo pairs and each real product is hi*hi + hi*lo + lo*hi,
-// accumulated in FP32.  One workgroup of four waves images one subaperture
-// (N = 64, nxsa = 32).
+// matrix product on the matrix cores (v_mfma_f32_32x32x16_bf16 and
+// v_mfma_f32_16x16x32_bf16).  Operands are split into BF16 hi + lo pairs and
+// each real product is hi*hi + hi*lo + lo*hi, accumulated in FP32.  One
+// workgroup of four waves images one subaperture (N = 64, nxsa = 32).
 //
-// In every stage the field is the A operand and the (symmetric) DFT matrix
-// the B operand, so a stage reads its input as rows that are contiguous
-// along the summed index and writes its output transposed, contiguous along
-// the index the next stage sums over.  The field therefore lives in LDS as
-// four BF16 planes (re hi, re lo, im hi, im lo), split once by the stage
-// that produces it; operands are plain 16-byte LDS reads.  Each wave keeps
-// its 32 columns of the DFT matrix in registers.
-//   1a Z[ky][x]   = sum_y W[x][y] F[y][ky]         (only the 32x32 block)
-//   1b Y[kx][ky]  = sum_x Z[ky][x] F[x][kx]  -> P = |Y|^2
-//   2a Q[uy][kx]  = sum_ky P[kx][ky] F[ky][uy]
-//   2b O[ux][uy]  = sum_kx Q[uy][kx] F[kx][ux] -> X = O conj(lotf) ETF
-//   3a R[ys][ux]  = sum_uy X[ux][uy] G[uy][ys]      (sampled image rows)
-//   3b I[ys][xs]  = Re sum_ux R[ys][ux] G[ux][xs]   (sampled image columns)
-// (G = conj F restricted to <= 16 sampled coordinates), then the si sampler,
-// pixel processing and FP64 i0 as in k_shwfs_step.
+// The field is the A operand (LDS rows contiguous along the summed index) and
+// the symmetric DFT matrix F[k][n] = exp(-2 pi i k n / 64) the B operand.  A
+// stage writes its output transposed, split into four BF16 planes (re hi, re
+// lo, im hi, im lo), contiguous along the index the next stage sums over.
+//   1a Z[ky][x]  = sum_y W[x][y] F[y][ky]       (32x32 band, K = 32)
+//   1b Y[kx][ky] = sum_x Z[ky][x] F[x][kx]      -> P = |Y|^2
+//   2a Q[uy][kx] = sum_ky P[kx][ky] F[ky][uy]   (uy < 32)
+//   2b O[ux][uy] = sum_kx Q[uy][kx] F[kx][ux]   -> X = O conj(lotf) ETF
+//   3a R[ys][ux] = sum_uy X[ux][uy] w(uy) G[uy][ys]   (sampled image rows)
+//   3b I[ys][xs] = Re sum_ux R[ys][ux] G[ux][xs]      (sampled image columns)
+// (G = conj F at <= 16 sampled coordinates), then the si sampler, pixel
+// processing and FP64 i0 as in k_shwfs_step.
+//
+// Two properties halve the inverse transform:
+// - P is real, so O and X = O conj(lotf) ETF are Hermitian (every factor is
+//   the transform of a real function); the field fills 32 of 64 samples per
+//   axis, so O (its autocorrelation) is zero at lag 32.  The half plane
+//   uy < 32 carries all of X, with weight w = 1 at uy = 0 and 2 elsewhere.
+// - Radix-2 split of the 64-point sums in 2a and 2b: F[k + 32][n] = (-1)^n
+//   F[k][n], so even outputs sum (x[k] + x[k+32]) and odd outputs (x[k] -
+//   x[k+32]) over k < 32.  The producing stage forms both in FP32 before the
+//   split (its C tile holds k and k + 32 in the same lane) and stores them side
+//   by side, columns [0, 32) and [32, 64).
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -669,9 +694,14 @@ constexpr int MX_MFMA_NS = 16;  // sampled image rows/columns kept by stage 3
 // layout).  The negated imaginary part lets every complex product accumulate
 // with additions only.
 constexpr int MX_TPARTS = 6;
+// DFT table entries (each MX_TPARTS x 64 bf16x8)
+constexpr int MX_F32 = 0;  // + 2 nb + ks: 32x32x16, n = 32 nb + l%32, k = 16 + 16 ks + 8 (l/32) + j
+constexpr int MX_F1A = 4;  // + nt: 16x16x32, n = 16 nt + l%16, k = 16 + 8 (l/16) + j
+constexpr int MX_F2 = 8;   // + 2 e + hf: 16x16x32, n = 2 (16 hf + l%16) + e, k = 8 (l/16) + j
+constexpr int MX_FENT = 12;
 struct MfmaWvl {
-    const float2 *etf_t;     // [nsa][4 tiles][16][64] ETF (nominal fused), or null
-    const float *nominal_t;  // [4][16][64] nominal when there is no ETF, or null
+    const float2 *etf_t;     // [nsa][MX_HALF] ETF (nominal fused), half_slot order, or null
+    const float *nominal_t;  // [MX_HALF] nominal when there is no ETF, or null
     const bf16x8 *gx;        // [2 ks][6][64] conj(F) at the sampled columns
     const bf16x8 *gy;        // [6][64] w(uy) conj(F) at the sampled rows, uy < 32
     const int4 *si_c;        // [npix] compact indices (iy*16 + ix) into the 16x16 image
@@ -683,9 +713,9 @@ struct MfmaArgs {
     int nsa, npix, pixpsax, pixpsay, nwvl, nwfs;
     const float *amp;
     const float *opd;
-    const bf16x8 *F;          // [2 blocks][4 ks][6][64]
+    const bf16x8 *F;          // [MX_FENT][6][64]
     const MfmaWvl *wv;
-    const float2 *lotf_t;     // [nwfs][nwvl][4][16][64] conj(lotfc), or null
+    const float2 *lotf_t;     // [nwfs][nwvl][MX_HALF] conj(lotfc), or null
     const DevWfs *wfs;
     int phytype, sigmatch;
     float cogthres, cogoff, pixthetax, pixthetay, sigtot;
@@ -758,20 +788,20 @@ __device__ __forceinline__ void st4(__bf16 *ph, __bf16 *pl, int row, int k, floa
     *reinterpret_cast<uint2 *>(pl + row * MLDB + k) = l;
 }
 
-// write a 32x32 C tile transposed: element (m0 + 8g + 4h + i, n0 + l%32) goes
-// to plane row n0 + l%32, column m0 + 8g + 4h + i
-__device__ __forceinline__ void st_tile32(__bf16 *const (&L)[4], const f32x16 &cr, const f32x16 &ci,
-                                          int m0, int n0, int lane, bool real) {
-    const int row = n0 + lane % 32, h = lane / 32;
-#pragma unroll
-    for (int g = 0; g < 4; g++) {
-        const int k = m0 + 8 * g + 4 * h;
-        st4(L[0], L[1], row, k, cr[4 * g], cr[4 * g + 1], cr[4 * g + 2], cr[4 * g + 3]);
-        if (!real) st4(L[2], L[3], row, k, ci[4 * g], ci[4 * g + 1], ci[4 * g + 2], ci[4 * g + 3]);
-    }
+// amp * exp(+i 2 pi x) for x in revolutions: the phase is reduced to
+// [-1/2, 1/2] exactly, then the hardware sine and cosine (argument in
+// revolutions) are used
+__device__ __forceinline__ void cis_rev(float x, float am, float &re, float &im) {
+    const float r = x - rintf(x);
+    re = am * __builtin_amdgcn_cosf(r);
+    im = am * __builtin_amdgcn_sinf(r);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_shwfs_mfma(MfmaArgs a) {
+// waves per SIMD the register budget is sized for
+#ifndef MX_MFMA_WAVES
+#define MX_MFMA_WAVES 4
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAVES))) void k_shwfs_mfma(MfmaArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 Lb[4 * MPLANE];
     __shared__ float img[MX_MFMA_NS * MX_MFMA_NS];
     __shared__ float red[4];
@@ -782,131 +812,160 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
     const DevWfs &W = a.wfs[iwfs];
     constexpr int NX = 32, B0 = 16; // nonzero block [16, 48)
-    // The stages give the four waves unequal work; the role of a wave (lw)
-    // rotates with the block so that the SIMDs of a CU share it evenly.
+    // role of the wave; it rotates with the block so that the SIMDs of a CU
+    // share the unequal parts (stage 3b) evenly
     const int lw = (wave + (int)blockIdx.x) & 3;
-    const int mb = lw % 2, nb = lw / 2; // m block (rows of the field), n block (frequencies)
-    const int h32 = lane / 32, l32 = lane % 32;
-    // this wave's 32 DFT columns n = 32 nb + l%32, k = 16 ks + 8 (l/32) + j
-    bf16x8 f[4][3][2];
-#pragma unroll
-    for (int ks = 0; ks < 4; ks++) loadT(a.F + (nb * 4 + ks) * MX_TPARTS * 64, lane, f[ks]);
+    const int mb = lw % 2, nb = lw / 2;
+    const int h32 = lane / 32, l32 = lane % 32, h16 = lane / 16, l16 = lane % 16;
     // OPD and amplitude: thread (ix = tid%32, iy = 4 (tid/32) + j)
     const int ix = tid % 32, iy0 = 4 * (tid / 32);
-    float op[4], am[4];
+    const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
+    const float *ag = a.amp + (long)isa * NX * NX;
+    float mean;
     {
-        const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
-        const float *ag = a.amp + (long)isa * NX * NX;
         float s = 0.f;
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            op[j] = og[(iy0 + j) * NX + ix];
-            am[j] = ag[(iy0 + j) * NX + ix];
-            s += op[j];
-        }
+        for (int j = 0; j < 4; j++) s += og[(iy0 + j) * NX + ix];
         seg_sum<64>(s);
         if (lane == 0) red[wave] = s;
         __syncthreads();
-        const float mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
-#pragma unroll
-        for (int j = 0; j < 4; j++) op[j] -= mean;
+        mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
     }
     float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
     for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
         const MfmaWvl &WV = a.wv[iwvl];
-        // embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf) as W[x][y], y contiguous
+        // embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf) as W[x][y], y contiguous;
+        // OPD and amplitude are read again per wavelength (cache hits) rather
+        // than held in registers
         {
             float re[4], im[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                float sn, cs;
-                sincospif(2.f * op[j] * WV.inv_wvl, &sn, &cs);
-                re[j] = am[j] * cs;
-                im[j] = am[j] * sn;
-            }
+            for (int j = 0; j < 4; j++)
+                cis_rev((og[(iy0 + j) * NX + ix] - mean) * WV.inv_wvl, ag[(iy0 + j) * NX + ix], re[j], im[j]);
             st4(L[0], L[1], B0 + ix, B0 + iy0, re[0], re[1], re[2], re[3]);
             st4(L[2], L[3], B0 + ix, B0 + iy0, im[0], im[1], im[2], im[3]);
         }
         __syncthreads();
-        // 1a: Z[ky][x] = sum_{y in band} W[x][y] F[y][ky]  (roles 1 and 3, one ky block each)
-        f32x16 cr = {}, ci = {};
-        const bool r1a = (lw & 1) != 0;
-        if (r1a) {
+        // 1a: C[x][ky] = sum_{y in band} W[x][y] F[y][ky]; wave: ky in [16 lw, 16 lw + 16),
+        // both 16-row blocks of x; stored as Z row ky, columns x
+        {
+            bf16x8 b[3][2];
+            loadT(a.F + (MX_F1A + lw) * MX_TPARTS * 64, lane, b);
+            f32x4 zr[2] = {}, zi[2] = {};
 #pragma unroll
-            for (int ks = 1; ks <= 2; ks++)
-                cmacd<f32x16, 32, false>(cr, ci, Lc, B0 + l32, 16 * ks + 8 * h32, f[ks]);
-        }
-        __syncthreads();
-        if (r1a) st_tile32(L, cr, ci, B0, 32 * nb, lane, false); // Z row ky, column x
-        __syncthreads();
-        // 1b: Y[kx][ky] = sum_{x in band} Z[ky][x] F[x][kx]; P = |Y|^2 stored [kx][ky]
-        cr = f32x16{};
-        ci = f32x16{};
+            for (int mt = 0; mt < 2; mt++)
+                cmacd<f32x4, 16, false>(zr[mt], zi[mt], Lc, B0 + 16 * mt + l16, B0 + 8 * h16, b);
+            __syncthreads();
 #pragma unroll
-        for (int ks = 1; ks <= 2; ks++)
-            cmacd<f32x16, 32, false>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, f[ks]);
-        __syncthreads();
-#pragma unroll
-        for (int v = 0; v < 16; v++) cr[v] = cr[v] * cr[v] + ci[v] * ci[v];
-        st_tile32(L, cr, ci, 32 * mb, 32 * nb, lane, true);
-        __syncthreads();
-        // P is real, so its transform O is Hermitian, and so is X = O conj(lotf)
-        // ETF (every factor is the transform of a real function).  The field
-        // fills 32 of the 64 samples per axis, so O (its autocorrelation) is
-        // zero at lag 32: the half plane uy in [0, 32) carries all of X.
-        // 2a: Q[uy][kx] = sum_ky P[kx][ky] F[ky][uy], uy < 32 (roles 0 and 1, one kx block each)
-        const bool r2a = lw < 2;
-        cr = f32x16{};
-        ci = f32x16{};
-        if (r2a) {
-#pragma unroll
-            for (int ks = 0; ks < 4; ks++)
-                cmacd<f32x16, 32, true>(cr, ci, Lc, 32 * mb + l32, 16 * ks + 8 * h32, f[ks]);
-        }
-        __syncthreads();
-        if (r2a) st_tile32(L, cr, ci, 32 * mb, 0, lane, false); // row uy, column kx
-        __syncthreads();
-        // 2b: O[ux][uy] = sum_kx Q[uy][kx] F[kx][ux], uy < 32; X = O conj(lotf) ETF
-        // (roles 0 and 2, one ux block each)
-        const bool r2b = (lw & 1) == 0;
-        cr = f32x16{};
-        ci = f32x16{};
-        if (r2b) {
-#pragma unroll
-            for (int ks = 0; ks < 4; ks++)
-                cmacd<f32x16, 32, false>(cr, ci, Lc, l32, 16 * ks + 8 * h32, f[ks]);
-            // element v of this lane: uy = 8 (v/4) + 4 h + v%4, ux = 32 nb + l%32
-            const int tile = nb;
-            if (a.lotf_t) {
-                const float2 *lt = a.lotf_t + (((long)iwfs * a.nwvl + iwvl) * 4 + tile) * 16 * 64 + lane;
-#pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    const float2 e = lt[v * 64];
-                    const float re = cr[v] * e.x - ci[v] * e.y, im = cr[v] * e.y + ci[v] * e.x;
-                    cr[v] = re;
-                    ci[v] = im;
-                }
-            }
-            if (WV.etf_t) {
-                const float2 *et = WV.etf_t + ((long)isa * 4 + tile) * 16 * 64 + lane;
-#pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    const float2 e = et[v * 64];
-                    const float re = cr[v] * e.x - ci[v] * e.y, im = cr[v] * e.y + ci[v] * e.x;
-                    cr[v] = re;
-                    ci[v] = im;
-                }
-            } else if (WV.nominal_t) {
-                const float *nm = WV.nominal_t + tile * 16 * 64 + lane;
-#pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    cr[v] *= nm[v * 64];
-                    ci[v] *= nm[v * 64];
-                }
+            for (int mt = 0; mt < 2; mt++) {
+                const int row = 16 * lw + l16, col = B0 + 16 * mt + 4 * h16;
+                st4(L[0], L[1], row, col, zr[mt][0], zr[mt][1], zr[mt][2], zr[mt][3]);
+                st4(L[2], L[3], row, col, zi[mt][0], zi[mt][1], zi[mt][2], zi[mt][3]);
             }
         }
         __syncthreads();
-        if (r2b) st_tile32(L, cr, ci, 0, 32 * nb, lane, false); // row ux, column uy
+        // 1b: C[ky][kx] = sum_{x in band} Z[ky][x] F[x][kx]; P = |C|^2.  Tile
+        // row t of the wave reads Z row ky = 16 mb + t%16 + 32 (t/16), so a
+        // lane's elements v and v + 8 are ky and ky + 32: P+ = P[ky] + P[ky+32]
+        // and P- = P[ky] - P[ky+32] (ky < 32) go to row kx, columns ky and 32 + ky
+        {
+            // DFT columns kx = 32 nb + l%32 over the band k in [16, 48)
+            f32x16 cr = {}, ci = {};
+            const int row = 16 * mb + (l32 % 16) + 32 * (l32 / 16);
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                bf16x8 f[3][2];
+                loadT(a.F + (MX_F32 + 2 * nb + ks) * MX_TPARTS * 64, lane, f);
+                cmacd<f32x16, 32, false>(cr, ci, Lc, row, B0 + 16 * ks + 8 * h32, f);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < 16; v++) cr[v] = cr[v] * cr[v] + ci[v] * ci[v];
+            const int kx = 32 * nb + l32;
+#pragma unroll
+            for (int g = 0; g < 2; g++) {
+                const int col = 16 * mb + 8 * g + 4 * h32, v = 4 * g;
+                st4(L[0], L[1], kx, col, cr[v] + cr[v + 8], cr[v + 1] + cr[v + 9], cr[v + 2] + cr[v + 10],
+                    cr[v + 3] + cr[v + 11]);
+                st4(L[0], L[1], kx, 32 + col, cr[v] - cr[v + 8], cr[v + 1] - cr[v + 9], cr[v + 2] - cr[v + 10],
+                    cr[v + 3] - cr[v + 11]);
+            }
+        }
+        __syncthreads();
+        // 2a: Q[uy][kx] = sum_{k<32} P+-[kx][k] F[k][uy] for uy < 32 of parity e
+        // (P+ for even uy, P- for odd); wave: e = lw/2, kx in [16 mb, 16 mb + 16)
+        // and the same + 32.  Q+ = Q[kx] + Q[kx+32], Q- = Q[kx] - Q[kx+32]
+        // (kx < 32) go to row uy, columns kx and 32 + kx
+        {
+            const int e = nb;
+            bf16x8 b[3][2];
+            loadT(a.F + (MX_F2 + 2 * e) * MX_TPARTS * 64, lane, b);
+            f32x4 qr[2] = {}, qi[2] = {};
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+                cmacd<f32x4, 16, true>(qr[mt], qi[mt], Lc, 16 * mb + 32 * mt + l16, 32 * e + 8 * h16, b);
+            __syncthreads();
+            const int uy = 2 * l16 + e, col = 16 * mb + 4 * h16;
+            st4(L[0], L[1], uy, col, qr[0][0] + qr[1][0], qr[0][1] + qr[1][1], qr[0][2] + qr[1][2],
+                qr[0][3] + qr[1][3]);
+            st4(L[2], L[3], uy, col, qi[0][0] + qi[1][0], qi[0][1] + qi[1][1], qi[0][2] + qi[1][2],
+                qi[0][3] + qi[1][3]);
+            st4(L[0], L[1], uy, 32 + col, qr[0][0] - qr[1][0], qr[0][1] - qr[1][1], qr[0][2] - qr[1][2],
+                qr[0][3] - qr[1][3]);
+            st4(L[2], L[3], uy, 32 + col, qi[0][0] - qi[1][0], qi[0][1] - qi[1][1], qi[0][2] - qi[1][2],
+                qi[0][3] - qi[1][3]);
+        }
+        __syncthreads();
+        // 2b: O[uy][ux] = sum_{k<32} Q+-[uy][k] F[k][ux], uy < 32, for ux of parity
+        // e (Q+ for even ux, Q- for odd); wave: e = lw/2, ux = 2 (16 hf + l%16) + e
+        // with hf = lw%2, both 16-row blocks of uy.  X = O conj(lotf) ETF goes
+        // to row ux, columns uy
+        {
+            const int e = nb, hf = mb;
+            bf16x8 b[3][2];
+            loadT(a.F + (MX_F2 + 2 * e + hf) * MX_TPARTS * 64, lane, b);
+            f32x4 xr[2] = {}, xi[2] = {};
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+                cmacd<f32x4, 16, false>(xr[mt], xi[mt], Lc, 16 * mt + l16, 32 * e + 8 * h16, b);
+            // element (mt, v) of this lane is table slot ((2 lw + mt) 4 + v) 64 + lane
+            const long tbase = (long)(2 * lw) * 4 * 64 + lane;
+            const float2 *lt = a.lotf_t ? a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * MX_HALF + tbase : nullptr;
+            const float2 *et = WV.etf_t ? WV.etf_t + (long)isa * MX_HALF + tbase : nullptr;
+            const float *nm = WV.nominal_t ? WV.nominal_t + tbase : nullptr;
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int o = (mt * 4 + v) * 64;
+                    float re = xr[mt][v], im = xi[mt][v];
+                    if (lt) {
+                        const float2 t = lt[o];
+                        const float r2 = re * t.x - im * t.y;
+                        im = re * t.y + im * t.x;
+                        re = r2;
+                    }
+                    if (et) {
+                        const float2 t = et[o];
+                        const float r2 = re * t.x - im * t.y;
+                        im = re * t.y + im * t.x;
+                        re = r2;
+                    } else if (nm) {
+                        re *= nm[o];
+                        im *= nm[o];
+                    }
+                    xr[mt][v] = re;
+                    xi[mt][v] = im;
+                }
+            __syncthreads();
+            const int ux = 2 * (16 * hf + l16) + e;
+#pragma unroll
+            for (int mt = 0; mt < 2; mt++) {
+                const int col = 16 * mt + 4 * h16;
+                st4(L[0], L[1], ux, col, xr[mt][0], xr[mt][1], xr[mt][2], xr[mt][3]);
+                st4(L[2], L[3], ux, col, xi[mt][0], xi[mt][1], xi[mt][2], xi[mt][3]);
+            }
+        }
         __syncthreads();
         // 3a: R[ys][ux] = sum_{uy < 32} X[ux][uy] w(uy) Gy[uy][ys], w = 1 at uy = 0
         // and 2 elsewhere (folded into the table); role r: ux in [16r, 16r+16)
@@ -1024,6 +1083,7 @@ struct mx_wfsgrad_s {
     float4 *d_si_w[MX_MAXWVL] = {};
     bool dtf_set[MX_MAXWVL] = {};
     float *d_llt_amp = nullptr;
+    float *d_llt_opd = nullptr;        // [nwfs][llt_nx^2] uplink OPD
     float2 *d_lotf = nullptr;
     DevWvl *d_wv = nullptr;
     DevWvl h_wv[MX_MAXWVL] = {};
@@ -1031,7 +1091,7 @@ struct mx_wfsgrad_s {
     int variant = MX_VARIANT_AUTO;     // requested form
     bool mfma_size = false;            // N = 64, nxsa = 32
     bool mfma_wvl_ok[MX_MAXWVL] = {};  // sampled rows/columns fit in 16
-    uint16_t *d_F = nullptr;           // bf16x8 table [2][4][6][64]
+    uint16_t *d_F = nullptr;           // bf16x8 table [MX_FENT][6][64]
     uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};
     int4 *d_sic[MX_MAXWVL] = {};
     float2 *d_etf_t[MX_MAXWVL] = {};
@@ -1120,23 +1180,33 @@ void put_entry(uint16_t *dst, const double (*re)[8], const double (*im)[8]) {
             }
         }
 }
-// forward DFT matrix F[k][n] = exp(-2 pi i k n / 64) (symmetric) as B
-// operands of v_mfma_f32_32x32x16_bf16: entry (nb, ks), lane l, j ->
-// n = 32 nb + l%32, k = 16 ks + 8 (l/32) + j
-std::vector<uint16_t> build_F64() {
-    std::vector<uint16_t> t(2 * 4 * MX_TPARTS * 64 * 8);
+// one B-operand entry of the forward DFT matrix F[k][n] = exp(-2 pi i k n /
+// 64): lane l, j -> column ncol(l % w), k = kbase + 8 (l / w) + j, with w = 32
+// for v_mfma_f32_32x32x16 and 16 for v_mfma_f32_16x16x32
+template <typename NCOL>
+void put_F(uint16_t *dst, int w, int kbase, NCOL ncol) {
     double re[64][8], im[64][8];
-    for (int mt = 0; mt < 2; mt++)
-        for (int ks = 0; ks < 4; ks++) {
-            for (int lane = 0; lane < 64; lane++)
-                for (int j = 0; j < 8; j++) {
-                    const int m = 32 * mt + lane % 32, k = 16 * ks + 8 * (lane / 32) + j;
-                    const double ang = -2. * M_PI * (double)((m * k) % 64) / 64.;
-                    re[lane][j] = std::cos(ang);
-                    im[lane][j] = std::sin(ang);
-                }
-            put_entry(t.data() + (size_t)(mt * 4 + ks) * MX_TPARTS * 64 * 8, re, im);
+    for (int lane = 0; lane < 64; lane++)
+        for (int j = 0; j < 8; j++) {
+            const int n = ncol(lane % w), k = kbase + 8 * (lane / w) + j;
+            const double ang = -2. * M_PI * (double)((k * n) % 64) / 64.;
+            re[lane][j] = std::cos(ang);
+            im[lane][j] = std::sin(ang);
         }
+    put_entry(dst, re, im);
+}
+// the DFT tables of k_shwfs_mfma (entries MX_F32, MX_F1A, MX_F2)
+std::vector<uint16_t> build_F64() {
+    const size_t ent = (size_t)MX_TPARTS * 64 * 8;
+    std::vector<uint16_t> t(MX_FENT * ent);
+    for (int nb = 0; nb < 2; nb++)
+        for (int ks = 0; ks < 2; ks++)
+            put_F(t.data() + (MX_F32 + 2 * nb + ks) * ent, 32, 16 + 16 * ks, [nb](int c) { return 32 * nb + c; });
+    for (int nt = 0; nt < 4; nt++)
+        put_F(t.data() + (MX_F1A + nt) * ent, 16, 16, [nt](int c) { return 16 * nt + c; });
+    for (int e = 0; e < 2; e++)
+        for (int hf = 0; hf < 2; hf++)
+            put_F(t.data() + (MX_F2 + 2 * e + hf) * ent, 16, 0, [e, hf](int c) { return 2 * (16 * hf + c) + e; });
     return t;
 }
 // the inverse DFT conj(F) at the sampled coordinates rows[0..n) as B
@@ -1183,14 +1253,6 @@ std::vector<uint16_t> build_Gh(const std::vector<int> &rows) {
         }
     put_entry(t.data(), re, im);
     return t;
-}
-// element (ux, uy) of an N=64 table -> its slot [tile][v][lane] in the C tile
-// of stage 2b (rows uy, columns ux of v_mfma_f32_32x32x16): tile = 2 (uy/32)
-// + ux/32, uy%32 = 8 (v/4) + 4 (lane/32) + v%4, ux%32 = lane%32
-inline long tile_slot(int ux, int uy) {
-    const int r = uy % 32;
-    const int v = 4 * (r / 8) + (r % 4), l = 32 * ((r % 8) / 4) + ux % 32;
-    return ((long)((uy / 32) * 2 + ux / 32) * 16 + v) * 64 + l;
 }
 // form used by the next step: the requested one, or for AUTO the MFMA form
 // whenever its size and sampling limits hold; -1 if the request cannot run
@@ -1271,11 +1333,12 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
     h->mfma_size = (N == 64 && h->nxsa == 32);
     if (p->llt_nx > 0) {
         ALLOC(h->d_llt_amp, p->llt_nx * p->llt_nx);
+        ALLOC(h->d_llt_opd, nwfs * p->llt_nx * p->llt_nx);
         ALLOC(h->d_lotf, nwfs * h->nwvl * N * N);
-        if (h->mfma_size) ALLOC(h->d_lotf_t, nwfs * h->nwvl * N * N);
+        if (h->mfma_size) ALLOC(h->d_lotf_t, nwfs * h->nwvl * MX_HALF);
     }
     if (h->mfma_size) {
-        ALLOC(h->d_F, 2 * 4 * MX_TPARTS * 64 * 8);
+        ALLOC(h->d_F, MX_FENT * MX_TPARTS * 64 * 8);
         ALLOC(h->d_mwv, MX_MAXWVL);
     }
 #undef ALLOC
@@ -1333,7 +1396,7 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (!h) return MX_OK;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_lotf,
+    void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
                     h->d_lotf_t, h->d_F, h->d_mwv,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
     for (void *q : ptrs)
@@ -1430,11 +1493,11 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
             HIPCHK(hipMemcpy(h->d_gy[iwvl], gy.data(), gy.size() * 2, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(h->d_sic[iwvl], sic.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
             if (nominal && h->cfg.llt_nx == 0) {
-                std::vector<float> nt(N * N);
-                for (long uy = 0; uy < N; uy++)
-                    for (long ux = 0; ux < N; ux++) nt[tile_slot((int)ux, (int)uy)] = (float)nominal[ux + uy * N];
-                if ((rc = dev_alloc(&h->d_nominal_t[iwvl], N * N))) return rc;
-                HIPCHK(hipMemcpy(h->d_nominal_t[iwvl], nt.data(), sizeof(float) * N * N, hipMemcpyHostToDevice));
+                std::vector<float> nt(MX_HALF);
+                for (long uy = 0; uy < N / 2; uy++)
+                    for (long ux = 0; ux < N; ux++) nt[half_slot((int)ux, (int)uy)] = (float)nominal[ux + uy * N];
+                if ((rc = dev_alloc(&h->d_nominal_t[iwvl], MX_HALF))) return rc;
+                HIPCHK(hipMemcpy(h->d_nominal_t[iwvl], nt.data(), sizeof(float) * MX_HALF, hipMemcpyHostToDevice));
             }
         }
     }
@@ -1464,13 +1527,13 @@ int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const do
         HIPCHK(hipMemcpy(*dst, t.data(), sizeof(float2) * nsa * N * N, hipMemcpyHostToDevice));
         if (h->mfma_size && !which) {
             for (long isa = 0; isa < nsa; isa++)
-                for (long uy = 0; uy < N; uy++)
+                for (long uy = 0; uy < N / 2; uy++)
                     for (long ux = 0; ux < N; ux++) {
                         const double *e = src + 2 * (isa * N * N + ux + uy * N);
-                        t[isa * N * N + tile_slot((int)ux, (int)uy)] = make_float2((float)e[0], (float)e[1]);
+                        t[isa * MX_HALF + half_slot((int)ux, (int)uy)] = make_float2((float)e[0], (float)e[1]);
                     }
-            if ((rc = dev_alloc(&h->d_etf_t[iwvl], nsa * N * N))) return rc;
-            HIPCHK(hipMemcpy(h->d_etf_t[iwvl], t.data(), sizeof(float2) * nsa * N * N, hipMemcpyHostToDevice));
+            if ((rc = dev_alloc(&h->d_etf_t[iwvl], nsa * MX_HALF))) return rc;
+            HIPCHK(hipMemcpy(h->d_etf_t[iwvl], t.data(), sizeof(float2) * nsa * MX_HALF, hipMemcpyHostToDevice));
         }
     }
     h->etf2wt = etf2 ? (float)etf2wt : 0.f;
@@ -1579,45 +1642,25 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
             std::memcpy(h->h_wv, hw, sizeof(hw));
         }
     }
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    // One stream: ray trace (the subaperture grids and the LLT grids), LLT
+    // uplink OTF, imaging.  Timing events: e2/e3 bracket the ray trace, e0/e1
+    // the LLT kernel, e4/e5 the imaging kernel.
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr, e4 = nullptr, e5 = nullptr;
     if (h->timing) {
-        if (h->ev.size() < 4) {
-            h->ev.resize(4);
+        if (h->ev.size() < 6) {
+            for (auto &e : h->ev) HIPCHK(hipEventDestroy(e));
+            h->ev.assign(6, nullptr);
             for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
         }
         e0 = h->ev[0];
         e1 = h->ev[1];
         e2 = h->ev[2];
         e3 = h->ev[3];
-        HIPCHK(hipEventRecord(e0, h->stream));
+        e4 = h->ev[4];
+        e5 = h->ev[5];
     }
-    if (h->cfg.llt_nx > 0) {
-        LltArgs la{};
-        la.nwvl = (int)h->nwvl;
-        la.nwfs = h->nwfs;
-        la.nps = g_atm.nps;
-        la.ptsdx = h->cfg.dx;
-        la.origx = h->cfg.llt_origx;
-        la.origy = h->cfg.llt_origy;
-        la.ox = h->cfg.llt_ox;
-        la.oy = h->cfg.llt_oy;
-        la.amp = h->d_llt_amp;
-        la.wv = h->d_wv;
-        const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
-        la.norm = (float)(1. / (nlwvf * nlwvf));
-        la.layers = g_atm.d_layers;
-        la.atmscale = g_atm.atmscale;
-        la.wfs = h->d_wfs;
-        la.isim = isim;
-        la.dt = dt;
-        la.lotf = h->d_lotf;
-        la.lotf_t = h->d_lotf_t;
-        dim3 grid((unsigned)(h->nwfs * h->nwvl));
-        if (N == 64) hipLaunchKernelGGL((k_llt_otf<64, 32>), grid, dim3(64), 0, h->stream, la);
-        else hipLaunchKernelGGL((k_llt_otf<32, 16>), grid, dim3(64), 0, h->stream, la);
-        HIPCHK(hipGetLastError());
-    }
-    if (h->timing) HIPCHK(hipEventRecord(e1, h->stream));
+    const bool llt = h->cfg.llt_nx > 0;
+    if (h->timing) HIPCHK(hipEventRecord(e2, h->stream));
     {
         OpdArgs o{};
         o.nsa = (int)h->nsa;
@@ -1632,12 +1675,37 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         o.isim = isim;
         o.dt = dt;
         o.opd = h->d_opd;
-        dim3 grid((unsigned)(h->nsa * h->nwfs));
+        o.nllt = llt ? h->nwfs : 0;
+        o.llt_origx = h->cfg.llt_origx;
+        o.llt_origy = h->cfg.llt_origy;
+        o.llt_ox = h->cfg.llt_ox;
+        o.llt_oy = h->cfg.llt_oy;
+        o.llt_opd = h->d_llt_opd;
+        dim3 grid((unsigned)(h->nsa * h->nwfs + o.nllt));
         if (h->nxsa == 32) hipLaunchKernelGGL((k_wfs_opd<32>), grid, dim3(64), 0, h->stream, o);
         else hipLaunchKernelGGL((k_wfs_opd<16>), grid, dim3(64), 0, h->stream, o);
         HIPCHK(hipGetLastError());
     }
-    if (h->timing) HIPCHK(hipEventRecord(e2, h->stream));
+    if (h->timing) HIPCHK(hipEventRecord(e3, h->stream));
+    if (llt) {
+        if (h->timing) HIPCHK(hipEventRecord(e0, h->stream));
+        LltArgs la{};
+        la.nwvl = (int)h->nwvl;
+        la.nwfs = h->nwfs;
+        la.amp = h->d_llt_amp;
+        la.opd = h->d_llt_opd;
+        la.wv = h->d_wv;
+        const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
+        la.norm = (float)(1. / (nlwvf * nlwvf));
+        la.lotf = h->d_lotf;
+        la.lotf_t = h->d_lotf_t;
+        dim3 grid((unsigned)(h->nwfs * h->nwvl));
+        if (N == 64) hipLaunchKernelGGL((k_llt_otf<64, 32>), grid, dim3(64), 0, h->stream, la);
+        else hipLaunchKernelGGL((k_llt_otf<32, 16>), grid, dim3(64), 0, h->stream, la);
+        HIPCHK(hipGetLastError());
+        if (h->timing) HIPCHK(hipEventRecord(e1, h->stream));
+    }
+    if (h->timing) HIPCHK(hipEventRecord(e4, h->stream));
     KArgs a{};
     a.nsa = (int)h->nsa;
     a.npix = (int)h->npix;
@@ -1727,12 +1795,12 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     }
     HIPCHK(hipGetLastError());
     if (h->timing) {
-        HIPCHK(hipEventRecord(e3, h->stream));
-        HIPCHK(hipEventSynchronize(e3));
+        HIPCHK(hipEventRecord(e5, h->stream));
+        HIPCHK(hipEventSynchronize(e5));
         float m1 = 0, m2 = 0, m3 = 0;
-        HIPCHK(hipEventElapsedTime(&m1, e0, e1));
-        HIPCHK(hipEventElapsedTime(&m2, e1, e2));
-        HIPCHK(hipEventElapsedTime(&m3, e2, e3));
+        if (llt) HIPCHK(hipEventElapsedTime(&m1, e0, e1));
+        HIPCHK(hipEventElapsedTime(&m2, e2, e3));
+        HIPCHK(hipEventElapsedTime(&m3, e4, e5));
         h->t_llt += m1;
         h->t_opd += m2;
         h->t_ints += m3;

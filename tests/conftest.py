@@ -43,3 +43,22 @@ def lgs_small():
     p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
     layers = genatm(cfg["atm"])
     return cfg, p, layers
+
+
+@pytest.fixture(scope="session")
+def ngs_two_wvl():
+    """A 4 m NGS case on the N=64 geometry with no LLT and two wavelengths
+    (nominal DTF only): 8x8 subapertures of 0.5 m at 1/64 m, notf 64, 6x6
+    pixels of 0.3", CoG; two von Karman layers of 512^2 at 1/64 m."""
+    import math
+    from maos_amd.powfs import PTYPE_COG, AperCfg, AtmCfg, Powfs, PowfsCfg, SimCfg, genatm
+    pcfg = PowfsCfg(order=8, dsa=0.5, dx=1.0 / 64, saat=0.3, hs=math.inf, embfac=2,
+                    wvl=[0.7e-6, 0.9e-6], wvlwts=[0.6, 0.4], pixpsa=6, pixtheta=0.3,
+                    pixblur=0.3, notf=64, phytype=PTYPE_COG, sigmatch=1, siglev=1000.0)
+    sim = SimCfg(dt=1.0 / 800)
+    atm = AtmCfg(r0=0.15, L0=30.0, ht=[0.0, 5000.0], wt=[0.7, 0.3], ws=[10.0, 15.0],
+                 wddeg=[0.0, 60.0], dx=1.0 / 64, nx=512, seeds=[11, 12])
+    cfg = dict(name="ngs_two_wvl", aper=AperCfg(4.0, 0.6), powfs=pcfg, sim=sim, atm=atm,
+               wfs=[dict(thetax=0.0, thetay=0.0), dict(thetax=5e-5, thetay=-3e-5)])
+    p = Powfs(cfg["aper"], pcfg, sim)
+    return cfg, p, genatm(atm)

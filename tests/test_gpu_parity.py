@@ -97,6 +97,33 @@ def test_lgs_step_parity(lgs_small, variant):
     eng.close()
 
 
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16"])
+def test_ngs_two_wavelengths(ngs_two_wvl, variant):
+    """N=64 without LLT or ETF (nominal DTF only), two wavelengths with their
+    own si and weights, two WFS off axis: both imaging forms against the
+    oracle."""
+    from maos_amd import _lib
+    cfg, p, layers = ngs_two_wvl
+    print("case: ngs two wavelengths nsa=%d notf=%d, %s" % (p.nsa, p.notf, variant), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    assert eng.variant() == _lib.VARIANT_MFMA_BF16  # AUTO picks the MFMA form here
+    want = {"fft32": _lib.VARIANT_FFT32, "mfma_bf16": _lib.VARIANT_MFMA_BF16}[variant]
+    eng.set_variant(want)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    for isim in (0, 41):
+        eng.queue(isim, dt)
+        for iwfs in range(len(wfs)):
+            g, ints = eng.sync(iwfs)
+            ri, rg, _, _ = o.step(wfs[iwfs], isim, dt)
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(g - rg).max() * RAD2MAS
+            print("isim=%d wfs=%d ints=%.2e grad=%.3e mas" % (isim, iwfs, e_int, e_g))
+            assert e_int <= INTS_TOL
+            assert e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
 def test_i0_accumulation_fp64(lgs_small):
     """Long-exposure i0 is accumulated in FP64 on the GPU: it equals the sum
     of the per-frame images to double rounding, and the oracle's sum to the
