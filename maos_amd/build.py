@@ -49,7 +49,7 @@ def build_setup():
 
 
 def build_oracle():
-    _run(["make", "-C", os.path.join(ROOT, "oracle")])
+    _run(["make", "-B", "-C", os.path.join(ROOT, "oracle")])
 
 
 def build_all(oracle=True):
