@@ -11,6 +11,7 @@
 // runs in one launch with no intermediate array in HBM
 // (maos/wfsgrad.c:122-501, maos/wfsints.c:79-302, maos/utils.c:983-1122).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include <cmath>
 #include <cstdarg>
@@ -252,11 +253,12 @@ __device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i,
 }
 
 // OPD of one NXSA x NXSA grid (origin origx, origy, step ptsdx) summed over
-// the layers, by N lanes.  Per layer, lanes [0, 2 NXSA) find the column and
-// row coordinates (cidx, cfrac: 2 NXSA entries), every lane finds the tile
-// origin itself, the (NXSA+2)^2 screen tile that the grid touches (its step
-// on the screen is <= 1 pixel) is staged in LDS with coalesced loads,
-// wrapping periodically, and the bilinear interpolation runs from LDS.
+// the layers, by the N lanes of one wave.  Per layer, lanes [0, 2 NXSA) find
+// the column and row coordinates (cidx, cfrac) in FP64, the (NXSA+2)^2 screen
+// tile that the grid touches (its step on the screen is <= 1 pixel) is
+// staged in LDS with coalesced loads, and the bilinear interpolation runs
+// from LDS.  A tile that crosses the screen edge wraps periodically; the
+// common case that does not (a uniform test) skips the wrap arithmetic.
 template <int N, int NXSA>
 __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, double hs,
                                           double thetax, double thetay, double misregx,
@@ -276,10 +278,6 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
     for (int l = 0; l < nps; l++) {
         const DevLayer L = layers[l];
         if (hs != 0. && L.ht > hs) continue; // layer above the source (uniform)
-        int tx0, ty0;
-        float f0;
-        screen_coord(L, true, 0, hs, thetax, misregx, shift0x, isim, dt, ptsdx, origx, tx0, f0);
-        screen_coord(L, false, 0, hs, thetay, misregy, shift0y, isim, dt, ptsdx, origy, ty0, f0);
         __syncthreads(); // the previous layer's coordinates and tile fully read
         if (line < 2 * NXSA) {
             const bool isx = line < NXSA;
@@ -290,34 +288,57 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
             cidx[line] = ii;
             cfrac[line] = ff;
         }
+        __syncthreads();
+        const int tx0 = cidx[0], ty0 = cidx[NXSA];
         const int nx = L.nx, ny = L.ny;
+        const bool wrap = tx0 > nx - TW || ty0 > ny - TW; // uniform
         if (act) {
+            if (!wrap) {
+                const float *base = L.p + (long)ty0 * nx + tx0;
 #pragma unroll
-            for (int k = 0; k < NFILL; k++) {
-                const int e = line + k * N;
-                if (e < TW * TW) {
-                    const int r = e / TW, c = e - r * TW;
-                    int gx = tx0 + c, gy = ty0 + r;
-                    gx -= gx >= nx ? nx : 0;
-                    gy -= gy >= ny ? ny : 0;
-                    tile[e] = L.p[(long)gy * nx + gx];
+                for (int k = 0; k < NFILL; k++) {
+                    const int e = line + k * N;
+                    if (e < TW * TW) {
+                        const int r = e / TW, c = e - r * TW;
+                        tile[e] = base[r * nx + c];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < NFILL; k++) {
+                    const int e = line + k * N;
+                    if (e < TW * TW) {
+                        const int r = e / TW, c = e - r * TW;
+                        int gx = tx0 + c, gy = ty0 + r;
+                        gx -= gx >= nx ? nx : 0;
+                        gy -= gy >= ny ? ny : 0;
+                        tile[e] = L.p[(long)gy * nx + gx];
+                    }
                 }
             }
         }
         __syncthreads();
-        if (act) {
+        // bilinear interpolation as two lerps, a + f (b - a)
+        auto interp = [&](auto wrapped) {
 #pragma unroll
             for (int j = 0; j < PPT; j++) {
                 const int pt = line + j * N;
                 const int ix = pt % NXSA, iy = pt / NXSA;
                 int lc = cidx[ix] - tx0, lr = cidx[NXSA + iy] - ty0;
-                lc += lc < 0 ? nx : 0;
-                lr += lr < 0 ? ny : 0;
+                if (decltype(wrapped)::value) {
+                    lc += lc < 0 ? nx : 0;
+                    lr += lr < 0 ? ny : 0;
+                }
                 const float fx = cfrac[ix], fy = cfrac[NXSA + iy];
                 const float *t0 = tile + lr * TW + lc;
-                acc[j] += (1.f - fy) * ((1.f - fx) * t0[0] + fx * t0[1]) +
-                          fy * ((1.f - fx) * t0[TW] + fx * t0[TW + 1]);
+                const float r0 = fmaf(fx, t0[1] - t0[0], t0[0]);
+                const float r1 = fmaf(fx, t0[TW + 1] - t0[TW], t0[TW]);
+                acc[j] += fmaf(fy, r1 - r0, r0);
             }
+        };
+        if (act) {
+            if (wrap) interp(std::true_type{});
+            else interp(std::false_type{});
         }
     }
     if (act)
@@ -540,11 +561,13 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
 // wfsgrad_iwfs, maos/wfsgrad.c:175-194).  Small register and LDS footprint,
 // so many waves per SIMD hide the screen-fetch latency; the FP32 OPD goes to
 // HBM for the imaging kernel.
+// threads of a ray-trace block (one grid per block, one wave)
+constexpr int MX_OPD_THREADS = 64;
 #ifndef MX_OPD_WAVES
 #define MX_OPD_WAVES 4
 #endif
 template <int NXSA>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MX_OPD_WAVES))) void k_wfs_opd(OpdArgs a) {
+__global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(MX_OPD_WAVES))) void k_wfs_opd(OpdArgs a) {
     __shared__ int cidx[2 * NXSA];
     __shared__ float cfrac[2 * NXSA];
     __shared__ float tile[(NXSA + 2) * (NXSA + 2)];
@@ -554,7 +577,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MX_OPD_WAVES
         // shift o - v dt isim.  These blocks come first so that they start
         // early.
         const DevWfs &W = a.wfs[b];
-        ray_trace<64, NXSA>(a.layers, a.nps, W.hs, W.thetax - a.llt_ox / W.hs, W.thetay - a.llt_oy / W.hs, 0.,
+        ray_trace<MX_OPD_THREADS, NXSA>(a.layers, a.nps, W.hs, W.thetax - a.llt_ox / W.hs, W.thetay - a.llt_oy / W.hs, 0.,
                             0., a.llt_ox, a.llt_oy, a.isim, a.dt, a.atmscale, a.ptsdx, a.llt_origx, a.llt_origy,
                             threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile);
         return;
@@ -562,7 +585,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MX_OPD_WAVES
     const int unit = xcd_unit(b - a.nllt, a.nsa * a.nwfs);
     const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
     const DevWfs &W = a.wfs[iwfs];
-    ray_trace<64, NXSA>(a.layers, a.nps, W.hs, W.thetax, W.thetay, W.misregx, W.misregy, 0., 0.,
+    ray_trace<MX_OPD_THREADS, NXSA>(a.layers, a.nps, W.hs, W.thetax, W.thetay, W.misregx, W.misregy, 0., 0.,
                         a.isim, a.dt, a.atmscale, a.ptsdx, a.ptsx[isa], a.ptsy[isa], threadIdx.x,
                         a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac, tile);
 }
@@ -1682,8 +1705,8 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         o.llt_oy = h->cfg.llt_oy;
         o.llt_opd = h->d_llt_opd;
         dim3 grid((unsigned)(h->nsa * h->nwfs + o.nllt));
-        if (h->nxsa == 32) hipLaunchKernelGGL((k_wfs_opd<32>), grid, dim3(64), 0, h->stream, o);
-        else hipLaunchKernelGGL((k_wfs_opd<16>), grid, dim3(64), 0, h->stream, o);
+        if (h->nxsa == 32) hipLaunchKernelGGL((k_wfs_opd<32>), grid, dim3(MX_OPD_THREADS), 0, h->stream, o);
+        else hipLaunchKernelGGL((k_wfs_opd<16>), grid, dim3(MX_OPD_THREADS), 0, h->stream, o);
         HIPCHK(hipGetLastError());
     }
     if (h->timing) HIPCHK(hipEventRecord(e3, h->stream));
