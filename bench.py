@@ -56,13 +56,12 @@ def flops_per_subap(N, nxsa, nps, npix, llt=True, etf=True):
 
 def mfma_flops_per_subap():
     """BF16 MFMA FLOPs the MFMA imaging kernel executes per subaperture-
-    wavelength at N=64, nxsa=32, three products per split multiply (four
-    waves): forward transform over the 32-wide band, 96 16x16x32 (first
-    axis) + 96 32x32x16 (second axis, |E|^2); half-plane inverse transform
-    with radix-2 halved sums, 48 + 96 16x16x32; detector rows and columns,
-    48 + 12 16x16x32."""
-    n32 = 96
-    n16 = 96 + 48 + 96 + 48 + 12
+    wavelength at N=64, nxsa=32: 16x16x32 products, three per split multiply,
+    four waves.  Forward transform over the 32-wide band, 96 (first axis) +
+    192 (second axis, |E|^2); half-plane inverse transform with radix-2
+    halved sums, 48 + 96; detector rows and columns, 48 + 12."""
+    n32 = 0
+    n16 = 96 + 192 + 48 + 96 + 48 + 12
     return n32 * 32 * 32 * 16 * 2 + n16 * 16 * 16 * 32 * 2
 
 

@@ -708,8 +708,19 @@ typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int MLDB = 72;        // bf16 row stride of the LDS planes
+// LDS plane layout: element (row, k) of a plane sits at row * MLDB + (k ^
+// swizzle(row)).  The swizzle permutes whole 8-element (16-byte) chunks, so
+// the 16-byte operand reads and the 8-byte stores stay contiguous.
+#ifndef MX_LDS_SWZ
+#define MX_LDS_SWZ 2
+#endif
+constexpr int MLDB = MX_LDS_SWZ ? 64 : 72; // bf16 row stride of the LDS planes
 constexpr int MPLANE = 64 * MLDB;
+__device__ __forceinline__ int lds_at(int row, int k) {
+    if constexpr (MX_LDS_SWZ == 1) return row * MLDB + (k ^ ((row & 7) << 3));
+    else if constexpr (MX_LDS_SWZ == 2) return row * MLDB + (k ^ (((row >> 1) & 7) << 3));
+    else return row * MLDB + k;
+}
 constexpr int MX_MFMA_NS = 16;  // sampled image rows/columns kept by stage 3
 
 // Table entries: [.. entry ..][part][lane] bf16x8, parts re hi, re lo, im hi,
@@ -717,11 +728,11 @@ constexpr int MX_MFMA_NS = 16;  // sampled image rows/columns kept by stage 3
 // layout).  The negated imaginary part lets every complex product accumulate
 // with additions only.
 constexpr int MX_TPARTS = 6;
-// DFT table entries (each MX_TPARTS x 64 bf16x8)
-constexpr int MX_F32 = 0;  // + 2 nb + ks: 32x32x16, n = 32 nb + l%32, k = 16 + 16 ks + 8 (l/32) + j
-constexpr int MX_F1A = 4;  // + nt: 16x16x32, n = 16 nt + l%16, k = 16 + 8 (l/16) + j
-constexpr int MX_F2 = 8;   // + 2 e + hf: 16x16x32, n = 2 (16 hf + l%16) + e, k = 8 (l/16) + j
-constexpr int MX_FENT = 12;
+// DFT table entries (each MX_TPARTS x 64 bf16x8), B operands of
+// v_mfma_f32_16x16x32_bf16
+constexpr int MX_F1A = 0;  // + nt: n = 16 nt + l%16, k = 16 + 8 (l/16) + j (the band)
+constexpr int MX_F2 = 4;   // + 2 e + hf: n = 2 (16 hf + l%16) + e, k = 8 (l/16) + j
+constexpr int MX_FENT = 8;
 struct MfmaWvl {
     const float2 *etf_t;     // [nsa][MX_HALF] ETF (nominal fused), half_slot order, or null
     const float *nominal_t;  // [MX_HALF] nominal when there is no ETF, or null
@@ -770,13 +781,13 @@ __device__ __forceinline__ void mac3(ACC &c, bf16x8 ah, bf16x8 al, bf16x8 bh, bf
 template <typename ACC, int SZ, bool DREAL>
 __device__ __forceinline__ void cmacd(ACC &cr, ACC &ci, const __bf16 *const (&L)[4], int row, int k,
                                       const bf16x8 (&b)[3][2]) {
-    const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(L[0] + row * MLDB + k);
-    const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(L[1] + row * MLDB + k);
+    const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(L[0] + lds_at(row, k));
+    const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(L[1] + lds_at(row, k));
     mac3<ACC, SZ>(cr, dh, dl, b[0][0], b[0][1]);
     mac3<ACC, SZ>(ci, dh, dl, b[1][0], b[1][1]);
     if (!DREAL) {
-        const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(L[2] + row * MLDB + k);
-        const bf16x8 el = *reinterpret_cast<const bf16x8 *>(L[3] + row * MLDB + k);
+        const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(L[2] + lds_at(row, k));
+        const bf16x8 el = *reinterpret_cast<const bf16x8 *>(L[3] + lds_at(row, k));
         mac3<ACC, SZ>(cr, eh, el, b[2][0], b[2][1]);
         mac3<ACC, SZ>(ci, eh, el, b[0][0], b[0][1]);
     }
@@ -807,8 +818,8 @@ __device__ __forceinline__ void st4(__bf16 *ph, __bf16 *pl, int row, int k, floa
     uint2 h, l;
     split2(a0, a1, h.x, l.x);
     split2(a2, a3, h.y, l.y);
-    *reinterpret_cast<uint2 *>(ph + row * MLDB + k) = h;
-    *reinterpret_cast<uint2 *>(pl + row * MLDB + k) = l;
+    *reinterpret_cast<uint2 *>(ph + lds_at(row, k)) = h;
+    *reinterpret_cast<uint2 *>(pl + lds_at(row, k)) = l;
 }
 
 // amp * exp(+i 2 pi x) for x in revolutions: the phase is reduced to
@@ -857,6 +868,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
     float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
     for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
         const MfmaWvl &WV = a.wv[iwvl];
+        // B operand of stages 1a and 1b: F over the band [16, 48) at the 16
+        // columns 16 lw + l%16; fetched before the embed so that it arrives
+        // meanwhile
+        bf16x8 b1[3][2];
+        loadT(a.F + (MX_F1A + lw) * MX_TPARTS * 64, lane, b1);
         // embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf) as W[x][y], y contiguous;
         // OPD and amplitude are read again per wavelength (cache hits) rather
         // than held in registers
@@ -872,12 +888,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // 1a: C[x][ky] = sum_{y in band} W[x][y] F[y][ky]; wave: ky in [16 lw, 16 lw + 16),
         // both 16-row blocks of x; stored as Z row ky, columns x
         {
-            bf16x8 b[3][2];
-            loadT(a.F + (MX_F1A + lw) * MX_TPARTS * 64, lane, b);
             f32x4 zr[2] = {}, zi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<f32x4, 16, false>(zr[mt], zi[mt], Lc, B0 + 16 * mt + l16, B0 + 8 * h16, b);
+                cmacd<f32x4, 16, false>(zr[mt], zi[mt], Lc, B0 + 16 * mt + l16, B0 + 8 * h16, b1);
             __syncthreads();
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
@@ -887,31 +901,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             }
         }
         __syncthreads();
-        // 1b: C[ky][kx] = sum_{x in band} Z[ky][x] F[x][kx]; P = |C|^2.  Tile
-        // row t of the wave reads Z row ky = 16 mb + t%16 + 32 (t/16), so a
-        // lane's elements v and v + 8 are ky and ky + 32: P+ = P[ky] + P[ky+32]
-        // and P- = P[ky] - P[ky+32] (ky < 32) go to row kx, columns ky and 32 + ky
+        // 1b: C[ky][kx] = sum_{x in band} Z[ky][x] F[x][kx]; P = |C|^2.  Wave:
+        // kx in [16 lw, 16 lw + 16) (the same B operand as 1a, F being
+        // symmetric), the four 16-row blocks t of ky; blocks t and t + 2 hold ky
+        // and ky + 32 in the same lane: P+ = P[ky] + P[ky+32] and P- = P[ky] -
+        // P[ky+32] (ky < 32) go to row kx, columns ky and 32 + ky
+        bf16x8 b2[3][2];
         {
-            // DFT columns kx = 32 nb + l%32 over the band k in [16, 48)
-            f32x16 cr = {}, ci = {};
-            const int row = 16 * mb + (l32 % 16) + 32 * (l32 / 16);
+            f32x4 yr[4] = {}, yi[4] = {};
 #pragma unroll
-            for (int ks = 0; ks < 2; ks++) {
-                bf16x8 f[3][2];
-                loadT(a.F + (MX_F32 + 2 * nb + ks) * MX_TPARTS * 64, lane, f);
-                cmacd<f32x16, 32, false>(cr, ci, Lc, row, B0 + 16 * ks + 8 * h32, f);
-            }
+            for (int t = 0; t < 4; t++) cmacd<f32x4, 16, false>(yr[t], yi[t], Lc, 16 * t + l16, B0 + 8 * h16, b1);
+            loadT(a.F + (MX_F2 + 2 * nb) * MX_TPARTS * 64, lane, b2); // stage 2a's operand
             __syncthreads();
 #pragma unroll
-            for (int v = 0; v < 16; v++) cr[v] = cr[v] * cr[v] + ci[v] * ci[v];
-            const int kx = 32 * nb + l32;
+            for (int t = 0; t < 4; t++)
 #pragma unroll
-            for (int g = 0; g < 2; g++) {
-                const int col = 16 * mb + 8 * g + 4 * h32, v = 4 * g;
-                st4(L[0], L[1], kx, col, cr[v] + cr[v + 8], cr[v + 1] + cr[v + 9], cr[v + 2] + cr[v + 10],
-                    cr[v + 3] + cr[v + 11]);
-                st4(L[0], L[1], kx, 32 + col, cr[v] - cr[v + 8], cr[v + 1] - cr[v + 9], cr[v + 2] - cr[v + 10],
-                    cr[v + 3] - cr[v + 11]);
+                for (int v = 0; v < 4; v++) yr[t][v] = yr[t][v] * yr[t][v] + yi[t][v] * yi[t][v];
+            const int kx = 16 * lw + l16;
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int col = 16 * t + 4 * h16;
+                st4(L[0], L[1], kx, col, yr[t][0] + yr[t + 2][0], yr[t][1] + yr[t + 2][1], yr[t][2] + yr[t + 2][2],
+                    yr[t][3] + yr[t + 2][3]);
+                st4(L[0], L[1], kx, 32 + col, yr[t][0] - yr[t + 2][0], yr[t][1] - yr[t + 2][1],
+                    yr[t][2] - yr[t + 2][2], yr[t][3] - yr[t + 2][3]);
             }
         }
         __syncthreads();
@@ -921,12 +934,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // (kx < 32) go to row uy, columns kx and 32 + kx
         {
             const int e = nb;
-            bf16x8 b[3][2];
-            loadT(a.F + (MX_F2 + 2 * e) * MX_TPARTS * 64, lane, b);
             f32x4 qr[2] = {}, qi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<f32x4, 16, true>(qr[mt], qi[mt], Lc, 16 * mb + 32 * mt + l16, 32 * e + 8 * h16, b);
+                cmacd<f32x4, 16, true>(qr[mt], qi[mt], Lc, 16 * mb + 32 * mt + l16, 32 * e + 8 * h16, b2);
+            loadT(a.F + (MX_F2 + 2 * nb + mb) * MX_TPARTS * 64, lane, b1); // stage 2b's operand
             __syncthreads();
             const int uy = 2 * l16 + e, col = 16 * mb + 4 * h16;
             st4(L[0], L[1], uy, col, qr[0][0] + qr[1][0], qr[0][1] + qr[1][1], qr[0][2] + qr[1][2],
@@ -945,12 +957,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // to row ux, columns uy
         {
             const int e = nb, hf = mb;
-            bf16x8 b[3][2];
-            loadT(a.F + (MX_F2 + 2 * e + hf) * MX_TPARTS * 64, lane, b);
             f32x4 xr[2] = {}, xi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<f32x4, 16, false>(xr[mt], xi[mt], Lc, 16 * mt + l16, 32 * e + 8 * h16, b);
+                cmacd<f32x4, 16, false>(xr[mt], xi[mt], Lc, 16 * mt + l16, 32 * e + 8 * h16, b1);
+            loadT(WV.gy, lane, b2); // stage 3a's operand
             // element (mt, v) of this lane is table slot ((2 lw + mt) 4 + v) 64 + lane
             const long tbase = (long)(2 * lw) * 4 * 64 + lane;
             const float2 *lt = a.lotf_t ? a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * MX_HALF + tbase : nullptr;
@@ -994,10 +1005,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // and 2 elsewhere (folded into the table); role r: ux in [16r, 16r+16)
         {
             f32x4 rr = {}, ri = {};
-            const int l16 = lane % 16, h16 = lane / 16;
-            bf16x8 g[3][2];
-            loadT(WV.gy, lane, g);
-            cmacd<f32x4, 16, false>(rr, ri, Lc, 16 * lw + l16, 8 * h16, g);
+            cmacd<f32x4, 16, false>(rr, ri, Lc, 16 * lw + l16, 8 * h16, b2);
+            if (lw == 3) loadT(WV.gx, lane, b1); // stage 3b's operand, first K step
             __syncthreads();
             // element v: ux = 16 lw + 4 (l/16) + v, ys = l%16 -> row ys, columns ux
             st4(L[0], L[1], l16, 16 * lw + 4 * h16, rr[0], rr[1], rr[2], rr[3]);
@@ -1007,16 +1016,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // 3b: I[ys][xs] = Re sum_ux R[ys][ux] Gx[ux][xs]  (role 3)
         if (lw == 3) {
             f32x4 ir = {};
-            const int l16 = lane % 16, h16 = lane / 16;
+            loadT(WV.gx + MX_TPARTS * 64, lane, b2);
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) {
-                bf16x8 g[3][2];
-                loadT(WV.gx + ks * MX_TPARTS * 64, lane, g);
+                const bf16x8(&g)[3][2] = ks ? b2 : b1;
                 const int k = 32 * ks + 8 * h16;
-                const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(Lc[0] + l16 * MLDB + k);
-                const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(Lc[1] + l16 * MLDB + k);
-                const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(Lc[2] + l16 * MLDB + k);
-                const bf16x8 el = *reinterpret_cast<const bf16x8 *>(Lc[3] + l16 * MLDB + k);
+                const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(Lc[0] + lds_at(l16, k));
+                const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(Lc[1] + lds_at(l16, k));
+                const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(Lc[2] + lds_at(l16, k));
+                const bf16x8 el = *reinterpret_cast<const bf16x8 *>(Lc[3] + lds_at(l16, k));
                 mac3<f32x4, 16>(ir, dh, dl, g[0][0], g[0][1]); // Re R * Re G
                 mac3<f32x4, 16>(ir, eh, el, g[2][0], g[2][1]); // Im R * (-Im G)
             }
@@ -1218,13 +1226,10 @@ void put_F(uint16_t *dst, int w, int kbase, NCOL ncol) {
         }
     put_entry(dst, re, im);
 }
-// the DFT tables of k_shwfs_mfma (entries MX_F32, MX_F1A, MX_F2)
+// the DFT tables of k_shwfs_mfma (entries MX_F1A, MX_F2)
 std::vector<uint16_t> build_F64() {
     const size_t ent = (size_t)MX_TPARTS * 64 * 8;
     std::vector<uint16_t> t(MX_FENT * ent);
-    for (int nb = 0; nb < 2; nb++)
-        for (int ks = 0; ks < 2; ks++)
-            put_F(t.data() + (MX_F32 + 2 * nb + ks) * ent, 32, 16 + 16 * ks, [nb](int c) { return 32 * nb + c; });
     for (int nt = 0; nt < 4; nt++)
         put_F(t.data() + (MX_F1A + nt) * ent, 16, 16, [nt](int c) { return 16 * nt + c; });
     for (int e = 0; e < 2; e++)
