@@ -757,6 +757,12 @@ struct MfmaArgs {
     float saat;
     float *ints, *grad;
     double *i0acc;
+    // uplink mode (one workgroup per WFS and wavelength): LLT OPD
+    // [nwfs][32*32] and amplitude, 1/nlwvf^2, and the two outputs conj(lotfc)
+    // [nwfs][nwvl][64*64] ([ux][uy]) and its half plane [nwfs][nwvl][MX_HALF]
+    const float *llt_opd, *llt_amp;
+    float llt_norm;
+    float2 *lotf_full, *lotf_half;
 };
 
 #define MF32(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0)
@@ -835,6 +841,10 @@ __device__ __forceinline__ void cis_rev(float x, float am, float &re, float &im)
 #ifndef MX_MFMA_WAVES
 #define MX_MFMA_WAVES 4
 #endif
+// UPLINK: the same stages through 2b on the LLT grid give the uplink OTF
+// lotfc = FFT(|FFT(embed)|^2) / nlwvf^2 (maos/wfsints.c:149-199); one
+// workgroup per (WFS, wavelength) writes conj(lotfc) and stops there.
+template <bool UPLINK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAVES))) void k_shwfs_mfma(MfmaArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 Lb[4 * MPLANE];
     __shared__ float img[MX_MFMA_NS * MX_MFMA_NS];
@@ -842,8 +852,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
     __bf16 *const L[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
     const __bf16 *const Lc[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
-    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    int iwfs, isa, iwvl0, iwvl1;
+    if constexpr (UPLINK) {
+        iwfs = (int)blockIdx.x / a.nwvl;
+        isa = 0;
+        iwvl0 = (int)blockIdx.x % a.nwvl;
+        iwvl1 = iwvl0 + 1;
+    } else {
+        const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
+        iwfs = unit % a.nwfs;
+        isa = unit / a.nwfs;
+        iwvl0 = 0;
+        iwvl1 = a.nwvl;
+    }
     const DevWfs &W = a.wfs[iwfs];
     constexpr int NX = 32, B0 = 16; // nonzero block [16, 48)
     // role of the wave; it rotates with the block so that the SIMDs of a CU
@@ -853,8 +874,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
     const int h32 = lane / 32, l32 = lane % 32, h16 = lane / 16, l16 = lane % 16;
     // OPD and amplitude: thread (ix = tid%32, iy = 4 (tid/32) + j)
     const int ix = tid % 32, iy0 = 4 * (tid / 32);
-    const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
-    const float *ag = a.amp + (long)isa * NX * NX;
+    const float *og = UPLINK ? a.llt_opd + (long)iwfs * NX * NX : a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
+    const float *ag = UPLINK ? a.llt_amp : a.amp + (long)isa * NX * NX;
     float mean;
     {
         float s = 0.f;
@@ -866,7 +887,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
     }
     float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
-    for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
+    for (int iwvl = iwvl0; iwvl < iwvl1; iwvl++) {
         const MfmaWvl &WV = a.wv[iwvl];
         // B operand of stages 1a and 1b: F over the band [16, 48) at the 16
         // columns 16 lw + l%16; fetched before the embed so that it arrives
@@ -961,9 +982,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
                 cmacd<f32x4, 16, false>(xr[mt], xi[mt], Lc, 16 * mt + l16, 32 * e + 8 * h16, b1);
-            loadT(WV.gy, lane, b2); // stage 3a's operand
             // element (mt, v) of this lane is table slot ((2 lw + mt) 4 + v) 64 + lane
             const long tbase = (long)(2 * lw) * 4 * 64 + lane;
+            const int ux = 2 * (16 * hf + l16) + e;
+            if constexpr (UPLINK) {
+                // conj(lotfc): the half plane in slot order, and the full plane
+                // [ux][uy] with uy > 32 from conj(lotfc)(-u) = lotfc(u) and the
+                // lag-32 row zero (the LLT field fills 32 of 64 samples)
+                float2 *hp = a.lotf_half + ((long)iwfs * a.nwvl + iwvl) * MX_HALF + tbase;
+                float2 *fp = a.lotf_full + ((long)iwfs * a.nwvl + iwvl) * 64 * 64;
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int uy = 16 * mt + 4 * h16 + v;
+                        const float re = xr[mt][v] * a.llt_norm, im = xi[mt][v] * a.llt_norm;
+                        hp[(mt * 4 + v) * 64] = make_float2(re, -im);
+                        fp[ux * 64 + uy] = make_float2(re, -im);
+                        if (uy) fp[((64 - ux) & 63) * 64 + (64 - uy)] = make_float2(re, im);
+                    }
+                if (h16 == 0) fp[ux * 64 + 32] = make_float2(0.f, 0.f);
+                continue;
+            }
+            loadT(WV.gy, lane, b2); // stage 3a's operand
             const float2 *lt = a.lotf_t ? a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * MX_HALF + tbase : nullptr;
             const float2 *et = WV.etf_t ? WV.etf_t + (long)isa * MX_HALF + tbase : nullptr;
             const float *nm = WV.nominal_t ? WV.nominal_t + tbase : nullptr;
@@ -992,7 +1033,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
                     xi[mt][v] = im;
                 }
             __syncthreads();
-            const int ux = 2 * (16 * hf + l16) + e;
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
                 const int col = 16 * mt + 4 * h16;
@@ -1039,6 +1079,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         }
         __syncthreads();
     }
+    if constexpr (UPLINK) return;
     // outputs and pixel processing (shwfs_grad, maos/utils.c:983-1122), wave 0
     if (wave == 0) {
         const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
@@ -1715,25 +1756,6 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         HIPCHK(hipGetLastError());
     }
     if (h->timing) HIPCHK(hipEventRecord(e3, h->stream));
-    if (llt) {
-        if (h->timing) HIPCHK(hipEventRecord(e0, h->stream));
-        LltArgs la{};
-        la.nwvl = (int)h->nwvl;
-        la.nwfs = h->nwfs;
-        la.amp = h->d_llt_amp;
-        la.opd = h->d_llt_opd;
-        la.wv = h->d_wv;
-        const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
-        la.norm = (float)(1. / (nlwvf * nlwvf));
-        la.lotf = h->d_lotf;
-        la.lotf_t = h->d_lotf_t;
-        dim3 grid((unsigned)(h->nwfs * h->nwvl));
-        if (N == 64) hipLaunchKernelGGL((k_llt_otf<64, 32>), grid, dim3(64), 0, h->stream, la);
-        else hipLaunchKernelGGL((k_llt_otf<32, 16>), grid, dim3(64), 0, h->stream, la);
-        HIPCHK(hipGetLastError());
-        if (h->timing) HIPCHK(hipEventRecord(e1, h->stream));
-    }
-    if (h->timing) HIPCHK(hipEventRecord(e4, h->stream));
     KArgs a{};
     a.nsa = (int)h->nsa;
     a.npix = (int)h->npix;
@@ -1770,6 +1792,7 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     a.grad = h->d_grad;
     a.i0acc = h->i0_enable ? h->d_i0acc : nullptr;
     a.opd = h->d_opd;
+    MfmaArgs m{};
     if (variant == MX_VARIANT_MFMA_BF16) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
@@ -1786,7 +1809,6 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
             HIPCHK(hipStreamSynchronize(h->stream));
             std::memcpy(h->h_mwv, hw, sizeof(hw));
         }
-        MfmaArgs m{};
         m.nsa = a.nsa;
         m.npix = a.npix;
         m.pixpsax = a.pixpsax;
@@ -1813,7 +1835,39 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         m.ints = a.ints;
         m.grad = a.grad;
         m.i0acc = a.i0acc;
-        hipLaunchKernelGGL(k_shwfs_mfma, dim3((unsigned)(h->nsa * h->nwfs)), dim3(256), 0, h->stream, m);
+        m.llt_opd = h->d_llt_opd;
+        m.llt_amp = h->d_llt_amp;
+        m.llt_norm = (float)(1. / ((double)h->cfg.llt_nx * h->embfac * (double)h->cfg.llt_nx * h->embfac));
+        m.lotf_full = h->d_lotf;
+        m.lotf_half = h->d_lotf_t;
+    }
+    // uplink OTF: the MFMA kernel's uplink mode with the MFMA form, else the
+    // FP32 FFT kernel
+    if (llt) {
+        if (h->timing) HIPCHK(hipEventRecord(e0, h->stream));
+        if (variant == MX_VARIANT_MFMA_BF16) {
+            hipLaunchKernelGGL(k_shwfs_mfma<true>, dim3((unsigned)(h->nwfs * h->nwvl)), dim3(256), 0, h->stream, m);
+        } else {
+            LltArgs la{};
+            la.nwvl = (int)h->nwvl;
+            la.nwfs = h->nwfs;
+            la.amp = h->d_llt_amp;
+            la.opd = h->d_llt_opd;
+            la.wv = h->d_wv;
+            const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
+            la.norm = (float)(1. / (nlwvf * nlwvf));
+            la.lotf = h->d_lotf;
+            la.lotf_t = h->d_lotf_t;
+            dim3 grid((unsigned)(h->nwfs * h->nwvl));
+            if (N == 64) hipLaunchKernelGGL((k_llt_otf<64, 32>), grid, dim3(64), 0, h->stream, la);
+            else hipLaunchKernelGGL((k_llt_otf<32, 16>), grid, dim3(64), 0, h->stream, la);
+        }
+        HIPCHK(hipGetLastError());
+        if (h->timing) HIPCHK(hipEventRecord(e1, h->stream));
+    }
+    if (h->timing) HIPCHK(hipEventRecord(e4, h->stream));
+    if (variant == MX_VARIANT_MFMA_BF16) {
+        hipLaunchKernelGGL(k_shwfs_mfma<false>, dim3((unsigned)(h->nsa * h->nwfs)), dim3(256), 0, h->stream, m);
     } else {
         const int spw = (int)(64 / N);
         const long ngroup = (h->nsa + spw - 1) / spw;

@@ -91,7 +91,9 @@ def test_lgs_step_parity(lgs_small, variant):
             print("isim=%d wfs=%d opd=%.2e lotf=%.2e ints=%.2e grad=%.3e mas"
                   % (isim, iwfs, e_opd, e_lotf, e_int, e_g))
             assert e_opd < 1e-5
-            assert e_lotf < 1e-5
+            # the MFMA form computes the uplink OTF with the same split-BF16
+            # products as the image, so it gets the image's bar
+            assert e_lotf < (1e-5 if variant == "fft32" else INTS_TOL)
             assert e_int <= INTS_TOL
             assert e_g <= GRAD_TOL_MAS
     eng.close()
