@@ -124,6 +124,7 @@ struct DevLayer {
     int nx, ny;
     double dx1, dy1;   // 1/dx, 1/dy
     double ox, oy, ht, vx, vy;
+    double inx, iny;   // 1/nx, 1/ny
 };
 
 struct DevWfs {
@@ -228,10 +229,13 @@ __device__ __forceinline__ void seg_sum(float &v) {
 // L, FP64 (prop_grid_pts, lib/accphi_grid_do.c:408-446, with the frozen-flow
 // displacement of maos/wfsgrad.c:175-194): disp = ht theta + scale misreg +
 // shift0 - v dt isim, scale = 1 - ht/hs (cone), periodic wrap.
-__device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i, double hs, double theta,
+// ihs = 1/hs (0 for an NGS).  The wrap count floor(X/n) comes from X * (1/n)
+// except within 1e-9 of an integer quotient, where the division decides (the
+// wrapped coordinate then matches the division's rounding exactly).
+__device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i, double ihs, double theta,
                                              double misreg, double shift0, long isim, double dt,
                                              double ptsdx, double orig, int &ii, float &ff) {
-    const double scale = hs != 0. ? 1. - L.ht / hs : 1.;
+    const double scale = 1. - L.ht * ihs;
     const double disp = L.ht * theta + scale * misreg + shift0 - (isx ? L.vx : L.vy) * dt * (double)isim;
     const double d1 = isx ? L.dx1 : L.dy1;
     const double nn = (double)(isx ? L.nx : L.ny);
@@ -239,7 +243,10 @@ __device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i,
     const double dout = ptsdx * d1 * scale;
     const double oout = orig * d1 * scale + (disp - o0) * d1;
     double X = fma((double)i, dout, oout);
-    X -= floor(X / nn) * nn;
+    const double q = X * (isx ? L.inx : L.iny);
+    double fq = floor(q);
+    if (q - fq < 1e-9 || q - fq > 1. - 1e-9) fq = floor(X / nn);
+    X -= fq * nn;
     const double fl = floor(X);
     ii = (int)fl;
     double fr = X - fl;
@@ -272,6 +279,7 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
     constexpr int NFILL = (TW * TW + N - 1) / N;
     static_assert(PPT * N == NXSA * NXSA, "points per lane");
     const bool act = line < N;
+    const double ihs = hs != 0. ? 1. / hs : 0.;
     float acc[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) acc[j] = 0.f;
@@ -283,7 +291,7 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
             const bool isx = line < NXSA;
             int ii;
             float ff;
-            screen_coord(L, isx, isx ? line : line - NXSA, hs, isx ? thetax : thetay, isx ? misregx : misregy,
+            screen_coord(L, isx, isx ? line : line - NXSA, ihs, isx ? thetax : thetay, isx ? misregx : misregy,
                          isx ? shift0x : shift0y, isim, dt, ptsdx, isx ? origx : origy, ii, ff);
             cidx[line] = ii;
             cfrac[line] = ff;
@@ -1662,6 +1670,8 @@ int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
         D.ny = (int)L.ny;
         D.dx1 = 1. / L.dx;
         D.dy1 = 1. / L.dy;
+        D.inx = 1. / (double)L.nx;
+        D.iny = 1. / (double)L.ny;
         D.ox = L.ox;
         D.oy = L.oy;
         D.ht = L.ht;
