@@ -1012,34 +1012,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
                 if (h16 == 0) fp[ux * 64 + 32] = make_float2(0.f, 0.f);
                 continue;
             }
-            loadT(WV.gy, lane, b2); // stage 3a's operand
-            const float2 *lt = a.lotf_t ? a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * MX_HALF + tbase : nullptr;
-            const float2 *et = WV.etf_t ? WV.etf_t + (long)isa * MX_HALF + tbase : nullptr;
-            const float *nm = WV.nominal_t ? WV.nominal_t + tbase : nullptr;
+            // transfer functions: conj(lotf) and the ETF (LGS), or the nominal
+            // (NGS); one uniform branch, every load issued before the first use
+            if (a.lotf_t && WV.etf_t) {
+                const float2 *lt = a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * MX_HALF + tbase;
+                const float2 *et = WV.etf_t + (long)isa * MX_HALF + tbase;
 #pragma unroll
-            for (int mt = 0; mt < 2; mt++)
+                for (int mt = 0; mt < 2; mt++) {
+                    float2 tl[4], te[4];
 #pragma unroll
-                for (int v = 0; v < 4; v++) {
-                    const int o = (mt * 4 + v) * 64;
-                    float re = xr[mt][v], im = xi[mt][v];
-                    if (lt) {
-                        const float2 t = lt[o];
-                        const float r2 = re * t.x - im * t.y;
-                        im = re * t.y + im * t.x;
-                        re = r2;
+                    for (int v = 0; v < 4; v++) {
+                        tl[v] = lt[(mt * 4 + v) * 64];
+                        te[v] = et[(mt * 4 + v) * 64];
                     }
-                    if (et) {
-                        const float2 t = et[o];
-                        const float r2 = re * t.x - im * t.y;
-                        im = re * t.y + im * t.x;
-                        re = r2;
-                    } else if (nm) {
-                        re *= nm[o];
-                        im *= nm[o];
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const float2 t = make_float2(tl[v].x * te[v].x - tl[v].y * te[v].y,
+                                                     tl[v].x * te[v].y + tl[v].y * te[v].x);
+                        const float re = xr[mt][v], im = xi[mt][v];
+                        xr[mt][v] = re * t.x - im * t.y;
+                        xi[mt][v] = re * t.y + im * t.x;
                     }
-                    xr[mt][v] = re;
-                    xi[mt][v] = im;
                 }
+            } else if (WV.nominal_t) {
+                const float *nm = WV.nominal_t + tbase;
+                float tn[8];
+#pragma unroll
+                for (int i = 0; i < 8; i++) tn[i] = nm[i * 64];
+#pragma unroll
+                for (int i = 0; i < 8; i++) {
+                    xr[i / 4][i % 4] *= tn[i];
+                    xi[i / 4][i % 4] *= tn[i];
+                }
+            }
+            loadT(WV.gy, lane, b2); // stage 3a's operand
             __syncthreads();
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
