@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--variant", choices=("auto", "fft32", "mfma"), default="auto")
     ap.add_argument("--copy-calib", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel, not one graph per step")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the WFS path of this build is single-GPU "
@@ -155,6 +156,8 @@ def main():
         eng.set_variant(_lib.VARIANT_FFT32 if args.variant == "fft32" else _lib.VARIANT_MFMA_BF16)
     variant = eng.variant()
     log("imaging form: %s" % KERNELS[variant])
+    if not args.no_graph:
+        eng.set_graph(True)
 
     # matched filter from the FP64 long-exposure i0 (mtch_cell on the host)
     eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
@@ -228,7 +231,8 @@ def main():
                       "OPD pts -> 64x64 FFT, 6x6 px of 1\", 7 layers 4096^2 @1/64 m, LLT "
                       "uplink, ETF, matched filter, 800 Hz" % p.nsa,
                       "nwfs": nwfs, "nsa": p.nsa, "notf": p.notf, "npix": p.npix,
-                      "variant": {0: "fp32-radix-fft", 1: "mfma-split-bf16"}[variant]},
+                      "variant": {0: "fp32-radix-fft", 1: "mfma-split-bf16"}[variant],
+                      "launch": "kernel by kernel" if args.no_graph else "one HIP graph per step"},
            "roofline": roof}
     if other is not None:
         out["other_form"] = other

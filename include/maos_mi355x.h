@@ -171,6 +171,14 @@ int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
 int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant);
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant);
 
+/* Launch each step's kernels as one HIP graph (enable=1) or one by one
+ * (enable=0, the default).  No counterpart in the reference, whose GPU code
+ * launches kernels directly; the graph is built on the next queued step and
+ * rebuilt when the kernel set changes (imaging form, uplink present); every
+ * step updates its nodes' arguments.  Steps with timing enabled are launched
+ * one by one. */
+int mx_wfsgrad_set_graph(mx_wfsgrad_t *h, int enable);
+
 /* Wait for all queued work of the engine. */
 int mx_wfsgrad_wait(mx_wfsgrad_t *h);
 

@@ -90,6 +90,7 @@ HIP_API = {
     "mx_wfsgrad_wait": (C.c_int, [C.c_void_p]),
     "mx_wfsgrad_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_wfsgrad_get_variant": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "mx_wfsgrad_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
 }
 
 VARIANT_AUTO, VARIANT_FFT32, VARIANT_MFMA_BF16 = -1, 0, 1

@@ -1151,6 +1151,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 struct mx_wfsgrad_s {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the argument blocks of the last queued step, and the step's graph
+    OpdArgs arg_opd{};
+    KArgs arg_step{};
+    MfmaArgs arg_mfma{};
+    LltArgs arg_llt{};
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipGraphNode_t gnode[3] = {};
+    int gkey = 0;
     long nsa = 0, nxsa = 0, embfac = 0, nwvf = 0, notf = 0, npix = 0, pixpsax = 0, pixpsay = 0,
          nwvl = 0;
     int nwfs = 0;
@@ -1475,10 +1485,20 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
     return MX_OK;
 }
 
+// release the step graph of a handle (the stream holds no launch of it)
+static void drop_graph(mx_wfsgrad_t *h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+    h->gkey = 0;
+}
+
 int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (!h) return MX_OK;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
+    drop_graph(h);
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
                     h->d_lotf_t, h->d_F, h->d_mwv,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
@@ -1727,52 +1747,30 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
             std::memcpy(h->h_wv, hw, sizeof(hw));
         }
     }
-    // One stream: ray trace (the subaperture grids and the LLT grids), LLT
-    // uplink OTF, imaging.  Timing events: e2/e3 bracket the ray trace, e0/e1
-    // the LLT kernel, e4/e5 the imaging kernel.
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr, e4 = nullptr, e5 = nullptr;
-    if (h->timing) {
-        if (h->ev.size() < 6) {
-            for (auto &e : h->ev) HIPCHK(hipEventDestroy(e));
-            h->ev.assign(6, nullptr);
-            for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
-        }
-        e0 = h->ev[0];
-        e1 = h->ev[1];
-        e2 = h->ev[2];
-        e3 = h->ev[3];
-        e4 = h->ev[4];
-        e5 = h->ev[5];
-    }
+    // Argument blocks live in the handle (a graph's kernel nodes copy them).
     const bool llt = h->cfg.llt_nx > 0;
-    if (h->timing) HIPCHK(hipEventRecord(e2, h->stream));
-    {
-        OpdArgs o{};
-        o.nsa = (int)h->nsa;
-        o.nwfs = h->nwfs;
-        o.nps = g_atm.nps;
-        o.ptsdx = h->cfg.dx;
-        o.ptsx = h->d_ptsx;
-        o.ptsy = h->d_ptsy;
-        o.layers = g_atm.d_layers;
-        o.atmscale = g_atm.atmscale;
-        o.wfs = h->d_wfs;
-        o.isim = isim;
-        o.dt = dt;
-        o.opd = h->d_opd;
-        o.nllt = llt ? h->nwfs : 0;
-        o.llt_origx = h->cfg.llt_origx;
-        o.llt_origy = h->cfg.llt_origy;
-        o.llt_ox = h->cfg.llt_ox;
-        o.llt_oy = h->cfg.llt_oy;
-        o.llt_opd = h->d_llt_opd;
-        dim3 grid((unsigned)(h->nsa * h->nwfs + o.nllt));
-        if (h->nxsa == 32) hipLaunchKernelGGL((k_wfs_opd<32>), grid, dim3(MX_OPD_THREADS), 0, h->stream, o);
-        else hipLaunchKernelGGL((k_wfs_opd<16>), grid, dim3(MX_OPD_THREADS), 0, h->stream, o);
-        HIPCHK(hipGetLastError());
-    }
-    if (h->timing) HIPCHK(hipEventRecord(e3, h->stream));
-    KArgs a{};
+    OpdArgs &o = h->arg_opd;
+    o = OpdArgs{};
+    o.nsa = (int)h->nsa;
+    o.nwfs = h->nwfs;
+    o.nps = g_atm.nps;
+    o.ptsdx = h->cfg.dx;
+    o.ptsx = h->d_ptsx;
+    o.ptsy = h->d_ptsy;
+    o.layers = g_atm.d_layers;
+    o.atmscale = g_atm.atmscale;
+    o.wfs = h->d_wfs;
+    o.isim = isim;
+    o.dt = dt;
+    o.opd = h->d_opd;
+    o.nllt = llt ? h->nwfs : 0;
+    o.llt_origx = h->cfg.llt_origx;
+    o.llt_origy = h->cfg.llt_origy;
+    o.llt_ox = h->cfg.llt_ox;
+    o.llt_oy = h->cfg.llt_oy;
+    o.llt_opd = h->d_llt_opd;
+    KArgs &a = h->arg_step;
+    a = KArgs{};
     a.nsa = (int)h->nsa;
     a.npix = (int)h->npix;
     a.pixpsax = (int)h->pixpsax;
@@ -1808,7 +1806,8 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     a.grad = h->d_grad;
     a.i0acc = h->i0_enable ? h->d_i0acc : nullptr;
     a.opd = h->d_opd;
-    MfmaArgs m{};
+    MfmaArgs &m = h->arg_mfma;
+    m = MfmaArgs{};
     if (variant == MX_VARIANT_MFMA_BF16) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
@@ -1857,52 +1856,111 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         m.lotf_full = h->d_lotf;
         m.lotf_half = h->d_lotf_t;
     }
-    // uplink OTF: the MFMA kernel's uplink mode with the MFMA form, else the
-    // FP32 FFT kernel
+    LltArgs &la = h->arg_llt;
+    la = LltArgs{};
+    la.nwvl = (int)h->nwvl;
+    la.nwfs = h->nwfs;
+    la.amp = h->d_llt_amp;
+    la.opd = h->d_llt_opd;
+    la.wv = h->d_wv;
+    const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
+    la.norm = (float)(1. / (nlwvf * nlwvf));
+    la.lotf = h->d_lotf;
+    la.lotf_t = h->d_lotf_t;
+    // The step in stream order: ray trace (subaperture and LLT grids), uplink
+    // OTF (the MFMA kernel's uplink mode with the MFMA form, else the FP32 FFT
+    // kernel), imaging.
+    struct Launch {
+        const void *f;
+        dim3 grid, block;
+        void *arg;
+    };
+    Launch L[3];
+    int nl = 0;
+    const bool mf = variant == MX_VARIANT_MFMA_BF16;
+    L[nl++] = {h->nxsa == 32 ? reinterpret_cast<const void *>(&k_wfs_opd<32>)
+                             : reinterpret_cast<const void *>(&k_wfs_opd<16>),
+               dim3((unsigned)(h->nsa * h->nwfs + o.nllt)), dim3(MX_OPD_THREADS), &o};
+    const int slot_llt = llt ? nl : -1;
     if (llt) {
-        if (h->timing) HIPCHK(hipEventRecord(e0, h->stream));
-        if (variant == MX_VARIANT_MFMA_BF16) {
-            hipLaunchKernelGGL(k_shwfs_mfma<true>, dim3((unsigned)(h->nwfs * h->nwvl)), dim3(256), 0, h->stream, m);
-        } else {
-            LltArgs la{};
-            la.nwvl = (int)h->nwvl;
-            la.nwfs = h->nwfs;
-            la.amp = h->d_llt_amp;
-            la.opd = h->d_llt_opd;
-            la.wv = h->d_wv;
-            const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
-            la.norm = (float)(1. / (nlwvf * nlwvf));
-            la.lotf = h->d_lotf;
-            la.lotf_t = h->d_lotf_t;
-            dim3 grid((unsigned)(h->nwfs * h->nwvl));
-            if (N == 64) hipLaunchKernelGGL((k_llt_otf<64, 32>), grid, dim3(64), 0, h->stream, la);
-            else hipLaunchKernelGGL((k_llt_otf<32, 16>), grid, dim3(64), 0, h->stream, la);
-        }
-        HIPCHK(hipGetLastError());
-        if (h->timing) HIPCHK(hipEventRecord(e1, h->stream));
+        if (mf)
+            L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<true>), dim3((unsigned)(h->nwfs * h->nwvl)),
+                       dim3(256), &m};
+        else
+            L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_llt_otf<64, 32>)
+                               : reinterpret_cast<const void *>(&k_llt_otf<32, 16>),
+                       dim3((unsigned)(h->nwfs * h->nwvl)), dim3(64), &la};
     }
-    if (h->timing) HIPCHK(hipEventRecord(e4, h->stream));
-    if (variant == MX_VARIANT_MFMA_BF16) {
-        hipLaunchKernelGGL(k_shwfs_mfma<false>, dim3((unsigned)(h->nsa * h->nwfs)), dim3(256), 0, h->stream, m);
+    const int slot_img = nl;
+    if (mf) {
+        L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<false>), dim3((unsigned)(h->nsa * h->nwfs)),
+                   dim3(256), &m};
     } else {
-        const int spw = (int)(64 / N);
-        const long ngroup = (h->nsa + spw - 1) / spw;
-        dim3 grid((unsigned)(ngroup * h->nwfs));
-        if (N == 64) hipLaunchKernelGGL((k_shwfs_step<64, 32, 1>), grid, dim3(64), 0, h->stream, a);
-        else hipLaunchKernelGGL((k_shwfs_step<32, 16, 2>), grid, dim3(64), 0, h->stream, a);
+        const long spw = 64 / N, ngroup = (h->nsa + spw - 1) / spw;
+        L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_shwfs_step<64, 32, 1>)
+                           : reinterpret_cast<const void *>(&k_shwfs_step<32, 16, 2>),
+                   dim3((unsigned)(ngroup * h->nwfs)), dim3(64), &a};
     }
-    HIPCHK(hipGetLastError());
-    if (h->timing) {
-        HIPCHK(hipEventRecord(e5, h->stream));
-        HIPCHK(hipEventSynchronize(e5));
-        float m1 = 0, m2 = 0, m3 = 0;
-        if (llt) HIPCHK(hipEventElapsedTime(&m1, e0, e1));
-        HIPCHK(hipEventElapsedTime(&m2, e2, e3));
-        HIPCHK(hipEventElapsedTime(&m3, e4, e5));
-        h->t_llt += m1;
-        h->t_opd += m2;
-        h->t_ints += m3;
-        h->t_n++;
+    if (h->use_graph && !h->timing) {
+        // one graph launch per step: the graph is built for the kernel set and
+        // its nodes get this step's arguments
+        const int key = 1 + (variant << 4) + (llt ? 2 : 0) + (N == 64 ? 1 : 0);
+        hipKernelNodeParams kp[3];
+        void *argv[3][1];
+        for (int i = 0; i < nl; i++) {
+            argv[i][0] = L[i].arg;
+            kp[i] = hipKernelNodeParams{};
+            kp[i].func = const_cast<void *>(L[i].f);
+            kp[i].gridDim = L[i].grid;
+            kp[i].blockDim = L[i].block;
+            kp[i].sharedMemBytes = 0;
+            kp[i].kernelParams = argv[i];
+            kp[i].extra = nullptr;
+        }
+        if (!h->gexec || h->gkey != key) {
+            drop_graph(h);
+            HIPCHK(hipGraphCreate(&h->graph, 0));
+            for (int i = 0; i < nl; i++)
+                HIPCHK(hipGraphAddKernelNode(&h->gnode[i], h->graph, i ? &h->gnode[i - 1] : nullptr, i ? 1 : 0,
+                                             &kp[i]));
+            HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+            h->gkey = key;
+        } else {
+            for (int i = 0; i < nl; i++) HIPCHK(hipGraphExecKernelNodeSetParams(h->gexec, h->gnode[i], &kp[i]));
+        }
+        HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+    } else {
+        // Timing events: e2/e3 bracket the ray trace, e0/e1 the uplink OTF,
+        // e4/e5 the imaging kernel.
+        hipEvent_t ev[6] = {};
+        if (h->timing) {
+            if (h->ev.size() < 6) {
+                for (auto &e : h->ev) HIPCHK(hipEventDestroy(e));
+                h->ev.assign(6, nullptr);
+                for (auto &e : h->ev) HIPCHK(hipEventCreate(&e));
+            }
+            for (int i = 0; i < 6; i++) ev[i] = h->ev[i];
+        }
+        for (int i = 0; i < nl; i++) {
+            hipEvent_t eb = i == 0 ? ev[2] : i == slot_llt ? ev[0] : ev[4];
+            hipEvent_t ee = i == 0 ? ev[3] : i == slot_llt ? ev[1] : ev[5];
+            if (h->timing) HIPCHK(hipEventRecord(eb, h->stream));
+            void *argv[1] = {L[i].arg};
+            HIPCHK(hipLaunchKernel(L[i].f, L[i].grid, L[i].block, argv, 0, h->stream));
+            if (h->timing) HIPCHK(hipEventRecord(ee, h->stream));
+        }
+        (void)slot_img;
+        if (h->timing) {
+            HIPCHK(hipEventSynchronize(ev[5]));
+            float m1 = 0, m2 = 0, m3 = 0;
+            if (llt) HIPCHK(hipEventElapsedTime(&m1, ev[0], ev[1]));
+            HIPCHK(hipEventElapsedTime(&m2, ev[2], ev[3]));
+            HIPCHK(hipEventElapsedTime(&m3, ev[4], ev[5]));
+            h->t_llt += m1;
+            h->t_opd += m2;
+            h->t_ints += m3;
+            h->t_n++;
+        }
     }
     if (h->i0_enable) h->i0_nframe++;
     h->queued = true;
@@ -2000,6 +2058,15 @@ int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant) {
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant) {
     if (!h || !variant) return set_err(MX_EINVAL, "null argument");
     *variant = effective_variant(h);
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_graph(mx_wfsgrad_t *h, int enable) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (!enable) drop_graph(h);
+    h->use_graph = enable != 0;
     return MX_OK;
 }
 

@@ -128,6 +128,10 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_get_variant(self.h, C.byref(v)))
         return v.value
 
+    def set_graph(self, enable=True):
+        """Launch each step's kernels as one HIP graph."""
+        check(_lib.hip().mx_wfsgrad_set_graph(self.h, int(enable)))
+
     def variant_ok(self, variant):
         """True if this configuration runs `variant`; the current variant is kept."""
         cur = self.variant()
