@@ -99,6 +99,13 @@ const char *mx_last_error(void);
  * Copies geometry; tables are supplied with the calls below. */
 int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *powfs,
                     const mx_wfs_t *wfs, int nwfs);
+/* The same engine for nreal independent realisations advanced together
+ * (1 <= nreal <= 32): one MAOS run per realisation, each with its own
+ * screens (mx_atm2gpu_real), gradients, intensities, i0 and matched filter,
+ * sharing the powfs tables.  Every other call addresses WFS ireal*nwfs + iwfs.
+ * No counterpart in the reference, which runs one realisation per process. */
+int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *powfs,
+                          const mx_wfs_t *wfs, int nwfs, int nreal);
 int mx_wfsgrad_free(mx_wfsgrad_t *h);
 
 /* Detector transfer tables of one wavelength: powfs.dtf[iwvl].nominal (real
@@ -129,6 +136,11 @@ int mx_wfsgrad_update_ref(mx_wfsgrad_t *h, int iwfs, const double *mtche,
  * at maos/sim.c:79).  Values are stored as FP32; geometry stays FP64.
  * atmscale multiplies every layer (simu->atmscale). */
 int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale);
+/* The screens of realisation ireal (0 <= ireal < 32) for batched engines;
+ * mx_atm2gpu uploads realisation 0.  All realisations of an engine must have
+ * the same number of layers. */
+int mx_atm2gpu_real(int ireal, const mx_atm_layer_t *layers, int nps,
+                    double atmscale);
 
 /* Enqueue step isim for every WFS of the engine (gpu_wfsgrad_queue,
  * cuda/sim/wfsgrad.cu:483): ray trace, LLT uplink, imaging, gradients and

@@ -73,12 +73,15 @@ HIP_API = {
     "mx_init": (C.c_int, [C.c_int]),
     "mx_last_error": (C.c_char_p, []),
     "mx_wfsgrad_init": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(MxPowfs), C.POINTER(MxWfs), C.c_int]),
+    "mx_wfsgrad_init_batch": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(MxPowfs), C.POINTER(MxWfs), C.c_int,
+                                        C.c_int]),
     "mx_wfsgrad_free": (C.c_int, [C.c_void_p]),
     "mx_wfsgrad_set_dtf": (C.c_int, [C.c_void_p, C.c_int, c_double_p, C.c_long, C.c_long,
                                      c_long_p, c_long_p, c_double_p]),
     "mx_wfsgrad_update_etf": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, C.c_double]),
     "mx_wfsgrad_update_ref": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_atm2gpu": (C.c_int, [C.POINTER(MxAtmLayer), C.c_int, C.c_double]),
+    "mx_atm2gpu_real": (C.c_int, [C.c_int, C.POINTER(MxAtmLayer), C.c_int, C.c_double]),
     "mx_wfsgrad_queue": (C.c_int, [C.c_void_p, C.c_long, C.c_double]),
     "mx_wfsgrad_sync": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_wfsgrad_i0_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),

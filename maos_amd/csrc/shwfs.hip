@@ -25,6 +25,7 @@
 #include "twiddles.h"
 
 #define MX_MAXLAYER 16
+#define MX_MAXREAL 32 // realisations one engine can advance together
 #define MX_MAXWVL 4
 
 namespace {
@@ -130,6 +131,7 @@ struct DevLayer {
 struct DevWfs {
     double thetax, thetay, hs, misregx, misregy;
     float alpha[MX_MAXWVL]; // wvlwts[iwvl]*norm_ints
+    int ireal;              // realisation (atmosphere) of this WFS
 };
 
 // Per-wavelength tables; transfer functions are stored transposed
@@ -143,16 +145,10 @@ struct DevWvl {
 };
 
 struct KArgs {
-    int nsa, npix, pixpsax, pixpsay, nwvl, nwfs, nps;
-    double ptsdx;
-    const double *ptsx, *ptsy;
+    int nsa, npix, pixpsax, pixpsay, nwvl, nwfs;
     const float *amp;
     const DevWvl *wv;   // [nwvl]
-    const DevLayer *layers;
-    double atmscale;
     const DevWfs *wfs;
-    long isim;
-    double dt;
     float etf1wt, etf2wt;
     const float2 *lotf; // [nwfs][nwvl][N*N] conj(lotfc), or null
     // pixel processing
@@ -173,8 +169,8 @@ struct OpdArgs {
     int nsa, nwfs, nps;
     double ptsdx;
     const double *ptsx, *ptsy;
-    const DevLayer *layers;
-    double atmscale;
+    const DevLayer *layers;  // [MX_MAXREAL][MX_MAXLAYER]
+    const double *atmscale;  // [MX_MAXREAL]
     const DevWfs *wfs;
     long isim;
     double dt;
@@ -585,16 +581,18 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
         // shift o - v dt isim.  These blocks come first so that they start
         // early.
         const DevWfs &W = a.wfs[b];
-        ray_trace<MX_OPD_THREADS, NXSA>(a.layers, a.nps, W.hs, W.thetax - a.llt_ox / W.hs, W.thetay - a.llt_oy / W.hs, 0.,
-                            0., a.llt_ox, a.llt_oy, a.isim, a.dt, a.atmscale, a.ptsdx, a.llt_origx, a.llt_origy,
+        ray_trace<MX_OPD_THREADS, NXSA>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax - a.llt_ox / W.hs,
+                                        W.thetay - a.llt_oy / W.hs, 0., 0., a.llt_ox, a.llt_oy, a.isim, a.dt,
+                                        a.atmscale[W.ireal], a.ptsdx, a.llt_origx, a.llt_origy,
                             threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile);
         return;
     }
     const int unit = xcd_unit(b - a.nllt, a.nsa * a.nwfs);
     const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
     const DevWfs &W = a.wfs[iwfs];
-    ray_trace<MX_OPD_THREADS, NXSA>(a.layers, a.nps, W.hs, W.thetax, W.thetay, W.misregx, W.misregy, 0., 0.,
-                        a.isim, a.dt, a.atmscale, a.ptsdx, a.ptsx[isa], a.ptsy[isa], threadIdx.x,
+    ray_trace<MX_OPD_THREADS, NXSA>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax, W.thetay, W.misregx,
+                                    W.misregy, 0., 0., a.isim, a.dt, a.atmscale[W.ireal], a.ptsdx, a.ptsx[isa],
+                                    a.ptsy[isa], threadIdx.x,
                         a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac, tile);
 }
 
@@ -1150,6 +1148,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 // ======================================================================
 struct mx_wfsgrad_s {
     int device = 0;
+    int nreal = 1; // realisations; WFS r * (nwfs / nreal) + i is realisation r's WFS i
     hipStream_t stream = nullptr;
     // the argument blocks of the last queued step, and the step's graph
     OpdArgs arg_opd{};
@@ -1213,15 +1212,17 @@ struct mx_wfsgrad_s {
 };
 
 namespace {
+// screens of one realisation
 struct AtmState {
-    int nps = 0;
+    int nps = -1; // -1: not uploaded
     double atmscale = 1;
     DevLayer h_layers[MX_MAXLAYER];
-    DevLayer *d_layers = nullptr;
     float *d_data = nullptr;
     size_t bytes = 0;
 };
-AtmState g_atm;
+AtmState g_atm[MX_MAXREAL];
+DevLayer *g_dlayers = nullptr; // device [MX_MAXREAL][MX_MAXLAYER]
+double *g_datmscale = nullptr; // device [MX_MAXREAL]
 int g_device = 0;
 
 template <typename T>
@@ -1372,8 +1373,18 @@ int mx_init(int device) {
 }
 
 int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs, int nwfs) {
-    if (!out || !p || !wfs || nwfs <= 0) return set_err(MX_EINVAL, "mx_wfsgrad_init: null argument");
+    return mx_wfsgrad_init_batch(out, p, wfs, nwfs, 1);
+}
+
+int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs0, int nwfs0, int nreal) {
+    if (!out || !p || !wfs0 || nwfs0 <= 0) return set_err(MX_EINVAL, "mx_wfsgrad_init: null argument");
     *out = nullptr;
+    if (nreal < 1 || nreal > MX_MAXREAL) return set_err(MX_EINVAL, "nreal=%d out of range (1..%d)", nreal, MX_MAXREAL);
+    // realisation r's copy of WFS i is WFS r * nwfs0 + i of the engine
+    const int nwfs = nwfs0 * nreal;
+    std::vector<mx_wfs_t> wfs_all(nwfs);
+    for (int i = 0; i < nwfs; i++) wfs_all[i] = wfs0[i % nwfs0];
+    const mx_wfs_t *wfs = wfs_all.data();
     if (!supported(p->notf, p->nxsa, p->embfac))
         return set_err(MX_EINVAL, "unsupported size: notf=%ld nxsa=%ld embfac=%ld (supported: 64/32/2, 32/16/2)",
                        p->notf, p->nxsa, p->embfac);
@@ -1402,6 +1413,7 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
     h->npix = npix;
     h->nwvl = p->nwvl;
     h->nwfs = nwfs;
+    h->nreal = nreal;
     h->wvl.assign(p->wvl, p->wvl + p->nwvl);
     h->wfs_cfg.assign(wfs, wfs + nwfs);
     int rc = 0;
@@ -1465,6 +1477,7 @@ int mx_wfsgrad_init(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_t *wfs
         dw[i].hs = std::isinf(wfs[i].hs) ? 0. : wfs[i].hs;
         dw[i].misregx = wfs[i].misregx;
         dw[i].misregy = wfs[i].misregy;
+        dw[i].ireal = i / nwfs0;
         const double norm_ints = wfs[i].sigsim * norm_psf * norm_psf / ((double)N * N);
         for (int w = 0; w < MX_MAXWVL; w++)
             dw[i].alpha[w] = w < h->nwvl ? (float)(wfs[i].wvlwts[w] * norm_ints) : 0.f;
@@ -1664,24 +1677,30 @@ int mx_wfsgrad_update_ref(mx_wfsgrad_t *h, int iwfs, const double *mtche, const 
     return MX_OK;
 }
 
-int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
+int mx_atm2gpu_real(int ireal, const mx_atm_layer_t *layers, int nps, double atmscale) {
+    if (ireal < 0 || ireal >= MX_MAXREAL) return set_err(MX_EINVAL, "mx_atm2gpu_real: ireal=%d", ireal);
     if (nps < 0 || nps > MX_MAXLAYER || (nps && !layers)) return set_err(MX_EINVAL, "mx_atm2gpu: nps=%d", nps);
     HIPCHK(hipSetDevice(g_device));
+    AtmState &A = g_atm[ireal];
     size_t total = 0;
     for (int i = 0; i < nps; i++) {
         if (layers[i].nx < 2 || layers[i].ny < 2 || !layers[i].p || layers[i].dx <= 0 || layers[i].dy <= 0)
             return set_err(MX_EINVAL, "layer %d is invalid", i);
         total += (size_t)layers[i].nx * layers[i].ny;
     }
-    if (g_atm.d_data && g_atm.bytes < total * sizeof(float)) {
-        hipFree(g_atm.d_data);
-        g_atm.d_data = nullptr;
+    if (A.d_data && A.bytes < total * sizeof(float)) {
+        HIPCHK(hipDeviceSynchronize()); // no queued step reads the old screens any more
+        HIPCHK(hipFree(A.d_data));
+        A.d_data = nullptr;
     }
-    if (!g_atm.d_data && total) {
-        HIPCHK(hipMalloc(&g_atm.d_data, total * sizeof(float)));
-        g_atm.bytes = total * sizeof(float);
+    if (!A.d_data && total) {
+        HIPCHK(hipMalloc(&A.d_data, total * sizeof(float)));
+        A.bytes = total * sizeof(float);
     }
-    if (!g_atm.d_layers) HIPCHK(hipMalloc(&g_atm.d_layers, sizeof(DevLayer) * MX_MAXLAYER));
+    if (!g_dlayers) {
+        HIPCHK(hipMalloc(&g_dlayers, sizeof(DevLayer) * MX_MAXREAL * MX_MAXLAYER));
+        HIPCHK(hipMalloc(&g_datmscale, sizeof(double) * MX_MAXREAL));
+    }
     size_t off = 0;
     std::vector<float> tmp;
     for (int i = 0; i < nps; i++) {
@@ -1689,9 +1708,9 @@ int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
         size_t n = (size_t)L.nx * L.ny;
         tmp.resize(n);
         for (size_t k = 0; k < n; k++) tmp[k] = (float)L.p[k];
-        HIPCHK(hipMemcpy(g_atm.d_data + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
-        DevLayer &D = g_atm.h_layers[i];
-        D.p = g_atm.d_data + off;
+        HIPCHK(hipMemcpy(A.d_data + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        DevLayer &D = A.h_layers[i];
+        D.p = A.d_data + off;
         D.nx = (int)L.nx;
         D.ny = (int)L.ny;
         D.dx1 = 1. / L.dx;
@@ -1705,23 +1724,35 @@ int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
         D.vy = L.vy;
         off += n;
     }
-    HIPCHK(hipMemcpy(g_atm.d_layers, g_atm.h_layers, sizeof(DevLayer) * MX_MAXLAYER, hipMemcpyHostToDevice));
-    g_atm.nps = nps;
-    g_atm.atmscale = atmscale;
+    HIPCHK(hipMemcpy(g_dlayers + (size_t)ireal * MX_MAXLAYER, A.h_layers, sizeof(DevLayer) * MX_MAXLAYER,
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(g_datmscale + ireal, &atmscale, sizeof(double), hipMemcpyHostToDevice));
+    A.nps = nps;
+    A.atmscale = atmscale;
     return MX_OK;
+}
+
+int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
+    return mx_atm2gpu_real(0, layers, nps, atmscale);
 }
 
 int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     if (!h) return set_err(MX_EINVAL, "mx_wfsgrad_queue: null handle");
-    if (!g_atm.d_layers) return set_err(MX_ESTATE, "no screens uploaded (mx_atm2gpu)");
+    for (int r = 0; r < h->nreal; r++) {
+        if (g_atm[r].nps < 0) return set_err(MX_ESTATE, "no screens uploaded for realisation %d (mx_atm2gpu)", r);
+        if (g_atm[r].nps != g_atm[0].nps)
+            return set_err(MX_EINVAL, "realisation %d has %d layers, realisation 0 %d", r, g_atm[r].nps, g_atm[0].nps);
+    }
     for (int w = 0; w < h->nwvl; w++)
         if (!h->dtf_set[w]) return set_err(MX_ESTATE, "dtf of wavelength %d not set", w);
     if (h->cfg.llt_nx > 0)
         for (int w = 0; w < h->nwvl; w++)
             if (!h->d_etf[w]) return set_err(MX_ESTATE, "etf of wavelength %d not set", w);
-    for (int l = 0; l < g_atm.nps; l++) // the staged screen tile assumes a grid step <= 1 pixel
-        if (h->cfg.dx * g_atm.h_layers[l].dx1 > 1. + 1e-9 || h->cfg.dx * g_atm.h_layers[l].dy1 > 1. + 1e-9)
-            return set_err(MX_EINVAL, "layer %d is sampled more coarsely than the OPD grid (dx=%g)", l, h->cfg.dx);
+    for (int r = 0; r < h->nreal; r++) // the staged screen tile assumes a grid step <= 1 pixel
+        for (int l = 0; l < g_atm[r].nps; l++)
+            if (h->cfg.dx * g_atm[r].h_layers[l].dx1 > 1. + 1e-9 || h->cfg.dx * g_atm[r].h_layers[l].dy1 > 1. + 1e-9)
+                return set_err(MX_EINVAL, "realisation %d layer %d is sampled more coarsely than the OPD grid (dx=%g)",
+                               r, l, h->cfg.dx);
     if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche) return set_err(MX_ESTATE, "matched filter not set");
     if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum)
         return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=1)");
@@ -1753,12 +1784,12 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     o = OpdArgs{};
     o.nsa = (int)h->nsa;
     o.nwfs = h->nwfs;
-    o.nps = g_atm.nps;
+    o.nps = g_atm[0].nps;
     o.ptsdx = h->cfg.dx;
     o.ptsx = h->d_ptsx;
     o.ptsy = h->d_ptsy;
-    o.layers = g_atm.d_layers;
-    o.atmscale = g_atm.atmscale;
+    o.layers = g_dlayers;
+    o.atmscale = g_datmscale;
     o.wfs = h->d_wfs;
     o.isim = isim;
     o.dt = dt;
@@ -1777,19 +1808,11 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     a.pixpsay = (int)h->pixpsay;
     a.nwvl = (int)h->nwvl;
     a.nwfs = h->nwfs;
-    a.nps = g_atm.nps;
-    a.ptsdx = h->cfg.dx;
-    a.ptsx = h->d_ptsx;
-    a.ptsy = h->d_ptsy;
     a.amp = h->d_amp;
     a.wv = h->d_wv;
     a.etf2wt = h->etf2wt;
     a.etf1wt = 1.f - h->etf2wt;
-    a.layers = g_atm.d_layers;
-    a.atmscale = g_atm.atmscale;
     a.wfs = h->d_wfs;
-    a.isim = isim;
-    a.dt = dt;
     a.lotf = h->cfg.llt_nx > 0 ? h->d_lotf : nullptr;
     a.phytype = h->cfg.phytype;
     a.sigmatch = h->cfg.sigmatch;

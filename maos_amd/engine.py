@@ -32,8 +32,8 @@ def init(device=0):
     _initialised = True
 
 
-def atm2gpu(layers, atmscale=1.0):
-    """Upload screens (list of powfs.Layer)."""
+def atm2gpu(layers, atmscale=1.0, ireal=0):
+    """Upload screens (list of powfs.Layer) of realisation ireal."""
     if not _initialised:
         init(0)
     arr = (MxAtmLayer * max(1, len(layers)))()
@@ -42,15 +42,20 @@ def atm2gpu(layers, atmscale=1.0):
         p = np.ascontiguousarray(L.p, dtype=np.float64)
         keep.append(p)
         arr[i] = MxAtmLayer(dptr(p), L.nx, L.ny, L.dx, L.dx, L.ox, L.oy, L.ht, L.vx, L.vy)
-    check(_lib.hip().mx_atm2gpu(arr, len(layers), atmscale))
+    check(_lib.hip().mx_atm2gpu_real(ireal, arr, len(layers), atmscale))
 
 
 class WfsGrad:
-    def __init__(self, powfs, wfs_list):
+    """One powfs and its WFS.  With nreal > 1 the engine advances nreal
+    realisations together; WFS ireal * len(wfs_list) + i is realisation
+    ireal's copy of wfs_list[i] and uses the screens of atm2gpu(..., ireal)."""
+
+    def __init__(self, powfs, wfs_list, nreal=1):
         if not _initialised:
             init(0)
         self.powfs = powfs
-        self.nwfs = len(wfs_list)
+        self.nreal = nreal
+        self.nwfs = len(wfs_list) * nreal
         self.nsa, self.npix = powfs.nsa, powfs.npix
         p = powfs
         self._keep = [p.ptsx, p.ptsy, p.amp, p.saa, p.wvl]
@@ -66,12 +71,12 @@ class WfsGrad:
             llt_origy=getattr(p, "llt_origin", 0.0),
             llt_ox=p.llt.ox if p.llt else 0.0, llt_oy=p.llt.oy if p.llt else 0.0,
             llt_amp=dptr(llt_amp))
-        wa = (MxWfs * self.nwfs)()
+        wa = (MxWfs * len(wfs_list))()
         for i, w in enumerate(wfs_list):
             self._keep.append(w.wvlwts)
             wa[i] = MxWfs(w.thetax, w.thetay, w.hs, w.sigsim, w.misregx, w.misregy, dptr(w.wvlwts))
         h = C.c_void_p()
-        check(_lib.hip().mx_wfsgrad_init(C.byref(h), C.byref(cfg), wa, self.nwfs))
+        check(_lib.hip().mx_wfsgrad_init_batch(C.byref(h), C.byref(cfg), wa, len(wfs_list), nreal))
         self.h = h
         for iw in range(p.nwvl):
             check(_lib.hip().mx_wfsgrad_set_dtf(

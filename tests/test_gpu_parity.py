@@ -200,6 +200,50 @@ def test_graph_matches_direct(lgs_small, variant):
     graph.close()
 
 
+def test_batch_realisations(lgs_small):
+    """Two realisations in one engine (one launch per kernel per step):
+    realisation 0 is bit-identical to a one-realisation engine on the same
+    screens, realisation 1 (other seeds and r0) matches the oracle on its own
+    screens."""
+    import dataclasses
+    from maos_amd import configs, engine
+    from maos_amd.powfs import genatm
+    cfg, p, layers0 = lgs_small
+    atm1 = dataclasses.replace(cfg["atm"], r0=0.14, seeds=[s + 100 for s in cfg["atm"].seeds])
+    layers1 = genatm(atm1)
+    print("case: batch of 2 realisations, lgs order20", flush=True)
+    engine.init(0)
+    engine.atm2gpu(layers0, ireal=0)
+    engine.atm2gpu(layers1, ireal=1)
+    wfs = configs.wfs_specs(cfg, p)
+    nw = len(wfs)
+    batch = engine.WfsGrad(p, wfs, nreal=2)
+    single = engine.WfsGrad(p, wfs)
+    o0, o1 = _oracle(p, layers0), _oracle(p, layers1)
+    dt = cfg["sim"].dt
+    mtche, i0sum = _lgs_ref(p, o0, wfs, dt)
+    for eng in (batch, single):
+        eng.update_ref(mtche, i0sum)
+    o1.set_ref(mtche, i0sum)
+    isim = 17
+    batch.queue(isim, dt)
+    single.queue(isim, dt)
+    for iwfs in range(nw):
+        gb, ib = batch.sync(iwfs)
+        gs, is_ = single.sync(iwfs)
+        assert np.array_equal(gb, gs) and np.array_equal(ib, is_)
+        g1, i1 = batch.sync(nw + iwfs)
+        ri, rg, _, _ = o1.step(wfs[iwfs], isim, dt)
+        e_int = rel_l2(i1, ri)
+        e_g = np.abs(g1 - rg).max() * RAD2MAS
+        print("real 1 wfs=%d ints=%.2e grad=%.3e mas" % (iwfs, e_int, e_g))
+        assert e_int <= INTS_TOL
+        assert e_g <= GRAD_TOL_MAS
+        assert rel_l2(i1, ib) > 1e-2  # the realisations differ
+    batch.close()
+    single.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C
