@@ -117,6 +117,105 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def batch_main(args):
+    """nreal realisations of the lgs30m step in one engine: aggregate WFS
+    frames/s, per-step latency p50/p99, and the error against the FP64 oracle
+    on two realisations (the first and the last)."""
+    import dataclasses
+    from maos_amd import configs, engine
+    from maos_amd.powfs import Powfs, genatm
+
+    t0 = time.time()
+    cfg = configs.lgs30m(nx=args.nx, order=args.order)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    wfs = configs.wfs_specs(cfg, p)
+    nw, nreal = len(wfs), args.nreal
+    dt = cfg["sim"].dt
+    nps = len(cfg["atm"].ht)
+    r0s = np.random.RandomState(2025).uniform(0.12, 0.25, nreal)
+    engine.init(0)
+    kept = {}
+    for k in range(nreal):
+        # realisation k: its own r0 and layer seeds 7k+1 .. 7k+7
+        atm = dataclasses.replace(cfg["atm"], r0=float(r0s[k]), seeds=[nps * k + j + 1 for j in range(nps)])
+        layers = genatm(atm)
+        engine.atm2gpu(layers, ireal=k)
+        if k in (0, nreal - 1):
+            kept[k] = layers
+        log("realisation %d: r0=%.3f m (%.1fs)" % (k, r0s[k], time.time() - t0))
+    eng = engine.WfsGrad(p, wfs, nreal=nreal)
+    if not args.no_graph:
+        eng.set_graph(True)
+    eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    eng.i0_control(enable=True, reset=True)
+    for isim in range(args.i0_steps):
+        eng.queue(isim, dt)
+    eng.wait()
+    refs = {}
+    for iw in range(eng.nwfs):
+        i0, n = eng.i0_get(iw)
+        mtche, i0sum, _ = p.mtch(i0 / n)
+        eng.update_ref(mtche, i0sum, iwfs=iw)
+        refs[iw] = (mtche, i0sum)
+    eng.i0_control(enable=True, reset=True)
+    log("matched filters of %d WFS (%.1fs)" % (eng.nwfs, time.time() - t0))
+    isim = args.i0_steps
+    for _ in range(args.warmup):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    sec = time.perf_counter() - t1
+    fps = eng.nwfs * args.steps / sec
+    # latency of one step on an idle engine
+    lat = []
+    for _ in range(args.latency_steps):
+        ta = time.perf_counter()
+        eng.queue(isim, dt)
+        eng.wait()
+        lat.append((time.perf_counter() - ta) * 1e3)
+        isim += 1
+    lat = np.array(lat)
+    eng.timing_enable(True)
+    for _ in range(20):
+        eng.queue(isim, dt)
+        isim += 1
+    nl, ms_llt, ms_opd, ms_ints = eng.timing()
+    eng.timing_enable(False)
+    out = {"metric": METRIC, "value": round(fps, 2), "unit": "WFS frames/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+           "data": "synthetic (%d seeded von Karman atmospheres, Gaussian sodium profile)" % nreal,
+           "config": {"workload": "lgs30m x %d realisations in one engine (r0 ~ U[0.12, 0.25] m, layer "
+                                  "seeds 7k+1..7k+7), %d WFS x 2820 subapertures per step" % (nreal, eng.nwfs),
+                      "nreal": nreal, "nwfs": eng.nwfs, "nsa": p.nsa,
+                      "launch": "kernel by kernel" if args.no_graph else "one HIP graph per step"},
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                          "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
+           "kernels_us": {"raytrace": round(ms_opd / nl * 1e3, 1), "uplink": round(ms_llt / nl * 1e3, 1),
+                          "imaging": round(ms_ints / nl * 1e3, 1)}}
+    # error against the FP64 oracle on the first and the last realisation
+    import oracle
+    errs = []
+    eng.queue(isim, dt)
+    for k, layers in kept.items():
+        o = oracle.Oracle(p)
+        o.atm(layers)
+        iw = k * nw
+        o.set_ref(*refs[iw])
+        g, ints = eng.sync(iw)
+        ri, rg, _, _ = o.step(wfs[0], isim, dt)
+        errs.append({"realisation": k, "wfs": 0, "ints_rel_l2": float(np.linalg.norm(ints - ri) / np.linalg.norm(ri)),
+                     "grad_max_abs_mas": float(np.abs(g - rg).max() * 180 / math.pi * 3600e3)})
+    out["parity"] = errs
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,11 +229,17 @@ def main():
     ap.add_argument("--variant", choices=("auto", "fft32", "mfma"), default="auto")
     ap.add_argument("--copy-calib", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel, not one graph per step")
+    ap.add_argument("--nreal", type=int, default=1,
+                    help="advance this many independent realisations together (r0 ~ U[0.12, 0.25] m)")
+    ap.add_argument("--latency-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the WFS path of this build is single-GPU "
               "(see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.nreal > 1:
+        batch_main(args)
+        return
 
     from maos_amd import _lib, configs, engine
     from maos_amd.powfs import Powfs, genatm
