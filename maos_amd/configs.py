@@ -12,6 +12,13 @@ lgs30m: 30 m pupil (3.6 m obstruction), six LGS WFS at 90 km on a 35"
         [0,0.5,1,2,4,8,16] km with Cn2 [.45,.15,.10,.10,.08,.07,.05],
         r0=0.186 m, L0=30 m, 4096^2 at 1/64 m, seeds 1-7, winds U[5,20] m/s
         in seeded directions, dt=1/800 s, 500 steps.
+ngs_lo: three low-order NGS WFS on the 30 m pupil, 2x2 subapertures of 15 m,
+        dx=1/16 m (240x240 points), embfac 2 (480^2 field) padded to a 512^2
+        OTF, 1.25 and 1.65 um, 16x16 pixels of 8 mas, tCoG; the lgs30m
+        atmosphere.
+ngs_tt: one tip-tilt NGS WFS, 1x1 subaperture of 30 m, dx=1/16 m (480x480
+        points, 960^2 field) padded to a 1024^2 OTF, 2.2 um, 16x16 pixels of
+        8 mas, tCoG; the lgs30m atmosphere.
 """
 import math
 
@@ -55,6 +62,32 @@ def lgs30m(nx=4096, nwfs=6, order=60):
     wfs = [dict(thetax=LGS_THETAX[i] * AS2RAD, thetay=LGS_THETAY[i] * AS2RAD)
            for i in range(nwfs)]
     return dict(name="lgs30m", aper=aper, powfs=pcfg, sim=sim, atm=atm, wfs=wfs, nstep=500)
+
+
+# NGS asterism of the low-order WFS (arcsec): three stars on a 60" ring
+NGS_THETAX = [60.0, -30.0, -30.0]
+NGS_THETAY = [0.0, 51.9615, -51.9615]
+
+
+def ngs_lo(nx=4096, nwfs=3):
+    base = lgs30m(nx=nx)
+    pcfg = PowfsCfg(order=2, dsa=15.0, dx=1.0 / 16, saat=0.3, hs=math.inf, embfac=2,
+                    wvl=[1.25e-6, 1.65e-6], wvlwts=[0.5, 0.5], pixpsa=16, pixtheta=0.008,
+                    pixblur=0.3, notf=512, phytype=PTYPE_COG, sigmatch=1, siglev=2000.0)
+    wfs = [dict(thetax=NGS_THETAX[i] * AS2RAD, thetay=NGS_THETAY[i] * AS2RAD)
+           for i in range(nwfs)]
+    return dict(name="ngs_lo", aper=AperCfg(d=30.0, din=3.6), powfs=pcfg, sim=base["sim"],
+                atm=base["atm"], wfs=wfs, nstep=1000)
+
+
+def ngs_tt(nx=4096):
+    base = lgs30m(nx=nx)
+    pcfg = PowfsCfg(order=1, dsa=30.0, dx=1.0 / 16, saat=0.3, hs=math.inf, embfac=2,
+                    wvl=[2.2e-6], wvlwts=[1.0], pixpsa=16, pixtheta=0.008, pixblur=0.3,
+                    notf=1024, phytype=PTYPE_COG, sigmatch=1, siglev=5000.0)
+    wfs = [dict(thetax=20.0 * AS2RAD, thetay=-15.0 * AS2RAD)]
+    return dict(name="ngs_tt", aper=AperCfg(d=30.0, din=3.6), powfs=pcfg, sim=base["sim"],
+                atm=base["atm"], wfs=wfs, nstep=1000)
 
 
 def wfs_specs(cfg, powfs):

@@ -180,6 +180,8 @@ struct OpdArgs {
     int nllt;
     double llt_origx, llt_origy, llt_ox, llt_oy;
     float *llt_opd; // [nwfs][nxsa^2]
+    int tiles;      // a subaperture grid is tiles x tiles blocks of the kernel's NXSA
+    int direct;     // some screen is finer than the grid: gather without the LDS tile
 };
 
 // Transfer-function tables of the MFMA imaging kernel (N = 64) cover the half
@@ -262,13 +264,16 @@ __device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i,
 // staged in LDS with coalesced loads, and the bilinear interpolation runs
 // from LDS.  A tile that crosses the screen edge wraps periodically; the
 // common case that does not (a uniform test) skips the wrap arithmetic.
+// direct = true (a grid coarser than some screen, whose footprint exceeds
+// the tile) reads the four neighbours of each point from the screen itself.
 template <int N, int NXSA>
 __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, double hs,
                                           double thetax, double thetay, double misregx,
                                           double misregy, double shift0x, double shift0y,
                                           long isim, double dt, double atmscale, double ptsdx,
                                           double origx, double origy, int line, float *opd,
-                                          int *cidx, float *cfrac, float *tile) {
+                                          int *cidx, float *cfrac, float *tile, bool direct,
+                                          int ostride = NXSA) {
     static_assert(2 * NXSA <= N, "coordinate lanes");
     constexpr int PPT = NXSA * NXSA / N;
     constexpr int TW = NXSA + 2;
@@ -293,8 +298,24 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
             cfrac[line] = ff;
         }
         __syncthreads();
-        const int tx0 = cidx[0], ty0 = cidx[NXSA];
         const int nx = L.nx, ny = L.ny;
+        if (direct) { // uniform
+            if (act)
+#pragma unroll
+                for (int j = 0; j < PPT; j++) {
+                    const int pt = line + j * N;
+                    const int ix = pt % NXSA, iy = pt / NXSA;
+                    const int c0 = cidx[ix], r0i = cidx[NXSA + iy];
+                    const int c1 = c0 + 1 == nx ? 0 : c0 + 1, r1i = r0i + 1 == ny ? 0 : r0i + 1;
+                    const float *p0 = L.p + (long)r0i * nx, *p1 = L.p + (long)r1i * nx;
+                    const float fx = cfrac[ix], fy = cfrac[NXSA + iy];
+                    const float r0 = fmaf(fx, p0[c1] - p0[c0], p0[c0]);
+                    const float r1 = fmaf(fx, p1[c1] - p1[c0], p1[c0]);
+                    acc[j] += fmaf(fy, r1 - r0, r0);
+                }
+            continue;
+        }
+        const int tx0 = cidx[0], ty0 = cidx[NXSA];
         const bool wrap = tx0 > nx - TW || ty0 > ny - TW; // uniform
         if (act) {
             if (!wrap) {
@@ -347,7 +368,10 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
     }
     if (act)
 #pragma unroll
-        for (int j = 0; j < PPT; j++) opd[line + j * N] = (float)atmscale * acc[j];
+        for (int j = 0; j < PPT; j++) {
+            const int pt = line + j * N;
+            opd[(pt / NXSA) * ostride + pt % NXSA] = (float)atmscale * acc[j];
+        }
 }
 
 // XCD-aware unit index: consecutive units (the WFS of one subaperture
@@ -584,16 +608,23 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
         ray_trace<MX_OPD_THREADS, NXSA>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax - a.llt_ox / W.hs,
                                         W.thetay - a.llt_oy / W.hs, 0., 0., a.llt_ox, a.llt_oy, a.isim, a.dt,
                                         a.atmscale[W.ireal], a.ptsdx, a.llt_origx, a.llt_origy,
-                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile);
+                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile, a.direct != 0);
         return;
     }
-    const int unit = xcd_unit(b - a.nllt, a.nsa * a.nwfs);
-    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    // unit: (WFS, subaperture, tile); a subaperture grid of side NXSA * tiles
+    // is traced as tiles x tiles blocks of NXSA x NXSA points
+    const int T2 = a.tiles * a.tiles;
+    const int unit = xcd_unit(b - a.nllt, a.nsa * a.nwfs * T2);
+    const int iwfs = unit % a.nwfs, rest = unit / a.nwfs;
+    const int isa = rest / T2, it = rest % T2, tx = it % a.tiles, ty = it / a.tiles;
     const DevWfs &W = a.wfs[iwfs];
+    const int side = NXSA * a.tiles;
     ray_trace<MX_OPD_THREADS, NXSA>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax, W.thetay, W.misregx,
-                                    W.misregy, 0., 0., a.isim, a.dt, a.atmscale[W.ireal], a.ptsdx, a.ptsx[isa],
-                                    a.ptsy[isa], threadIdx.x,
-                        a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac, tile);
+                                    W.misregy, 0., 0., a.isim, a.dt, a.atmscale[W.ireal], a.ptsdx,
+                                    a.ptsx[isa] + a.ptsdx * (double)(NXSA * tx), a.ptsy[isa] + a.ptsdx * (double)(NXSA * ty),
+                                    threadIdx.x,
+                                    a.opd + ((long)iwfs * a.nsa + isa) * side * side + (long)ty * NXSA * side + tx * NXSA,
+                                    cidx, cfrac, tile, a.direct != 0, side);
 }
 
 // ------------------------------------------------------------ uplink OTF
@@ -1141,11 +1172,297 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         }
     }
 }
+
+// ------------------------------------------------------------ large fields
+// Fields larger than a workgroup holds (nwvf, notf up to 1024, lengths whose
+// prime factors are 2, 3 and 5; the 2x2 and 1x1 NGS of 15 and 30 m, or the
+// 128^2 padding of an LGS subaperture).  Each (WFS, subaperture, wavelength)
+// chain c = (iwfs nsa + isa) nwvl + iwvl has two global buffers, B1 (nwvf^2
+// complex) and B2 (notf^2 complex), and an image (notf^2 real).  The 2-D
+// transforms are batched 1-D line transforms, one workgroup per line with
+// the line in LDS, in five fused passes:
+//   1 embed amp exp(+i 2 pi opd/wvl) (cembed_wvf) and FFT the rows of B1
+//   2 FFT the columns of B1, |E|^2, corner copy into B2 (ccpcorner,
+//     math/matcomp.c:176-216; zero elsewhere)
+//   3 FFT the rows of B2
+//   4 FFT the columns of B2, x nominal DTF, inverse FFT the columns
+//   5 inverse FFT the rows of B2, real part -> image
+// then k_lf_sample applies si and the pixel processing.
+constexpr int MX_LF_MAXN = 1024;
+constexpr int MX_LF_MAXRAD = 12;
+constexpr int MX_LF_THREADS = 256;
+struct LfPlan {
+    int n, nrad;
+    int rad[MX_LF_MAXRAD]; // radices (4, 2, 3, 5), product n
+};
+struct LfArgs {
+    LfPlan pn, pm; // plans of nwvf and notf
+    int nwvf, notf, nxsa, nsa, nwfs, nwvl;
+    const float *opd, *amp;
+    const DevWvl *wv;
+    float2 *b1, *b2;
+    float *img;
+};
+
+// out[k] = sum_r v[r] exp(t 2 pi i r k / R), t = +-1, R in {2, 3, 4, 5}
+__device__ __forceinline__ float2 mul_ti(float2 z, float t) { return make_float2(-t * z.y, t * z.x); }
+__device__ __forceinline__ void small_dft(float2 (&v)[5], int R, float t) {
+    if (R == 2) {
+        const float2 a = v[0], b = v[1];
+        v[0] = cadd(a, b);
+        v[1] = csub(a, b);
+    } else if (R == 4) {
+        const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const float2 s13 = cadd(v[1], v[3]), d13 = mul_ti(csub(v[1], v[3]), t);
+        v[0] = cadd(s02, s13);
+        v[2] = csub(s02, s13);
+        v[1] = cadd(d02, d13);
+        v[3] = csub(d02, d13);
+    } else if (R == 3) {
+        const float h = 0.86602540378443865f; // sqrt(3)/2
+        const float2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+        const float2 m = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+        const float2 r = mul_ti(make_float2(h * d.x, h * d.y), t);
+        v[0] = cadd(v[0], s);
+        v[1] = cadd(m, r);
+        v[2] = csub(m, r);
+    } else { // 5
+        const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+        const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
+        const float2 a1 = cadd(v[1], v[4]), a2 = cadd(v[2], v[3]);
+        const float2 b1 = csub(v[1], v[4]), b2 = csub(v[2], v[3]);
+        const float2 p1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+        const float2 p2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+        const float2 q1 = mul_ti(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y), t);
+        const float2 q2 = mul_ti(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y), t);
+        v[0] = cadd(v[0], cadd(a1, a2));
+        v[1] = cadd(p1, q1);
+        v[4] = csub(p1, q1);
+        v[2] = cadd(p2, q2);
+        v[3] = csub(p2, q2);
+    }
+}
+
+// Stockham autosort FFT of one line in LDS (A holds the input; the result is
+// in the returned buffer, A or B): per stage of radix R and span Ns, item j
+// takes A[j + r n/R], twiddles by exp(t 2 pi i r (j%Ns)/(Ns R)), transforms,
+// and writes B[(j/Ns) Ns R + j%Ns + r Ns].  t = -1 forward (FFTW_FORWARD).
+__device__ float2 *lds_fft(float2 *A, float2 *B, const LfPlan &P, float t) {
+    float2 *src = A, *dst = B;
+    int Ns = 1;
+    for (int s = 0; s < P.nrad; s++) {
+        const int R = P.rad[s], nb = P.n / R;
+        for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+            const int k = j % Ns;
+            float sn, cs;
+            sincospif(t * 2.f * (float)k / (float)(Ns * R), &sn, &cs);
+            const float2 w1 = make_float2(cs, sn);
+            float2 v[5];
+            float2 w = make_float2(1.f, 0.f);
+            for (int r = 0; r < R; r++) {
+                v[r] = cmul(src[j + r * nb], w);
+                w = cmul(w, w1);
+            }
+            small_dft(v, R, t);
+            const int d = (j / Ns) * Ns * R + k;
+            for (int r = 0; r < R; r++) dst[d + r * Ns] = v[r];
+        }
+        __syncthreads();
+        float2 *tmp = src;
+        src = dst;
+        dst = tmp;
+        Ns *= R;
+    }
+    return src;
+}
+
+__device__ __forceinline__ void lf_chain(int c, int nsa, int nwvl, int &iwfs, int &isa, int &iwvl) {
+    iwvl = c % nwvl;
+    isa = (c / nwvl) % nsa;
+    iwfs = c / (nwvl * nsa);
+}
+
+// pass 1: embed and FFT the rows of B1 (rows outside the field band are zero)
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_embed_rows(LfArgs a) {
+    __shared__ float2 A[MX_LF_MAXN], B[MX_LF_MAXN];
+    const int y = blockIdx.x, c = blockIdx.y, n = a.nwvf;
+    int iwfs, isa, iwvl;
+    lf_chain(c, a.nsa, a.nwvl, iwfs, isa, iwvl);
+    float2 *out = a.b1 + (long)c * n * n + (long)y * n;
+    const int skip = (n - a.nxsa) / 2, iy = y - skip;
+    if (iy < 0 || iy >= a.nxsa) {
+        for (int x = threadIdx.x; x < n; x += blockDim.x) out[x] = make_float2(0.f, 0.f);
+        return;
+    }
+    const long np = (long)a.nxsa * a.nxsa;
+    const float *op = a.opd + ((long)iwfs * a.nsa + isa) * np + (long)iy * a.nxsa;
+    const float *am = a.amp + (long)isa * np + (long)iy * a.nxsa;
+    const float inv_wvl = a.wv[iwvl].inv_wvl;
+    for (int x = threadIdx.x; x < n; x += blockDim.x) {
+        const int ix = x - skip;
+        float2 e = make_float2(0.f, 0.f);
+        if (ix >= 0 && ix < a.nxsa) {
+            float sn, cs;
+            sincospif(2.f * op[ix] * inv_wvl, &sn, &cs);
+            e = make_float2(am[ix] * cs, am[ix] * sn);
+        }
+        A[x] = e;
+    }
+    __syncthreads();
+    const float2 *r = lds_fft(A, B, a.pn, -1.f);
+    for (int x = threadIdx.x; x < n; x += blockDim.x) out[x] = r[x];
+}
+
+// pass 2: FFT the columns of B1, |E|^2, and the corner copy into B2; the
+// blocks beyond nwvf zero the middle columns of B2
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_psf_cols(LfArgs a) {
+    __shared__ float2 A[MX_LF_MAXN], B[MX_LF_MAXN];
+    const int xb = blockIdx.x, c = blockIdx.y, n = a.nwvf, m = a.notf, h = n / 2;
+    float2 *o = a.b2 + (long)c * m * m;
+    if (xb >= n) { // middle column of B2, all zero
+        const int x2 = h + (xb - n);
+        for (int y2 = threadIdx.x; y2 < m; y2 += blockDim.x) o[(long)y2 * m + x2] = make_float2(0.f, 0.f);
+        return;
+    }
+    const float2 *in = a.b1 + (long)c * n * n + xb;
+    for (int y = threadIdx.x; y < n; y += blockDim.x) A[y] = in[(long)y * n];
+    __syncthreads();
+    const float2 *r = lds_fft(A, B, a.pn, -1.f);
+    const int x2 = xb < h ? xb : xb + (m - n);
+    for (int y2 = threadIdx.x; y2 < m; y2 += blockDim.x) {
+        const int y = y2 < h ? y2 : (y2 >= m - h ? y2 - (m - n) : -1);
+        float2 v = make_float2(0.f, 0.f);
+        if (y >= 0) v.x = r[y].x * r[y].x + r[y].y * r[y].y;
+        o[(long)y2 * m + x2] = v;
+    }
+}
+
+// passes 3-5 on B2: MODE 0 rows forward; 1 columns forward, x nominal,
+// columns inverse; 2 rows inverse, real part to the image
+template <int MODE>
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_otf(LfArgs a) {
+    __shared__ float2 A[MX_LF_MAXN], B[MX_LF_MAXN];
+    const int l = blockIdx.x, c = blockIdx.y, m = a.notf;
+    float2 *base = a.b2 + (long)c * m * m;
+    const long estride = MODE == 1 ? m : 1, lstride = MODE == 1 ? 1 : m;
+    float2 *line = base + (long)l * lstride;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) A[i] = line[i * estride];
+    __syncthreads();
+    float2 *r = lds_fft(A, B, a.pm, MODE == 2 ? 1.f : -1.f);
+    if (MODE == 1) {
+        int iwfs, isa, iwvl;
+        lf_chain(c, a.nsa, a.nwvl, iwfs, isa, iwvl);
+        const float *nm = a.wv[iwvl].nominal; // transposed: [ux][uy], ux = this column
+        if (nm)
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                const float f = nm[(long)l * m + i];
+                r[i] = make_float2(r[i].x * f, r[i].y * f);
+            }
+        __syncthreads();
+        float2 *other = r == A ? B : A;
+        r = lds_fft(r, other, a.pm, 1.f);
+    }
+    if (MODE == 2) {
+        float *im = a.img + (long)c * m * m + (long)l * m;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) im[i] = r[i].x;
+    } else {
+        for (int i = threadIdx.x; i < m; i += blockDim.x) line[i * estride] = r[i];
+    }
+}
+
+// si sampling over the wavelengths and the pixel processing of one (WFS,
+// subaperture) (dspmulcreal, maos/wfsints.c:286; shwfs_grad,
+// maos/utils.c:983-1122)
+struct LfSampleArgs {
+    int nsa, nwfs, nwvl, npix, pixpsax, pixpsay, notf;
+    const float *img;
+    const DevWvl *wv;
+    const DevWfs *wfs;
+    int phytype, sigmatch;
+    float cogthres, cogoff, pixthetax, pixthetay, sigtot;
+    const float *mtche, *i0sum, *saa;
+    float saat;
+    float *ints, *grad;
+    double *i0acc;
+};
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
+    __shared__ float red[3][MX_LF_THREADS / 64];
+    const int isa = blockIdx.x % a.nsa, iwfs = blockIdx.x / a.nsa;
+    const DevWfs &W = a.wfs[iwfs];
+    const long m2 = (long)a.notf * a.notf;
+    const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int p = threadIdx.x; p < a.npix; p += blockDim.x) {
+        float pv = 0.f;
+        for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
+            const float *im = a.img + (((long)iwfs * a.nsa + isa) * a.nwvl + iwvl) * m2;
+            const int4 id = a.wv[iwvl].si_idx[p];
+            const float4 w = a.wv[iwvl].si_w[p];
+            pv += W.alpha[iwvl] * (w.x * im[id.x] + w.y * im[id.y] + w.z * im[id.z] + w.w * im[id.w]);
+        }
+        a.ints[ibase + p] = pv;
+        if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
+        if (a.phytype == MX_PTYPE_MF) {
+            const float *mt = a.mtche + ibase * 2 + 2 * p;
+            s0 += mt[0] * pv;
+            s1 += mt[1] * pv;
+            s2 += pv;
+        } else {
+            const float iI = pv - a.cogoff;
+            if (iI > a.cogthres) {
+                s0 += iI * (float)(p % a.pixpsax);
+                s1 += iI * (float)(p / a.pixpsax);
+                s2 += iI;
+            }
+        }
+    }
+    seg_sum<64>(s0);
+    seg_sum<64>(s1);
+    seg_sum<64>(s2);
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (lane == 0) {
+        red[0][wave] = s0;
+        red[1][wave] = s1;
+        red[2][wave] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s0 = s1 = s2 = 0.f;
+        for (int w = 0; w < (int)blockDim.x / 64; w++) {
+            s0 += red[0][w];
+            s1 += red[1][w];
+            s2 += red[2][w];
+        }
+        if (!(a.saa[isa] < a.saat)) {
+            float gx = 0.f, gy = 0.f;
+            if (a.phytype == MX_PTYPE_MF) {
+                float scale = 1.f;
+                if (a.sigmatch == 1) {
+                    scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
+                    if (scale <= 0.f || scale > 10.f || !isfinite(scale)) scale = 1.f;
+                }
+                gx = s0 * scale;
+                gy = s1 * scale;
+            } else {
+                float sum = s2;
+                if (a.sigmatch == 0) sum = a.i0sum ? a.i0sum[(long)iwfs * a.nsa + isa] : a.sigtot * a.saa[isa];
+                if (sum > a.cogthres) {
+                    gx = (s0 / sum - (float)(a.pixpsax - 1) * 0.5f) * a.pixthetax;
+                    gy = (s1 / sum - (float)(a.pixpsay - 1) * 0.5f) * a.pixthetay;
+                }
+            }
+            a.grad[(long)iwfs * 2 * a.nsa + isa] = gx;
+            a.grad[(long)iwfs * 2 * a.nsa + a.nsa + isa] = gy;
+        }
+    }
+}
 } // namespace
 
 // ======================================================================
 // C ABI
 // ======================================================================
+constexpr int MX_MAXLAUNCH = 8; // kernels of one step
+
 struct mx_wfsgrad_s {
     int device = 0;
     int nreal = 1; // realisations; WFS r * (nwfs / nreal) + i is realisation r's WFS i
@@ -1158,7 +1475,7 @@ struct mx_wfsgrad_s {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    hipGraphNode_t gnode[3] = {};
+    hipGraphNode_t gnode[MX_MAXLAUNCH] = {};
     int gkey = 0;
     long nsa = 0, nxsa = 0, embfac = 0, nwvf = 0, notf = 0, npix = 0, pixpsax = 0, pixpsay = 0,
          nwvl = 0;
@@ -1194,6 +1511,13 @@ struct mx_wfsgrad_s {
     float2 *d_lotf_t = nullptr;
     MfmaWvl *d_mwv = nullptr;
     MfmaWvl h_mwv[MX_MAXWVL] = {};
+    // large-field form: line-FFT plans of nwvf and notf, per-chain buffers
+    bool lf = false;
+    LfPlan lf_pn{}, lf_pm{};
+    float2 *d_b1 = nullptr, *d_b2 = nullptr;
+    float *d_img = nullptr;
+    LfArgs arg_lf{};
+    LfSampleArgs arg_lfs{};
     float *d_mtche = nullptr, *d_i0sum = nullptr;
     bool have_mtche = false, have_i0sum = false;
     float *d_ints = nullptr, *d_grad = nullptr, *d_opd = nullptr;
@@ -1244,6 +1568,32 @@ void transpose_c(float2 *dst, const double *src, long N, bool conj) {
 
 bool supported(long notf, long nxsa, long embfac) {
     return (notf == 64 && nxsa == 32 && embfac == 2) || (notf == 32 && nxsa == 16 && embfac == 2);
+}
+
+// Stockham plan of a line length n <= MX_LF_MAXN whose prime factors are 2,
+// 3 and 5: radix 4 first, then 2, 3, 5.  False if n does not factor so.
+bool lf_plan(long n, LfPlan &P) {
+    P = LfPlan{};
+    if (n < 2 || n > MX_LF_MAXN) return false;
+    P.n = (int)n;
+    long r = n;
+    const int radix[4] = {4, 2, 3, 5};
+    for (int k = 0; k < 4; k++)
+        while (r % radix[k] == 0) {
+            if (P.nrad == MX_LF_MAXRAD) return false;
+            P.rad[P.nrad++] = radix[k];
+            r /= radix[k];
+        }
+    return r == 1;
+}
+
+// the large-field form applies: OPD grid a multiple of 16 points, even
+// field and OTF sizes that plan, notf >= nwvf (ccpcorner pads)
+bool lf_supported(long notf, long nxsa, long embfac) {
+    LfPlan a, b;
+    const long nwvf = nxsa * embfac;
+    return nxsa >= 16 && nxsa % 16 == 0 && embfac >= 1 && nwvf % 2 == 0 && notf % 2 == 0 && notf >= nwvf &&
+           lf_plan(nwvf, a) && lf_plan(notf, b);
 }
 
 // FP32 -> BF16 with round-to-nearest-even (finite inputs)
@@ -1385,13 +1735,20 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     std::vector<mx_wfs_t> wfs_all(nwfs);
     for (int i = 0; i < nwfs; i++) wfs_all[i] = wfs0[i % nwfs0];
     const mx_wfs_t *wfs = wfs_all.data();
-    if (!supported(p->notf, p->nxsa, p->embfac))
-        return set_err(MX_EINVAL, "unsupported size: notf=%ld nxsa=%ld embfac=%ld (supported: 64/32/2, 32/16/2)",
-                       p->notf, p->nxsa, p->embfac);
+    const bool lf = !supported(p->notf, p->nxsa, p->embfac);
+    if (lf && !lf_supported(p->notf, p->nxsa, p->embfac))
+        return set_err(MX_EINVAL,
+                       "unsupported size: notf=%ld nxsa=%ld embfac=%ld (supported: 64/32/2, 32/16/2, or nxsa a "
+                       "multiple of 16 with nxsa*embfac <= notf <= %d, both even with prime factors 2, 3 and 5)",
+                       p->notf, p->nxsa, p->embfac, MX_LF_MAXN);
+    if (lf && p->llt_nx > 0)
+        return set_err(MX_EINVAL, "an LLT needs notf=64/nxsa=32 or notf=32/nxsa=16 (got notf=%ld nxsa=%ld)", p->notf,
+                       p->nxsa);
     if (p->nwvl < 1 || p->nwvl > MX_MAXWVL) return set_err(MX_EINVAL, "nwvl=%ld out of range", p->nwvl);
     const long npix = p->pixpsax * p->pixpsay;
     const long ppl = (npix + p->notf - 1) / p->notf;
-    if (npix < 1 || ppl > 2) return set_err(MX_EINVAL, "npix=%ld too large for notf=%ld", npix, p->notf);
+    if (npix < 1 || (!lf && ppl > 2) || npix > p->notf * p->notf)
+        return set_err(MX_EINVAL, "npix=%ld too large for notf=%ld", npix, p->notf);
     if (p->phytype != MX_PTYPE_MF && p->phytype != MX_PTYPE_COG)
         return set_err(MX_EINVAL, "phytype %d unsupported", p->phytype);
     if (p->sigmatch < 0 || p->sigmatch > 1)
@@ -1436,6 +1793,15 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     ALLOC(h->d_mtche, nwfs * nsa * 2 * npix);
     ALLOC(h->d_i0sum, nwfs * nsa);
     h->mfma_size = (N == 64 && h->nxsa == 32);
+    h->lf = lf;
+    if (lf) {
+        lf_plan(h->nwvf, h->lf_pn);
+        lf_plan(N, h->lf_pm);
+        const long nchain = (long)nwfs * nsa * h->nwvl;
+        ALLOC(h->d_b1, nchain * h->nwvf * h->nwvf);
+        ALLOC(h->d_b2, nchain * N * N);
+        ALLOC(h->d_img, nchain * N * N);
+    }
     if (p->llt_nx > 0) {
         ALLOC(h->d_llt_amp, p->llt_nx * p->llt_nx);
         ALLOC(h->d_llt_opd, nwfs * p->llt_nx * p->llt_nx);
@@ -1513,7 +1879,7 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     drop_graph(h);
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
-                    h->d_lotf_t, h->d_F, h->d_mwv,
+                    h->d_lotf_t, h->d_F, h->d_mwv, h->d_b1, h->d_b2, h->d_img,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
     for (void *q : ptrs)
         if (q) hipFree(q);
@@ -1748,11 +2114,13 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     if (h->cfg.llt_nx > 0)
         for (int w = 0; w < h->nwvl; w++)
             if (!h->d_etf[w]) return set_err(MX_ESTATE, "etf of wavelength %d not set", w);
-    for (int r = 0; r < h->nreal; r++) // the staged screen tile assumes a grid step <= 1 pixel
+    // the staged screen tile holds a grid whose step is <= 1 screen pixel;
+    // a coarser grid gathers from the screens directly
+    bool direct = false;
+    for (int r = 0; r < h->nreal; r++)
         for (int l = 0; l < g_atm[r].nps; l++)
             if (h->cfg.dx * g_atm[r].h_layers[l].dx1 > 1. + 1e-9 || h->cfg.dx * g_atm[r].h_layers[l].dy1 > 1. + 1e-9)
-                return set_err(MX_EINVAL, "realisation %d layer %d is sampled more coarsely than the OPD grid (dx=%g)",
-                               r, l, h->cfg.dx);
+                direct = true;
     if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche) return set_err(MX_ESTATE, "matched filter not set");
     if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum)
         return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=1)");
@@ -1800,6 +2168,10 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     o.llt_ox = h->cfg.llt_ox;
     o.llt_oy = h->cfg.llt_oy;
     o.llt_opd = h->d_llt_opd;
+    // the large-field form traces a subaperture grid as tiles of 32 or 16
+    const int opd_nx = (h->lf && h->nxsa % 32 == 0) || h->nxsa == 32 ? 32 : 16;
+    o.tiles = (int)(h->nxsa / opd_nx);
+    o.direct = direct ? 1 : 0;
     KArgs &a = h->arg_step;
     a = KArgs{};
     a.nsa = (int)h->nsa;
@@ -1898,12 +2270,12 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         dim3 grid, block;
         void *arg;
     };
-    Launch L[3];
+    Launch L[MX_MAXLAUNCH];
     int nl = 0;
     const bool mf = variant == MX_VARIANT_MFMA_BF16;
-    L[nl++] = {h->nxsa == 32 ? reinterpret_cast<const void *>(&k_wfs_opd<32>)
-                             : reinterpret_cast<const void *>(&k_wfs_opd<16>),
-               dim3((unsigned)(h->nsa * h->nwfs + o.nllt)), dim3(MX_OPD_THREADS), &o};
+    L[nl++] = {opd_nx == 32 ? reinterpret_cast<const void *>(&k_wfs_opd<32>)
+                            : reinterpret_cast<const void *>(&k_wfs_opd<16>),
+               dim3((unsigned)(h->nsa * h->nwfs * o.tiles * o.tiles + o.nllt)), dim3(MX_OPD_THREADS), &o};
     const int slot_llt = llt ? nl : -1;
     if (llt) {
         if (mf)
@@ -1915,7 +2287,58 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
                        dim3((unsigned)(h->nwfs * h->nwvl)), dim3(64), &la};
     }
     const int slot_img = nl;
-    if (mf) {
+    if (h->lf) {
+        LfArgs &f = h->arg_lf;
+        f = LfArgs{};
+        f.pn = h->lf_pn;
+        f.pm = h->lf_pm;
+        f.nwvf = (int)h->nwvf;
+        f.notf = (int)N;
+        f.nxsa = (int)h->nxsa;
+        f.nsa = (int)h->nsa;
+        f.nwfs = h->nwfs;
+        f.nwvl = (int)h->nwvl;
+        f.opd = h->d_opd;
+        f.amp = h->d_amp;
+        f.wv = h->d_wv;
+        f.b1 = h->d_b1;
+        f.b2 = h->d_b2;
+        f.img = h->d_img;
+        LfSampleArgs &g = h->arg_lfs;
+        g = LfSampleArgs{};
+        g.nsa = a.nsa;
+        g.nwfs = a.nwfs;
+        g.nwvl = a.nwvl;
+        g.npix = a.npix;
+        g.pixpsax = a.pixpsax;
+        g.pixpsay = a.pixpsay;
+        g.notf = (int)N;
+        g.img = h->d_img;
+        g.wv = h->d_wv;
+        g.wfs = h->d_wfs;
+        g.phytype = a.phytype;
+        g.sigmatch = a.sigmatch;
+        g.cogthres = a.cogthres;
+        g.cogoff = a.cogoff;
+        g.pixthetax = a.pixthetax;
+        g.pixthetay = a.pixthetay;
+        g.sigtot = a.sigtot;
+        g.mtche = a.mtche;
+        g.i0sum = a.i0sum;
+        g.saa = a.saa;
+        g.saat = a.saat;
+        g.ints = a.ints;
+        g.grad = a.grad;
+        g.i0acc = a.i0acc;
+        const unsigned nchain = (unsigned)(h->nwfs * h->nsa * h->nwvl);
+        const dim3 blk(MX_LF_THREADS);
+        L[nl++] = {reinterpret_cast<const void *>(&k_lf_embed_rows), dim3((unsigned)h->nwvf, nchain), blk, &f};
+        L[nl++] = {reinterpret_cast<const void *>(&k_lf_psf_cols), dim3((unsigned)N, nchain), blk, &f};
+        L[nl++] = {reinterpret_cast<const void *>(&k_lf_otf<0>), dim3((unsigned)N, nchain), blk, &f};
+        L[nl++] = {reinterpret_cast<const void *>(&k_lf_otf<1>), dim3((unsigned)N, nchain), blk, &f};
+        L[nl++] = {reinterpret_cast<const void *>(&k_lf_otf<2>), dim3((unsigned)N, nchain), blk, &f};
+        L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)), blk, &g};
+    } else if (mf) {
         L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<false>), dim3((unsigned)(h->nsa * h->nwfs)),
                    dim3(256), &m};
     } else {
@@ -1927,9 +2350,9 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     if (h->use_graph && !h->timing) {
         // one graph launch per step: the graph is built for the kernel set and
         // its nodes get this step's arguments
-        const int key = 1 + (variant << 4) + (llt ? 2 : 0) + (N == 64 ? 1 : 0);
-        hipKernelNodeParams kp[3];
-        void *argv[3][1];
+        const int key = 1 + (variant << 4) + (llt ? 2 : 0) + (N == 64 ? 1 : 0) + (h->lf ? 4 : 0) + (opd_nx << 8);
+        hipKernelNodeParams kp[MX_MAXLAUNCH];
+        void *argv[MX_MAXLAUNCH][1];
         for (int i = 0; i < nl; i++) {
             argv[i][0] = L[i].arg;
             kp[i] = hipKernelNodeParams{};
@@ -1954,7 +2377,8 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         HIPCHK(hipGraphLaunch(h->gexec, h->stream));
     } else {
         // Timing events: e2/e3 bracket the ray trace, e0/e1 the uplink OTF,
-        // e4/e5 the imaging kernel.
+        // e4/e5 the imaging kernels (one, or the passes of the large-field
+        // form).
         hipEvent_t ev[6] = {};
         if (h->timing) {
             if (h->ev.size() < 6) {
@@ -1965,14 +2389,13 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
             for (int i = 0; i < 6; i++) ev[i] = h->ev[i];
         }
         for (int i = 0; i < nl; i++) {
-            hipEvent_t eb = i == 0 ? ev[2] : i == slot_llt ? ev[0] : ev[4];
-            hipEvent_t ee = i == 0 ? ev[3] : i == slot_llt ? ev[1] : ev[5];
-            if (h->timing) HIPCHK(hipEventRecord(eb, h->stream));
+            hipEvent_t eb = i == 0 ? ev[2] : i == slot_llt ? ev[0] : i == slot_img ? ev[4] : nullptr;
+            hipEvent_t ee = i == 0 ? ev[3] : i == slot_llt ? ev[1] : i == nl - 1 ? ev[5] : nullptr;
+            if (h->timing && eb) HIPCHK(hipEventRecord(eb, h->stream));
             void *argv[1] = {L[i].arg};
             HIPCHK(hipLaunchKernel(L[i].f, L[i].grid, L[i].block, argv, 0, h->stream));
-            if (h->timing) HIPCHK(hipEventRecord(ee, h->stream));
+            if (h->timing && ee) HIPCHK(hipEventRecord(ee, h->stream));
         }
-        (void)slot_img;
         if (h->timing) {
             HIPCHK(hipEventSynchronize(ev[5]));
             float m1 = 0, m2 = 0, m3 = 0;

@@ -19,8 +19,10 @@ typedef double complex cplx;
 
 /* ---------------------------------------------------------------------
  * FFT: cached radix-2 plans (bit reversal + twiddles); other sizes by a
- * direct DFT.  Sign and scaling follow FFTW as used by cfft2
- * (math/fft.c:253-281): dir=-1 is exp(-i 2 pi jk/n), no normalisation.
+ * recursive mixed-radix Cooley-Tukey decimation in time (smallest prime
+ * factor first, a direct DFT for a prime length).  Sign and scaling follow
+ * FFTW as used by cfft2 (math/fft.c:253-281): dir=-1 is exp(-i 2 pi jk/n),
+ * no normalisation.
  * ------------------------------------------------------------------- */
 typedef struct {
     long n;
@@ -88,29 +90,64 @@ static void fft_pow2(cplx *x, long n, int dir, const plan_t *p) {
     }
 }
 
+/* out[k] = sum_j in[j*stride] w^(j k) for the length n = ntop/S sub-transform;
+ * w^t = tw[(t*S) mod ntop] with tw[t] = exp(dir i 2 pi t/ntop).  scratch
+ * holds n values. */
+static void fft_mixed(const cplx *in, long stride, cplx *out, long n, long S, long ntop, const cplx *tw,
+                      cplx *scratch) {
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    long p = n;
+    for (long f = 2; f * f <= n; f++)
+        if (n % f == 0) {
+            p = f;
+            break;
+        }
+    if (p == n) { /* prime length: direct DFT */
+        for (long k = 0; k < n; k++) {
+            cplx s = 0;
+            for (long j = 0; j < n; j++) s += in[j * stride] * tw[((j * k) % n) * S % ntop];
+            scratch[k] = s;
+        }
+        memcpy(out, scratch, sizeof(cplx) * n);
+        return;
+    }
+    const long m = n / p;
+    for (long r = 0; r < p; r++) fft_mixed(in + r * stride, stride * p, out + r * m, m, S * p, ntop, tw, scratch);
+    /* X[k + q m] = sum_r w^(r (k + q m)) Y_r[k] */
+    for (long k = 0; k < m; k++)
+        for (long q = 0; q < p; q++) {
+            cplx s = 0;
+            for (long r = 0; r < p; r++) s += out[r * m + k] * tw[((r * (k + q * m)) % n) * S % ntop];
+            scratch[k + q * m] = s;
+        }
+    memcpy(out, scratch, sizeof(cplx) * n);
+}
+
+/* work holds 4 n values */
 static void fft_any(cplx *x, long n, int dir, cplx *work) {
     if (n <= 1) return;
     if ((n & (n - 1)) == 0) {
         fft_pow2(x, n, dir, get_plan(n));
         return;
     }
-    for (long k = 0; k < n; k++) {
-        cplx s = 0;
-        for (long j = 0; j < n; j++) {
-            double a = dir * 2. * M_PI * (double)((j * k) % n) / (double)n;
-            s += x[j] * (cos(a) + I * sin(a));
-        }
-        work[k] = s;
+    cplx *tw = work, *out = work + n, *scratch = work + 2 * n;
+    for (long t = 0; t < n; t++) {
+        double a = dir * 2. * M_PI * (double)t / (double)n;
+        tw[t] = cos(a) + I * sin(a);
     }
-    memcpy(x, work, sizeof(cplx) * n);
+    fft_mixed(x, 1, out, n, 1, n, tw, scratch);
+    memcpy(x, out, sizeof(cplx) * n);
 }
 
 void orc_fft2(double *c, long nx, long ny, int dir) {
     cplx *a = (cplx *)c;
     long nmax = nx > ny ? nx : ny;
-    cplx stackbuf[2 * 1024];
-    cplx *work = nmax <= 1024 ? stackbuf : (cplx *)malloc(sizeof(cplx) * 2 * nmax);
-    cplx *col = work + nmax;
+    cplx stackbuf[5 * 1024];
+    cplx *work = nmax <= 1024 ? stackbuf : (cplx *)malloc(sizeof(cplx) * 5 * nmax);
+    cplx *col = work + 4 * nmax;
     for (long iy = 0; iy < ny; iy++) fft_any(a + iy * nx, nx, dir, work);
     if (ny > 1) {
         for (long ix = 0; ix < nx; ix++) {

@@ -62,3 +62,44 @@ def ngs_two_wvl():
                wfs=[dict(thetax=0.0, thetay=0.0), dict(thetax=5e-5, thetay=-3e-5)])
     p = Powfs(cfg["aper"], pcfg, sim)
     return cfg, p, genatm(atm)
+
+
+@pytest.fixture(scope="session")
+def lf_mixed():
+    """A large-field case with mixed-radix sizes: 4 m pupil, 2x2 subapertures
+    of 2 m at 1/24 m (48^2 points, 96^2 field = 2^5 3) padded to a 120^2 OTF
+    (2^3 3 5), 1.25 and 1.65 um, 8x8 pixels of 30 mas, CoG; screens at 1/64 m
+    (finer than the OPD grid)."""
+    import math
+    from maos_amd.powfs import PTYPE_COG, AperCfg, AtmCfg, Powfs, PowfsCfg, SimCfg, genatm
+    pcfg = PowfsCfg(order=2, dsa=2.0, dx=1.0 / 24, saat=0.3, hs=math.inf, embfac=2,
+                    wvl=[1.25e-6, 1.65e-6], wvlwts=[0.7, 0.3], pixpsa=8, pixtheta=0.03,
+                    pixblur=0.3, notf=120, phytype=PTYPE_COG, sigmatch=1, siglev=1000.0)
+    sim = SimCfg(dt=1.0 / 800)
+    atm = AtmCfg(r0=0.15, L0=30.0, ht=[0.0, 6000.0], wt=[0.6, 0.4], ws=[8.0, 12.0],
+                 wddeg=[30.0, 200.0], dx=1.0 / 64, nx=512, seeds=[21, 22])
+    cfg = dict(name="lf_mixed", aper=AperCfg(4.0, 0.6), powfs=pcfg, sim=sim, atm=atm,
+               wfs=[dict(thetax=0.0, thetay=0.0), dict(thetax=-4e-5, thetay=6e-5)])
+    p = Powfs(cfg["aper"], pcfg, sim)
+    return cfg, p, genatm(atm)
+
+
+@pytest.fixture(scope="session")
+def ngs_lo_small():
+    """ngs_lo (3 WFS, 2x2 of 15 m, 480^2 field in a 512^2 OTF, two
+    wavelengths) with 1024^2 screens."""
+    from maos_amd import configs
+    from maos_amd.powfs import Powfs, genatm
+    cfg = configs.ngs_lo(nx=1024)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    return cfg, p, genatm(cfg["atm"])
+
+
+@pytest.fixture(scope="session")
+def ngs_tt_small():
+    """ngs_tt (1x1 of 30 m, 960^2 field in a 1024^2 OTF) with 1024^2 screens."""
+    from maos_amd import configs
+    from maos_amd.powfs import Powfs, genatm
+    cfg = configs.ngs_tt(nx=1024)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    return cfg, p, genatm(cfg["atm"])

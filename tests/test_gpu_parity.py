@@ -244,6 +244,44 @@ def test_batch_realisations(lgs_small):
     single.close()
 
 
+@pytest.mark.parametrize("case", ["lf_mixed", "ngs_lo_small", "ngs_tt_small"])
+def test_large_field_parity(case, request):
+    """The large-field form (line FFTs of any 2/3/5-smooth length <= 1024,
+    ccpcorner padding, screens finer than the OPD grid) against the oracle,
+    every WFS, two steps; and its FP64 i0 accumulation."""
+    from maos_amd import _lib
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: %s nsa=%d nxsa=%d nwvf=%d notf=%d nwvl=%d nwfs=%d"
+          % (case, p.nsa, p.nxsa, p.nxsa * p.cfg.embfac, p.notf, p.nwvl, len(cfg["wfs"])), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    assert eng.variant() == _lib.VARIANT_FFT32
+    assert not eng.variant_ok(_lib.VARIANT_MFMA_BF16)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    eng.i0_control(True, reset=True)
+    acc = [np.zeros((p.nsa, p.npix)) for _ in wfs]
+    for isim in (0, 311):
+        eng.queue(isim, dt)
+        for iw in range(len(wfs)):
+            g, ints = eng.sync(iw)
+            opd = eng.debug_opd(iw)
+            ri, rg, ropd, _ = o.step(wfs[iw], isim, dt, want_opd=True)
+            acc[iw] += ints
+            e_opd = np.abs(opd - ropd).max() / np.abs(ropd).max()
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(g - rg).max() * RAD2MAS
+            print("isim=%d wfs=%d opd maxrel=%.2e ints relL2=%.2e grad maxabs=%.3e mas"
+                  % (isim, iw, e_opd, e_int, e_g))
+            assert e_opd < 1e-5
+            assert e_int <= INTS_TOL
+            assert e_g <= GRAD_TOL_MAS
+    for iw in range(len(wfs)):
+        i0, n = eng.i0_get(iw)
+        assert n == 2
+        assert np.array_equal(i0, acc[iw])  # FP32 frames summed in FP64 exactly
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C

@@ -33,6 +33,24 @@ def test_fft2_matches_numpy_and_brute_force(nx, ny):
         assert np.allclose(fwd, brute, atol=1e-10)
 
 
+@pytest.mark.parametrize("n", [6, 15, 30, 96, 480, 960, 77])
+def test_fft_mixed_radix_lengths(n):
+    """Lengths with factors 3, 5, 7, 11 (the 480/960-point fields of the
+    large NGS subapertures) take the mixed-radix path: numpy along rows, and
+    a brute-force DFT where it is cheap."""
+    a = RNG.randn(2, n) + 1j * RNG.randn(2, n)
+    fwd = oracle.fft2(a.copy(), -1)
+    assert np.allclose(fwd, np.fft.fft2(a), atol=1e-12 * np.abs(a).sum())
+    bwd = oracle.fft2(a.copy(), +1)
+    assert np.allclose(bwd, np.fft.ifft2(a) * a.size, atol=1e-12 * np.abs(a).sum())
+    if n <= 96:
+        k = np.arange(n)
+        F = np.exp(-2j * np.pi * np.outer(k, k) / n)
+        brute = (a @ F.T)
+        brute = np.stack([brute[0] + brute[1], brute[0] - brute[1]])
+        assert np.allclose(fwd, brute, atol=1e-10 * np.abs(a).sum())
+
+
 # ------------------------------------------------------- orc_prop_grid_pts
 def _plane_screen(n, a, b, c):
     y, x = np.mgrid[0:n, 0:n].astype(float)
