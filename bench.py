@@ -8,6 +8,7 @@ before the timed region.  One frame is one WFS x one step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
                     [--variant auto|fft32|mfma] [--copy-calib BYTES]
+                    [--nreal N] [--hetero] [--no-graph]
 
 Prints one JSON line.  The CPU baseline is the FP64 oracle (a C restatement
 of MAOS's CPU path) timed on this host on a bounded sample of frames.
@@ -216,6 +217,122 @@ def batch_main(args):
     eng.close()
 
 
+def hetero_main(args):
+    """The heterogeneous step: the six lgs30m LGS WFS, three low-order NGS
+    WFS (2x2 of 15 m, 480^2 fields in 512^2 OTFs, two wavelengths) and one
+    tip-tilt NGS WFS (1x1 of 30 m, 960^2 in 1024^2) on one atmosphere,
+    stepped as one group graph per step.  Each engine is autotuned first;
+    per-class kernel times come from each engine stepped alone."""
+    from maos_amd import configs, engine
+    from maos_amd.powfs import Powfs, genatm
+
+    t0 = time.time()
+    cfgs = [configs.lgs30m(nx=args.nx, order=args.order), configs.ngs_lo(nx=args.nx),
+            configs.ngs_tt(nx=args.nx)]
+    layers = genatm(cfgs[0]["atm"])
+    engine.init(0)
+    engine.atm2gpu(layers)
+    dt = cfgs[0]["sim"].dt
+    classes = []
+    for cfg in cfgs:
+        p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+        wfs = configs.wfs_specs(cfg, p)
+        eng = engine.WfsGrad(p, wfs)
+        classes.append(dict(name=cfg["name"], cfg=cfg, p=p, wfs=wfs, eng=eng))
+        log("%s: nsa=%d nxsa=%d notf=%d nwvl=%d nwfs=%d (%.1fs)" % (cfg["name"], p.nsa, p.nxsa, p.notf,
+                                                                   p.nwvl, len(wfs), time.time() - t0))
+    # LGS matched filter from the FP64 i0 (as the single-class run)
+    lgs = classes[0]
+    eng, p = lgs["eng"], lgs["p"]
+    eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    eng.i0_control(enable=True, reset=True)
+    for isim in range(args.i0_steps):
+        eng.queue(isim, dt)
+    eng.wait()
+    for iw in range(len(lgs["wfs"])):
+        i0, n = eng.i0_get(iw)
+        mtche, i0sum, _ = p.mtch(i0 / n)
+        eng.update_ref(mtche, i0sum, iwfs=iw)
+        if iw == 0:
+            lgs["ref"] = (mtche, i0sum)
+    isim = args.i0_steps
+    for c in classes:
+        c["eng"].i0_control(enable=True, reset=True)
+        c["tune"] = c["eng"].autotune(isim, dt, nrep=10)
+        best = [t for t in c["tune"] if t["best"]][0]
+        log("%s autotune: %s" % (c["name"], ", ".join("%s/%d/%d %.1f us%s" % (
+            t["form"], t["opd_tile"], t["threads"], t["step_us"], "*" if t["best"] else "") for t in c["tune"])))
+        c["choice"] = best
+        isim += 12
+    # each class alone: kernel times with events
+    for c in classes:
+        e = c["eng"]
+        e.timing_enable(True)
+        for _ in range(20):
+            e.queue(isim, dt)
+            isim += 1
+        nl, ms_llt, ms_opd, ms_ints = e.timing()
+        e.timing_enable(False)
+        c["us"] = {"raytrace": round(ms_opd / nl * 1e3, 1), "uplink": round(ms_llt / nl * 1e3, 1),
+                   "imaging": round(ms_ints / nl * 1e3, 1),
+                   "step": round((ms_opd + ms_llt + ms_ints) / nl * 1e3, 1)}
+    group = engine.WfsGroup([c["eng"] for c in classes])
+    nwfs = sum(len(c["wfs"]) for c in classes)
+    for _ in range(args.warmup):
+        group.queue(isim, dt)
+        isim += 1
+    group.wait()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        group.queue(isim, dt)
+        isim += 1
+    group.wait()
+    sec = time.perf_counter() - t1
+    fps = nwfs * args.steps / sec
+    lat = []
+    for _ in range(args.latency_steps):
+        ta = time.perf_counter()
+        group.queue(isim, dt)
+        group.wait()
+        lat.append((time.perf_counter() - ta) * 1e3)
+        isim += 1
+    lat = np.array(lat)
+    log("group: %d steps in %.3fs -> %.1f frames/s" % (args.steps, sec, fps))
+    # parity of WFS 0 of each class against the FP64 oracle at one step
+    import oracle
+    group.queue(isim, dt)
+    group.wait()
+    parity = {}
+    for c in classes:
+        o = oracle.Oracle(c["p"])
+        o.atm(layers)
+        if "ref" in c:
+            o.set_ref(*c["ref"])
+        g, ints = c["eng"].sync(0)
+        ri, rg, _, _ = o.step(c["wfs"][0], isim, dt)
+        parity[c["name"]] = {"ints_rel_l2": float(np.linalg.norm(ints - ri) / np.linalg.norm(ri)),
+                             "grad_max_abs_mas": float(np.abs(g - rg).max() * 180 / math.pi * 3600e3)}
+    out = {"metric": METRIC, "value": round(fps, 2), "unit": "WFS frames/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3+fp32",
+           "data": "synthetic (seeded von Karman screens, Gaussian sodium profile)",
+           "config": {"workload": "heterogeneous step: lgs30m (6 LGS x %d subapertures, 64^2) + ngs_lo "
+                                  "(3 NGS x 2x2 of 15 m, 480^2 in 512^2, 2 wavelengths) + ngs_tt (1 NGS, "
+                                  "960^2 in 1024^2), 7 layers %d^2 shared, one group graph per step"
+                                  % (classes[0]["p"].nsa, args.nx),
+                      "nwfs": nwfs, "launch": "one HIP graph per step (one branch per powfs)"},
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                          "p99": round(float(np.percentile(lat, 99)), 3), "steps": len(lat)},
+           "classes": {c["name"]: {"nwfs": len(c["wfs"]), "nsa": c["p"].nsa, "notf": c["p"].notf,
+                                   "nwvl": c["p"].nwvl, "kernels_us_alone": c["us"],
+                                   "autotune": c["tune"]} for c in classes},
+           "parity": parity}
+    print(json.dumps(out), flush=True)
+    group.close()
+    for c in classes:
+        c["eng"].close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +349,8 @@ def main():
     ap.add_argument("--nreal", type=int, default=1,
                     help="advance this many independent realisations together (r0 ~ U[0.12, 0.25] m)")
     ap.add_argument("--latency-steps", type=int, default=100)
+    ap.add_argument("--hetero", action="store_true",
+                    help="LGS + low-order NGS + TT NGS stepped as one group graph per step")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the WFS path of this build is single-GPU "
@@ -239,6 +358,9 @@ def main():
         sys.exit(2)
     if args.nreal > 1:
         batch_main(args)
+        return
+    if args.hetero:
+        hetero_main(args)
         return
 
     from maos_amd import _lib, configs, engine

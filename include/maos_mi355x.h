@@ -191,6 +191,33 @@ int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant);
  * one by one. */
 int mx_wfsgrad_set_graph(mx_wfsgrad_t *h, int enable);
 
+/* A group of engines (e.g. the powfs of one simulation: LGS, low-order NGS,
+ * tip-tilt NGS) stepped together by one HIP graph launch per step, each
+ * engine a branch of the graph that runs concurrently with the others.  No
+ * counterpart in the reference, which queues each WFS on its own stream
+ * (gpu_wfsgrad_queue, cuda/sim/wfsgrad.cu:483).  mx_wfsgroup_queue plans
+ * every engine's step as mx_wfsgrad_queue would (same checks, same
+ * results), starts after each engine's earlier work, and makes each
+ * engine's later calls (sync, wait, queue) wait for it.  Group steps are
+ * not timed.  Engines must stay alive while the group exists. */
+typedef struct mx_wfsgroup_s mx_wfsgroup_t;
+int mx_wfsgroup_create(mx_wfsgroup_t **out, mx_wfsgrad_t *const *engines, int n);
+int mx_wfsgroup_queue(mx_wfsgroup_t *g, long isim, double dt);
+int mx_wfsgroup_free(mx_wfsgroup_t *g);
+
+/* Pick the fastest way to run this engine's step on this GPU (no
+ * counterpart in the reference, which always uses cuFFT): times nrep steps
+ * from isim (kernel time, HIP events) for each candidate -- imaging form
+ * (MFMA split-BF16 where it applies, FP32 FFT; the large-field FFT for
+ * other sizes), ray-trace tile side (32 or 16 points; an LLT grid is one
+ * tile), threads per large-field block (128, 256, 512) -- and keeps the
+ * fastest.  Every candidate gives results within the parity bar of the
+ * FP64 reference.  The i0 sums and frame count are left unchanged; ints and
+ * gradients hold the last tuning step.  report (may be NULL) receives one
+ * line per candidate, the chosen one marked with '*'. */
+int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep,
+                        char *report, long report_len);
+
 /* Wait for all queued work of the engine. */
 int mx_wfsgrad_wait(mx_wfsgrad_t *h);
 

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "../../include/maos_mi355x.h"
@@ -181,7 +182,6 @@ struct OpdArgs {
     double llt_origx, llt_origy, llt_ox, llt_oy;
     float *llt_opd; // [nwfs][nxsa^2]
     int tiles;      // a subaperture grid is tiles x tiles blocks of the kernel's NXSA
-    int direct;     // some screen is finer than the grid: gather without the LDS tile
 };
 
 // Transfer-function tables of the MFMA imaging kernel (N = 64) cover the half
@@ -264,16 +264,15 @@ __device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i,
 // staged in LDS with coalesced loads, and the bilinear interpolation runs
 // from LDS.  A tile that crosses the screen edge wraps periodically; the
 // common case that does not (a uniform test) skips the wrap arithmetic.
-// direct = true (a grid coarser than some screen, whose footprint exceeds
-// the tile) reads the four neighbours of each point from the screen itself.
-template <int N, int NXSA>
+// DIRECT (a grid coarser than some screen, whose footprint exceeds the
+// tile) reads the four neighbours of each point from the screen itself.
+template <int N, int NXSA, bool DIRECT>
 __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, double hs,
                                           double thetax, double thetay, double misregx,
                                           double misregy, double shift0x, double shift0y,
                                           long isim, double dt, double atmscale, double ptsdx,
                                           double origx, double origy, int line, float *opd,
-                                          int *cidx, float *cfrac, float *tile, bool direct,
-                                          int ostride = NXSA) {
+                                          int *cidx, float *cfrac, float *tile, int ostride = NXSA) {
     static_assert(2 * NXSA <= N, "coordinate lanes");
     constexpr int PPT = NXSA * NXSA / N;
     constexpr int TW = NXSA + 2;
@@ -299,7 +298,7 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
         }
         __syncthreads();
         const int nx = L.nx, ny = L.ny;
-        if (direct) { // uniform
+        if constexpr (DIRECT) {
             if (act)
 #pragma unroll
                 for (int j = 0; j < PPT; j++) {
@@ -594,7 +593,9 @@ constexpr int MX_OPD_THREADS = 64;
 #ifndef MX_OPD_WAVES
 #define MX_OPD_WAVES 4
 #endif
-template <int NXSA>
+// TILED: a subaperture grid is tiles x tiles blocks (else one block, with
+// the register budget of the one-tile case untouched).
+template <int NXSA, bool DIRECT, bool TILED>
 __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(MX_OPD_WAVES))) void k_wfs_opd(OpdArgs a) {
     __shared__ int cidx[2 * NXSA];
     __shared__ float cfrac[2 * NXSA];
@@ -605,10 +606,20 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
         // shift o - v dt isim.  These blocks come first so that they start
         // early.
         const DevWfs &W = a.wfs[b];
-        ray_trace<MX_OPD_THREADS, NXSA>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax - a.llt_ox / W.hs,
+        ray_trace<MX_OPD_THREADS, NXSA, DIRECT>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax - a.llt_ox / W.hs,
                                         W.thetay - a.llt_oy / W.hs, 0., 0., a.llt_ox, a.llt_oy, a.isim, a.dt,
                                         a.atmscale[W.ireal], a.ptsdx, a.llt_origx, a.llt_origy,
-                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile, a.direct != 0);
+                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile);
+        return;
+    }
+    if constexpr (!TILED) {
+        const int unit = xcd_unit(b - a.nllt, a.nsa * a.nwfs);
+        const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+        const DevWfs &W = a.wfs[iwfs];
+        ray_trace<MX_OPD_THREADS, NXSA, DIRECT>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax, W.thetay,
+                                                W.misregx, W.misregy, 0., 0., a.isim, a.dt, a.atmscale[W.ireal],
+                                                a.ptsdx, a.ptsx[isa], a.ptsy[isa], threadIdx.x,
+                                                a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac, tile);
         return;
     }
     // unit: (WFS, subaperture, tile); a subaperture grid of side NXSA * tiles
@@ -619,12 +630,12 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
     const int isa = rest / T2, it = rest % T2, tx = it % a.tiles, ty = it / a.tiles;
     const DevWfs &W = a.wfs[iwfs];
     const int side = NXSA * a.tiles;
-    ray_trace<MX_OPD_THREADS, NXSA>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax, W.thetay, W.misregx,
+    ray_trace<MX_OPD_THREADS, NXSA, DIRECT>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax, W.thetay, W.misregx,
                                     W.misregy, 0., 0., a.isim, a.dt, a.atmscale[W.ireal], a.ptsdx,
                                     a.ptsx[isa] + a.ptsdx * (double)(NXSA * tx), a.ptsy[isa] + a.ptsdx * (double)(NXSA * ty),
                                     threadIdx.x,
                                     a.opd + ((long)iwfs * a.nsa + isa) * side * side + (long)ty * NXSA * side + tx * NXSA,
-                                    cidx, cfrac, tile, a.direct != 0, side);
+                                    cidx, cfrac, tile, side);
 }
 
 // ------------------------------------------------------------ uplink OTF
@@ -1190,7 +1201,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 // then k_lf_sample applies si and the pixel processing.
 constexpr int MX_LF_MAXN = 1024;
 constexpr int MX_LF_MAXRAD = 12;
-constexpr int MX_LF_THREADS = 256;
+constexpr int MX_LF_THREADS = 512; // most threads of a large-field block (128, 256 or 512 run)
 struct LfPlan {
     int n, nrad;
     int rad[MX_LF_MAXRAD]; // radices (4, 2, 3, 5), product n
@@ -1463,6 +1474,22 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
 // ======================================================================
 constexpr int MX_MAXLAUNCH = 8; // kernels of one step
 
+// The kernels of one queued step in stream order; arg points into the
+// handle's argument blocks (a graph's kernel nodes copy them).
+struct StepLaunch {
+    const void *f;
+    dim3 grid, block;
+    void *arg;
+};
+struct StepPlan {
+    StepLaunch L[MX_MAXLAUNCH];
+    int nl = 0;
+    int key = 0;       // the kernel set: a graph built for one key fits any step of it
+    int slot_llt = -1; // index of the uplink-OTF kernel, or -1
+    int slot_img = 0;  // index of the first imaging kernel
+    bool llt = false;
+};
+
 struct mx_wfsgrad_s {
     int device = 0;
     int nreal = 1; // realisations; WFS r * (nwfs / nreal) + i is realisation r's WFS i
@@ -1513,6 +1540,8 @@ struct mx_wfsgrad_s {
     MfmaWvl h_mwv[MX_MAXWVL] = {};
     // large-field form: line-FFT plans of nwvf and notf, per-chain buffers
     bool lf = false;
+    int lf_threads = 256;              // threads per large-field block
+    int opd_tile = 0;                  // ray-trace tile side (16 or 32), 0: the default
     LfPlan lf_pn{}, lf_pm{};
     float2 *d_b1 = nullptr, *d_b2 = nullptr;
     float *d_img = nullptr;
@@ -2102,7 +2131,12 @@ int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
     return mx_atm2gpu_real(0, layers, nps, atmscale);
 }
 
-int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
+} // extern "C"
+
+namespace {
+// Check the engine's state and fill its argument blocks and kernel list for
+// step isim (the body of mx_wfsgrad_queue before any launch).
+int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     if (!h) return set_err(MX_EINVAL, "mx_wfsgrad_queue: null handle");
     for (int r = 0; r < h->nreal; r++) {
         if (g_atm[r].nps < 0) return set_err(MX_ESTATE, "no screens uploaded for realisation %d (mx_atm2gpu)", r);
@@ -2168,10 +2202,10 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     o.llt_ox = h->cfg.llt_ox;
     o.llt_oy = h->cfg.llt_oy;
     o.llt_opd = h->d_llt_opd;
-    // the large-field form traces a subaperture grid as tiles of 32 or 16
-    const int opd_nx = (h->lf && h->nxsa % 32 == 0) || h->nxsa == 32 ? 32 : 16;
+    // the ray trace covers a subaperture grid with tiles of 32 or 16 points
+    // a side (an LLT grid is one tile of nxsa)
+    const int opd_nx = h->opd_tile ? h->opd_tile : ((h->lf && h->nxsa % 32 == 0) || h->nxsa == 32 ? 32 : 16);
     o.tiles = (int)(h->nxsa / opd_nx);
-    o.direct = direct ? 1 : 0;
     KArgs &a = h->arg_step;
     a = KArgs{};
     a.nsa = (int)h->nsa;
@@ -2265,16 +2299,20 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     // The step in stream order: ray trace (subaperture and LLT grids), uplink
     // OTF (the MFMA kernel's uplink mode with the MFMA form, else the FP32 FFT
     // kernel), imaging.
-    struct Launch {
-        const void *f;
-        dim3 grid, block;
-        void *arg;
-    };
-    Launch L[MX_MAXLAUNCH];
+    StepLaunch *L = sp.L;
     int nl = 0;
     const bool mf = variant == MX_VARIANT_MFMA_BF16;
-    L[nl++] = {opd_nx == 32 ? reinterpret_cast<const void *>(&k_wfs_opd<32>)
-                            : reinterpret_cast<const void *>(&k_wfs_opd<16>),
+    const void *kopd_t[2][2][2] = {
+        {{reinterpret_cast<const void *>(&k_wfs_opd<16, false, false>),
+          reinterpret_cast<const void *>(&k_wfs_opd<16, false, true>)},
+         {reinterpret_cast<const void *>(&k_wfs_opd<16, true, false>),
+          reinterpret_cast<const void *>(&k_wfs_opd<16, true, true>)}},
+        {{reinterpret_cast<const void *>(&k_wfs_opd<32, false, false>),
+          reinterpret_cast<const void *>(&k_wfs_opd<32, false, true>)},
+         {reinterpret_cast<const void *>(&k_wfs_opd<32, true, false>),
+          reinterpret_cast<const void *>(&k_wfs_opd<32, true, true>)}}};
+    const void *kopd = kopd_t[opd_nx == 32][direct][o.tiles > 1];
+    L[nl++] = {kopd,
                dim3((unsigned)(h->nsa * h->nwfs * o.tiles * o.tiles + o.nllt)), dim3(MX_OPD_THREADS), &o};
     const int slot_llt = llt ? nl : -1;
     if (llt) {
@@ -2331,7 +2369,7 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
         g.grad = a.grad;
         g.i0acc = a.i0acc;
         const unsigned nchain = (unsigned)(h->nwfs * h->nsa * h->nwvl);
-        const dim3 blk(MX_LF_THREADS);
+        const dim3 blk((unsigned)h->lf_threads);
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_embed_rows), dim3((unsigned)h->nwvf, nchain), blk, &f};
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_psf_cols), dim3((unsigned)N, nchain), blk, &f};
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_otf<0>), dim3((unsigned)N, nchain), blk, &f};
@@ -2347,22 +2385,42 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
                            : reinterpret_cast<const void *>(&k_shwfs_step<32, 16, 2>),
                    dim3((unsigned)(ngroup * h->nwfs)), dim3(64), &a};
     }
+    sp.nl = nl;
+    sp.key = (1 << 30) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) | (direct ? 8 : 0) | (variant << 4) |
+             (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) | (h->lf ? h->lf_threads << 14 : 0);
+    sp.slot_llt = slot_llt;
+    sp.slot_img = slot_img;
+    sp.llt = llt;
+    return MX_OK;
+}
+
+// kernel-node parameters of a plan; argv holds the argument-pointer arrays
+void node_params(const StepPlan &sp, hipKernelNodeParams *kp, void *(*argv)[1]) {
+    for (int i = 0; i < sp.nl; i++) {
+        argv[i][0] = sp.L[i].arg;
+        kp[i] = hipKernelNodeParams{};
+        kp[i].func = const_cast<void *>(sp.L[i].f);
+        kp[i].gridDim = sp.L[i].grid;
+        kp[i].blockDim = sp.L[i].block;
+        kp[i].sharedMemBytes = 0;
+        kp[i].kernelParams = argv[i];
+        kp[i].extra = nullptr;
+    }
+}
+
+// launch a planned step on the engine's stream: one graph launch, or the
+// kernels one by one (with the timing events when timing is on)
+int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
+    const StepLaunch *L = sp.L;
+    const int nl = sp.nl, slot_llt = sp.slot_llt, slot_img = sp.slot_img;
+    const bool llt = sp.llt;
     if (h->use_graph && !h->timing) {
         // one graph launch per step: the graph is built for the kernel set and
         // its nodes get this step's arguments
-        const int key = 1 + (variant << 4) + (llt ? 2 : 0) + (N == 64 ? 1 : 0) + (h->lf ? 4 : 0) + (opd_nx << 8);
+        const int key = sp.key;
         hipKernelNodeParams kp[MX_MAXLAUNCH];
         void *argv[MX_MAXLAUNCH][1];
-        for (int i = 0; i < nl; i++) {
-            argv[i][0] = L[i].arg;
-            kp[i] = hipKernelNodeParams{};
-            kp[i].func = const_cast<void *>(L[i].f);
-            kp[i].gridDim = L[i].grid;
-            kp[i].blockDim = L[i].block;
-            kp[i].sharedMemBytes = 0;
-            kp[i].kernelParams = argv[i];
-            kp[i].extra = nullptr;
-        }
+        node_params(sp, kp, argv);
         if (!h->gexec || h->gkey != key) {
             drop_graph(h);
             HIPCHK(hipGraphCreate(&h->graph, 0));
@@ -2408,8 +2466,134 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
             h->t_n++;
         }
     }
+    return MX_OK;
+}
+
+// bookkeeping after a step was queued
+void step_queued(mx_wfsgrad_t *h) {
     if (h->i0_enable) h->i0_nframe++;
     h->queued = true;
+}
+} // namespace
+
+extern "C" {
+
+int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
+    StepPlan sp;
+    int rc = plan_step(h, isim, dt, sp);
+    if (rc) return rc;
+    if ((rc = launch_step(h, sp))) return rc;
+    step_queued(h);
+    return MX_OK;
+}
+
+} // extern "C"
+
+// Several engines stepped by one graph launch: engine e's kernels form
+// branch e of the graph, and the branches run concurrently.
+struct mx_wfsgroup_s {
+    int device = 0;
+    std::vector<mx_wfsgrad_t *> h;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;            // the group's step finished
+    std::vector<hipEvent_t> ready;        // engine e's earlier work finished
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<hipGraphNode_t> node;     // [e * MX_MAXLAUNCH + i]
+    std::vector<int> key;                 // kernel-set key of each branch
+};
+
+extern "C" {
+
+int mx_wfsgroup_create(mx_wfsgroup_t **out, mx_wfsgrad_t *const *h, int n) {
+    if (!out || !h || n < 1) return set_err(MX_EINVAL, "mx_wfsgroup_create: bad argument");
+    *out = nullptr;
+    for (int e = 0; e < n; e++) {
+        if (!h[e]) return set_err(MX_EINVAL, "mx_wfsgroup_create: engine %d is null", e);
+        if (h[e]->device != h[0]->device) return set_err(MX_EINVAL, "engines of a group must share one GPU");
+        for (int f = 0; f < e; f++)
+            if (h[f] == h[e]) return set_err(MX_EINVAL, "engine %d appears twice in the group", e);
+    }
+    HIPCHK(hipSetDevice(h[0]->device));
+    mx_wfsgroup_t *g = new mx_wfsgroup_t();
+    g->device = h[0]->device;
+    g->h.assign(h, h + n);
+    g->ready.assign(n, nullptr);
+    g->node.assign((size_t)n * MX_MAXLAUNCH, nullptr);
+    g->key.assign(n, 0);
+    bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&g->done, hipEventDisableTiming) == hipSuccess;
+    for (int e = 0; ok && e < n; e++) ok = hipEventCreateWithFlags(&g->ready[e], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        mx_wfsgroup_free(g);
+        return set_err(MX_EHIP, "mx_wfsgroup_create: stream or event creation failed");
+    }
+    *out = g;
+    return MX_OK;
+}
+
+int mx_wfsgroup_queue(mx_wfsgroup_t *g, long isim, double dt) {
+    if (!g) return set_err(MX_EINVAL, "mx_wfsgroup_queue: null group");
+    HIPCHK(hipSetDevice(g->device));
+    const int n = (int)g->h.size();
+    std::vector<StepPlan> sp(n);
+    for (int e = 0; e < n; e++) {
+        int rc = plan_step(g->h[e], isim, dt, sp[e]);
+        if (rc) return rc;
+    }
+    bool same = g->gexec != nullptr;
+    for (int e = 0; e < n && same; e++) same = g->key[e] == sp[e].key;
+    std::vector<hipKernelNodeParams> kp((size_t)n * MX_MAXLAUNCH);
+    std::vector<void *> argv((size_t)n * MX_MAXLAUNCH);
+    for (int e = 0; e < n; e++)
+        node_params(sp[e], kp.data() + (size_t)e * MX_MAXLAUNCH,
+                    reinterpret_cast<void *(*)[1]>(argv.data() + (size_t)e * MX_MAXLAUNCH));
+    if (!same) {
+        if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        g->gexec = nullptr;
+        g->graph = nullptr;
+        HIPCHK(hipGraphCreate(&g->graph, 0));
+        for (int e = 0; e < n; e++) {
+            hipGraphNode_t *nd = g->node.data() + (size_t)e * MX_MAXLAUNCH;
+            for (int i = 0; i < sp[e].nl; i++)
+                HIPCHK(hipGraphAddKernelNode(&nd[i], g->graph, i ? &nd[i - 1] : nullptr, i ? 1 : 0,
+                                             &kp[(size_t)e * MX_MAXLAUNCH + i]));
+            g->key[e] = sp[e].key;
+        }
+        HIPCHK(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
+    } else {
+        for (int e = 0; e < n; e++)
+            for (int i = 0; i < sp[e].nl; i++)
+                HIPCHK(hipGraphExecKernelNodeSetParams(g->gexec, g->node[(size_t)e * MX_MAXLAUNCH + i],
+                                                       &kp[(size_t)e * MX_MAXLAUNCH + i]));
+    }
+    // the step starts after each engine's earlier work; each engine's stream
+    // then waits for the step, so mx_wfsgrad_sync / wait see its results
+    for (int e = 0; e < n; e++) {
+        HIPCHK(hipEventRecord(g->ready[e], g->h[e]->stream));
+        HIPCHK(hipStreamWaitEvent(g->stream, g->ready[e], 0));
+    }
+    HIPCHK(hipGraphLaunch(g->gexec, g->stream));
+    HIPCHK(hipEventRecord(g->done, g->stream));
+    for (int e = 0; e < n; e++) {
+        HIPCHK(hipStreamWaitEvent(g->h[e]->stream, g->done, 0));
+        step_queued(g->h[e]);
+    }
+    return MX_OK;
+}
+
+int mx_wfsgroup_free(mx_wfsgroup_t *g) {
+    if (!g) return MX_OK;
+    hipSetDevice(g->device);
+    if (g->stream) hipStreamSynchronize(g->stream);
+    if (g->gexec) hipGraphExecDestroy(g->gexec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    for (hipEvent_t e : g->ready)
+        if (e) hipEventDestroy(e);
+    if (g->done) hipEventDestroy(g->done);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
     return MX_OK;
 }
 
@@ -2513,6 +2697,85 @@ int mx_wfsgrad_set_graph(mx_wfsgrad_t *h, int enable) {
     HIPCHK(hipStreamSynchronize(h->stream));
     if (!enable) drop_graph(h);
     h->use_graph = enable != 0;
+    return MX_OK;
+}
+
+int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep, char *report, long report_len) {
+    if (!h || nrep < 1) return set_err(MX_EINVAL, "mx_wfsgrad_autotune: bad argument");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    struct Cand {
+        int variant, tile, threads;
+        double us;
+    };
+    std::vector<Cand> cand;
+    const bool llt = h->cfg.llt_nx > 0;
+    std::vector<int> tiles;
+    if (llt) {
+        tiles.push_back((int)h->nxsa); // the LLT grid is one tile
+    } else {
+        if (h->nxsa % 32 == 0) tiles.push_back(32);
+        tiles.push_back(16);
+    }
+    std::vector<int> variants;
+    if (h->lf) {
+        variants.push_back(MX_VARIANT_FFT32);
+    } else {
+        const int saved = h->variant;
+        h->variant = MX_VARIANT_MFMA_BF16;
+        if (effective_variant(h) == MX_VARIANT_MFMA_BF16) variants.push_back(MX_VARIANT_MFMA_BF16);
+        h->variant = saved;
+        variants.push_back(MX_VARIANT_FFT32);
+    }
+    const std::vector<int> threads = h->lf ? std::vector<int>{128, 256, 512} : std::vector<int>{h->lf_threads};
+    for (int v : variants)
+        for (int t : tiles)
+            for (int th : threads) cand.push_back({v, t, th, 0.});
+    const int sv = h->variant, st = h->opd_tile, sth = h->lf_threads;
+    const bool si0 = h->i0_enable, stim = h->timing;
+    const long sn = h->i0_nframe;
+    h->i0_enable = false; // tuning steps leave the i0 sums alone
+    int rc = MX_OK;
+    for (Cand &c : cand) {
+        h->variant = c.variant;
+        h->opd_tile = c.tile;
+        h->lf_threads = c.threads;
+        h->timing = false;
+        for (int r = 0; r < 2 && !rc; r++) rc = mx_wfsgrad_queue(h, isim, dt); // warm the kernel set
+        if (rc) break;
+        mx_wfsgrad_timing_enable(h, 1);
+        for (int r = 0; r < nrep && !rc; r++) rc = mx_wfsgrad_queue(h, isim + r, dt);
+        if (rc) break;
+        c.us = (h->t_llt + h->t_opd + h->t_ints) / (double)h->t_n * 1e3;
+    }
+    h->timing = stim;
+    h->t_n = 0;
+    h->t_llt = h->t_opd = h->t_ints = 0;
+    h->i0_enable = si0;
+    h->i0_nframe = sn;
+    if (rc) {
+        h->variant = sv;
+        h->opd_tile = st;
+        h->lf_threads = sth;
+        return rc;
+    }
+    size_t best = 0;
+    for (size_t i = 1; i < cand.size(); i++)
+        if (cand[i].us < cand[best].us) best = i;
+    h->variant = cand[best].variant;
+    h->opd_tile = cand[best].tile;
+    h->lf_threads = cand[best].threads;
+    if (report && report_len > 0) {
+        std::string txt;
+        char line[160];
+        for (size_t i = 0; i < cand.size(); i++) {
+            std::snprintf(line, sizeof line, "%sform=%s opd_tile=%d threads=%d step_us=%.2f\n", i == best ? "*" : " ",
+                          cand[i].variant == MX_VARIANT_MFMA_BF16 ? "mfma_bf16" : (h->lf ? "lf_fft32" : "fft32"),
+                          cand[i].tile, h->lf ? cand[i].threads : 0, cand[i].us);
+            txt += line;
+        }
+        std::snprintf(report, (size_t)report_len, "%s", txt.c_str());
+    }
     return MX_OK;
 }
 

@@ -145,6 +145,20 @@ class WfsGrad:
         self.set_variant(cur)
         return True
 
+    def autotune(self, isim, dt, nrep=10):
+        """Time each candidate (imaging form, ray-trace tile, large-field
+        block size) and keep the fastest; returns a list of dicts, the chosen
+        one with best=True."""
+        buf = C.create_string_buffer(4096)
+        check(_lib.hip().mx_wfsgrad_autotune(self.h, isim, dt, nrep, buf, len(buf)))
+        out = []
+        for line in buf.value.decode().splitlines():
+            best = line.startswith("*")
+            kv = dict(t.split("=") for t in line[1:].split())
+            out.append({"form": kv["form"], "opd_tile": int(kv["opd_tile"]), "threads": int(kv["threads"]),
+                        "step_us": float(kv["step_us"]), "best": best})
+        return out
+
     def timing_enable(self, on=True):
         check(_lib.hip().mx_wfsgrad_timing_enable(self.h, int(on)))
 
@@ -162,6 +176,36 @@ class WfsGrad:
         if getattr(self, "h", None):
             _lib.hip().mx_wfsgrad_free(self.h)
             self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WfsGroup:
+    """Several WfsGrad engines stepped by one graph launch per step, each
+    engine a concurrent branch (mx_wfsgroup_*).  sync/wait stay per engine."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        g = C.c_void_p()
+        check(_lib.hip().mx_wfsgroup_create(C.byref(g), arr, len(self.engines)))
+        self.g = g
+
+    def queue(self, isim, dt):
+        check(_lib.hip().mx_wfsgroup_queue(self.g, isim, dt))
+
+    def wait(self):
+        for e in self.engines:
+            e.wait()
+
+    def close(self):
+        if getattr(self, "g", None):
+            _lib.hip().mx_wfsgroup_free(self.g)
+            self.g = None
 
     def __del__(self):
         try:

@@ -282,6 +282,88 @@ def test_large_field_parity(case, request):
     eng.close()
 
 
+def test_group_matches_single(lgs_small, ngs_lo_small, ngs_tt_small):
+    """A heterogeneous step (LGS on the MFMA form, low-order NGS and TT NGS
+    on the large-field form) queued as one group graph gives bit-identical
+    gradients, intensities and i0 sums to each engine queued on its own,
+    over steps whose kernel arguments change, and the group is rebuilt when
+    an engine's kernel set changes.  The three fixtures share one
+    atmosphere."""
+    from maos_amd import _lib, configs, engine
+    cfg_l, p_l, layers = lgs_small
+    print("case: group of lgs order20 (mfma) + ngs_lo + ngs_tt", flush=True)
+    engine.init(0)
+    engine.atm2gpu(layers)
+    cases = [(cfg_l, p_l), ngs_lo_small[:2], ngs_tt_small[:2]]
+    o = _oracle(p_l, layers)
+    dt = cfg_l["sim"].dt
+    wfs_l = configs.wfs_specs(cfg_l, p_l)
+    mtche, i0sum = _lgs_ref(p_l, o, wfs_l, dt)
+    pairs = []
+    for cfg, p in cases:
+        wfs = configs.wfs_specs(cfg, p)
+        a, b = engine.WfsGrad(p, wfs), engine.WfsGrad(p, wfs)
+        for eng in (a, b):
+            if p.llt_nx:
+                eng.update_ref(mtche, i0sum)
+            eng.i0_control(enable=True, reset=True)
+        pairs.append((a, b, len(wfs)))
+    group = engine.WfsGroup([b for _, b, _ in pairs])
+    for isim in (0, 1, 5, 6):
+        if isim == 5:  # another kernel set for the LGS engine: the group graph is rebuilt
+            for eng in pairs[0][:2]:
+                eng.set_variant(_lib.VARIANT_FFT32)
+        for a, _, _ in pairs:
+            a.queue(isim, dt)
+        group.queue(isim, dt)
+        for a, b, nw in pairs:
+            for iw in range(nw):
+                ga, ia = a.sync(iw)
+                gb, ib = b.sync(iw)
+                assert np.array_equal(ga, gb) and np.array_equal(ia, ib)
+    for a, b, nw in pairs:
+        for iw in range(nw):
+            i0a, na = a.i0_get(iw)
+            i0b, nb = b.i0_get(iw)
+            assert na == nb == 4
+            assert np.array_equal(i0a, i0b)
+    group.close()
+    for a, b, _ in pairs:
+        a.close()
+        b.close()
+
+
+@pytest.mark.parametrize("case", ["lgs_small", "ngs_lo_small"])
+def test_autotune(case, request):
+    """The autotuner times every candidate, keeps one, leaves the i0 sums
+    and frame count alone, and the chosen configuration matches the
+    oracle."""
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: autotune %s" % case, flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    if p.llt_nx:
+        mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+        o.set_ref(mtche, i0sum)
+        eng.update_ref(mtche, i0sum)
+    eng.i0_control(enable=True, reset=True)
+    eng.queue(0, dt)
+    i0a, na = eng.i0_get(0)
+    tune = eng.autotune(3, dt, nrep=3)
+    print(tune)
+    assert sum(t["best"] for t in tune) == 1
+    assert len(tune) >= 2 and all(t["step_us"] > 0 for t in tune)
+    i0b, nb = eng.i0_get(0)
+    assert na == nb == 1 and np.array_equal(i0a, i0b)
+    eng.queue(40, dt)
+    g, ints = eng.sync(0)
+    ri, rg, _, _ = o.step(wfs[0], 40, dt)
+    assert rel_l2(ints, ri) <= INTS_TOL
+    assert np.abs(g - rg).max() * RAD2MAS <= GRAD_TOL_MAS
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C
