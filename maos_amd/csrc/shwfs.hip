@@ -1185,51 +1185,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 }
 
 // ------------------------------------------------------------ large fields
-// Fields larger than a workgroup holds (nwvf, notf up to 1024, lengths whose
-// prime factors are 2, 3 and 5; the 2x2 and 1x1 NGS of 15 and 30 m, or the
-// 128^2 padding of an LGS subaperture).  Each (WFS, subaperture, wavelength)
-// chain c = (iwfs nsa + isa) nwvl + iwvl has two global buffers, B1 (nwvf^2
-// complex) and B2 (notf^2 complex), and an image (notf^2 real).  The 2-D
-// transforms are batched 1-D line transforms, one workgroup per line with
-// the line in LDS, in five fused passes:
-//   1 embed amp exp(+i 2 pi opd/wvl) (cembed_wvf) and FFT the rows of B1
-//   2 FFT the columns of B1, |E|^2, corner copy into B2 (ccpcorner,
-//     math/matcomp.c:176-216; zero elsewhere)
-//   3 FFT the rows of B2
-//   4 FFT the columns of B2, x nominal DTF, inverse FFT the columns
-//   5 inverse FFT the rows of B2, real part -> image
-// then k_lf_sample applies si and the pixel processing.
+// Fields larger than the one-workgroup subaperture forms hold: nwvf and notf
+// up to 1024 with prime factors 2, 3 and 5 (the 2x2 and 1x1 NGS of 15 and
+// 30 m, a 128^2 padding of an LGS subaperture).  Each (WFS, subaperture,
+// wavelength) chain c = (iwfs nsa + isa) nwvl + iwvl runs five batched
+// line-transform passes; a block transforms L lines held in LDS (rows, or L
+// adjacent columns read and written as L-wide row segments):
+//   A  embed amp exp(+i 2 pi opd/wvl) (cembed_wvf, math/matcomp.c:56-82) on
+//      the nxsa rows of the field band, FFT along x  -> B1 [nxsa][nwvf]
+//   B  FFT the columns of B1 (rows outside the band are zero), |E|^2
+//                                                     -> PSF [nwvf][nwvf]
+//   C  FFT the PSF rows padded to notf by the corner copy (ccpcorner,
+//      math/matcomp.c:176-216: x < nwvf/2 stays, the rest moves up by
+//      notf - nwvf, the middle is zero)              -> B2 [nwvf][notf]
+//   D  FFT the columns of B2 (rows padded the same way), x nominal DTF,
+//      inverse FFT, keep the rows the detector samples -> B3 [ny][notf]
+//   E  inverse FFT of those rows, keep the sampled columns (real part)
+//                                                     -> image [ny][nx]
+// then k_lf_sample applies si (maos/wfsints.c:286) and the pixel processing.
 constexpr int MX_LF_MAXN = 1024;
 constexpr int MX_LF_MAXRAD = 12;
-constexpr int MX_LF_THREADS = 512; // most threads of a large-field block (128, 256 or 512 run)
+constexpr int MX_LF_THREADS = 512; // most threads of a block (128, 256 or 512 run)
+constexpr int MX_LF_LDS = 4160;    // complex elements of one LDS line buffer
+constexpr int MX_LF_MAXS = 128;    // most sampled rows / columns kept (else the full image)
+
 struct LfPlan {
     int n, nrad;
     int rad[MX_LF_MAXRAD]; // radices (4, 2, 3, 5), product n
+    int off[MX_LF_MAXRAD]; // twiddle offset of each stage
+    const float2 *tw;      // stage s: exp(-2 pi i r k / (Ns R)) at off[s] + k (R-1) + r-1
+};
+struct LfWvl {
+    float inv_wvl;
+    const float *nominal; // [ux][uy] (transposed), or null
+    int ny, nx;           // sampled image rows / columns
+    const int *rows, *cols;
+    const int4 *si_c;     // si in compact indices iy * nx + ix
+    const float4 *si_w;
 };
 struct LfArgs {
-    LfPlan pn, pm; // plans of nwvf and notf
-    int nwvf, notf, nxsa, nsa, nwfs, nwvl;
+    LfPlan pn, pm;        // lengths nwvf and notf
+    int nwvf, notf, nxsa, nsa, nwfs, nwvl, nchain;
+    int L, lgL;           // lines per block in this pass (a power of two)
+    int maxny, maxnx;     // B3 rows and image capacity per chain
     const float *opd, *amp;
-    const DevWvl *wv;
-    float2 *b1, *b2;
-    float *img;
+    const LfWvl *wv;
+    float2 *b1, *b2, *b3;
+    float *psf, *img;
 };
 
-// out[k] = sum_r v[r] exp(t 2 pi i r k / R), t = +-1, R in {2, 3, 4, 5}
+// a / b for 0 <= a < 2^22 from a rounded reciprocal rb of b, corrected
+__device__ __forceinline__ int qdiv(int a, int b, float rb) {
+    int q = (int)((float)a * rb);
+    const int r = a - q * b;
+    q += r >= b ? 1 : (r < 0 ? -1 : 0);
+    return q;
+}
+
 __device__ __forceinline__ float2 mul_ti(float2 z, float t) { return make_float2(-t * z.y, t * z.x); }
-__device__ __forceinline__ void small_dft(float2 (&v)[5], int R, float t) {
-    if (R == 2) {
+// v[q] <- sum_r v[r] exp(t 2 pi i r q / R), t = +-1
+template <int R>
+__device__ __forceinline__ void small_dft(float2 (&v)[R], float t) {
+    if constexpr (R == 2) {
         const float2 a = v[0], b = v[1];
         v[0] = cadd(a, b);
         v[1] = csub(a, b);
-    } else if (R == 4) {
+    } else if constexpr (R == 4) {
         const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
         const float2 s13 = cadd(v[1], v[3]), d13 = mul_ti(csub(v[1], v[3]), t);
         v[0] = cadd(s02, s13);
         v[2] = csub(s02, s13);
         v[1] = cadd(d02, d13);
         v[3] = csub(d02, d13);
-    } else if (R == 3) {
+    } else if constexpr (R == 3) {
         const float h = 0.86602540378443865f; // sqrt(3)/2
         const float2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
         const float2 m = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
@@ -1237,7 +1265,7 @@ __device__ __forceinline__ void small_dft(float2 (&v)[5], int R, float t) {
         v[0] = cadd(v[0], s);
         v[1] = cadd(m, r);
         v[2] = csub(m, r);
-    } else { // 5
+    } else { // 5: cos and sin of 2 pi/5 and 4 pi/5
         const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
         const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
         const float2 a1 = cadd(v[1], v[4]), a2 = cadd(v[2], v[3]);
@@ -1254,130 +1282,211 @@ __device__ __forceinline__ void small_dft(float2 (&v)[5], int R, float t) {
     }
 }
 
-// Stockham autosort FFT of one line in LDS (A holds the input; the result is
-// in the returned buffer, A or B): per stage of radix R and span Ns, item j
-// takes A[j + r n/R], twiddles by exp(t 2 pi i r (j%Ns)/(Ns R)), transforms,
-// and writes B[(j/Ns) Ns R + j%Ns + r Ns].  t = -1 forward (FFTW_FORWARD).
-__device__ float2 *lds_fft(float2 *A, float2 *B, const LfPlan &P, float t) {
-    float2 *src = A, *dst = B;
+// one Stockham stage of radix R over L lines (line l at l S)
+template <int R, bool INV>
+__device__ __forceinline__ void lds_stage(const float2 *src, float2 *dst, int n, int Ns, int L, int S,
+                                          const float2 *tw) {
+    const int nb = n / R, total = L * nb;
+    const float rnb = 1.f / (float)nb, rns = 1.f / (float)Ns;
+    for (int it = threadIdx.x; it < total; it += blockDim.x) {
+        const int l = qdiv(it, nb, rnb), j = it - l * nb;
+        const int g = qdiv(j, Ns, rns), k = j - g * Ns;
+        const float2 *x = src + l * S + j;
+        const float2 *w = tw + k * (R - 1);
+        float2 v[R];
+        v[0] = x[0];
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            float2 wr = w[r - 1];
+            if (INV) wr.y = -wr.y;
+            v[r] = cmul(x[r * nb], wr);
+        }
+        small_dft<R>(v, INV ? 1.f : -1.f);
+        float2 *y = dst + l * S + g * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; r++) y[r * Ns] = v[r];
+    }
+}
+
+// Stockham autosort FFT of L lines of length P.n held in src (line l at
+// src + l S); returns the buffer that holds the result (src or dst).  Stage
+// s of radix R and span Ns: item (l, j), j < n/R, takes x[j + r n/R] times
+// w^(r k) (w = exp(-+ 2 pi i / (Ns R)), k = j % Ns), transforms it and
+// writes x'[(j/Ns) Ns R + k + r Ns].  Forward is FFTW_FORWARD (exp(-i...)).
+template <bool INV>
+__device__ float2 *lds_fft(float2 *src, float2 *dst, const LfPlan &P, int L, int S) {
     int Ns = 1;
     for (int s = 0; s < P.nrad; s++) {
-        const int R = P.rad[s], nb = P.n / R;
-        for (int j = threadIdx.x; j < nb; j += blockDim.x) {
-            const int k = j % Ns;
-            float sn, cs;
-            sincospif(t * 2.f * (float)k / (float)(Ns * R), &sn, &cs);
-            const float2 w1 = make_float2(cs, sn);
-            float2 v[5];
-            float2 w = make_float2(1.f, 0.f);
-            for (int r = 0; r < R; r++) {
-                v[r] = cmul(src[j + r * nb], w);
-                w = cmul(w, w1);
-            }
-            small_dft(v, R, t);
-            const int d = (j / Ns) * Ns * R + k;
-            for (int r = 0; r < R; r++) dst[d + r * Ns] = v[r];
-        }
+        const int R = P.rad[s];
+        const float2 *tw = P.tw + P.off[s];
+        if (R == 4) lds_stage<4, INV>(src, dst, P.n, Ns, L, S, tw);
+        else if (R == 2) lds_stage<2, INV>(src, dst, P.n, Ns, L, S, tw);
+        else if (R == 3) lds_stage<3, INV>(src, dst, P.n, Ns, L, S, tw);
+        else lds_stage<5, INV>(src, dst, P.n, Ns, L, S, tw);
         __syncthreads();
-        float2 *tmp = src;
+        float2 *t = src;
         src = dst;
-        dst = tmp;
+        dst = t;
         Ns *= R;
     }
     return src;
 }
 
-__device__ __forceinline__ void lf_chain(int c, int nsa, int nwvl, int &iwfs, int &isa, int &iwvl) {
-    iwvl = c % nwvl;
-    isa = (c / nwvl) % nsa;
-    iwfs = c / (nwvl * nsa);
+// block -> (line group g of ngroup, chain c); consecutive groups of a chain
+// share an XCD and its L2
+__device__ __forceinline__ void lf_unit(const LfArgs &a, int ngroup, int &g, int &c) {
+    const int u = xcd_unit((int)blockIdx.x, ngroup * a.nchain);
+    g = u % ngroup;
+    c = u / ngroup;
+}
+__device__ __forceinline__ void lf_chain(const LfArgs &a, int c, int &iwfs, int &isa, int &iwvl) {
+    iwvl = c % a.nwvl;
+    isa = (c / a.nwvl) % a.nsa;
+    iwfs = c / (a.nwvl * a.nsa);
+}
+// padded position -> source index of the corner copy (-1 in the zero middle)
+__device__ __forceinline__ int corner_src(int x2, int m, int n) {
+    const int h = n / 2;
+    return x2 < h ? x2 : (x2 >= m - h ? x2 - (m - n) : -1);
 }
 
-// pass 1: embed and FFT the rows of B1 (rows outside the field band are zero)
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_embed_rows(LfArgs a) {
-    __shared__ float2 A[MX_LF_MAXN], B[MX_LF_MAXN];
-    const int y = blockIdx.x, c = blockIdx.y, n = a.nwvf;
-    int iwfs, isa, iwvl;
-    lf_chain(c, a.nsa, a.nwvl, iwfs, isa, iwvl);
-    float2 *out = a.b1 + (long)c * n * n + (long)y * n;
-    const int skip = (n - a.nxsa) / 2, iy = y - skip;
-    if (iy < 0 || iy >= a.nxsa) {
-        for (int x = threadIdx.x; x < n; x += blockDim.x) out[x] = make_float2(0.f, 0.f);
-        return;
-    }
-    const long np = (long)a.nxsa * a.nxsa;
-    const float *op = a.opd + ((long)iwfs * a.nsa + isa) * np + (long)iy * a.nxsa;
-    const float *am = a.amp + (long)isa * np + (long)iy * a.nxsa;
+// pass A: embed the field-band rows and FFT them along x
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_a(LfArgs a) {
+    __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
+    const int n = a.nwvf, S = n + 1, L = a.L, nxsa = a.nxsa, skip = (n - nxsa) / 2;
+    int g, c, iwfs, isa, iwvl;
+    lf_unit(a, (nxsa + L - 1) / L, g, c);
+    lf_chain(a, c, iwfs, isa, iwvl);
+    const long np = (long)nxsa * nxsa;
+    const float *op = a.opd + ((long)iwfs * a.nsa + isa) * np;
+    const float *am = a.amp + (long)isa * np;
     const float inv_wvl = a.wv[iwvl].inv_wvl;
-    for (int x = threadIdx.x; x < n; x += blockDim.x) {
-        const int ix = x - skip;
-        float2 e = make_float2(0.f, 0.f);
-        if (ix >= 0 && ix < a.nxsa) {
-            float sn, cs;
-            sincospif(2.f * op[ix] * inv_wvl, &sn, &cs);
-            e = make_float2(am[ix] * cs, am[ix] * sn);
-        }
-        A[x] = e;
-    }
-    __syncthreads();
-    const float2 *r = lds_fft(A, B, a.pn, -1.f);
-    for (int x = threadIdx.x; x < n; x += blockDim.x) out[x] = r[x];
-}
-
-// pass 2: FFT the columns of B1, |E|^2, and the corner copy into B2; the
-// blocks beyond nwvf zero the middle columns of B2
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_psf_cols(LfArgs a) {
-    __shared__ float2 A[MX_LF_MAXN], B[MX_LF_MAXN];
-    const int xb = blockIdx.x, c = blockIdx.y, n = a.nwvf, m = a.notf, h = n / 2;
-    float2 *o = a.b2 + (long)c * m * m;
-    if (xb >= n) { // middle column of B2, all zero
-        const int x2 = h + (xb - n);
-        for (int y2 = threadIdx.x; y2 < m; y2 += blockDim.x) o[(long)y2 * m + x2] = make_float2(0.f, 0.f);
-        return;
-    }
-    const float2 *in = a.b1 + (long)c * n * n + xb;
-    for (int y = threadIdx.x; y < n; y += blockDim.x) A[y] = in[(long)y * n];
-    __syncthreads();
-    const float2 *r = lds_fft(A, B, a.pn, -1.f);
-    const int x2 = xb < h ? xb : xb + (m - n);
-    for (int y2 = threadIdx.x; y2 < m; y2 += blockDim.x) {
-        const int y = y2 < h ? y2 : (y2 >= m - h ? y2 - (m - n) : -1);
+    const float rn = 1.f / (float)n;
+    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
+        const int l = qdiv(e, n, rn), x = e - l * n, iy = g * L + l, ix = x - skip;
         float2 v = make_float2(0.f, 0.f);
-        if (y >= 0) v.x = r[y].x * r[y].x + r[y].y * r[y].y;
-        o[(long)y2 * m + x2] = v;
+        if (iy < nxsa && ix >= 0 && ix < nxsa) {
+            float sn, cs;
+            sincospif(2.f * op[(long)iy * nxsa + ix] * inv_wvl, &sn, &cs);
+            const float A = am[(long)iy * nxsa + ix];
+            v = make_float2(A * cs, A * sn);
+        }
+        X[l * S + x] = v;
+    }
+    __syncthreads();
+    const float2 *r = lds_fft<false>(X, Y, a.pn, L, S);
+    float2 *out = a.b1 + (long)c * nxsa * n;
+    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
+        const int l = qdiv(e, n, rn), x = e - l * n, iy = g * L + l;
+        if (iy < nxsa) out[(long)iy * n + x] = r[l * S + x];
     }
 }
 
-// passes 3-5 on B2: MODE 0 rows forward; 1 columns forward, x nominal,
-// columns inverse; 2 rows inverse, real part to the image
-template <int MODE>
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_otf(LfArgs a) {
-    __shared__ float2 A[MX_LF_MAXN], B[MX_LF_MAXN];
-    const int l = blockIdx.x, c = blockIdx.y, m = a.notf;
-    float2 *base = a.b2 + (long)c * m * m;
-    const long estride = MODE == 1 ? m : 1, lstride = MODE == 1 ? 1 : m;
-    float2 *line = base + (long)l * lstride;
-    for (int i = threadIdx.x; i < m; i += blockDim.x) A[i] = line[i * estride];
-    __syncthreads();
-    float2 *r = lds_fft(A, B, a.pm, MODE == 2 ? 1.f : -1.f);
-    if (MODE == 1) {
-        int iwfs, isa, iwvl;
-        lf_chain(c, a.nsa, a.nwvl, iwfs, isa, iwvl);
-        const float *nm = a.wv[iwvl].nominal; // transposed: [ux][uy], ux = this column
-        if (nm)
-            for (int i = threadIdx.x; i < m; i += blockDim.x) {
-                const float f = nm[(long)l * m + i];
-                r[i] = make_float2(r[i].x * f, r[i].y * f);
-            }
-        __syncthreads();
-        float2 *other = r == A ? B : A;
-        r = lds_fft(r, other, a.pm, 1.f);
+// pass B: FFT the columns of B1 and take |E|^2
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_b(LfArgs a) {
+    __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
+    const int n = a.nwvf, S = n + 1, L = a.L, nxsa = a.nxsa, skip = (n - nxsa) / 2;
+    int g, c;
+    lf_unit(a, (n + L - 1) / L, g, c);
+    const int x0 = g * L;
+    const float2 *in = a.b1 + (long)c * nxsa * n;
+    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
+        const int l = e & (L - 1), y = e >> a.lgL, x = x0 + l, iy = y - skip;
+        float2 v = make_float2(0.f, 0.f);
+        if (x < n && iy >= 0 && iy < nxsa) v = in[(long)iy * n + x];
+        X[l * S + y] = v;
     }
-    if (MODE == 2) {
-        float *im = a.img + (long)c * m * m + (long)l * m;
-        for (int i = threadIdx.x; i < m; i += blockDim.x) im[i] = r[i].x;
-    } else {
-        for (int i = threadIdx.x; i < m; i += blockDim.x) line[i * estride] = r[i];
+    __syncthreads();
+    const float2 *r = lds_fft<false>(X, Y, a.pn, L, S);
+    float *out = a.psf + (long)c * n * n;
+    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
+        const int l = e & (L - 1), y = e >> a.lgL, x = x0 + l;
+        if (x < n) {
+            const float2 v = r[l * S + y];
+            out[(long)y * n + x] = v.x * v.x + v.y * v.y;
+        }
+    }
+}
+
+// pass C: FFT the PSF rows, padded to notf by the corner copy
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_c(LfArgs a) {
+    __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
+    const int n = a.nwvf, m = a.notf, S = m + 1, L = a.L;
+    int g, c;
+    lf_unit(a, (n + L - 1) / L, g, c);
+    const float *in = a.psf + (long)c * n * n;
+    const float rm = 1.f / (float)m;
+    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
+        const int l = qdiv(e, m, rm), x2 = e - l * m, y = g * L + l, x = corner_src(x2, m, n);
+        X[l * S + x2] = make_float2(y < n && x >= 0 ? in[(long)y * n + x] : 0.f, 0.f);
+    }
+    __syncthreads();
+    const float2 *r = lds_fft<false>(X, Y, a.pm, L, S);
+    float2 *out = a.b2 + (long)c * n * m;
+    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
+        const int l = qdiv(e, m, rm), x2 = e - l * m, y = g * L + l;
+        if (y < n) out[(long)y * m + x2] = r[l * S + x2];
+    }
+}
+
+// pass D: FFT the columns of B2 (rows padded by the corner copy), x nominal,
+// inverse FFT, keep the sampled rows
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_d(LfArgs a) {
+    __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
+    const int n = a.nwvf, m = a.notf, S = m + 1, L = a.L;
+    int g, c, iwfs, isa, iwvl;
+    lf_unit(a, (m + L - 1) / L, g, c);
+    lf_chain(a, c, iwfs, isa, iwvl);
+    const int x0 = g * L;
+    const float2 *in = a.b2 + (long)c * n * m;
+    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
+        const int l = e & (L - 1), y2 = e >> a.lgL, x2 = x0 + l, y = corner_src(y2, m, n);
+        X[l * S + y2] = x2 < m && y >= 0 ? in[(long)y * m + x2] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    float2 *r = lds_fft<false>(X, Y, a.pm, L, S);
+    const LfWvl &W = a.wv[iwvl];
+    if (W.nominal) {
+        const float rm = 1.f / (float)m;
+        for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
+            const int l = qdiv(e, m, rm), y2 = e - l * m, x2 = x0 + l;
+            if (x2 < m) {
+                const float f = W.nominal[(long)x2 * m + y2];
+                r[l * S + y2] = make_float2(r[l * S + y2].x * f, r[l * S + y2].y * f);
+            }
+        }
+        __syncthreads();
+    }
+    const float2 *q = lds_fft<true>(r, r == X ? Y : X, a.pm, L, S);
+    float2 *out = a.b3 + (long)c * a.maxny * m;
+    for (int e = threadIdx.x; e < L * W.ny; e += blockDim.x) {
+        const int l = e & (L - 1), iyy = e >> a.lgL, x2 = x0 + l;
+        if (x2 < m) out[(long)iyy * m + x2] = q[l * S + W.rows[iyy]];
+    }
+}
+
+// pass E: inverse FFT of the sampled rows, real part at the sampled columns
+__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_e(LfArgs a) {
+    __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
+    const int m = a.notf, S = m + 1, L = a.L;
+    int g, c, iwfs, isa, iwvl;
+    lf_unit(a, (a.maxny + L - 1) / L, g, c);
+    lf_chain(a, c, iwfs, isa, iwvl);
+    const LfWvl &W = a.wv[iwvl];
+    if (g * L >= W.ny) return; // uniform: no sampled row in this group
+    const float2 *in = a.b3 + (long)c * a.maxny * m;
+    const float rm = 1.f / (float)m;
+    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
+        const int l = qdiv(e, m, rm), x = e - l * m, iyy = g * L + l;
+        X[l * S + x] = iyy < W.ny ? in[(long)iyy * m + x] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const float2 *r = lds_fft<true>(X, Y, a.pm, L, S);
+    float *out = a.img + (long)c * a.maxny * a.maxnx;
+    const float rnx = 1.f / (float)W.nx;
+    for (int e = threadIdx.x; e < L * W.nx; e += blockDim.x) {
+        const int l = qdiv(e, W.nx, rnx), ixx = e - l * W.nx, iyy = g * L + l;
+        if (iyy < W.ny) out[(long)iyy * W.nx + ixx] = r[l * S + W.cols[ixx]].x;
     }
 }
 
@@ -1385,9 +1494,9 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_otf(LfArgs a) {
 // subaperture) (dspmulcreal, maos/wfsints.c:286; shwfs_grad,
 // maos/utils.c:983-1122)
 struct LfSampleArgs {
-    int nsa, nwfs, nwvl, npix, pixpsax, pixpsay, notf;
+    int nsa, nwfs, nwvl, npix, pixpsax, pixpsay, imgstride;
     const float *img;
-    const DevWvl *wv;
+    const LfWvl *wv;
     const DevWfs *wfs;
     int phytype, sigmatch;
     float cogthres, cogoff, pixthetax, pixthetay, sigtot;
@@ -1400,14 +1509,13 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
     __shared__ float red[3][MX_LF_THREADS / 64];
     const int isa = blockIdx.x % a.nsa, iwfs = blockIdx.x / a.nsa;
     const DevWfs &W = a.wfs[iwfs];
-    const long m2 = (long)a.notf * a.notf;
     const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     for (int p = threadIdx.x; p < a.npix; p += blockDim.x) {
         float pv = 0.f;
         for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
-            const float *im = a.img + (((long)iwfs * a.nsa + isa) * a.nwvl + iwvl) * m2;
-            const int4 id = a.wv[iwvl].si_idx[p];
+            const float *im = a.img + (((long)iwfs * a.nsa + isa) * a.nwvl + iwvl) * a.imgstride;
+            const int4 id = a.wv[iwvl].si_c[p];
             const float4 w = a.wv[iwvl].si_w[p];
             pv += W.alpha[iwvl] * (w.x * im[id.x] + w.y * im[id.y] + w.z * im[id.z] + w.w * im[id.w]);
         }
@@ -1542,10 +1650,17 @@ struct mx_wfsgrad_s {
     bool lf = false;
     int lf_threads = 256;              // threads per large-field block
     int opd_tile = 0;                  // ray-trace tile side (16 or 32), 0: the default
-    LfPlan lf_pn{}, lf_pm{};
-    float2 *d_b1 = nullptr, *d_b2 = nullptr;
-    float *d_img = nullptr;
-    LfArgs arg_lf{};
+    LfPlan lf_pn{}, lf_pm{};           // twiddles in d_lf_tw
+    float2 *d_lf_tw = nullptr;
+    float2 *d_b1 = nullptr, *d_b2 = nullptr, *d_b3 = nullptr;
+    float *d_psf = nullptr, *d_img = nullptr;
+    int lf_ny[MX_MAXWVL] = {}, lf_nx[MX_MAXWVL] = {}; // sampled rows / columns per wavelength
+    int *d_lf_rows[MX_MAXWVL] = {}, *d_lf_cols[MX_MAXWVL] = {};
+    int4 *d_lf_sic[MX_MAXWVL] = {};
+    int lf_maxny = 0, lf_maxnx = 0;    // capacity of B3 and the image per chain
+    LfWvl *d_lfw = nullptr;
+    LfWvl h_lfw[MX_MAXWVL] = {};
+    LfArgs arg_lf[5] = {};
     LfSampleArgs arg_lfs{};
     float *d_mtche = nullptr, *d_i0sum = nullptr;
     bool have_mtche = false, have_i0sum = false;
@@ -1614,6 +1729,31 @@ bool lf_plan(long n, LfPlan &P) {
             r /= radix[k];
         }
     return r == 1;
+}
+
+// twiddles exp(-2 pi i r k / (Ns R)) of a plan's stages (k < Ns, 1 <= r <
+// R) appended to tw; sets P.off relative to the plan's first entry
+void lf_twiddles(LfPlan &P, std::vector<float2> &tw) {
+    const size_t base = tw.size();
+    int Ns = 1;
+    for (int s = 0; s < P.nrad; s++) {
+        const int R = P.rad[s];
+        P.off[s] = (int)(tw.size() - base);
+        for (int k = 0; k < Ns; k++)
+            for (int r = 1; r < R; r++) {
+                const double ang = -2. * M_PI * (double)(r * k) / (double)(Ns * R);
+                tw.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+            }
+        Ns *= R;
+    }
+}
+
+// lines of length n per large-field block: a power of two <= 32 whose lines
+// (stride n + 1) fit one LDS buffer
+int lf_lines(int n) {
+    int L = 32;
+    while (L > 1 && L * (n + 1) > MX_LF_LDS) L /= 2;
+    return L;
 }
 
 // the large-field form applies: OPD grid a multiple of 16 points, even
@@ -1823,13 +1963,21 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     ALLOC(h->d_i0sum, nwfs * nsa);
     h->mfma_size = (N == 64 && h->nxsa == 32);
     h->lf = lf;
+    std::vector<float2> lf_tw;
     if (lf) {
         lf_plan(h->nwvf, h->lf_pn);
         lf_plan(N, h->lf_pm);
+        lf_twiddles(h->lf_pn, lf_tw);
+        const size_t nt_n = lf_tw.size();
+        lf_twiddles(h->lf_pm, lf_tw);
         const long nchain = (long)nwfs * nsa * h->nwvl;
-        ALLOC(h->d_b1, nchain * h->nwvf * h->nwvf);
-        ALLOC(h->d_b2, nchain * N * N);
-        ALLOC(h->d_img, nchain * N * N);
+        ALLOC(h->d_lf_tw, lf_tw.size());
+        h->lf_pn.tw = h->d_lf_tw;
+        h->lf_pm.tw = h->d_lf_tw + nt_n;
+        ALLOC(h->d_b1, nchain * h->nxsa * h->nwvf);
+        ALLOC(h->d_psf, nchain * h->nwvf * h->nwvf);
+        ALLOC(h->d_b2, nchain * h->nwvf * N);
+        ALLOC(h->d_lfw, MX_MAXWVL);
     }
     if (p->llt_nx > 0) {
         ALLOC(h->d_llt_amp, p->llt_nx * p->llt_nx);
@@ -1879,6 +2027,8 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     }
     if (hipMemcpy(h->d_wfs, dw.data(), sizeof(DevWfs) * nwfs, hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_err(MX_EHIP, "upload of WFS parameters failed"));
+    if (lf && hipMemcpy(h->d_lf_tw, lf_tw.data(), sizeof(float2) * lf_tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(MX_EHIP, "upload of the FFT twiddles failed"));
     if (h->mfma_size) {
         std::vector<uint16_t> F = build_F64();
         if (hipMemcpy(h->d_F, F.data(), F.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
@@ -1908,7 +2058,8 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     drop_graph(h);
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
-                    h->d_lotf_t, h->d_F, h->d_mwv, h->d_b1, h->d_b2, h->d_img,
+                    h->d_lotf_t, h->d_F, h->d_mwv, h->d_lf_tw, h->d_b1, h->d_b2, h->d_b3, h->d_psf, h->d_img,
+                    h->d_lfw,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
     for (void *q : ptrs)
         if (q) hipFree(q);
@@ -1923,6 +2074,9 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
         if (h->d_etf2[w]) hipFree(h->d_etf2[w]);
         if (h->d_si_idx[w]) hipFree(h->d_si_idx[w]);
         if (h->d_si_w[w]) hipFree(h->d_si_w[w]);
+        if (h->d_lf_rows[w]) hipFree(h->d_lf_rows[w]);
+        if (h->d_lf_cols[w]) hipFree(h->d_lf_cols[w]);
+        if (h->d_lf_sic[w]) hipFree(h->d_lf_sic[w]);
     }
     for (hipEvent_t e : h->ev) hipEventDestroy(e);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -2010,6 +2164,66 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
                 if ((rc = dev_alloc(&h->d_nominal_t[iwvl], MX_HALF))) return rc;
                 HIPCHK(hipMemcpy(h->d_nominal_t[iwvl], nt.data(), sizeof(float) * MX_HALF, hipMemcpyHostToDevice));
             }
+        }
+    }
+    if (h->lf) {
+        // image rows / columns the sampler reads; all of them past MX_LF_MAXS
+        std::vector<int> xs, ys;
+        for (long i = 0; i < h->npix; i++)
+            for (int c = 0; c < 4; c++)
+                if ((&wt[i].x)[c] != 0.f) {
+                    xs.push_back((&idx[i].x)[c] % (int)N);
+                    ys.push_back((&idx[i].x)[c] / (int)N);
+                }
+        for (std::vector<int> *v : {&xs, &ys}) {
+            std::sort(v->begin(), v->end());
+            v->erase(std::unique(v->begin(), v->end()), v->end());
+            if (v->size() > (size_t)MX_LF_MAXS || v->empty()) {
+                v->resize(N);
+                for (long k = 0; k < N; k++) (*v)[k] = (int)k;
+            }
+        }
+        const int nx = (int)xs.size(), ny = (int)ys.size();
+        std::vector<int4> sic(h->npix);
+        for (long i = 0; i < h->npix; i++)
+            for (int c = 0; c < 4; c++) {
+                const int id = (&idx[i].x)[c];
+                const int cx = (int)(std::lower_bound(xs.begin(), xs.end(), id % (int)N) - xs.begin());
+                const int cy = (int)(std::lower_bound(ys.begin(), ys.end(), id / (int)N) - ys.begin());
+                (&sic[i].x)[c] = (&wt[i].x)[c] != 0.f ? cy * nx + cx : 0;
+            }
+        HIPCHK(hipStreamSynchronize(h->stream)); // no queued step reads the tables replaced here
+        for (int **pp : {&h->d_lf_rows[iwvl], &h->d_lf_cols[iwvl]})
+            if (*pp) {
+                HIPCHK(hipFree(*pp));
+                *pp = nullptr;
+            }
+        if ((rc = dev_alloc(&h->d_lf_rows[iwvl], (size_t)ny)) || (rc = dev_alloc(&h->d_lf_cols[iwvl], (size_t)nx)) ||
+            (rc = dev_alloc(&h->d_lf_sic[iwvl], h->npix)))
+            return rc;
+        HIPCHK(hipMemcpy(h->d_lf_rows[iwvl], ys.data(), sizeof(int) * ny, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_lf_cols[iwvl], xs.data(), sizeof(int) * nx, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_lf_sic[iwvl], sic.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
+        h->lf_ny[iwvl] = ny;
+        h->lf_nx[iwvl] = nx;
+        // B3 and the image hold the largest sampled set of any wavelength
+        int maxny = 0, maxnx = 0;
+        for (int w = 0; w < h->nwvl; w++) {
+            maxny = std::max(maxny, h->lf_ny[w]);
+            maxnx = std::max(maxnx, h->lf_nx[w]);
+        }
+        if (maxny > h->lf_maxny || maxnx > h->lf_maxnx) {
+            const long nchain = (long)h->nwfs * h->nsa * h->nwvl;
+            if (h->d_b3) HIPCHK(hipFree(h->d_b3));
+            if (h->d_img) HIPCHK(hipFree(h->d_img));
+            h->d_b3 = nullptr;
+            h->d_img = nullptr;
+            h->lf_maxny = h->lf_maxnx = 0;
+            if ((rc = dev_alloc(&h->d_b3, (size_t)nchain * maxny * N)) ||
+                (rc = dev_alloc(&h->d_img, (size_t)nchain * maxny * maxnx)))
+                return rc;
+            h->lf_maxny = maxny;
+            h->lf_maxnx = maxnx;
         }
     }
     h->dtf_set[iwvl] = true;
@@ -2326,8 +2540,24 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     }
     const int slot_img = nl;
     if (h->lf) {
-        LfArgs &f = h->arg_lf;
-        f = LfArgs{};
+        LfWvl hw[MX_MAXWVL] = {};
+        for (int w = 0; w < h->nwvl; w++) {
+            hw[w].inv_wvl = (float)(1. / h->wvl[w]);
+            hw[w].nominal = h->d_nominal[w];
+            hw[w].ny = h->lf_ny[w];
+            hw[w].nx = h->lf_nx[w];
+            hw[w].rows = h->d_lf_rows[w];
+            hw[w].cols = h->d_lf_cols[w];
+            hw[w].si_c = h->d_lf_sic[w];
+            hw[w].si_w = h->d_si_w[w];
+        }
+        if (std::memcmp(hw, h->h_lfw, sizeof(hw)) != 0) {
+            HIPCHK(hipMemcpyAsync(h->d_lfw, hw, sizeof(hw), hipMemcpyHostToDevice, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            std::memcpy(h->h_lfw, hw, sizeof(hw));
+        }
+        const int nchain = (int)(h->nwfs * h->nsa * h->nwvl);
+        LfArgs f{};
         f.pn = h->lf_pn;
         f.pm = h->lf_pm;
         f.nwvf = (int)h->nwvf;
@@ -2336,12 +2566,34 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         f.nsa = (int)h->nsa;
         f.nwfs = h->nwfs;
         f.nwvl = (int)h->nwvl;
+        f.nchain = nchain;
+        f.maxny = h->lf_maxny;
+        f.maxnx = h->lf_maxnx;
         f.opd = h->d_opd;
         f.amp = h->d_amp;
-        f.wv = h->d_wv;
+        f.wv = h->d_lfw;
         f.b1 = h->d_b1;
         f.b2 = h->d_b2;
+        f.b3 = h->d_b3;
+        f.psf = h->d_psf;
         f.img = h->d_img;
+        const int Ln = lf_lines((int)h->nwvf), Lm = lf_lines((int)N), Le = std::min(Lm, 4);
+        // pass: kernel, lines per block, line groups per chain
+        const struct {
+            const void *k;
+            int L, ngroup;
+        } pass[5] = {{reinterpret_cast<const void *>(&k_lf_a), Ln, (int)((h->nxsa + Ln - 1) / Ln)},
+                     {reinterpret_cast<const void *>(&k_lf_b), Ln, (int)((h->nwvf + Ln - 1) / Ln)},
+                     {reinterpret_cast<const void *>(&k_lf_c), Lm, (int)((h->nwvf + Lm - 1) / Lm)},
+                     {reinterpret_cast<const void *>(&k_lf_d), Lm, (int)((N + Lm - 1) / Lm)},
+                     {reinterpret_cast<const void *>(&k_lf_e), Le, (h->lf_maxny + Le - 1) / Le}};
+        const dim3 blk((unsigned)h->lf_threads);
+        for (int q = 0; q < 5; q++) {
+            h->arg_lf[q] = f;
+            h->arg_lf[q].L = pass[q].L;
+            h->arg_lf[q].lgL = ilog2c(pass[q].L);
+            L[nl++] = {pass[q].k, dim3((unsigned)(pass[q].ngroup * nchain)), blk, &h->arg_lf[q]};
+        }
         LfSampleArgs &g = h->arg_lfs;
         g = LfSampleArgs{};
         g.nsa = a.nsa;
@@ -2350,9 +2602,9 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         g.npix = a.npix;
         g.pixpsax = a.pixpsax;
         g.pixpsay = a.pixpsay;
-        g.notf = (int)N;
+        g.imgstride = h->lf_maxny * h->lf_maxnx;
         g.img = h->d_img;
-        g.wv = h->d_wv;
+        g.wv = h->d_lfw;
         g.wfs = h->d_wfs;
         g.phytype = a.phytype;
         g.sigmatch = a.sigmatch;
@@ -2368,14 +2620,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         g.ints = a.ints;
         g.grad = a.grad;
         g.i0acc = a.i0acc;
-        const unsigned nchain = (unsigned)(h->nwfs * h->nsa * h->nwvl);
-        const dim3 blk((unsigned)h->lf_threads);
-        L[nl++] = {reinterpret_cast<const void *>(&k_lf_embed_rows), dim3((unsigned)h->nwvf, nchain), blk, &f};
-        L[nl++] = {reinterpret_cast<const void *>(&k_lf_psf_cols), dim3((unsigned)N, nchain), blk, &f};
-        L[nl++] = {reinterpret_cast<const void *>(&k_lf_otf<0>), dim3((unsigned)N, nchain), blk, &f};
-        L[nl++] = {reinterpret_cast<const void *>(&k_lf_otf<1>), dim3((unsigned)N, nchain), blk, &f};
-        L[nl++] = {reinterpret_cast<const void *>(&k_lf_otf<2>), dim3((unsigned)N, nchain), blk, &f};
-        L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)), blk, &g};
+        L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)),
+                   dim3((unsigned)std::min(h->lf_threads, 256)), &g};
     } else if (mf) {
         L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<false>), dim3((unsigned)(h->nsa * h->nwfs)),
                    dim3(256), &m};
