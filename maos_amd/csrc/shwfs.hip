@@ -1206,12 +1206,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 constexpr int MX_LF_MAXN = 1024;
 constexpr int MX_LF_MAXRAD = 12;
 constexpr int MX_LF_THREADS = 512; // most threads of a block (128, 256 or 512 run)
-constexpr int MX_LF_LDS = 4160;    // complex elements of one LDS line buffer
+constexpr int MX_LF_LDS = 4352;    // complex elements of one LDS line buffer
 constexpr int MX_LF_MAXS = 128;    // most sampled rows / columns kept (else the full image)
+
+// LDS position of element e of a line: one pad element after every 32, so
+// that the power-of-two write strides of the early Stockham stages spread
+// over the LDS banks
+__host__ __device__ constexpr int lf_pos(int e) { return e + (e >> 5); }
+// LDS stride of lines of length n: the padded length, made odd
+__host__ __device__ constexpr int lf_stride(int n) { return lf_pos(n) | 1; }
+// lines of length n per block: a power of two <= 32 that one buffer holds
+__host__ __device__ constexpr int lf_lines(int n) {
+    int L = 32;
+    while (L > 1 && L * lf_stride(n) > MX_LF_LDS) L /= 2;
+    return L;
+}
+// radix of stage s in the plan of n (radix 4 first, then 2, 3, 5); 0 past
+// the last stage or if n has another prime factor
+__host__ __device__ constexpr int lf_radix(int n, int s) {
+    const int rad[4] = {4, 2, 3, 5};
+    int r = n, c = 0;
+    for (int k = 0; k < 4; k++)
+        while (r > 1 && r % rad[k] == 0) {
+            if (c == s) return rad[k];
+            c++;
+            r /= rad[k];
+        }
+    return 0;
+}
 
 struct LfPlan {
     int n, nrad;
-    int rad[MX_LF_MAXRAD]; // radices (4, 2, 3, 5), product n
+    int rad[MX_LF_MAXRAD]; // lf_radix(n, s)
     int off[MX_LF_MAXRAD]; // twiddle offset of each stage
     const float2 *tw;      // stage s: exp(-2 pi i r k / (Ns R)) at off[s] + k (R-1) + r-1
 };
@@ -1226,7 +1252,7 @@ struct LfWvl {
 struct LfArgs {
     LfPlan pn, pm;        // lengths nwvf and notf
     int nwvf, notf, nxsa, nsa, nwfs, nwvl, nchain;
-    int L, lgL;           // lines per block in this pass (a power of two)
+    int L, lgL;           // lines per block in this pass (lf_lines of its length)
     int maxny, maxnx;     // B3 rows and image capacity per chain
     const float *opd, *amp;
     const LfWvl *wv;
@@ -1240,6 +1266,12 @@ __device__ __forceinline__ int qdiv(int a, int b, float rb) {
     const int r = a - q * b;
     q += r >= b ? 1 : (r < 0 ? -1 : 0);
     return q;
+}
+// a / b: by the compiler when b is a constant (CT), else by qdiv
+template <bool CT>
+__device__ __forceinline__ int ldiv(int a, int b, float rb) {
+    if constexpr (CT) return a / b;
+    else return qdiv(a, b, rb);
 }
 
 __device__ __forceinline__ float2 mul_ti(float2 z, float t) { return make_float2(-t * z.y, t * z.x); }
@@ -1282,47 +1314,49 @@ __device__ __forceinline__ void small_dft(float2 (&v)[R], float t) {
     }
 }
 
-// one Stockham stage of radix R over L lines (line l at l S)
-template <int R, bool INV>
+// One Stockham stage of radix R and span Ns over L lines (line l at l S,
+// element e at lf_pos(e)) by a block of T threads: item (l, j), j < n/R,
+// takes x[j + r n/R] times w^(r k) (w = exp(-+ 2 pi i / (Ns R)), k = j % Ns),
+// transforms and writes x'[(j/Ns) Ns R + k + r Ns].  CT: n, Ns and L are
+// compile-time constants.  Forward is FFTW_FORWARD (exp(-i ...)).
+template <int R, bool INV, bool CT, int T>
 __device__ __forceinline__ void lds_stage(const float2 *src, float2 *dst, int n, int Ns, int L, int S,
                                           const float2 *tw) {
     const int nb = n / R, total = L * nb;
     const float rnb = 1.f / (float)nb, rns = 1.f / (float)Ns;
-    for (int it = threadIdx.x; it < total; it += blockDim.x) {
-        const int l = qdiv(it, nb, rnb), j = it - l * nb;
-        const int g = qdiv(j, Ns, rns), k = j - g * Ns;
-        const float2 *x = src + l * S + j;
+    for (int it = (int)threadIdx.x; it < total; it += T) {
+        const int l = ldiv<CT>(it, nb, rnb), j = it - l * nb;
+        const int g = ldiv<CT>(j, Ns, rns), k = j - g * Ns;
+        const float2 *x = src + l * S;
         const float2 *w = tw + k * (R - 1);
         float2 v[R];
-        v[0] = x[0];
+        v[0] = x[lf_pos(j)];
 #pragma unroll
         for (int r = 1; r < R; r++) {
             float2 wr = w[r - 1];
             if (INV) wr.y = -wr.y;
-            v[r] = cmul(x[r * nb], wr);
+            v[r] = cmul(x[lf_pos(j + r * nb)], wr);
         }
         small_dft<R>(v, INV ? 1.f : -1.f);
-        float2 *y = dst + l * S + g * Ns * R + k;
+        float2 *y = dst + l * S;
+        const int d = g * Ns * R + k;
 #pragma unroll
-        for (int r = 0; r < R; r++) y[r * Ns] = v[r];
+        for (int r = 0; r < R; r++) y[lf_pos(d + r * Ns)] = v[r];
     }
 }
 
-// Stockham autosort FFT of L lines of length P.n held in src (line l at
-// src + l S); returns the buffer that holds the result (src or dst).  Stage
-// s of radix R and span Ns: item (l, j), j < n/R, takes x[j + r n/R] times
-// w^(r k) (w = exp(-+ 2 pi i / (Ns R)), k = j % Ns), transforms it and
-// writes x'[(j/Ns) Ns R + k + r Ns].  Forward is FFTW_FORWARD (exp(-i...)).
-template <bool INV>
-__device__ float2 *lds_fft(float2 *src, float2 *dst, const LfPlan &P, int L, int S) {
+// Stockham autosort FFT of L lines of length P.n in src; returns the buffer
+// that holds the result (src or dst).  Stages from the plan at run time.
+template <bool INV, int T>
+__device__ float2 *lds_fft_rt(float2 *src, float2 *dst, const LfPlan &P, int L, int S) {
     int Ns = 1;
     for (int s = 0; s < P.nrad; s++) {
         const int R = P.rad[s];
         const float2 *tw = P.tw + P.off[s];
-        if (R == 4) lds_stage<4, INV>(src, dst, P.n, Ns, L, S, tw);
-        else if (R == 2) lds_stage<2, INV>(src, dst, P.n, Ns, L, S, tw);
-        else if (R == 3) lds_stage<3, INV>(src, dst, P.n, Ns, L, S, tw);
-        else lds_stage<5, INV>(src, dst, P.n, Ns, L, S, tw);
+        if (R == 4) lds_stage<4, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
+        else if (R == 2) lds_stage<2, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
+        else if (R == 3) lds_stage<3, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
+        else lds_stage<5, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
         __syncthreads();
         float2 *t = src;
         src = dst;
@@ -1330,6 +1364,25 @@ __device__ float2 *lds_fft(float2 *src, float2 *dst, const LfPlan &P, int L, int
         Ns *= R;
     }
     return src;
+}
+// the same for a compile-time length N, from stage SI of span NS on
+template <bool INV, int T, int N, int SI, int NS>
+__device__ __forceinline__ float2 *lds_fft_ct(float2 *src, float2 *dst, const float2 *tw) {
+    constexpr int R = lf_radix(N, SI);
+    if constexpr (R == 0) {
+        return src;
+    } else {
+        constexpr int L = lf_lines(N), S = lf_stride(N);
+        lds_stage<R, INV, true, T>(src, dst, N, NS, L, S, tw);
+        __syncthreads();
+        return lds_fft_ct<INV, T, N, SI + 1, NS * R>(dst, src, tw + NS * (R - 1));
+    }
+}
+// FFT of the block's lines: compile-time plan for N > 0, else P's
+template <bool INV, int T, int N>
+__device__ __forceinline__ float2 *lf_fft(float2 *src, float2 *dst, const LfPlan &P, int L, int S) {
+    if constexpr (N > 0) return lds_fft_ct<INV, T, N, 0, 1>(src, dst, P.tw);
+    else return lds_fft_rt<INV, T>(src, dst, P, L, S);
 }
 
 // block -> (line group g of ngroup, chain c); consecutive groups of a chain
@@ -1350,10 +1403,18 @@ __device__ __forceinline__ int corner_src(int x2, int m, int n) {
     return x2 < h ? x2 : (x2 >= m - h ? x2 - (m - n) : -1);
 }
 
+// Each pass is a template on the block size T and the length N of its lines
+// (0: taken from the arguments at run time).
+#define MX_LF_GEOM(LEN, ARGL)                                                   \
+    constexpr int LC = N > 0 ? lf_lines(N) : 1, SC = N > 0 ? lf_stride(N) : 1; \
+    const int n_ = N > 0 ? N : (LEN), L = N > 0 ? LC : (ARGL), S = N > 0 ? SC : lf_stride(n_);
+
 // pass A: embed the field-band rows and FFT them along x
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_a(LfArgs a) {
+template <int T, int N>
+__global__ __launch_bounds__(T) void k_lf_a(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
-    const int n = a.nwvf, S = n + 1, L = a.L, nxsa = a.nxsa, skip = (n - nxsa) / 2;
+    MX_LF_GEOM(a.nwvf, a.L)
+    const int n = n_, nxsa = a.nxsa, skip = (n - nxsa) / 2;
     int g, c, iwfs, isa, iwvl;
     lf_unit(a, (nxsa + L - 1) / L, g, c);
     lf_chain(a, c, iwfs, isa, iwvl);
@@ -1362,8 +1423,8 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_a(LfArgs a) {
     const float *am = a.amp + (long)isa * np;
     const float inv_wvl = a.wv[iwvl].inv_wvl;
     const float rn = 1.f / (float)n;
-    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
-        const int l = qdiv(e, n, rn), x = e - l * n, iy = g * L + l, ix = x - skip;
+    for (int e = threadIdx.x; e < L * n; e += T) {
+        const int l = ldiv<(N > 0)>(e, n, rn), x = e - l * n, iy = g * L + l, ix = x - skip;
         float2 v = make_float2(0.f, 0.f);
         if (iy < nxsa && ix >= 0 && ix < nxsa) {
             float sn, cs;
@@ -1371,104 +1432,113 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_a(LfArgs a) {
             const float A = am[(long)iy * nxsa + ix];
             v = make_float2(A * cs, A * sn);
         }
-        X[l * S + x] = v;
+        X[l * S + lf_pos(x)] = v;
     }
     __syncthreads();
-    const float2 *r = lds_fft<false>(X, Y, a.pn, L, S);
+    const float2 *r = lf_fft<false, T, N>(X, Y, a.pn, L, S);
     float2 *out = a.b1 + (long)c * nxsa * n;
-    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
-        const int l = qdiv(e, n, rn), x = e - l * n, iy = g * L + l;
-        if (iy < nxsa) out[(long)iy * n + x] = r[l * S + x];
+    for (int e = threadIdx.x; e < L * n; e += T) {
+        const int l = ldiv<(N > 0)>(e, n, rn), x = e - l * n, iy = g * L + l;
+        if (iy < nxsa) out[(long)iy * n + x] = r[l * S + lf_pos(x)];
     }
 }
 
 // pass B: FFT the columns of B1 and take |E|^2
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_b(LfArgs a) {
+template <int T, int N>
+__global__ __launch_bounds__(T) void k_lf_b(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
-    const int n = a.nwvf, S = n + 1, L = a.L, nxsa = a.nxsa, skip = (n - nxsa) / 2;
+    MX_LF_GEOM(a.nwvf, a.L)
+    const int n = n_, nxsa = a.nxsa, skip = (n - nxsa) / 2, lgL = N > 0 ? ilog2c(LC) : a.lgL;
     int g, c;
     lf_unit(a, (n + L - 1) / L, g, c);
     const int x0 = g * L;
     const float2 *in = a.b1 + (long)c * nxsa * n;
-    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
-        const int l = e & (L - 1), y = e >> a.lgL, x = x0 + l, iy = y - skip;
+    for (int e = threadIdx.x; e < L * n; e += T) {
+        const int l = e & (L - 1), y = e >> lgL, x = x0 + l, iy = y - skip;
         float2 v = make_float2(0.f, 0.f);
         if (x < n && iy >= 0 && iy < nxsa) v = in[(long)iy * n + x];
-        X[l * S + y] = v;
+        X[l * S + lf_pos(y)] = v;
     }
     __syncthreads();
-    const float2 *r = lds_fft<false>(X, Y, a.pn, L, S);
+    const float2 *r = lf_fft<false, T, N>(X, Y, a.pn, L, S);
     float *out = a.psf + (long)c * n * n;
-    for (int e = threadIdx.x; e < L * n; e += blockDim.x) {
-        const int l = e & (L - 1), y = e >> a.lgL, x = x0 + l;
+    for (int e = threadIdx.x; e < L * n; e += T) {
+        const int l = e & (L - 1), y = e >> lgL, x = x0 + l;
         if (x < n) {
-            const float2 v = r[l * S + y];
+            const float2 v = r[l * S + lf_pos(y)];
             out[(long)y * n + x] = v.x * v.x + v.y * v.y;
         }
     }
 }
 
 // pass C: FFT the PSF rows, padded to notf by the corner copy
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_c(LfArgs a) {
+template <int T, int N>
+__global__ __launch_bounds__(T) void k_lf_c(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
-    const int n = a.nwvf, m = a.notf, S = m + 1, L = a.L;
+    MX_LF_GEOM(a.notf, a.L)
+    const int n = a.nwvf, m = n_;
     int g, c;
     lf_unit(a, (n + L - 1) / L, g, c);
     const float *in = a.psf + (long)c * n * n;
     const float rm = 1.f / (float)m;
-    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
-        const int l = qdiv(e, m, rm), x2 = e - l * m, y = g * L + l, x = corner_src(x2, m, n);
-        X[l * S + x2] = make_float2(y < n && x >= 0 ? in[(long)y * n + x] : 0.f, 0.f);
+    for (int e = threadIdx.x; e < L * m; e += T) {
+        const int l = ldiv<(N > 0)>(e, m, rm), x2 = e - l * m, y = g * L + l, x = corner_src(x2, m, n);
+        X[l * S + lf_pos(x2)] = make_float2(y < n && x >= 0 ? in[(long)y * n + x] : 0.f, 0.f);
     }
     __syncthreads();
-    const float2 *r = lds_fft<false>(X, Y, a.pm, L, S);
+    const float2 *r = lf_fft<false, T, N>(X, Y, a.pm, L, S);
     float2 *out = a.b2 + (long)c * n * m;
-    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
-        const int l = qdiv(e, m, rm), x2 = e - l * m, y = g * L + l;
-        if (y < n) out[(long)y * m + x2] = r[l * S + x2];
+    for (int e = threadIdx.x; e < L * m; e += T) {
+        const int l = ldiv<(N > 0)>(e, m, rm), x2 = e - l * m, y = g * L + l;
+        if (y < n) out[(long)y * m + x2] = r[l * S + lf_pos(x2)];
     }
 }
 
 // pass D: FFT the columns of B2 (rows padded by the corner copy), x nominal,
 // inverse FFT, keep the sampled rows
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_d(LfArgs a) {
+template <int T, int N>
+__global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
-    const int n = a.nwvf, m = a.notf, S = m + 1, L = a.L;
+    MX_LF_GEOM(a.notf, a.L)
+    const int n = a.nwvf, m = n_, lgL = N > 0 ? ilog2c(LC) : a.lgL;
     int g, c, iwfs, isa, iwvl;
     lf_unit(a, (m + L - 1) / L, g, c);
     lf_chain(a, c, iwfs, isa, iwvl);
     const int x0 = g * L;
     const float2 *in = a.b2 + (long)c * n * m;
-    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
-        const int l = e & (L - 1), y2 = e >> a.lgL, x2 = x0 + l, y = corner_src(y2, m, n);
-        X[l * S + y2] = x2 < m && y >= 0 ? in[(long)y * m + x2] : make_float2(0.f, 0.f);
+    for (int e = threadIdx.x; e < L * m; e += T) {
+        const int l = e & (L - 1), y2 = e >> lgL, x2 = x0 + l, y = corner_src(y2, m, n);
+        X[l * S + lf_pos(y2)] = x2 < m && y >= 0 ? in[(long)y * m + x2] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    float2 *r = lds_fft<false>(X, Y, a.pm, L, S);
+    float2 *r = lf_fft<false, T, N>(X, Y, a.pm, L, S);
     const LfWvl &W = a.wv[iwvl];
     if (W.nominal) {
         const float rm = 1.f / (float)m;
-        for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
-            const int l = qdiv(e, m, rm), y2 = e - l * m, x2 = x0 + l;
+        for (int e = threadIdx.x; e < L * m; e += T) {
+            const int l = ldiv<(N > 0)>(e, m, rm), y2 = e - l * m, x2 = x0 + l;
             if (x2 < m) {
                 const float f = W.nominal[(long)x2 * m + y2];
-                r[l * S + y2] = make_float2(r[l * S + y2].x * f, r[l * S + y2].y * f);
+                float2 &z = r[l * S + lf_pos(y2)];
+                z = make_float2(z.x * f, z.y * f);
             }
         }
         __syncthreads();
     }
-    const float2 *q = lds_fft<true>(r, r == X ? Y : X, a.pm, L, S);
+    const float2 *q = lf_fft<true, T, N>(r, r == X ? Y : X, a.pm, L, S);
     float2 *out = a.b3 + (long)c * a.maxny * m;
-    for (int e = threadIdx.x; e < L * W.ny; e += blockDim.x) {
-        const int l = e & (L - 1), iyy = e >> a.lgL, x2 = x0 + l;
-        if (x2 < m) out[(long)iyy * m + x2] = q[l * S + W.rows[iyy]];
+    for (int e = threadIdx.x; e < L * W.ny; e += T) {
+        const int l = e & (L - 1), iyy = e >> lgL, x2 = x0 + l;
+        if (x2 < m) out[(long)iyy * m + x2] = q[l * S + lf_pos(W.rows[iyy])];
     }
 }
 
 // pass E: inverse FFT of the sampled rows, real part at the sampled columns
-__global__ __launch_bounds__(MX_LF_THREADS) void k_lf_e(LfArgs a) {
+template <int T, int N>
+__global__ __launch_bounds__(T) void k_lf_e(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
-    const int m = a.notf, S = m + 1, L = a.L;
+    MX_LF_GEOM(a.notf, a.L)
+    const int m = n_;
     int g, c, iwfs, isa, iwvl;
     lf_unit(a, (a.maxny + L - 1) / L, g, c);
     lf_chain(a, c, iwfs, isa, iwvl);
@@ -1476,19 +1546,20 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_e(LfArgs a) {
     if (g * L >= W.ny) return; // uniform: no sampled row in this group
     const float2 *in = a.b3 + (long)c * a.maxny * m;
     const float rm = 1.f / (float)m;
-    for (int e = threadIdx.x; e < L * m; e += blockDim.x) {
-        const int l = qdiv(e, m, rm), x = e - l * m, iyy = g * L + l;
-        X[l * S + x] = iyy < W.ny ? in[(long)iyy * m + x] : make_float2(0.f, 0.f);
+    for (int e = threadIdx.x; e < L * m; e += T) {
+        const int l = ldiv<(N > 0)>(e, m, rm), x = e - l * m, iyy = g * L + l;
+        X[l * S + lf_pos(x)] = iyy < W.ny ? in[(long)iyy * m + x] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    const float2 *r = lds_fft<true>(X, Y, a.pm, L, S);
+    const float2 *r = lf_fft<true, T, N>(X, Y, a.pm, L, S);
     float *out = a.img + (long)c * a.maxny * a.maxnx;
     const float rnx = 1.f / (float)W.nx;
-    for (int e = threadIdx.x; e < L * W.nx; e += blockDim.x) {
+    for (int e = threadIdx.x; e < L * W.nx; e += T) {
         const int l = qdiv(e, W.nx, rnx), ixx = e - l * W.nx, iyy = g * L + l;
-        if (iyy < W.ny) out[(long)iyy * W.nx + ixx] = r[l * S + W.cols[ixx]].x;
+        if (iyy < W.ny) out[(long)iyy * W.nx + ixx] = r[l * S + lf_pos(W.cols[ixx])].x;
     }
 }
+#undef MX_LF_GEOM
 
 // si sampling over the wavelengths and the pixel processing of one (WFS,
 // subaperture) (dspmulcreal, maos/wfsints.c:286; shwfs_grad,
@@ -1720,15 +1791,12 @@ bool lf_plan(long n, LfPlan &P) {
     P = LfPlan{};
     if (n < 2 || n > MX_LF_MAXN) return false;
     P.n = (int)n;
-    long r = n;
-    const int radix[4] = {4, 2, 3, 5};
-    for (int k = 0; k < 4; k++)
-        while (r % radix[k] == 0) {
-            if (P.nrad == MX_LF_MAXRAD) return false;
-            P.rad[P.nrad++] = radix[k];
-            r /= radix[k];
-        }
-    return r == 1;
+    long prod = 1;
+    while (P.nrad < MX_LF_MAXRAD && lf_radix((int)n, P.nrad)) {
+        P.rad[P.nrad] = lf_radix((int)n, P.nrad);
+        prod *= P.rad[P.nrad++];
+    }
+    return prod == n;
 }
 
 // twiddles exp(-2 pi i r k / (Ns R)) of a plan's stages (k < Ns, 1 <= r <
@@ -1748,12 +1816,38 @@ void lf_twiddles(LfPlan &P, std::vector<float2> &tw) {
     }
 }
 
-// lines of length n per large-field block: a power of two <= 32 whose lines
-// (stride n + 1) fit one LDS buffer
-int lf_lines(int n) {
-    int L = 32;
-    while (L > 1 && L * (n + 1) > MX_LF_LDS) L /= 2;
-    return L;
+// pass q (0..4 = A..E) of the large-field form for blocks of T threads and
+// lines of length n: a kernel compiled for that length where there is one
+// (the field and OTF sizes of the configured NGS and padded-LGS powfs)
+template <int T, int N>
+const void *lf_kernel_ab(int q) {
+    return q == 0 ? reinterpret_cast<const void *>(&k_lf_a<T, N>) : reinterpret_cast<const void *>(&k_lf_b<T, N>);
+}
+template <int T, int N>
+const void *lf_kernel_cde(int q) {
+    return q == 2 ? reinterpret_cast<const void *>(&k_lf_c<T, N>)
+                  : q == 3 ? reinterpret_cast<const void *>(&k_lf_d<T, N>)
+                           : reinterpret_cast<const void *>(&k_lf_e<T, N>);
+}
+template <int T>
+const void *lf_kernel_t(int q, int n) {
+    if constexpr (T == 512) {
+        if (q < 2) {
+            if (n == 96) return lf_kernel_ab<T, 96>(q);
+            if (n == 128) return lf_kernel_ab<T, 128>(q);
+            if (n == 480) return lf_kernel_ab<T, 480>(q);
+            if (n == 960) return lf_kernel_ab<T, 960>(q);
+        } else {
+            if (n == 120) return lf_kernel_cde<T, 120>(q);
+            if (n == 128) return lf_kernel_cde<T, 128>(q);
+            if (n == 512) return lf_kernel_cde<T, 512>(q);
+            if (n == 1024) return lf_kernel_cde<T, 1024>(q);
+        }
+    }
+    return q < 2 ? lf_kernel_ab<T, 0>(q) : lf_kernel_cde<T, 0>(q);
+}
+const void *lf_kernel(int q, int T, int n) {
+    return T == 128 ? lf_kernel_t<128>(q, n) : T == 256 ? lf_kernel_t<256>(q, n) : lf_kernel_t<512>(q, n);
 }
 
 // the large-field form applies: OPD grid a multiple of 16 points, even
@@ -2577,16 +2671,16 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         f.b3 = h->d_b3;
         f.psf = h->d_psf;
         f.img = h->d_img;
-        const int Ln = lf_lines((int)h->nwvf), Lm = lf_lines((int)N), Le = std::min(Lm, 4);
+        const int Ln = lf_lines((int)h->nwvf), Lm = lf_lines((int)N);
         // pass: kernel, lines per block, line groups per chain
         const struct {
             const void *k;
             int L, ngroup;
-        } pass[5] = {{reinterpret_cast<const void *>(&k_lf_a), Ln, (int)((h->nxsa + Ln - 1) / Ln)},
-                     {reinterpret_cast<const void *>(&k_lf_b), Ln, (int)((h->nwvf + Ln - 1) / Ln)},
-                     {reinterpret_cast<const void *>(&k_lf_c), Lm, (int)((h->nwvf + Lm - 1) / Lm)},
-                     {reinterpret_cast<const void *>(&k_lf_d), Lm, (int)((N + Lm - 1) / Lm)},
-                     {reinterpret_cast<const void *>(&k_lf_e), Le, (h->lf_maxny + Le - 1) / Le}};
+        } pass[5] = {{lf_kernel(0, h->lf_threads, (int)h->nwvf), Ln, (int)((h->nxsa + Ln - 1) / Ln)},
+                     {lf_kernel(1, h->lf_threads, (int)h->nwvf), Ln, (int)((h->nwvf + Ln - 1) / Ln)},
+                     {lf_kernel(2, h->lf_threads, (int)N), Lm, (int)((h->nwvf + Lm - 1) / Lm)},
+                     {lf_kernel(3, h->lf_threads, (int)N), Lm, (int)((N + Lm - 1) / Lm)},
+                     {lf_kernel(4, h->lf_threads, (int)N), Lm, (h->lf_maxny + Lm - 1) / Lm}};
         const dim3 blk((unsigned)h->lf_threads);
         for (int q = 0; q < 5; q++) {
             h->arg_lf[q] = f;
