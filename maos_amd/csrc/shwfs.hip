@@ -1244,6 +1244,7 @@ struct LfPlan {
 struct LfWvl {
     float inv_wvl;
     const float *nominal; // [ux][uy] (transposed), or null
+    const float2 *etf, *etf2; // [nsa][ux][uy] ETF with the nominal fused (LLT), second profile or null
     int ny, nx;           // sampled image rows / columns
     const int *rows, *cols;
     const int4 *si_c;     // si in compact indices iy * nx + ix
@@ -1258,6 +1259,9 @@ struct LfArgs {
     const LfWvl *wv;
     float2 *b1, *b2, *b3;
     float *psf, *img;
+    float psf_scale;      // pass C multiplies the PSF by this (1/nlwvf^2 for the uplink)
+    float2 *lotf;         // [nwfs][nwvl][ux][uy] conj(lotfc): written by the uplink, read by imaging; or null
+    float etf1wt, etf2wt; // ETF profile weights (maos/wfsints.c:215-223)
 };
 
 // a / b for 0 <= a < 2^22 from a rounded reciprocal rb of b, corrected
@@ -1483,7 +1487,7 @@ __global__ __launch_bounds__(T) void k_lf_c(LfArgs a) {
     const float rm = 1.f / (float)m;
     for (int e = threadIdx.x; e < L * m; e += T) {
         const int l = ldiv<(N > 0)>(e, m, rm), x2 = e - l * m, y = g * L + l, x = corner_src(x2, m, n);
-        X[l * S + lf_pos(x2)] = make_float2(y < n && x >= 0 ? in[(long)y * n + x] : 0.f, 0.f);
+        X[l * S + lf_pos(x2)] = make_float2(y < n && x >= 0 ? in[(long)y * n + x] * a.psf_scale : 0.f, 0.f);
     }
     __syncthreads();
     const float2 *r = lf_fft<false, T, N>(X, Y, a.pm, L, S);
@@ -1494,9 +1498,11 @@ __global__ __launch_bounds__(T) void k_lf_c(LfArgs a) {
     }
 }
 
-// pass D: FFT the columns of B2 (rows padded by the corner copy), x nominal,
-// inverse FFT, keep the sampled rows
-template <int T, int N>
+// pass D: FFT the columns of B2 (rows padded by the corner copy), x conj(lotf)
+// x ETF (LLT) or x nominal, inverse FFT, keep the sampled rows.  UPLINK: the
+// chains are (WFS, wavelength) LLT fields and the column FFT is conj-stored
+// as the uplink OTF lotfc (maos/wfsints.c:149-199).
+template <int T, int N, bool UPLINK>
 __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
     MX_LF_GEOM(a.notf, a.L)
@@ -1512,8 +1518,37 @@ __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
     }
     __syncthreads();
     float2 *r = lf_fft<false, T, N>(X, Y, a.pm, L, S);
+    if constexpr (UPLINK) {
+        float2 *lo = a.lotf + (long)c * m * m;
+        for (int e = threadIdx.x; e < L * m; e += T) {
+            const int l = e & (L - 1), y2 = e >> lgL, x2 = x0 + l;
+            if (x2 < m) {
+                const float2 z = r[l * S + lf_pos(y2)];
+                lo[(long)x2 * m + y2] = make_float2(z.x, -z.y);
+            }
+        }
+        return;
+    }
     const LfWvl &W = a.wv[iwvl];
-    if (W.nominal) {
+    if (a.lotf) { // x conj(lotfc) x ETF (apply_dtf_etf, maos/wfsints.c:38-64)
+        const float2 *lo = a.lotf + ((long)iwfs * a.nwvl + iwvl) * m * m;
+        const float2 *e1 = W.etf + (long)isa * m * m, *e2 = W.etf2 ? W.etf2 + (long)isa * m * m : nullptr;
+        const float rm = 1.f / (float)m;
+        for (int e = threadIdx.x; e < L * m; e += T) {
+            const int l = ldiv<(N > 0)>(e, m, rm), y2 = e - l * m, x2 = x0 + l;
+            if (x2 < m) {
+                const long i = (long)x2 * m + y2;
+                float2 et = e1[i];
+                if (e2) {
+                    const float2 t2 = e2[i];
+                    et = make_float2(et.x * a.etf1wt + t2.x * a.etf2wt, et.y * a.etf1wt + t2.y * a.etf2wt);
+                }
+                float2 &z = r[l * S + lf_pos(y2)];
+                z = cmul(cmul(z, lo[i]), et);
+            }
+        }
+        __syncthreads();
+    } else if (W.nominal) {
         const float rm = 1.f / (float)m;
         for (int e = threadIdx.x; e < L * m; e += T) {
             const int l = ldiv<(N > 0)>(e, m, rm), y2 = e - l * m, x2 = x0 + l;
@@ -1651,7 +1686,7 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
 // ======================================================================
 // C ABI
 // ======================================================================
-constexpr int MX_MAXLAUNCH = 8; // kernels of one step
+constexpr int MX_MAXLAUNCH = 12; // kernels of one step
 
 // The kernels of one queued step in stream order; arg points into the
 // handle's argument blocks (a graph's kernel nodes copy them).
@@ -1664,7 +1699,7 @@ struct StepPlan {
     StepLaunch L[MX_MAXLAUNCH];
     int nl = 0;
     int key = 0;       // the kernel set: a graph built for one key fits any step of it
-    int slot_llt = -1; // index of the uplink-OTF kernel, or -1
+    int slot_llt = -1; // index of the first uplink-OTF kernel, or -1 (they run up to slot_img)
     int slot_img = 0;  // index of the first imaging kernel
     bool llt = false;
 };
@@ -1731,7 +1766,7 @@ struct mx_wfsgrad_s {
     int lf_maxny = 0, lf_maxnx = 0;    // capacity of B3 and the image per chain
     LfWvl *d_lfw = nullptr;
     LfWvl h_lfw[MX_MAXWVL] = {};
-    LfArgs arg_lf[5] = {};
+    LfArgs arg_lf_base{}, arg_lf[5] = {}, arg_lf_up[4] = {};
     LfSampleArgs arg_lfs{};
     float *d_mtche = nullptr, *d_i0sum = nullptr;
     bool have_mtche = false, have_i0sum = false;
@@ -1826,7 +1861,8 @@ const void *lf_kernel_ab(int q) {
 template <int T, int N>
 const void *lf_kernel_cde(int q) {
     return q == 2 ? reinterpret_cast<const void *>(&k_lf_c<T, N>)
-                  : q == 3 ? reinterpret_cast<const void *>(&k_lf_d<T, N>)
+                  : q == 3 ? reinterpret_cast<const void *>(&k_lf_d<T, N, false>)
+                  : q == 5 ? reinterpret_cast<const void *>(&k_lf_d<T, N, true>)
                            : reinterpret_cast<const void *>(&k_lf_e<T, N>);
 }
 template <int T>
@@ -1846,6 +1882,7 @@ const void *lf_kernel_t(int q, int n) {
     }
     return q < 2 ? lf_kernel_ab<T, 0>(q) : lf_kernel_cde<T, 0>(q);
 }
+// q: 0..4 passes A..E, 5 the uplink's column pass
 const void *lf_kernel(int q, int T, int n) {
     return T == 128 ? lf_kernel_t<128>(q, n) : T == 256 ? lf_kernel_t<256>(q, n) : lf_kernel_t<512>(q, n);
 }
@@ -2004,9 +2041,10 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
                        "unsupported size: notf=%ld nxsa=%ld embfac=%ld (supported: 64/32/2, 32/16/2, or nxsa a "
                        "multiple of 16 with nxsa*embfac <= notf <= %d, both even with prime factors 2, 3 and 5)",
                        p->notf, p->nxsa, p->embfac, MX_LF_MAXN);
-    if (lf && p->llt_nx > 0)
-        return set_err(MX_EINVAL, "an LLT needs notf=64/nxsa=32 or notf=32/nxsa=16 (got notf=%ld nxsa=%ld)", p->notf,
-                       p->nxsa);
+    if (lf && p->llt_nx > 0 &&
+        !(p->llt_nx == p->nxsa && (p->nxsa == 16 || p->nxsa == 32) && p->llt_nx * p->embfac == p->notf))
+        return set_err(MX_EINVAL, "with an LLT the large-field form needs llt_nx == nxsa (16 or 32) and "
+                       "llt_nx*embfac == notf (got llt_nx=%ld nxsa=%ld notf=%ld)", p->llt_nx, p->nxsa, p->notf);
     if (p->nwvl < 1 || p->nwvl > MX_MAXWVL) return set_err(MX_EINVAL, "nwvl=%ld out of range", p->nwvl);
     const long npix = p->pixpsax * p->pixpsay;
     const long ppl = (npix + p->notf - 1) / p->notf;
@@ -2622,17 +2660,6 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     const void *kopd = kopd_t[opd_nx == 32][direct][o.tiles > 1];
     L[nl++] = {kopd,
                dim3((unsigned)(h->nsa * h->nwfs * o.tiles * o.tiles + o.nllt)), dim3(MX_OPD_THREADS), &o};
-    const int slot_llt = llt ? nl : -1;
-    if (llt) {
-        if (mf)
-            L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<true>), dim3((unsigned)(h->nwfs * h->nwvl)),
-                       dim3(256), &m};
-        else
-            L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_llt_otf<64, 32>)
-                               : reinterpret_cast<const void *>(&k_llt_otf<32, 16>),
-                       dim3((unsigned)(h->nwfs * h->nwvl)), dim3(64), &la};
-    }
-    const int slot_img = nl;
     if (h->lf) {
         LfWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
@@ -2644,6 +2671,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             hw[w].cols = h->d_lf_cols[w];
             hw[w].si_c = h->d_lf_sic[w];
             hw[w].si_w = h->d_si_w[w];
+            hw[w].etf = h->d_etf[w];
+            hw[w].etf2 = h->d_etf2[w];
         }
         if (std::memcmp(hw, h->h_lfw, sizeof(hw)) != 0) {
             HIPCHK(hipMemcpyAsync(h->d_lfw, hw, sizeof(hw), hipMemcpyHostToDevice, h->stream));
@@ -2651,7 +2680,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             std::memcpy(h->h_lfw, hw, sizeof(hw));
         }
         const int nchain = (int)(h->nwfs * h->nsa * h->nwvl);
-        LfArgs f{};
+        LfArgs &f = h->arg_lf_base;
+        f = LfArgs{};
         f.pn = h->lf_pn;
         f.pm = h->lf_pm;
         f.nwvf = (int)h->nwvf;
@@ -2671,6 +2701,49 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         f.b3 = h->d_b3;
         f.psf = h->d_psf;
         f.img = h->d_img;
+        f.psf_scale = 1.f;
+        f.lotf = llt ? h->d_lotf : nullptr;
+        f.etf1wt = 1.f - h->etf2wt;
+        f.etf2wt = h->etf2wt;
+    }
+    const int slot_llt = llt ? nl : -1;
+    if (llt && h->lf) {
+        // uplink OTF of each (WFS, wavelength) by passes A-C and the uplink
+        // column pass on the LLT grids (nlwvf == nwvf == notf)
+        const int Ln = lf_lines((int)h->nwvf), Lm = lf_lines((int)N), nch = (int)(h->nwfs * h->nwvl);
+        const int nlx = (int)h->cfg.llt_nx;
+        const struct {
+            int q, len, L, ngroup;
+        } up[4] = {{0, (int)h->nwvf, Ln, (nlx + Ln - 1) / Ln},
+                   {1, (int)h->nwvf, Ln, (int)((h->nwvf + Ln - 1) / Ln)},
+                   {2, (int)N, Lm, (int)((h->nwvf + Lm - 1) / Lm)},
+                   {5, (int)N, Lm, (int)((N + Lm - 1) / Lm)}};
+        for (int q = 0; q < 4; q++) {
+            LfArgs &u = h->arg_lf_up[q];
+            u = h->arg_lf_base;
+            u.nsa = 1;
+            u.nxsa = nlx;
+            u.nchain = nch;
+            u.opd = h->d_llt_opd;
+            u.amp = h->d_llt_amp;
+            u.psf_scale = (float)(1. / ((double)h->nwvf * (double)h->nwvf));
+            u.L = up[q].L;
+            u.lgL = ilog2c(up[q].L);
+            L[nl++] = {lf_kernel(up[q].q, h->lf_threads, up[q].len), dim3((unsigned)(up[q].ngroup * nch)),
+                       dim3((unsigned)h->lf_threads), &u};
+        }
+    } else if (llt) {
+        if (mf)
+            L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<true>), dim3((unsigned)(h->nwfs * h->nwvl)),
+                       dim3(256), &m};
+        else
+            L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_llt_otf<64, 32>)
+                               : reinterpret_cast<const void *>(&k_llt_otf<32, 16>),
+                       dim3((unsigned)(h->nwfs * h->nwvl)), dim3(64), &la};
+    }
+    const int slot_img = nl;
+    if (h->lf) {
+        const LfArgs &f = h->arg_lf_base;
         const int Ln = lf_lines((int)h->nwvf), Lm = lf_lines((int)N);
         // pass: kernel, lines per block, line groups per chain
         const struct {
@@ -2686,7 +2759,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             h->arg_lf[q] = f;
             h->arg_lf[q].L = pass[q].L;
             h->arg_lf[q].lgL = ilog2c(pass[q].L);
-            L[nl++] = {pass[q].k, dim3((unsigned)(pass[q].ngroup * nchain)), blk, &h->arg_lf[q]};
+            L[nl++] = {pass[q].k, dim3((unsigned)(pass[q].ngroup * f.nchain)), blk, &h->arg_lf[q]};
         }
         LfSampleArgs &g = h->arg_lfs;
         g = LfSampleArgs{};
@@ -2788,7 +2861,7 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
         }
         for (int i = 0; i < nl; i++) {
             hipEvent_t eb = i == 0 ? ev[2] : i == slot_llt ? ev[0] : i == slot_img ? ev[4] : nullptr;
-            hipEvent_t ee = i == 0 ? ev[3] : i == slot_llt ? ev[1] : i == nl - 1 ? ev[5] : nullptr;
+            hipEvent_t ee = i == 0 ? ev[3] : (slot_llt >= 0 && i == slot_img - 1) ? ev[1] : i == nl - 1 ? ev[5] : nullptr;
             if (h->timing && eb) HIPCHK(hipEventRecord(eb, h->stream));
             void *argv[1] = {L[i].arg};
             HIPCHK(hipLaunchKernel(L[i].f, L[i].grid, L[i].block, argv, 0, h->stream));

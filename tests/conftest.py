@@ -103,3 +103,14 @@ def ngs_tt_small():
     cfg = configs.ngs_tt(nx=1024)
     p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
     return cfg, p, genatm(cfg["atm"])
+
+
+@pytest.fixture(scope="session")
+def lgs_pad128():
+    """lgs_small with embfac 4: 32^2 points padded to a 128^2 field and OTF
+    (the 2x Nyquist variant), LLT grid 32^2 -> nlwvf 128, ETF at 128^2."""
+    from maos_amd import configs
+    from maos_amd.powfs import Powfs, genatm
+    cfg = configs.lgs30m(nx=1024, order=20, embfac=4)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    return cfg, p, genatm(cfg["atm"])

@@ -364,6 +364,37 @@ def test_autotune(case, request):
     eng.close()
 
 
+def test_lgs_pad128_parity(lgs_pad128):
+    """The 128^2 padding variant of the LGS (embfac 4) on the large-field
+    form: uplink OTF from the LLT grid, x conj(lotf) x ETF per subaperture,
+    matched filter; against the oracle for every WFS at two steps."""
+    from maos_amd import _lib
+    cfg, p, layers = lgs_pad128
+    print("case: lgs order20 embfac4 nsa=%d notf=%d nwfs=%d" % (p.nsa, p.notf, len(cfg["wfs"])), flush=True)
+    assert p.notf == 128 and p.llt_nx * p.cfg.embfac == 128
+    eng, wfs = _engine(cfg, p, layers)
+    assert eng.variant() == _lib.VARIANT_FFT32
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+    o.set_ref(mtche, i0sum)
+    eng.update_ref(mtche, i0sum)
+    for isim in (0, 123):
+        eng.queue(isim, dt)
+        for iwfs in range(len(wfs)):
+            g, ints = eng.sync(iwfs)
+            ri, rg, _, rlotf = o.step(wfs[iwfs], isim, dt, want_lotf=True)
+            lotf = eng.debug_lotf(iwfs)
+            e_lotf = rel_l2(lotf, rlotf)
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(g - rg).max() * RAD2MAS
+            print("isim=%d wfs=%d lotf=%.2e ints=%.2e grad=%.3e mas" % (isim, iwfs, e_lotf, e_int, e_g))
+            assert e_lotf < 1e-5
+            assert e_int <= INTS_TOL
+            assert e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C
