@@ -38,7 +38,8 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_shwfs_mfma_lgs30m_traffic.txt")
-KERNELS = {0: "k_shwfs_step<64,32,1>", 1: "k_shwfs_mfma"}
+KERNELS = {0: "k_shwfs_step<64,32,1>", 1: "k_shwfs_mfma<false, __bf16>", 2: "k_shwfs_mfma<false, _Float16>"}
+FORMS = {0: "fp32-radix-fft", 1: "mfma-split-bf16", 2: "mfma-split-fp16"}
 
 
 def flops_per_subap(N, nxsa, nps, npix, llt=True, etf=True):
@@ -343,7 +344,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--order", type=int, default=60)
     ap.add_argument("--nx", type=int, default=4096)
-    ap.add_argument("--variant", choices=("auto", "fft32", "mfma"), default="auto")
+    ap.add_argument("--variant", choices=("auto", "fft32", "mfma", "fp16"), default="auto")
     ap.add_argument("--copy-calib", type=int, default=0)
     ap.add_argument("--no-graph", action="store_true", help="launch kernel by kernel, not one graph per step")
     ap.add_argument("--nreal", type=int, default=1,
@@ -380,7 +381,8 @@ def main():
     engine.atm2gpu(layers)
     eng = engine.WfsGrad(p, wfs)
     if args.variant != "auto":
-        eng.set_variant(_lib.VARIANT_FFT32 if args.variant == "fft32" else _lib.VARIANT_MFMA_BF16)
+        eng.set_variant({"fft32": _lib.VARIANT_FFT32, "mfma": _lib.VARIANT_MFMA_BF16,
+                         "fp16": _lib.VARIANT_MFMA_FP16}[args.variant])
     variant = eng.variant()
     log("imaging form: %s" % KERNELS[variant])
     if not args.no_graph:
@@ -392,10 +394,12 @@ def main():
     for isim in range(args.i0_steps):
         eng.queue(isim, dt)
     eng.wait()
+    refs = []
     for iwfs in range(nwfs):
         i0, n = eng.i0_get(iwfs)
         mtche, i0sum, _ = p.mtch(i0 / n)
         eng.update_ref(mtche, i0sum, iwfs=iwfs)
+        refs.append((mtche, i0sum))
     eng.i0_control(enable=True, reset=True)
     log("matched filter from %d-frame FP64 i0 (%.1fs)" % (args.i0_steps, time.time() - t0))
 
@@ -430,13 +434,14 @@ def main():
     flop_launch = flops_per_subap(p.notf, p.nxsa, 0, p.npix) * p.nsa * nwfs
     achieved = flop_launch / t_ints / 1e12
     traffic = read_traffic() if variant == _lib.VARIANT_MFMA_BF16 else None
+    is_mfma = variant in (_lib.VARIANT_MFMA_BF16, _lib.VARIANT_MFMA_FP16)
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
             "kernel": KERNELS[variant], "avg_launch_us": round(t_ints * 1e6, 2),
             "llt_kernel_us": round(t_llt * 1e6, 2), "raytrace_kernel_us": round(t_opd * 1e6, 2),
             "flop_per_launch": flop_launch,
             "peak_kind": "fp32 dense (MI355X published): the path's results are FP32-accurate"}
-    if variant == _lib.VARIANT_MFMA_BF16:
+    if is_mfma:
         ex = mfma_flops_per_subap() * p.nsa * nwfs
         roof["mfma_executed_tflops"] = round(ex / t_ints / 1e12, 2)
         roof["mfma_executed_frac_of_bf16_peak"] = round(ex / t_ints / 1e12 / PEAK_BF16_TFLOPS, 4)
@@ -444,25 +449,26 @@ def main():
             roof["traffic_source"] = os.path.relpath(TRAFFIC_FILE, ROOT)
     if args.copy_calib > 0:
         copy_calibration(args.copy_calib)
-    other = None
-    if eng.variant_ok(1 - variant):
-        fps_other, isim = time_variant(eng, 1 - variant, isim, dt, 100)
-        other = {"kernel": KERNELS[1 - variant], "frames_per_s": round(fps_other * nwfs, 2)}
-        eng.set_variant(variant)
+    other = []
+    for v in (0, 1, 2):
+        if v != variant and eng.variant_ok(v):
+            fps_other, isim = time_variant(eng, v, isim, dt, 100)
+            other.append({"form": FORMS[v], "kernel": KERNELS[v], "frames_per_s": round(fps_other * nwfs, 2)})
+    eng.set_variant(variant)
 
     out = {"metric": METRIC, "value": round(fps, 2), "unit": "WFS frames/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3" if variant == _lib.VARIANT_MFMA_BF16 else "fp32",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": {0: "fp32", 1: "bf16x3", 2: "fp16x3"}[variant],
            "data": "synthetic (seeded von Karman screens, Gaussian sodium profile)",
            "config": {"workload": "lgs30m: 30 m pupil, 6 LGS WFS x 60x60 (nsa=%d), 32x32 "
                       "OPD pts -> 64x64 FFT, 6x6 px of 1\", 7 layers 4096^2 @1/64 m, LLT "
                       "uplink, ETF, matched filter, 800 Hz" % p.nsa,
                       "nwfs": nwfs, "nsa": p.nsa, "notf": p.notf, "npix": p.npix,
-                      "variant": {0: "fp32-radix-fft", 1: "mfma-split-bf16"}[variant],
+                      "variant": FORMS[variant],
                       "launch": "kernel by kernel" if args.no_graph else "one HIP graph per step"},
            "roofline": roof}
-    if other is not None:
-        out["other_form"] = other
+    if other:
+        out["other_forms"] = other
 
     # CPU baseline: FP64 oracle on the host cores, bounded sample of frames
     if not args.no_cpu:
@@ -471,18 +477,15 @@ def main():
         o = oracle.Oracle(p)
         o.atm(layers)
         ref_isim = isim
-        eng.queue(ref_isim, dt)
-        g_gpu, ints_gpu = eng.sync(0)
-        mt = np.zeros(p.nsa * 2 * p.npix)
-        o.set_ref(mt, np.ones(p.nsa))
+        o.set_ref(*refs[0])
         nf = 0
         tc0 = time.perf_counter()
-        ri = None
+        ri = rg = None
         while True:
             iw = nf % nwfs
             r = o.step(wfs[iw], ref_isim, dt, nthread=threads)
             if nf == 0:
-                ri = r[0]
+                ri, rg = r[0], r[1]
             nf += 1
             if time.perf_counter() - tc0 > args.cpu_seconds:
                 break
@@ -493,8 +496,18 @@ def main():
                                "sample": "%d full WFS frames (nsa=%d) of the same workload, FP64 "
                                          "oracle with OpenMP over subapertures" % (nf, p.nsa)}
         out["gpu_over_cpu"] = round(fps / cpu_fps, 1)
-        rel = float(np.linalg.norm(ints_gpu - ri) / np.linalg.norm(ri))
-        out["parity"] = {"ints_rel_l2_wfs0": rel}
+        # every imaging form against the FP64 oracle on WFS 0 at one step
+        parity = {}
+        for v in (0, 1, 2):
+            if eng.variant_ok(v):
+                eng.set_variant(v)
+                eng.queue(ref_isim, dt)
+                g_v, ints_v = eng.sync(0)
+                parity[FORMS[v]] = {
+                    "ints_rel_l2": float(np.linalg.norm(ints_v - ri) / np.linalg.norm(ri)),
+                    "grad_max_abs_mas": float(np.abs(g_v - rg).max() * 180 / math.pi * 3600e3)}
+        eng.set_variant(variant)
+        out["parity"] = {"wfs": 0, "isim": ref_isim, "forms": parity}
     print(json.dumps(out), flush=True)
     eng.close()
 

@@ -32,6 +32,7 @@ extern "C" {
 #define MX_VARIANT_AUTO -1      /* MFMA form where it applies, else FP32 FFT */
 #define MX_VARIANT_FFT32 0      /* FP32 radix-2 FFTs on the vector ALUs */
 #define MX_VARIANT_MFMA_BF16 1  /* DFTs as split-BF16 matrix products on MFMA */
+#define MX_VARIANT_MFMA_FP16 2  /* the same with split-FP16 operands (comparison form) */
 
 #define MX_PTYPE_MF 1  /* matched filter (PTYPE_MF) */
 #define MX_PTYPE_COG 2 /* thresholded centre of gravity (PTYPE_COG) */
@@ -177,8 +178,9 @@ int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
 
 /* Select how the per-subaperture transforms are computed (no counterpart in
  * the reference, whose GPU code always uses cuFFT).  MX_VARIANT_MFMA_BF16
- * needs notf=64, nxsa=32, at most 16 sampled image rows and columns per
- * wavelength and no second ETF; AUTO (the default) uses it when those hold.
+ * and MX_VARIANT_MFMA_FP16 need notf=64, nxsa=32, at most 16 sampled image
+ * rows and columns per wavelength and no second ETF; AUTO (the default) uses
+ * the BF16 form when those hold.  Other sizes run the FP32 FFT forms.
  * get_variant reports the form the next step will use. */
 int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant);
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant);

@@ -100,7 +100,7 @@ HIP_API = {
     "mx_wfsgroup_free": (C.c_int, [C.c_void_p]),
 }
 
-VARIANT_AUTO, VARIANT_FFT32, VARIANT_MFMA_BF16 = -1, 0, 1
+VARIANT_AUTO, VARIANT_FFT32, VARIANT_MFMA_BF16, VARIANT_MFMA_FP16 = -1, 0, 1, 2
 
 # name -> (restype, argtypes) of every symbol in include/maos_setup.h
 SETUP_API = {

@@ -720,10 +720,17 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
 
 // ------------------------------------------------------------ MFMA form
 // The same imaging chain with every 1-D DFT stage computed as a complex
-// matrix product on the matrix cores (v_mfma_f32_32x32x16_bf16 and
-// v_mfma_f32_16x16x32_bf16).  Operands are split into BF16 hi + lo pairs and
-// each real product is hi*hi + hi*lo + lo*hi, accumulated in FP32.  One
-// workgroup of four waves images one subaperture (N = 64, nxsa = 32).
+// matrix product on the matrix cores (v_mfma_f32_16x16x32_bf16, or _f16).
+// Operands are split into hi + lo pairs of the element type E (BF16, or FP16
+// for comparison) and each real product is hi*hi + hi*lo + lo*hi,
+// accumulated in FP32.  One workgroup of four waves images one subaperture
+// (N = 64, nxsa = 32).  FP16 has 11 significant bits against BF16's 8 but
+// the range of +-65504, and a lo part below 2^-14 is subnormal: every stored
+// operand is scaled by a power of two so that its bound is 2^14 (field |E| <=
+// amp <= 1; Z <= 32; P+- <= 2^21; Q+- <= 2^23; X <= 2^22 times the uplink OTF
+// at zero, the sum of the LLT amplitude squared, <= 1024; R <= 64 max X), the
+// DFT tables by 2^13, and the image by the product of the scales removed at
+// the end.
 //
 // The field is the A operand (LDS rows contiguous along the summed index) and
 // the symmetric DFT matrix F[k][n] = exp(-2 pi i k n / 64) the B operand.  A
@@ -751,6 +758,11 @@ __global__ __launch_bounds__(64) void k_llt_otf(LltArgs a) {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+template <typename E>
+using v8_t = E __attribute__((ext_vector_type(8)));
+template <typename E>
+constexpr bool is_f16 = std::is_same<E, _Float16>::value;
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float float2v __attribute__((ext_vector_type(2)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -771,7 +783,8 @@ __device__ __forceinline__ int lds_at(int row, int k) {
 }
 constexpr int MX_MFMA_NS = 16;  // sampled image rows/columns kept by stage 3
 
-// Table entries: [.. entry ..][part][lane] bf16x8, parts re hi, re lo, im hi,
+// Table entries: [.. entry ..][part][lane] 8 elements (BF16 or FP16 tables),
+// parts re hi, re lo, im hi,
 // im lo, -im hi, -im lo; a lane holds 8 consecutive k of one n (B-operand
 // layout).  The negated imaginary part lets every complex product accumulate
 // with additions only.
@@ -784,8 +797,8 @@ constexpr int MX_FENT = 8;
 struct MfmaWvl {
     const float2 *etf_t;     // [nsa][MX_HALF] ETF (nominal fused), half_slot order, or null
     const float *nominal_t;  // [MX_HALF] nominal when there is no ETF, or null
-    const bf16x8 *gx;        // [2 ks][6][64] conj(F) at the sampled columns
-    const bf16x8 *gy;        // [6][64] w(uy) conj(F) at the sampled rows, uy < 32
+    const void *gx;          // [2 ks][6][64] conj(F) at the sampled columns
+    const void *gy;          // [6][64] w(uy) conj(F) at the sampled rows, uy < 32
     const int4 *si_c;        // [npix] compact indices (iy*16 + ix) into the 16x16 image
     const float4 *si_w;      // [npix]
     float inv_wvl;
@@ -795,7 +808,7 @@ struct MfmaArgs {
     int nsa, npix, pixpsax, pixpsay, nwvl, nwfs;
     const float *amp;
     const float *opd;
-    const bf16x8 *F;          // [MX_FENT][6][64]
+    const void *F;            // [MX_FENT][6][64], element type of the form
     const MfmaWvl *wv;
     const float2 *lotf_t;     // [nwfs][nwvl][MX_HALF] conj(lotfc), or null
     const DevWfs *wfs;
@@ -813,41 +826,39 @@ struct MfmaArgs {
     float2 *lotf_full, *lotf_half;
 };
 
-#define MF32(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0)
-#define MF16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0)
+template <typename E>
+__device__ __forceinline__ f32x4 mf16(v8_t<E> a, v8_t<E> b, f32x4 c) {
+    if constexpr (is_f16<E>) return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
 // c += A * B for one K step with split operands: hi*hi + hi*lo + lo*hi
-template <typename ACC, int SZ>
-__device__ __forceinline__ void mac3(ACC &c, bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl) {
-    if constexpr (SZ == 32) {
-        c = MF32(ah, bh, c);
-        c = MF32(ah, bl, c);
-        c = MF32(al, bh, c);
-    } else {
-        c = MF16(ah, bh, c);
-        c = MF16(ah, bl, c);
-        c = MF16(al, bh, c);
-    }
+template <typename E>
+__device__ __forceinline__ void mac3(f32x4 &c, v8_t<E> ah, v8_t<E> al, v8_t<E> bh, v8_t<E> bl) {
+    c = mf16<E>(ah, bh, c);
+    c = mf16<E>(ah, bl, c);
+    c = mf16<E>(al, bh, c);
 }
 
 // (cr, ci) += D * B with D the field (A operand, from LDS) and B = b[0] + i
 // b[1]; b[2] = -b[1]
-template <typename ACC, int SZ, bool DREAL>
-__device__ __forceinline__ void cmacd(ACC &cr, ACC &ci, const __bf16 *const (&L)[4], int row, int k,
-                                      const bf16x8 (&b)[3][2]) {
-    const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(L[0] + lds_at(row, k));
-    const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(L[1] + lds_at(row, k));
-    mac3<ACC, SZ>(cr, dh, dl, b[0][0], b[0][1]);
-    mac3<ACC, SZ>(ci, dh, dl, b[1][0], b[1][1]);
+template <typename E, bool DREAL>
+__device__ __forceinline__ void cmacd(f32x4 &cr, f32x4 &ci, const E *const (&L)[4], int row, int k,
+                                      const v8_t<E> (&b)[3][2]) {
+    const v8_t<E> dh = *reinterpret_cast<const v8_t<E> *>(L[0] + lds_at(row, k));
+    const v8_t<E> dl = *reinterpret_cast<const v8_t<E> *>(L[1] + lds_at(row, k));
+    mac3<E>(cr, dh, dl, b[0][0], b[0][1]);
+    mac3<E>(ci, dh, dl, b[1][0], b[1][1]);
     if (!DREAL) {
-        const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(L[2] + lds_at(row, k));
-        const bf16x8 el = *reinterpret_cast<const bf16x8 *>(L[3] + lds_at(row, k));
-        mac3<ACC, SZ>(cr, eh, el, b[2][0], b[2][1]);
-        mac3<ACC, SZ>(ci, eh, el, b[0][0], b[0][1]);
+        const v8_t<E> eh = *reinterpret_cast<const v8_t<E> *>(L[2] + lds_at(row, k));
+        const v8_t<E> el = *reinterpret_cast<const v8_t<E> *>(L[3] + lds_at(row, k));
+        mac3<E>(cr, eh, el, b[2][0], b[2][1]);
+        mac3<E>(ci, eh, el, b[0][0], b[0][1]);
     }
 }
 
-__device__ __forceinline__ void loadT(const bf16x8 *T, int lane, bf16x8 (&b)[3][2]) {
+template <typename E>
+__device__ __forceinline__ void loadT(const v8_t<E> *T, int lane, v8_t<E> (&b)[3][2]) {
 #pragma unroll
     for (int p = 0; p < 3; p++) {
         b[p][0] = T[(2 * p) * 64 + lane];
@@ -855,23 +866,31 @@ __device__ __forceinline__ void loadT(const bf16x8 *T, int lane, bf16x8 (&b)[3][
     }
 }
 
-// hi = bf16(a) (round to nearest even), lo = bf16(a - hi) for a pair, packed
-// two per dword: one packed convert, two unpacks, two subtractions, one
-// packed convert
+// hi = E(a) (round to nearest even), lo = E(a - hi) for a pair, packed two
+// per dword: one packed convert, two unpacks, two subtractions, one packed
+// convert
+template <typename E>
 __device__ __forceinline__ void split2(float a0, float a1, unsigned &h, unsigned &l) {
-    const bf16x2 hb = __builtin_convertvector((float2v){a0, a1}, bf16x2);
-    h = __builtin_bit_cast(unsigned, hb);
-    const float h0 = __uint_as_float(h << 16), h1 = __uint_as_float(h & 0xffff0000u);
-    const bf16x2 lb = __builtin_convertvector((float2v){a0 - h0, a1 - h1}, bf16x2);
-    l = __builtin_bit_cast(unsigned, lb);
+    if constexpr (is_f16<E>) {
+        const f16x2 hb = __builtin_convertvector((float2v){a0, a1}, f16x2);
+        h = __builtin_bit_cast(unsigned, hb);
+        const f16x2 lb = __builtin_convertvector((float2v){a0 - (float)hb[0], a1 - (float)hb[1]}, f16x2);
+        l = __builtin_bit_cast(unsigned, lb);
+    } else {
+        const bf16x2 hb = __builtin_convertvector((float2v){a0, a1}, bf16x2);
+        h = __builtin_bit_cast(unsigned, hb);
+        const float h0 = __uint_as_float(h << 16), h1 = __uint_as_float(h & 0xffff0000u);
+        const bf16x2 lb = __builtin_convertvector((float2v){a0 - h0, a1 - h1}, bf16x2);
+        l = __builtin_bit_cast(unsigned, lb);
+    }
 }
 
 // split 4 values into hi/lo and store them at plane row, columns k..k+3
-__device__ __forceinline__ void st4(__bf16 *ph, __bf16 *pl, int row, int k, float a0, float a1, float a2,
-                                    float a3) {
+template <typename E>
+__device__ __forceinline__ void st4(E *ph, E *pl, int row, int k, float a0, float a1, float a2, float a3) {
     uint2 h, l;
-    split2(a0, a1, h.x, l.x);
-    split2(a2, a3, h.y, l.y);
+    split2<E>(a0, a1, h.x, l.x);
+    split2<E>(a2, a3, h.y, l.y);
     *reinterpret_cast<uint2 *>(ph + lds_at(row, k)) = h;
     *reinterpret_cast<uint2 *>(pl + lds_at(row, k)) = l;
 }
@@ -892,13 +911,22 @@ __device__ __forceinline__ void cis_rev(float x, float am, float &re, float &im)
 // UPLINK: the same stages through 2b on the LLT grid give the uplink OTF
 // lotfc = FFT(|FFT(embed)|^2) / nlwvf^2 (maos/wfsints.c:149-199); one
 // workgroup per (WFS, wavelength) writes conj(lotfc) and stops there.
-template <bool UPLINK>
+template <bool UPLINK, typename E>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAVES))) void k_shwfs_mfma(MfmaArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 Lb[4 * MPLANE];
+    using V8 = v8_t<E>;
+    // range scales of the FP16 form (powers of two; tables carry 2^13): the
+    // field, the stage outputs 1a, 1b (|Y|^2), 2a, 2b, 3a, the image and the
+    // uplink OTF
+    constexpr bool H = is_f16<E>;
+    constexpr float SE = H ? 0x1p14f : 1.f, F1 = H ? 0x1p-18f : 1.f, F2 = H ? 0x1p-51f : 1.f,
+                    F3 = H ? 0x1p-15f : 1.f, F4 = H ? 0x1p-22f : 1.f, F5 = H ? 0x1p-19f : 1.f,
+                    FI = H ? 0x1p11f : 1.f, FO = H ? 0x1p-4f : 1.f;
+    __shared__ __attribute__((aligned(16))) E Lb[4 * MPLANE];
     __shared__ float img[MX_MFMA_NS * MX_MFMA_NS];
     __shared__ float red[4];
-    __bf16 *const L[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
-    const __bf16 *const Lc[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
+    E *const L[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
+    const E *const Lc[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
+    const V8 *const F = static_cast<const V8 *>(a.F);
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     int iwfs, isa, iwvl0, iwvl1;
     if constexpr (UPLINK) {
@@ -940,8 +968,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // B operand of stages 1a and 1b: F over the band [16, 48) at the 16
         // columns 16 lw + l%16; fetched before the embed so that it arrives
         // meanwhile
-        bf16x8 b1[3][2];
-        loadT(a.F + (MX_F1A + lw) * MX_TPARTS * 64, lane, b1);
+        V8 b1[3][2];
+        loadT<E>(F + (MX_F1A + lw) * MX_TPARTS * 64, lane, b1);
         // embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf) as W[x][y], y contiguous;
         // OPD and amplitude are read again per wavelength (cache hits) rather
         // than held in registers
@@ -949,7 +977,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             float re[4], im[4];
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                cis_rev((og[(iy0 + j) * NX + ix] - mean) * WV.inv_wvl, ag[(iy0 + j) * NX + ix], re[j], im[j]);
+                cis_rev((og[(iy0 + j) * NX + ix] - mean) * WV.inv_wvl, ag[(iy0 + j) * NX + ix] * SE, re[j], im[j]);
             st4(L[0], L[1], B0 + ix, B0 + iy0, re[0], re[1], re[2], re[3]);
             st4(L[2], L[3], B0 + ix, B0 + iy0, im[0], im[1], im[2], im[3]);
         }
@@ -960,13 +988,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             f32x4 zr[2] = {}, zi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<f32x4, 16, false>(zr[mt], zi[mt], Lc, B0 + 16 * mt + l16, B0 + 8 * h16, b1);
+                cmacd<E, false>(zr[mt], zi[mt], Lc, B0 + 16 * mt + l16, B0 + 8 * h16, b1);
             __syncthreads();
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
                 const int row = 16 * lw + l16, col = B0 + 16 * mt + 4 * h16;
-                st4(L[0], L[1], row, col, zr[mt][0], zr[mt][1], zr[mt][2], zr[mt][3]);
-                st4(L[2], L[3], row, col, zi[mt][0], zi[mt][1], zi[mt][2], zi[mt][3]);
+                st4(L[0], L[1], row, col, zr[mt][0] * F1, zr[mt][1] * F1, zr[mt][2] * F1, zr[mt][3] * F1);
+                st4(L[2], L[3], row, col, zi[mt][0] * F1, zi[mt][1] * F1, zi[mt][2] * F1, zi[mt][3] * F1);
             }
         }
         __syncthreads();
@@ -975,17 +1003,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // symmetric), the four 16-row blocks t of ky; blocks t and t + 2 hold ky
         // and ky + 32 in the same lane: P+ = P[ky] + P[ky+32] and P- = P[ky] -
         // P[ky+32] (ky < 32) go to row kx, columns ky and 32 + ky
-        bf16x8 b2[3][2];
+        V8 b2[3][2];
         {
             f32x4 yr[4] = {}, yi[4] = {};
 #pragma unroll
-            for (int t = 0; t < 4; t++) cmacd<f32x4, 16, false>(yr[t], yi[t], Lc, 16 * t + l16, B0 + 8 * h16, b1);
-            loadT(a.F + (MX_F2 + 2 * nb) * MX_TPARTS * 64, lane, b2); // stage 2a's operand
+            for (int t = 0; t < 4; t++) cmacd<E, false>(yr[t], yi[t], Lc, 16 * t + l16, B0 + 8 * h16, b1);
+            loadT<E>(F + (MX_F2 + 2 * nb) * MX_TPARTS * 64, lane, b2); // stage 2a's operand
             __syncthreads();
 #pragma unroll
             for (int t = 0; t < 4; t++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) yr[t][v] = yr[t][v] * yr[t][v] + yi[t][v] * yi[t][v];
+                for (int v = 0; v < 4; v++) yr[t][v] = (yr[t][v] * yr[t][v] + yi[t][v] * yi[t][v]) * F2;
             const int kx = 16 * lw + l16;
 #pragma unroll
             for (int t = 0; t < 2; t++) {
@@ -1006,10 +1034,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             f32x4 qr[2] = {}, qi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<f32x4, 16, true>(qr[mt], qi[mt], Lc, 16 * mb + 32 * mt + l16, 32 * e + 8 * h16, b2);
-            loadT(a.F + (MX_F2 + 2 * nb + mb) * MX_TPARTS * 64, lane, b1); // stage 2b's operand
+                cmacd<E, true>(qr[mt], qi[mt], Lc, 16 * mb + 32 * mt + l16, 32 * e + 8 * h16, b2);
+            loadT<E>(F + (MX_F2 + 2 * nb + mb) * MX_TPARTS * 64, lane, b1); // stage 2b's operand
             __syncthreads();
             const int uy = 2 * l16 + e, col = 16 * mb + 4 * h16;
+            if constexpr (H)
+#pragma unroll
+                for (int mt = 0; mt < 2; mt++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        qr[mt][v] *= F3;
+                        qi[mt][v] *= F3;
+                    }
             st4(L[0], L[1], uy, col, qr[0][0] + qr[1][0], qr[0][1] + qr[1][1], qr[0][2] + qr[1][2],
                 qr[0][3] + qr[1][3]);
             st4(L[2], L[3], uy, col, qi[0][0] + qi[1][0], qi[0][1] + qi[1][1], qi[0][2] + qi[1][2],
@@ -1029,7 +1065,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             f32x4 xr[2] = {}, xi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<f32x4, 16, false>(xr[mt], xi[mt], Lc, 16 * mt + l16, 32 * e + 8 * h16, b1);
+                cmacd<E, false>(xr[mt], xi[mt], Lc, 16 * mt + l16, 32 * e + 8 * h16, b1);
             // element (mt, v) of this lane is table slot ((2 lw + mt) 4 + v) 64 + lane
             const long tbase = (long)(2 * lw) * 4 * 64 + lane;
             const int ux = 2 * (16 * hf + l16) + e;
@@ -1044,7 +1080,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 #pragma unroll
                     for (int v = 0; v < 4; v++) {
                         const int uy = 16 * mt + 4 * h16 + v;
-                        const float re = xr[mt][v] * a.llt_norm, im = xi[mt][v] * a.llt_norm;
+                        const float re = xr[mt][v] * (a.llt_norm * FO), im = xi[mt][v] * (a.llt_norm * FO);
                         hp[(mt * 4 + v) * 64] = make_float2(re, -im);
                         fp[ux * 64 + uy] = make_float2(re, -im);
                         if (uy) fp[((64 - ux) & 63) * 64 + (64 - uy)] = make_float2(re, im);
@@ -1085,13 +1121,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
                     xi[i / 4][i % 4] *= tn[i];
                 }
             }
-            loadT(WV.gy, lane, b2); // stage 3a's operand
+            loadT<E>(static_cast<const V8 *>(WV.gy), lane, b2); // stage 3a's operand
             __syncthreads();
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
                 const int col = 16 * mt + 4 * h16;
-                st4(L[0], L[1], ux, col, xr[mt][0], xr[mt][1], xr[mt][2], xr[mt][3]);
-                st4(L[2], L[3], ux, col, xi[mt][0], xi[mt][1], xi[mt][2], xi[mt][3]);
+                st4(L[0], L[1], ux, col, xr[mt][0] * F4, xr[mt][1] * F4, xr[mt][2] * F4, xr[mt][3] * F4);
+                st4(L[2], L[3], ux, col, xi[mt][0] * F4, xi[mt][1] * F4, xi[mt][2] * F4, xi[mt][3] * F4);
             }
         }
         __syncthreads();
@@ -1099,31 +1135,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // and 2 elsewhere (folded into the table); role r: ux in [16r, 16r+16)
         {
             f32x4 rr = {}, ri = {};
-            cmacd<f32x4, 16, false>(rr, ri, Lc, 16 * lw + l16, 8 * h16, b2);
-            if (lw == 3) loadT(WV.gx, lane, b1); // stage 3b's operand, first K step
+            cmacd<E, false>(rr, ri, Lc, 16 * lw + l16, 8 * h16, b2);
+            if (lw == 3) loadT<E>(static_cast<const V8 *>(WV.gx), lane, b1); // stage 3b's operand, first K step
             __syncthreads();
             // element v: ux = 16 lw + 4 (l/16) + v, ys = l%16 -> row ys, columns ux
-            st4(L[0], L[1], l16, 16 * lw + 4 * h16, rr[0], rr[1], rr[2], rr[3]);
-            st4(L[2], L[3], l16, 16 * lw + 4 * h16, ri[0], ri[1], ri[2], ri[3]);
+            st4(L[0], L[1], l16, 16 * lw + 4 * h16, rr[0] * F5, rr[1] * F5, rr[2] * F5, rr[3] * F5);
+            st4(L[2], L[3], l16, 16 * lw + 4 * h16, ri[0] * F5, ri[1] * F5, ri[2] * F5, ri[3] * F5);
         }
         __syncthreads();
         // 3b: I[ys][xs] = Re sum_ux R[ys][ux] Gx[ux][xs]  (role 3)
         if (lw == 3) {
             f32x4 ir = {};
-            loadT(WV.gx + MX_TPARTS * 64, lane, b2);
+            loadT<E>(static_cast<const V8 *>(WV.gx) + MX_TPARTS * 64, lane, b2);
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) {
-                const bf16x8(&g)[3][2] = ks ? b2 : b1;
+                const V8(&g)[3][2] = ks ? b2 : b1;
                 const int k = 32 * ks + 8 * h16;
-                const bf16x8 dh = *reinterpret_cast<const bf16x8 *>(Lc[0] + lds_at(l16, k));
-                const bf16x8 dl = *reinterpret_cast<const bf16x8 *>(Lc[1] + lds_at(l16, k));
-                const bf16x8 eh = *reinterpret_cast<const bf16x8 *>(Lc[2] + lds_at(l16, k));
-                const bf16x8 el = *reinterpret_cast<const bf16x8 *>(Lc[3] + lds_at(l16, k));
-                mac3<f32x4, 16>(ir, dh, dl, g[0][0], g[0][1]); // Re R * Re G
-                mac3<f32x4, 16>(ir, eh, el, g[2][0], g[2][1]); // Im R * (-Im G)
+                const V8 dh = *reinterpret_cast<const V8 *>(Lc[0] + lds_at(l16, k));
+                const V8 dl = *reinterpret_cast<const V8 *>(Lc[1] + lds_at(l16, k));
+                const V8 eh = *reinterpret_cast<const V8 *>(Lc[2] + lds_at(l16, k));
+                const V8 el = *reinterpret_cast<const V8 *>(Lc[3] + lds_at(l16, k));
+                mac3<E>(ir, dh, dl, g[0][0], g[0][1]); // Re R * Re G
+                mac3<E>(ir, eh, el, g[2][0], g[2][1]); // Im R * (-Im G)
             }
 #pragma unroll
-            for (int v = 0; v < 4; v++) img[(4 * (lane / 16) + v) * MX_MFMA_NS + lane % 16] = ir[v];
+            for (int v = 0; v < 4; v++) img[(4 * (lane / 16) + v) * MX_MFMA_NS + lane % 16] = ir[v] * FI;
         }
         __syncthreads();
         if (tid < a.npix) {
@@ -1743,9 +1779,11 @@ struct mx_wfsgrad_s {
     // MFMA form
     int variant = MX_VARIANT_AUTO;     // requested form
     bool mfma_size = false;            // N = 64, nxsa = 32
+    double amp_max = 0;                // largest subaperture or LLT amplitude (the FP16 form needs <= 1)
     bool mfma_wvl_ok[MX_MAXWVL] = {};  // sampled rows/columns fit in 16
-    uint16_t *d_F = nullptr;           // bf16x8 table [MX_FENT][6][64]
-    uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};
+    uint16_t *d_F = nullptr, *d_F16 = nullptr; // DFT tables [MX_FENT][6][64] x 8, BF16 and FP16
+    uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};     // BF16
+    uint16_t *d_gx16[MX_MAXWVL] = {}, *d_gy16[MX_MAXWVL] = {}; // FP16
     int4 *d_sic[MX_MAXWVL] = {};
     float2 *d_etf_t[MX_MAXWVL] = {};
     float *d_nominal_t[MX_MAXWVL] = {};
@@ -1914,6 +1952,15 @@ void split_bf(double x, uint16_t &hi, uint16_t &lo) {
     hi = f2bf((float)x);
     lo = f2bf((float)(x - (double)bf2f(hi)));
 }
+// hi/lo FP16 pair (round to nearest even, subnormals kept)
+void split_h(double x, uint16_t &hi, uint16_t &lo) {
+    const _Float16 h = (_Float16)(float)x, l = (_Float16)(float)(x - (double)(float)h);
+    std::memcpy(&hi, &h, 2);
+    std::memcpy(&lo, &l, 2);
+}
+// element type of the table entries being built (BF16 unless set); FP16
+// entries carry a scale of 2^13 (the kernel's range plan)
+bool g_tab_f16 = false;
 // one table entry (6 parts x 64 lanes x 8) of complex values: re, im, -im
 void put_entry(uint16_t *dst, const double (*re)[8], const double (*im)[8]) {
     for (int lane = 0; lane < 64; lane++)
@@ -1921,7 +1968,8 @@ void put_entry(uint16_t *dst, const double (*re)[8], const double (*im)[8]) {
             const double v[3] = {re[lane][j], im[lane][j], -im[lane][j]};
             for (int p = 0; p < 3; p++) {
                 uint16_t h, l;
-                split_bf(v[p], h, l);
+                if (g_tab_f16) split_h(v[p] * 8192., h, l);
+                else split_bf(v[p], h, l);
                 dst[((2 * p) * 64 + lane) * 8 + j] = h;
                 dst[((2 * p + 1) * 64 + lane) * 8 + j] = l;
             }
@@ -2004,7 +2052,7 @@ int effective_variant(const mx_wfsgrad_t *h) {
     bool ok = h->mfma_size;
     for (int w = 0; w < h->nwvl; w++) ok = ok && h->mfma_wvl_ok[w] && !h->d_etf2[w];
     if (h->variant == MX_VARIANT_FFT32) return MX_VARIANT_FFT32;
-    if (h->variant == MX_VARIANT_MFMA_BF16) return ok ? MX_VARIANT_MFMA_BF16 : -1;
+    if (h->variant == MX_VARIANT_MFMA_BF16 || h->variant == MX_VARIANT_MFMA_FP16) return ok ? h->variant : -1;
     return ok ? MX_VARIANT_MFMA_BF16 : MX_VARIANT_FFT32;
 }
 } // namespace
@@ -2119,6 +2167,7 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     }
     if (h->mfma_size) {
         ALLOC(h->d_F, MX_FENT * MX_TPARTS * 64 * 8);
+        ALLOC(h->d_F16, MX_FENT * MX_TPARTS * 64 * 8);
         ALLOC(h->d_mwv, MX_MAXWVL);
     }
 #undef ALLOC
@@ -2129,7 +2178,11 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(MX_EHIP, "hipStreamCreate failed"));
     std::vector<float> tmp(nsa * npsa);
-    for (long i = 0; i < nsa * npsa; i++) tmp[i] = (float)p->amp[i];
+    for (long i = 0; i < nsa * npsa; i++) {
+        tmp[i] = (float)p->amp[i];
+        h->amp_max = std::max(h->amp_max, std::fabs(p->amp[i]));
+    }
+    for (long i = 0; i < p->llt_nx * p->llt_nx; i++) h->amp_max = std::max(h->amp_max, std::fabs(p->llt_amp[i]));
     if (hipMemcpy(h->d_amp, tmp.data(), sizeof(float) * nsa * npsa, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->d_ptsx, p->ptsx, sizeof(double) * nsa, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->d_ptsy, p->ptsy, sizeof(double) * nsa, hipMemcpyHostToDevice) != hipSuccess)
@@ -2162,8 +2215,12 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     if (lf && hipMemcpy(h->d_lf_tw, lf_tw.data(), sizeof(float2) * lf_tw.size(), hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_err(MX_EHIP, "upload of the FFT twiddles failed"));
     if (h->mfma_size) {
+        g_tab_f16 = true;
+        std::vector<uint16_t> F16 = build_F64();
+        g_tab_f16 = false;
         std::vector<uint16_t> F = build_F64();
-        if (hipMemcpy(h->d_F, F.data(), F.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(h->d_F, F.data(), F.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(h->d_F16, F16.data(), F16.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_err(MX_EHIP, "upload of the DFT table failed"));
     }
     if (hipMemset(h->d_grad, 0, sizeof(float) * nwfs * 2 * nsa) != hipSuccess ||
@@ -2190,7 +2247,7 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     drop_graph(h);
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
-                    h->d_lotf_t, h->d_F, h->d_mwv, h->d_lf_tw, h->d_b1, h->d_b2, h->d_b3, h->d_psf, h->d_img,
+                    h->d_lotf_t, h->d_F, h->d_F16, h->d_mwv, h->d_lf_tw, h->d_b1, h->d_b2, h->d_b3, h->d_psf, h->d_img,
                     h->d_lfw,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
     for (void *q : ptrs)
@@ -2199,6 +2256,8 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
         if (h->d_nominal[w]) hipFree(h->d_nominal[w]);
         if (h->d_gx[w]) hipFree(h->d_gx[w]);
         if (h->d_gy[w]) hipFree(h->d_gy[w]);
+        if (h->d_gx16[w]) hipFree(h->d_gx16[w]);
+        if (h->d_gy16[w]) hipFree(h->d_gy16[w]);
         if (h->d_sic[w]) hipFree(h->d_sic[w]);
         if (h->d_etf_t[w]) hipFree(h->d_etf_t[w]);
         if (h->d_nominal_t[w]) hipFree(h->d_nominal_t[w]);
@@ -2282,7 +2341,14 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
                     op[c] = (&wt[i].x)[c] != 0.f ? cy * MX_MFMA_NS + cx : 0;
                 }
             }
+            g_tab_f16 = true;
+            std::vector<uint16_t> gx16 = build_G(xs), gy16 = build_Gh(ys);
+            g_tab_f16 = false;
             std::vector<uint16_t> gx = build_G(xs), gy = build_Gh(ys);
+            if ((rc = dev_alloc(&h->d_gx16[iwvl], gx16.size())) || (rc = dev_alloc(&h->d_gy16[iwvl], gy16.size())))
+                return rc;
+            HIPCHK(hipMemcpy(h->d_gx16[iwvl], gx16.data(), gx16.size() * 2, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->d_gy16[iwvl], gy16.data(), gy16.size() * 2, hipMemcpyHostToDevice));
             if ((rc = dev_alloc(&h->d_gx[iwvl], gx.size())) || (rc = dev_alloc(&h->d_gy[iwvl], gy.size())) ||
                 (rc = dev_alloc(&h->d_sic[iwvl], h->npix)))
                 return rc;
@@ -2583,13 +2649,14 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     a.opd = h->d_opd;
     MfmaArgs &m = h->arg_mfma;
     m = MfmaArgs{};
-    if (variant == MX_VARIANT_MFMA_BF16) {
+    const bool f16 = variant == MX_VARIANT_MFMA_FP16;
+    if (variant == MX_VARIANT_MFMA_BF16 || f16) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
             hw[w].etf_t = h->d_etf_t[w];
             hw[w].nominal_t = h->d_nominal_t[w];
-            hw[w].gx = reinterpret_cast<const bf16x8 *>(h->d_gx[w]);
-            hw[w].gy = reinterpret_cast<const bf16x8 *>(h->d_gy[w]);
+            hw[w].gx = f16 ? h->d_gx16[w] : h->d_gx[w];
+            hw[w].gy = f16 ? h->d_gy16[w] : h->d_gy[w];
             hw[w].si_c = h->d_sic[w];
             hw[w].si_w = h->d_si_w[w];
             hw[w].inv_wvl = (float)(1. / h->wvl[w]);
@@ -2607,7 +2674,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         m.nwfs = a.nwfs;
         m.amp = a.amp;
         m.opd = a.opd;
-        m.F = reinterpret_cast<const bf16x8 *>(h->d_F);
+        m.F = f16 ? h->d_F16 : h->d_F;
         m.wv = h->d_mwv;
         m.lotf_t = h->cfg.llt_nx > 0 ? h->d_lotf_t : nullptr;
         m.wfs = a.wfs;
@@ -2647,7 +2714,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     // kernel), imaging.
     StepLaunch *L = sp.L;
     int nl = 0;
-    const bool mf = variant == MX_VARIANT_MFMA_BF16;
+    const bool mf = variant == MX_VARIANT_MFMA_BF16 || f16;
     const void *kopd_t[2][2][2] = {
         {{reinterpret_cast<const void *>(&k_wfs_opd<16, false, false>),
           reinterpret_cast<const void *>(&k_wfs_opd<16, false, true>)},
@@ -2734,7 +2801,9 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         }
     } else if (llt) {
         if (mf)
-            L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<true>), dim3((unsigned)(h->nwfs * h->nwvl)),
+            L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<true, _Float16>)
+                           : reinterpret_cast<const void *>(&k_shwfs_mfma<true, __bf16>),
+                       dim3((unsigned)(h->nwfs * h->nwvl)),
                        dim3(256), &m};
         else
             L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_llt_otf<64, 32>)
@@ -2790,7 +2859,9 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)),
                    dim3((unsigned)std::min(h->lf_threads, 256)), &g};
     } else if (mf) {
-        L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_mfma<false>), dim3((unsigned)(h->nsa * h->nwfs)),
+        L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16>)
+                       : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16>),
+                   dim3((unsigned)(h->nsa * h->nwfs)),
                    dim3(256), &m};
     } else {
         const long spw = 64 / N, ngroup = (h->nsa + spw - 1) / spw;
@@ -3090,10 +3161,13 @@ int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf) {
 
 int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant) {
     if (!h) return set_err(MX_EINVAL, "null handle");
-    if (variant != MX_VARIANT_AUTO && variant != MX_VARIANT_FFT32 && variant != MX_VARIANT_MFMA_BF16)
+    if (variant != MX_VARIANT_AUTO && variant != MX_VARIANT_FFT32 && variant != MX_VARIANT_MFMA_BF16 &&
+        variant != MX_VARIANT_MFMA_FP16)
         return set_err(MX_EINVAL, "unknown variant %d", variant);
-    if (variant == MX_VARIANT_MFMA_BF16 && !h->mfma_size)
+    if ((variant == MX_VARIANT_MFMA_BF16 || variant == MX_VARIANT_MFMA_FP16) && !h->mfma_size)
         return set_err(MX_EINVAL, "the MFMA form needs notf=64 and nxsa=32");
+    if (variant == MX_VARIANT_MFMA_FP16 && h->amp_max > 1. + 1e-6)
+        return set_err(MX_EINVAL, "the FP16 form's range plan needs amplitudes <= 1 (largest %g)", h->amp_max);
     h->variant = variant;
     return MX_OK;
 }
@@ -3136,7 +3210,10 @@ int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep, char *r
     } else {
         const int saved = h->variant;
         h->variant = MX_VARIANT_MFMA_BF16;
-        if (effective_variant(h) == MX_VARIANT_MFMA_BF16) variants.push_back(MX_VARIANT_MFMA_BF16);
+        if (effective_variant(h) == MX_VARIANT_MFMA_BF16) {
+            variants.push_back(MX_VARIANT_MFMA_BF16);
+            if (h->amp_max <= 1. + 1e-6) variants.push_back(MX_VARIANT_MFMA_FP16);
+        }
         h->variant = saved;
         variants.push_back(MX_VARIANT_FFT32);
     }
@@ -3183,7 +3260,10 @@ int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep, char *r
         char line[160];
         for (size_t i = 0; i < cand.size(); i++) {
             std::snprintf(line, sizeof line, "%sform=%s opd_tile=%d threads=%d step_us=%.2f\n", i == best ? "*" : " ",
-                          cand[i].variant == MX_VARIANT_MFMA_BF16 ? "mfma_bf16" : (h->lf ? "lf_fft32" : "fft32"),
+                          cand[i].variant == MX_VARIANT_MFMA_BF16   ? "mfma_bf16"
+                          : cand[i].variant == MX_VARIANT_MFMA_FP16 ? "mfma_fp16"
+                          : h->lf                                   ? "lf_fft32"
+                                                                    : "fft32",
                           cand[i].tile, h->lf ? cand[i].threads : 0, cand[i].us);
             txt += line;
         }

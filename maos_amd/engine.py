@@ -125,7 +125,7 @@ class WfsGrad:
         return out
 
     def set_variant(self, variant):
-        """_lib.VARIANT_AUTO, VARIANT_FFT32 or VARIANT_MFMA_BF16."""
+        """_lib.VARIANT_AUTO, VARIANT_FFT32, VARIANT_MFMA_BF16 or VARIANT_MFMA_FP16."""
         check(_lib.hip().mx_wfsgrad_set_variant(self.h, variant))
 
     def variant(self):

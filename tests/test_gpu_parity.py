@@ -62,14 +62,16 @@ def _lgs_ref(p, o, wfs, dt, isim=3):
     return mtche, i0sum
 
 
-@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16"])
+VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2}  # _lib.VARIANT_*
+
+
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16"])
 def test_lgs_step_parity(lgs_small, variant):
-    from maos_amd import _lib
     cfg, p, layers = lgs_small
     print("case: lgs order20 nsa=%d notf=%d nwfs=%d 1024^2 screens, %s"
           % (p.nsa, p.notf, len(cfg["wfs"]), variant), flush=True)
     eng, wfs = _engine(cfg, p, layers)
-    want = {"fft32": _lib.VARIANT_FFT32, "mfma_bf16": _lib.VARIANT_MFMA_BF16}[variant]
+    want = VARIANTS[variant]
     eng.set_variant(want)
     assert eng.variant() == want
     o = _oracle(p, layers)
@@ -91,15 +93,15 @@ def test_lgs_step_parity(lgs_small, variant):
             print("isim=%d wfs=%d opd=%.2e lotf=%.2e ints=%.2e grad=%.3e mas"
                   % (isim, iwfs, e_opd, e_lotf, e_int, e_g))
             assert e_opd < 1e-5
-            # the MFMA form computes the uplink OTF with the same split-BF16
-            # products as the image, so it gets the image's bar
+            # the MFMA forms compute the uplink OTF with the same split
+            # products as the image, so they get the image's bar
             assert e_lotf < (1e-5 if variant == "fft32" else INTS_TOL)
             assert e_int <= INTS_TOL
             assert e_g <= GRAD_TOL_MAS
     eng.close()
 
 
-@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16"])
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16"])
 def test_ngs_two_wavelengths(ngs_two_wvl, variant):
     """N=64 without LLT or ETF (nominal DTF only), two wavelengths with their
     own si and weights, two WFS off axis: both imaging forms against the
@@ -109,7 +111,7 @@ def test_ngs_two_wavelengths(ngs_two_wvl, variant):
     print("case: ngs two wavelengths nsa=%d notf=%d, %s" % (p.nsa, p.notf, variant), flush=True)
     eng, wfs = _engine(cfg, p, layers)
     assert eng.variant() == _lib.VARIANT_MFMA_BF16  # AUTO picks the MFMA form here
-    want = {"fft32": _lib.VARIANT_FFT32, "mfma_bf16": _lib.VARIANT_MFMA_BF16}[variant]
+    want = VARIANTS[variant]
     eng.set_variant(want)
     o = _oracle(p, layers)
     dt = cfg["sim"].dt
