@@ -185,6 +185,21 @@ int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
 int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant);
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant);
 
+/* Photon and read noise on the pixel intensities of every later step
+ * (addnoise, lib/libmisc.c:31-57, as applied at maos/wfsgrad.c:423-442;
+ * seeding replaces gpu_wfsgrad_seeding, cuda/sim/wfsinit.cu:532):
+ * ints = randp(ints + bkgrnd) + rne N(0,1) - bkgrndc with MAOS's randp
+ * (math/random.c:186-209: a normal approximation above 200, else the
+ * multiplication method in chunks of 12).  bkgrnd is powfs.bkgrnd*dtrat and
+ * bkgrndc bkgrnd*powfs.bkgrndc in photons per pixel per frame.  The uniforms
+ * come from Philox4x32-10 keyed by seed, one stream per (step, WFS,
+ * subaperture, pixel), so the result does not depend on launch order; the
+ * streams differ from MAOS's MT19937 and the reference GPU code's curand by
+ * design.  Gradients and the i0 sums use the noisy intensities.  Give each
+ * engine its own seed.  enable=0 (the default) is noise-free. */
+int mx_wfsgrad_set_noise(mx_wfsgrad_t *h, int enable, double rne, double bkgrnd,
+                         double bkgrndc, unsigned long long seed);
+
 /* Launch each step's kernels as one HIP graph (enable=1) or one by one
  * (enable=0, the default).  No counterpart in the reference, whose GPU code
  * launches kernels directly; the graph is built on the next queued step and

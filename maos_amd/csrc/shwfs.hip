@@ -145,6 +145,88 @@ struct DevWvl {
     const float4 *si_w;
 };
 
+// ------------------------------------------------------------ pixel noise
+// addnoise (lib/libmisc.c:31-57) as applied per frame at
+// maos/wfsgrad.c:423-442: ints = randp(ints + bkgrnd) + rne N(0,1) - bkgrndc,
+// with randp (math/random.c:186-209): a normal approximation above 200
+// photons, else the multiplication method in chunks of at most 12.  The
+// uniforms come from Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC 2011) with counter (block, pixel,
+// WFS*nsa + subaperture, step) and the 64-bit seed as key: one stream per
+// pixel and step, independent of launch order.  Normals are Box-Muller (one
+// per pair of uniforms).  Poisson arithmetic is FP64.
+struct NoiseArgs {
+    int on;
+    float bkgrnd, bkgrndc, rne;
+    unsigned key0, key1, isim;
+};
+__device__ __forceinline__ void philox4x32_10(unsigned (&c)[4], unsigned k0, unsigned k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const unsigned lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+        const unsigned lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const unsigned n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+struct NoiseStream {
+    unsigned k0, k1, pix, unit, step, blk = 0, buf[4] = {};
+    int left = 0;
+    // uniform in (0, 1]
+    __device__ double uniform() {
+        if (left == 0) {
+            unsigned c[4] = {blk++, pix, unit, step};
+            philox4x32_10(c, k0, k1);
+            buf[0] = c[0];
+            buf[1] = c[1];
+            buf[2] = c[2];
+            buf[3] = c[3];
+            left = 4;
+        }
+        return ((double)buf[4 - left--] + 1.) * 2.3283064365386962890625e-10;
+    }
+    __device__ double normal() {
+        const double u1 = uniform(), u2 = uniform();
+        return sqrt(-2. * log(u1)) * cospi(2. * u2);
+    }
+    // randp: a multiplication loop ends after at most 1000 factors (its
+    // chance of running that long for a mean <= 12 is below 1e-300)
+    __device__ double poisson(double xm) {
+        if (xm > 200.) return rint(xm + normal() * sqrt(xm));
+        double x = 0.;
+        while (xm > 0.) {
+            const double xmu = xm > 12. ? 12. : xm;
+            xm -= xmu;
+            const double g = exp(-xmu);
+            double t = 1.;
+            int xu = -1;
+            while (t > g && xu < 1000) {
+                xu++;
+                t *= uniform();
+            }
+            x += xu;
+        }
+        return x;
+    }
+};
+// noisy value of pixel pix of (WFS iwfs, subaperture isa)
+__device__ __forceinline__ float add_noise(const NoiseArgs &n, float v, int pix, int iwfs, int nsa, int isa) {
+    NoiseStream s;
+    s.k0 = n.key0;
+    s.k1 = n.key1;
+    s.pix = (unsigned)pix;
+    s.unit = (unsigned)(iwfs * nsa + isa);
+    s.step = n.isim;
+    const double tot = (double)v + (double)n.bkgrnd;
+    const double c = s.poisson(tot);
+    return (float)(c + (double)n.rne * s.normal() - (double)n.bkgrndc);
+}
+
 struct KArgs {
     int nsa, npix, pixpsax, pixpsay, nwvl, nwfs;
     const float *amp;
@@ -164,6 +246,7 @@ struct KArgs {
     float *ints;     // [nwfs][nsa][npix]
     float *grad;     // [nwfs][2*nsa]
     double *i0acc;   // [nwfs][nsa][npix] or null
+    NoiseArgs noise;
 };
 
 struct OpdArgs {
@@ -533,6 +616,12 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
     // outputs and pixel processing (shwfs_grad, maos/utils.c:983-1122)
     const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    if (a.noise.on)
+#pragma unroll
+        for (int j = 0; j < PPL; j++) {
+            const int p = line + j * N;
+            if (p < a.npix && valid) pv[j] = add_noise(a.noise, pv[j], p, iwfs, a.nsa, isa);
+        }
 #pragma unroll
     for (int j = 0; j < PPL; j++) {
         const int p = line + j * N;
@@ -818,6 +907,7 @@ struct MfmaArgs {
     float saat;
     float *ints, *grad;
     double *i0acc;
+    NoiseArgs noise;
     // uplink mode (one workgroup per WFS and wavelength): LLT OPD
     // [nwfs][32*32] and amplitude, 1/nlwvf^2, and the two outputs conj(lotfc)
     // [nwfs][nwvl][64*64] ([ux][uy]) and its half plane [nwfs][nwvl][MX_HALF]
@@ -1176,6 +1266,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         float s0 = 0.f, s1 = 0.f, s2 = 0.f;
         const int p = lane;
         if (p < a.npix) {
+            if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
             a.ints[ibase + p] = pv;
             if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
             if (a.phytype == MX_PTYPE_MF) {
@@ -1646,6 +1737,7 @@ struct LfSampleArgs {
     float saat;
     float *ints, *grad;
     double *i0acc;
+    NoiseArgs noise;
 };
 __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
     __shared__ float red[3][MX_LF_THREADS / 64];
@@ -1661,6 +1753,7 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
             const float4 w = a.wv[iwvl].si_w[p];
             pv += W.alpha[iwvl] * (w.x * im[id.x] + w.y * im[id.y] + w.z * im[id.z] + w.w * im[id.w]);
         }
+        if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
         a.ints[ibase + p] = pv;
         if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
         if (a.phytype == MX_PTYPE_MF) {
@@ -1812,6 +1905,7 @@ struct mx_wfsgrad_s {
     double *d_i0acc = nullptr;
     bool i0_enable = false;
     long i0_nframe = 0;
+    NoiseArgs noise{}; // isim set per step
     bool queued = false;
     // timing
     bool timing = false;
@@ -2646,6 +2740,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     a.ints = h->d_ints;
     a.grad = h->d_grad;
     a.i0acc = h->i0_enable ? h->d_i0acc : nullptr;
+    a.noise = h->noise;
+    a.noise.isim = (unsigned)isim;
     a.opd = h->d_opd;
     MfmaArgs &m = h->arg_mfma;
     m = MfmaArgs{};
@@ -2692,6 +2788,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         m.ints = a.ints;
         m.grad = a.grad;
         m.i0acc = a.i0acc;
+        m.noise = a.noise;
         m.llt_opd = h->d_llt_opd;
         m.llt_amp = h->d_llt_amp;
         m.llt_norm = (float)(1. / ((double)h->cfg.llt_nx * h->embfac * (double)h->cfg.llt_nx * h->embfac));
@@ -2856,6 +2953,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         g.ints = a.ints;
         g.grad = a.grad;
         g.i0acc = a.i0acc;
+        g.noise = a.noise;
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)),
                    dim3((unsigned)std::min(h->lf_threads, 256)), &g};
     } else if (mf) {
@@ -3175,6 +3273,23 @@ int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant) {
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant) {
     if (!h || !variant) return set_err(MX_EINVAL, "null argument");
     *variant = effective_variant(h);
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_noise(mx_wfsgrad_t *h, int enable, double rne, double bkgrnd, double bkgrndc,
+                         unsigned long long seed) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    if (enable && (!(rne >= 0.) || !(bkgrnd >= 0.) || !std::isfinite(bkgrndc)))
+        return set_err(MX_EINVAL, "noise: rne=%g bkgrnd=%g bkgrndc=%g", rne, bkgrnd, bkgrndc);
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->noise = NoiseArgs{};
+    h->noise.on = enable != 0;
+    h->noise.rne = (float)rne;
+    h->noise.bkgrnd = (float)bkgrnd;
+    h->noise.bkgrndc = (float)bkgrndc;
+    h->noise.key0 = (unsigned)(seed & 0xffffffffu);
+    h->noise.key1 = (unsigned)(seed >> 32);
     return MX_OK;
 }
 

@@ -133,6 +133,10 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_get_variant(self.h, C.byref(v)))
         return v.value
 
+    def set_noise(self, enable=True, rne=0.0, bkgrnd=0.0, bkgrndc=0.0, seed=1):
+        """Photon and read noise on every later step's intensities."""
+        check(_lib.hip().mx_wfsgrad_set_noise(self.h, int(enable), rne, bkgrnd, bkgrndc, seed))
+
     def set_graph(self, enable=True):
         """Launch each step's kernels as one HIP graph."""
         check(_lib.hip().mx_wfsgrad_set_graph(self.h, int(enable)))

@@ -86,8 +86,36 @@ def lib():
         L.orc_wfsgrad_iwfs.argtypes = [dp, dp, C.POINTER(OrcPowfs), C.POINTER(OrcWfs),
                                        C.POINTER(OrcAtm), C.c_long, C.c_double, C.c_double,
                                        C.c_int, dp, dp]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint, C.c_uint]
+        L.orc_randp.restype = C.c_double
+        L.orc_randp.argtypes = [C.c_double, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
+                                C.POINTER(C.c_uint), C.POINTER(C.c_int)]
+        L.orc_addnoise.argtypes = [dp, C.c_long, C.c_double, C.c_double, C.c_double, C.c_uint,
+                                   C.c_uint, C.c_uint, C.c_uint]
         _lib = L
     return _lib
+
+
+def philox(ctr, key0, key1):
+    """Philox4x32-10 of a 4-word counter; returns the 4 output words."""
+    c = (C.c_uint * 4)(*ctr)
+    lib().orc_philox4x32_10(c, key0, key1)
+    return list(c)
+
+
+def randp_samples(xm, n, key0=1, key1=2, pix=0, unit=0, step=0):
+    """n successive randp draws of mean xm from one stream."""
+    blk, used = C.c_uint(0), C.c_int(0)
+    return np.array([lib().orc_randp(xm, key0, key1, pix, unit, step, C.byref(blk), C.byref(used))
+                     for _ in range(n)])
+
+
+def addnoise(ints, bkgrnd, bkgrndc, rne, seed, unit, step):
+    """addnoise on one subaperture's pixels (a copy); seed is 64-bit."""
+    a = np.array(ints, dtype=np.float64).ravel().copy()
+    lib().orc_addnoise(_d(a), a.size, bkgrnd, bkgrndc, rne, seed & 0xffffffff, (seed >> 32) & 0xffffffff,
+                       unit, step)
+    return a
 
 
 def fft2(a, dir):
@@ -166,6 +194,15 @@ class Oracle:
         self._atm = OrcAtm(len(layers), L0.nx, L0.ny, L0.dx, L0.ox, L0.oy, arr, _d(self._ht),
                            _d(self._vx), _d(self._vy))
         self._arr = arr
+
+    def grad(self, ints):
+        """shwfs_grad on given intensities [nsa, npix] (this oracle's MF / CoG
+        settings)."""
+        p = self.p
+        a = np.ascontiguousarray(ints, dtype=np.float64).reshape(-1)
+        g = np.zeros(2 * p.nsa)
+        lib().orc_shwfs_grad(_d(g), _d(a), C.byref(self.s), 0.0)
+        return g
 
     def step(self, wfs, isim, dt, atmscale=1.0, nthread=0, want_opd=False, want_lotf=False):
         """One wfsgrad_iwfs step for WfsSpec wfs; returns (ints, grad, opd, lotf)."""

@@ -565,3 +565,90 @@ void orc_wfsgrad_iwfs(double *ints, double *grad, const orc_powfs_t *p, const or
     if (lotfc && lotfc != lotf_out) free(lotfc);
     if (opd != opd_out) free(opd);
 }
+
+/* ---------------------------------------------------------------------
+ * Pixel noise: lib/libmisc.c:31-57 and math/random.c:186-209, with the
+ * uniforms of a Philox4x32-10 stream per (step, unit, pixel).
+ * ------------------------------------------------------------------- */
+void orc_philox4x32_10(unsigned ctr[4], unsigned k0, unsigned k1) {
+    for (int r = 0; r < 10; r++) {
+        const unsigned long long p0 = (unsigned long long)0xD2511F53u * ctr[0];
+        const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * ctr[2];
+        const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+        const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+        const unsigned n0 = hi1 ^ ctr[1] ^ k0, n2 = hi0 ^ ctr[3] ^ k1;
+        ctr[0] = n0;
+        ctr[1] = lo1;
+        ctr[2] = n2;
+        ctr[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+typedef struct {
+    unsigned k0, k1, pix, unit, step, blk;
+    unsigned buf[4];
+    int left;
+} orc_stream;
+
+/* uniform in (0, 1] */
+static double st_uniform(orc_stream *s) {
+    if (s->left == 0) {
+        unsigned c[4] = {s->blk++, s->pix, s->unit, s->step};
+        orc_philox4x32_10(c, s->k0, s->k1);
+        memcpy(s->buf, c, sizeof c);
+        s->left = 4;
+    }
+    return ((double)s->buf[4 - s->left--] + 1.) * 2.3283064365386962890625e-10;
+}
+/* Box-Muller, one normal per pair of uniforms */
+static double st_normal(orc_stream *s) {
+    const double u1 = st_uniform(s), u2 = st_uniform(s);
+    return sqrt(-2. * log(u1)) * cos(2. * M_PI * u2);
+}
+/* randp; the multiplication loop stops after 1000 factors */
+static double st_poisson(orc_stream *s, double xm) {
+    if (xm > 200.) return rint(xm + st_normal(s) * sqrt(xm));
+    double x = 0.;
+    while (xm > 0.) {
+        const double xmu = xm > 12. ? 12. : xm;
+        xm -= xmu;
+        const double g = exp(-xmu);
+        double t = 1.;
+        int xu = -1;
+        while (t > g && xu < 1000) {
+            xu++;
+            t *= st_uniform(s);
+        }
+        x += xu;
+    }
+    return x;
+}
+
+double orc_randp(double xm, unsigned key0, unsigned key1, unsigned pix, unsigned unit, unsigned step,
+                 unsigned *blk, int *used) {
+    orc_stream s = {key0, key1, pix, unit, step, *blk, {0, 0, 0, 0}, 0};
+    /* resume inside the current block of four */
+    if (*used) {
+        s.blk = *blk - 1;
+        unsigned c[4] = {s.blk++, pix, unit, step};
+        orc_philox4x32_10(c, key0, key1);
+        memcpy(s.buf, c, sizeof c);
+        s.left = 4 - *used;
+    }
+    const double x = st_poisson(&s, xm);
+    *blk = s.blk;
+    *used = s.left ? 4 - s.left : 0;
+    return x;
+}
+
+void orc_addnoise(double *ints, long npix, double bkgrnd, double bkgrndc, double rne, unsigned key0,
+                  unsigned key1, unsigned unit, unsigned step) {
+    for (long p = 0; p < npix; p++) {
+        orc_stream s = {key0, key1, (unsigned)p, unit, step, 0, {0, 0, 0, 0}, 0};
+        const double tot = ints[p] + bkgrnd;
+        const double c = st_poisson(&s, tot);
+        ints[p] = c + rne * st_normal(&s) - bkgrndc;
+    }
+}

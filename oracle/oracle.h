@@ -116,6 +116,23 @@ void orc_wfsgrad_iwfs(double *ints, double *grad, const orc_powfs_t *p,
                       double dt, double atmscale, int nthread, double *opd_out,
                       double *lotf_out);
 
+/* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
+ * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ctr is replaced
+ * by the four output words. */
+void orc_philox4x32_10(unsigned ctr[4], unsigned key0, unsigned key1);
+
+/* Poisson sample of mean xm by randp (math/random.c:186-209) drawing from the
+ * stream (key, pixel pix, unit, step); blk and used carry the stream position
+ * between calls (start at 0). */
+double orc_randp(double xm, unsigned key0, unsigned key1, unsigned pix, unsigned unit,
+                 unsigned step, unsigned *blk, int *used);
+
+/* addnoise (lib/libmisc.c:31-57) on the npix pixels of subaperture unit
+ * (= WFS * nsa + subaperture) at step: ints = randp(ints + bkgrnd) +
+ * rne N(0,1) - bkgrndc, one Philox stream per pixel, normals by Box-Muller. */
+void orc_addnoise(double *ints, long npix, double bkgrnd, double bkgrndc, double rne,
+                  unsigned key0, unsigned key1, unsigned unit, unsigned step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -397,6 +397,55 @@ def test_lgs_pad128_parity(lgs_pad128):
     eng.close()
 
 
+@pytest.mark.parametrize("case,variant", [("ngs_two_wvl", "mfma_bf16"), ("ngs_two_wvl", "fft32"),
+                                          ("lgs_small", "mfma_bf16"), ("lf_mixed", "fft32")])
+def test_noise_matches_oracle(case, variant, request):
+    """Photon and read noise: a noisy engine against the oracle's addnoise
+    (the same Philox streams) applied to a noise-free engine's intensities of
+    the same step; gradients and i0 sums use the noisy intensities."""
+    from maos_amd import configs, engine
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: noise %s %s" % (case, variant), flush=True)
+    engine.init(0)
+    engine.atm2gpu(layers)
+    wfs = configs.wfs_specs(cfg, p)
+    quiet, noisy = engine.WfsGrad(p, wfs), engine.WfsGrad(p, wfs)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    if p.llt_nx:
+        mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+        o.set_ref(mtche, i0sum)
+    seed, rne, bkgrnd, bkgrndc = (1 << 40) + 12345, 3.0, 5.0, 4.0
+    for eng in (quiet, noisy):
+        eng.set_variant(VARIANTS[variant])
+        if p.llt_nx:
+            eng.update_ref(mtche, i0sum)
+        eng.i0_control(enable=True, reset=True)
+    noisy.set_noise(True, rne=rne, bkgrnd=bkgrnd, bkgrndc=bkgrndc, seed=seed)
+    isim = 9
+    quiet.queue(isim, dt)
+    noisy.queue(isim, dt)
+    import oracle
+    for iw in range(len(wfs)):
+        _, ints_q = quiet.sync(iw)
+        g_n, ints_n = noisy.sync(iw)
+        ref = np.stack([oracle.addnoise(ints_q[isa], bkgrnd, bkgrndc, rne, seed, iw * p.nsa + isa, isim)
+                        for isa in range(p.nsa)])
+        err = np.abs(ints_n - ref) / np.maximum(1.0, np.abs(ref))
+        print("wfs=%d max rel diff %.2e, mean noise %.3f" % (iw, err.max(), (ints_n - ints_q).mean()))
+        assert err.max() < 1e-5
+        assert np.abs(ints_n - ints_q).max() > 1.0  # noise present
+        g_ref = o.grad(ints_n)
+        vign = p.saa < p.cfg.saat
+        e_g = np.abs(g_n - g_ref)[np.concatenate([~vign, ~vign])].max() * RAD2MAS
+        print("   grad from the noisy ints: %.3e mas" % e_g)
+        assert e_g < 0.01
+        i0, n = noisy.i0_get(iw)
+        assert n == 1 and np.allclose(i0, ints_n, rtol=0, atol=0)
+    quiet.close()
+    noisy.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C

@@ -292,3 +292,55 @@ def test_mf_identities(lgs_small):
         # sigmatch=1 rescales by i0sum/sum(ints) which is 1 to first order
         assert np.abs(grad[:p.nsa][live] - expect).max() < 2e-3 * 1e-8 + 1e-14
         assert np.abs(grad[p.nsa:][live]).max() < 2e-3 * 1e-8 + 1e-14
+
+
+# Philox4x32-10 known-answer vectors of the Random123 distribution
+# (kat_vectors, philox4x32 10 rounds): counter, key -> output
+PHILOX_KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", PHILOX_KAT)
+def test_philox_known_answers(ctr, key, out):
+    import oracle
+    assert oracle.philox(ctr, *key) == list(out)
+
+
+@pytest.mark.parametrize("xm", [0.3, 4.0, 11.5, 30.0, 150.0, 199.0, 450.0])
+def test_randp_distribution(xm):
+    """randp's samples have the Poisson mean and variance (4 sigma bounds on
+    20000 draws) and are integers; above 200 the normal approximation."""
+    import oracle
+    n = 20000
+    x = oracle.randp_samples(xm, n)
+    assert np.all(x == np.round(x)) and np.all(x >= 0)
+    m, v = x.mean(), x.var()
+    assert abs(m - xm) < 4 * np.sqrt(xm / n)
+    # variance of the sample variance ~ (2 xm^2 + xm) / n for a Poisson law
+    assert abs(v - xm) < 4 * np.sqrt((2 * xm * xm + xm) / n)
+    if xm < 12:
+        # P(0) = exp(-xm)
+        p0 = (x == 0).mean()
+        assert abs(p0 - np.exp(-xm)) < 4 * np.sqrt(np.exp(-xm) * (1 - np.exp(-xm)) / n)
+
+
+def test_addnoise_structure():
+    """rne = bkgrnd = 0 on zero intensities gives zeros; the same stream
+    repeats exactly; other steps, units or seeds give other samples; read
+    noise alone is N(0, rne^2) minus bkgrndc."""
+    import oracle
+    z = np.zeros(64)
+    assert np.all(oracle.addnoise(z, 0.0, 0.0, 0.0, 7, 3, 5) == 0)
+    ints = np.linspace(0, 300, 64)
+    a = oracle.addnoise(ints, 2.0, 1.0, 3.0, 7, 3, 5)
+    assert np.array_equal(a, oracle.addnoise(ints, 2.0, 1.0, 3.0, 7, 3, 5))
+    for args in ((7, 3, 6), (7, 4, 5), (8, 3, 5)):
+        assert not np.array_equal(a, oracle.addnoise(ints, 2.0, 1.0, 3.0, *args))
+    r = np.concatenate([oracle.addnoise(np.zeros(64), 0.0, 0.5, 2.0, 11, u, 0) for u in range(400)])
+    n = r.size
+    assert abs(r.mean() + 0.5) < 4 * 2.0 / np.sqrt(n)
+    assert abs(r.std() - 2.0) < 4 * 2.0 / np.sqrt(2 * n)
