@@ -12,6 +12,7 @@
  */
 #ifndef MAOS_SETUP_H
 #define MAOS_SETUP_H
+#include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -111,6 +112,42 @@ int mxs_mtch(double *mtche, double *sanea, double *i0sum, double *i0sumsum,
              long pixpsax, long pixpsay, double bkgrnd, double bkgrndc,
              double rne, double pixthetax, double pixthetay, int mtchcr,
              double sigratio);
+
+/* MAOS .bin files (sys/bin.c:693-800, sys/bin.h:39-84): header = optional
+ * comment block, M_SKIP, magic, nx, ny (64-bit), then nx*ny elements, x
+ * fastest; ".gz" names are gzip-compressed (zfopen, sys/bin.c:301).  Every
+ * call returns 0 on success, 1 at the end of a file (read_header), negative
+ * on error. */
+#define MXS_M_DBL 0x6402u     /* double */
+#define MXS_M_INT64 0x6403u   /* int64 */
+#define MXS_M_CMP 0x6404u     /* double complex */
+#define MXS_M_FLT 0x6408u     /* float */
+#define MXS_MCC_ANY 0x6421u   /* cell of any type */
+#define MXS_M_COMMENT 0x6500u /* keyword text */
+#define MXS_M_SKIP 0x6600u    /* padding word */
+#define MXS_M_EOD 0x64FFu     /* end of data */
+typedef struct mxs_bin_s mxs_bin_t;
+int mxs_bin_open(mxs_bin_t **out, const char *fn, int write);
+int mxs_bin_close(mxs_bin_t *f);
+int mxs_bin_write_header(mxs_bin_t *f, uint32_t magic, uint64_t nx, uint64_t ny,
+                         const char *keywords);
+int mxs_bin_write_data(mxs_bin_t *f, const void *data, uint64_t bytes);
+/* keywords (may be NULL) receives the concatenated comment text, up to
+ * klen - 1 characters */
+int mxs_bin_read_header(mxs_bin_t *f, uint32_t *magic, uint64_t *nx, uint64_t *ny,
+                        char *keywords, uint64_t klen);
+int mxs_bin_read_data(mxs_bin_t *f, void *data, uint64_t bytes);
+
+/* zfarr (math/zfarr.c:33-110): a cell file of nstep x 1 elements written one
+ * per step, as MAOS saves telemetry (wfs%d_gradcl_%d.bin and the like,
+ * maos/sim_utils.c:955-998).  Steps not pushed are written as empty
+ * elements; push takes steps in increasing order. */
+typedef struct mxs_zfarr_s mxs_zfarr_t;
+int mxs_zfarr_open(mxs_zfarr_t **out, const char *fn, long nstep,
+                   const char *keywords);
+int mxs_zfarr_push(mxs_zfarr_t *z, long istep, uint32_t magic, uint64_t nx,
+                   uint64_t ny, const void *data, uint64_t elsize);
+int mxs_zfarr_close(mxs_zfarr_t *z);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,17 @@ VARIANT_AUTO, VARIANT_FFT32, VARIANT_MFMA_BF16, VARIANT_MFMA_FP16 = -1, 0, 1, 2
 
 # name -> (restype, argtypes) of every symbol in include/maos_setup.h
 SETUP_API = {
+    "mxs_bin_open": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p, C.c_int]),
+    "mxs_bin_close": (C.c_int, [C.c_void_p]),
+    "mxs_bin_write_header": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint64, C.c_char_p]),
+    "mxs_bin_write_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mxs_bin_read_header": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                      C.POINTER(C.c_uint64), C.c_char_p, C.c_uint64]),
+    "mxs_bin_read_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "mxs_zfarr_open": (C.c_int, [C.POINTER(C.c_void_p), C.c_char_p, C.c_long, C.c_char_p]),
+    "mxs_zfarr_push": (C.c_int, [C.c_void_p, C.c_long, C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p,
+                                 C.c_uint64]),
+    "mxs_zfarr_close": (C.c_int, [C.c_void_p]),
     "mxs_fft2": (None, [c_double_p, C.c_long, C.c_long, C.c_int]),
     "mxs_nextfftsize": (C.c_long, [C.c_long]),
     "mxs_saloc": (C.c_long, [C.c_int, C.c_double, C.c_double, C.c_double, c_double_p,

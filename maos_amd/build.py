@@ -41,10 +41,10 @@ def build_hip():
 
 def build_setup():
     os.makedirs(LIBDIR, exist_ok=True)
-    src = os.path.join(PKG, "host", "setup.c")
+    srcs = [os.path.join(PKG, "host", f) for f in ("setup.c", "binio.c")]
     tmp = SETUP_LIB + ".tmp"
     _run(["gcc", "-O2", "-std=gnu11", "-fopenmp", "-fPIC", "-shared", "-Wall",
-          "-o", tmp, src, "-lm"])
+          "-o", tmp] + srcs + ["-lz", "-lm"])
     os.replace(tmp, SETUP_LIB)
 
 

@@ -39,14 +39,26 @@ class LltCfg:
 
 @dataclass
 class SodiumCfg:
-    """Gaussian sodium profile sampled on [hmin, hmax] with spacing dh."""
+    """Gaussian sodium profile sampled on [hmin, hmax] with spacing dh, or a
+    MAOS profile file (llt.fnprof, e.g. NapMean.bin): heights and column
+    density as a 2 x nh .bin array."""
     center: float = 90e3
     fwhm: float = 10e3
     hmin: float = 70e3
     hmax: float = 110e3
     dh: float = 20.0
+    fnprof: Optional[str] = None
 
     def table(self):
+        if self.fnprof:
+            from .telemetry import read_bin
+            a = np.asarray(read_bin(self.fnprof), dtype=np.float64)
+            if a.ndim != 2 or a.shape[0] != 2:
+                raise ValueError("%s: expected a 2 x nh profile" % self.fnprof)
+            return np.ascontiguousarray(a.reshape(-1))  # nh x 2, column-major
+        return self._gaussian()
+
+    def _gaussian(self):
         h = np.arange(self.hmin, self.hmax + 0.5 * self.dh, self.dh)
         sig = self.fwhm / (2.0 * math.sqrt(2.0 * math.log(2.0)))
         na = np.exp(-0.5 * ((h - self.center) / sig) ** 2)

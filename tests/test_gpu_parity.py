@@ -446,6 +446,29 @@ def test_noise_matches_oracle(case, variant, request):
     noisy.close()
 
 
+def test_recorder_writes_maos_telemetry(ngs_small, tmp_path):
+    """Recorder writes each step's gradients and intensities as MAOS zfarr
+    files that read back equal to what sync returned."""
+    from maos_amd import telemetry
+    cfg, p, layers = ngs_small
+    print("case: telemetry recorder ngs8m", flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    rec = telemetry.Recorder(str(tmp_path), eng, 4, seed=1, ints=True)
+    got = []
+    for isim in range(3):
+        eng.queue(isim, cfg["sim"].dt)
+        rec.record(isim)
+        got.append(eng.sync(0))
+    rec.close()
+    grads = telemetry.read_bin(str(tmp_path / "wfs0_gradcl_1.bin"))
+    ints = telemetry.read_bin(str(tmp_path / "wfs0_intsnf_1.bin"))
+    assert len(grads) == 4 and grads[3] is None
+    for isim in range(3):
+        assert np.array_equal(grads[isim].ravel(), got[isim][0])
+        assert np.array_equal(ints[isim].reshape(p.nsa, p.npix), got[isim][1])
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C
