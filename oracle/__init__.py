@@ -86,6 +86,7 @@ def lib():
         L.orc_wfsgrad_iwfs.argtypes = [dp, dp, C.POINTER(OrcPowfs), C.POINTER(OrcWfs),
                                        C.POINTER(OrcAtm), C.c_long, C.c_double, C.c_double,
                                        C.c_int, dp, dp]
+        L.orc_ccpcorner.argtypes = [dp, C.c_long, C.c_long, dp, C.c_long, C.c_long]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint, C.c_uint]
         L.orc_randp.restype = C.c_double
         L.orc_randp.argtypes = [C.c_double, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
@@ -94,6 +95,15 @@ def lib():
                                    C.c_uint, C.c_uint, C.c_uint]
         _lib = L
     return _lib
+
+
+def ccpcorner(a, nx, ny):
+    """ccpcorner of a complex [niny, ninx] array into [ny, nx]."""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    out = np.zeros((ny, nx), dtype=np.complex128)
+    lib().orc_ccpcorner(out.view(np.float64).ctypes.data_as(dp), nx, ny,
+                        a.view(np.float64).ctypes.data_as(dp), a.shape[1], a.shape[0])
+    return out
 
 
 def philox(ctr, key0, key1):

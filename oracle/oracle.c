@@ -323,6 +323,10 @@ void orc_embed_wvf(double *wvf_, long nwvf, const double *opd, const double *amp
 }
 
 /* ccpcorner, math/matcomp.c:176-216 (C_FULL) */
+static void ccpcorner(cplx *out, long nx, long ny, const cplx *in, long ninx, long niny);
+void orc_ccpcorner(double *out, long nx, long ny, const double *in, long ninx, long niny) {
+    ccpcorner((cplx *)out, nx, ny, (const cplx *)in, ninx, niny);
+}
 static void ccpcorner(cplx *out, long nx, long ny, const cplx *in, long ninx, long niny) {
     memset(out, 0, sizeof(cplx) * nx * ny);
     const long ny2 = (ny < niny) ? ny / 2 : niny / 2;

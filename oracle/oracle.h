@@ -116,6 +116,11 @@ void orc_wfsgrad_iwfs(double *ints, double *grad, const orc_powfs_t *p,
                       double dt, double atmscale, int nthread, double *opd_out,
                       double *lotf_out);
 
+/* ccpcorner (math/matcomp.c:176-216): the four corner quadrants of the
+ * ninx x niny complex array in (FFT order) copied into the corners of the
+ * nx x ny array out, zero elsewhere (padding or cropping). */
+void orc_ccpcorner(double *out, long nx, long ny, const double *in, long ninx, long niny);
+
 /* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
  * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ctr is replaced
  * by the four output words. */

@@ -344,3 +344,22 @@ def test_addnoise_structure():
     n = r.size
     assert abs(r.mean() + 0.5) < 4 * 2.0 / np.sqrt(n)
     assert abs(r.std() - 2.0) < 4 * 2.0 / np.sqrt(2 * n)
+
+
+@pytest.mark.parametrize("n,m", [(48, 96), (30, 60), (64, 64)])
+def test_ccpcorner_padding_keeps_the_transform(n, m):
+    """Padding a PSF (FFT order, negative lags in the upper half) to m = 2n
+    by the corner copy interpolates its transform: the even samples of the
+    m-point 2-D FFT equal the n-point FFT exactly, and the padded array keeps
+    every value (sum and energy); a shift of the corner map breaks both."""
+    import oracle
+    rs = np.random.RandomState(n)
+    psf = rs.random_sample((n, n)) + 0j
+    pad = oracle.ccpcorner(psf, m, m)
+    assert np.isclose(pad.sum(), psf.sum()) and np.isclose((abs(pad) ** 2).sum(), (abs(psf) ** 2).sum())
+    big = oracle.fft2(pad.copy(), -1)
+    small = oracle.fft2(psf.copy(), -1)
+    r = m // n
+    assert np.allclose(big[::r, ::r], small, rtol=0, atol=1e-9 * abs(small).max())
+    # cropping back recovers the original
+    assert np.array_equal(oracle.ccpcorner(pad, n, n), psf)
