@@ -185,6 +185,23 @@ int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
 int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant);
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant);
 
+/* Closed-loop inputs of the WFS OPD, added after the ray trace of every
+ * later step (not to the LLT): a static surface OPD per WFS
+ * (powfs.opdadd, maos/wfsgrad.c:228-230; nsa*nxsa*nxsa values in the OPD
+ * layout, iwfs=-1 for every WFS, opd=NULL for zero) and a piston / tip /
+ * tilt per WFS (loc_add_ptt, maos/wfsgrad.c:243-256: opd += ptt[0] +
+ * ptt[1]*x + ptt[2]*y with the TTM/FSM commands; the values are copied at
+ * the call and used from the next queued step on). */
+int mx_wfsgrad_set_opdadd(mx_wfsgrad_t *h, int iwfs, const double *opd);
+int mx_wfsgrad_set_ptt(mx_wfsgrad_t *h, int iwfs, const double *ptt);
+
+/* Gradient type: mode 0 (the default) physical optics; mode 1 geometric
+ * Zernike-tilt gradients from the OPD (pts_ztilt, math/loc.c:839-880, as
+ * maos/wfsgrad.c:303-306 computes them for gtype_sim = GTYPE_Z): the step
+ * runs the ray trace and one reduction per subaperture, leaves ints
+ * untouched, and needs no DTF, ETF or matched filter. */
+int mx_wfsgrad_set_geom(mx_wfsgrad_t *h, int mode);
+
 /* Photon and read noise on the pixel intensities of every later step
  * (addnoise, lib/libmisc.c:31-57, as applied at maos/wfsgrad.c:423-442;
  * seeding replaces gpu_wfsgrad_seeding, cuda/sim/wfsinit.cu:532):

@@ -94,6 +94,9 @@ HIP_API = {
     "mx_wfsgrad_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_wfsgrad_get_variant": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mx_wfsgrad_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mx_wfsgrad_set_opdadd": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "mx_wfsgrad_set_ptt": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
+    "mx_wfsgrad_set_geom": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_wfsgrad_set_noise": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_ulonglong]),
     "mx_wfsgrad_autotune": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_int, C.c_char_p, C.c_long]),
     "mx_wfsgroup_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]),

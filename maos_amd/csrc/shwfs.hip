@@ -727,6 +727,81 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
                                     cidx, cfrac, tile, side);
 }
 
+// ------------------------------------------------------------ OPD inputs
+// Static surface OPD (powfs.opdadd, maos/wfsgrad.c:228-230) and per-step
+// piston/tip/tilt (loc_add_ptt, maos/wfsgrad.c:243-256: TTM/FSM commands)
+// added to the traced OPD of every subaperture point:
+// opd += opdadd + ptt0 + ptt1 x + ptt2 y, x = origx + ix dx.
+struct OpdAddArgs {
+    int nsa, nxsa, nwfs;
+    double dx;
+    const double *ptsx, *ptsy;
+    const float *opdadd; // [nwfs][nsa][nxsa^2] or null
+    const double *ptt;   // [nwfs][3] or null
+    float *opd;
+};
+__global__ __launch_bounds__(256) void k_opd_add(OpdAddArgs a) {
+    const long np = (long)a.nxsa * a.nxsa, total = np * a.nsa * a.nwfs;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+        const int pt = (int)(i % np);
+        const long sw = i / np;
+        const int isa = (int)(sw % a.nsa), iwfs = (int)(sw / a.nsa);
+        float v = a.opd[i];
+        if (a.opdadd) v += a.opdadd[i];
+        if (a.ptt) {
+            const double *t = a.ptt + 3 * iwfs;
+            const double x = a.ptsx[isa] + a.dx * (double)(pt % a.nxsa), y = a.ptsy[isa] + a.dx * (double)(pt / a.nxsa);
+            v += (float)(t[0] + t[1] * x + t[2] * y);
+        }
+        a.opd[i] = v;
+    }
+}
+
+// ------------------------------------------------------------ geometric
+// Zernike-tilt gradients of each (WFS, subaperture) from its OPD (pts_ztilt,
+// math/loc.c:839-880, used at maos/wfsgrad.c:303-306): moments
+// a = sum amp opd [1, x, y] in FP64, times the inverse of
+// M = sum amp [1, x, y]^T [1, x, y] (pts_mcc_ptt, math/loc.c:612-650,
+// inverted on the host); the gradients are the x and y terms.
+struct ZtiltArgs {
+    int nsa, nxsa, nwfs;
+    double dx;
+    const double *ptsx, *ptsy;
+    const float *amp;   // [nsa][nxsa^2]
+    const double *imcc; // [nsa][9] inverse moment matrices
+    const float *opd;
+    const float *saa;
+    float saat;
+    float *grad;        // [nwfs][2*nsa]
+};
+__global__ __launch_bounds__(64) void k_ztilt(ZtiltArgs a) {
+    const int unit = xcd_unit((int)blockIdx.x, a.nsa * a.nwfs);
+    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    const long np = (long)a.nxsa * a.nxsa;
+    const float *op = a.opd + ((long)iwfs * a.nsa + isa) * np, *am = a.amp + (long)isa * np;
+    double a0 = 0., a1 = 0., a2 = 0.;
+    for (int pt = threadIdx.x; pt < np; pt += 64) {
+        const float w = am[pt];
+        if (w != 0.f) {
+            const double t = (double)w * (double)op[pt];
+            a0 += t;
+            a1 += t * (a.ptsx[isa] + a.dx * (double)(pt % a.nxsa));
+            a2 += t * (a.ptsy[isa] + a.dx * (double)(pt / a.nxsa));
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_xor(a0, o, 64);
+        a1 += __shfl_xor(a1, o, 64);
+        a2 += __shfl_xor(a2, o, 64);
+    }
+    if (threadIdx.x == 0 && !(a.saa[isa] < a.saat)) {
+        const double *m = a.imcc + 9 * (long)isa; // column-major 3x3
+        a.grad[(long)iwfs * 2 * a.nsa + isa] = (float)(m[1] * a0 + m[4] * a1 + m[7] * a2);
+        a.grad[(long)iwfs * 2 * a.nsa + a.nsa + isa] = (float)(m[2] * a0 + m[5] * a1 + m[8] * a2);
+    }
+}
+
 // ------------------------------------------------------------ uplink OTF
 // lotfc of wfsints (maos/wfsints.c:149-199) for nlwvf == nwvf == notf: from
 // the LLT OPD (traced by k_wfs_opd), embed, FFT, |.|^2/nlwvf^2, FFT.
@@ -1828,6 +1903,7 @@ struct StepPlan {
     StepLaunch L[MX_MAXLAUNCH];
     int nl = 0;
     int key = 0;       // the kernel set: a graph built for one key fits any step of it
+    int slot_opd_last = 0; // last kernel of the OPD stage (ray trace, OPD inputs)
     int slot_llt = -1; // index of the first uplink-OTF kernel, or -1 (they run up to slot_img)
     int slot_img = 0;  // index of the first imaging kernel
     bool llt = false;
@@ -1906,6 +1982,16 @@ struct mx_wfsgrad_s {
     bool i0_enable = false;
     long i0_nframe = 0;
     NoiseArgs noise{}; // isim set per step
+    // closed-loop OPD inputs and geometric gradients
+    int geom = 0;                     // 1: Zernike-tilt gradients from the OPD, no imaging
+    double *d_imcc = nullptr;         // [nsa][9] inverse tip/tilt moment matrices
+    float *d_opdadd = nullptr;        // [nwfs][nsa][nxsa^2] static surface OPD
+    bool opdadd_set = false;
+    double *d_ptt = nullptr;          // [nwfs][3] piston, tip, tilt per step
+    std::vector<double> h_ptt;
+    bool ptt_set = false, ptt_dirty = false;
+    OpdAddArgs arg_opdadd{};
+    ZtiltArgs arg_ztilt{};
     bool queued = false;
     // timing
     bool timing = false;
@@ -2317,6 +2403,35 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
             hipMemcpy(h->d_F16, F16.data(), F16.size() * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
             return fail(set_err(MX_EHIP, "upload of the DFT table failed"));
     }
+    {
+        // inverse tip/tilt moment matrix of each subaperture (pts_mcc_ptt,
+        // math/loc.c:612-650, inverted as dcellinvspd_each does)
+        std::vector<double> imcc(9 * nsa, 0.);
+        for (long isa = 0; isa < nsa; isa++) {
+            double m00 = 0, m01 = 0, m02 = 0, m11 = 0, m12 = 0, m22 = 0;
+            for (long iy = 0; iy < h->nxsa; iy++)
+                for (long ix = 0; ix < h->nxsa; ix++) {
+                    const double x = p->ptsx[isa] + ix * p->dx, y = p->ptsy[isa] + iy * p->dx;
+                    const double w = p->amp[isa * npsa + iy * h->nxsa + ix];
+                    m00 += w;
+                    m01 += w * x;
+                    m02 += w * y;
+                    m11 += w * x * x;
+                    m12 += w * x * y;
+                    m22 += w * y * y;
+                }
+            const double c00 = m11 * m22 - m12 * m12, c01 = m02 * m12 - m01 * m22, c02 = m01 * m12 - m02 * m11;
+            const double c11 = m00 * m22 - m02 * m02, c12 = m01 * m02 - m00 * m12, c22 = m00 * m11 - m01 * m01;
+            const double det = m00 * c00 + m01 * c01 + m02 * c02;
+            if (det > 0) {
+                const double r[9] = {c00, c01, c02, c01, c11, c12, c02, c12, c22};
+                for (int k = 0; k < 9; k++) imcc[9 * isa + k] = r[k] / det;
+            }
+        }
+        if (dev_alloc(&h->d_imcc, imcc.size()) ||
+            hipMemcpy(h->d_imcc, imcc.data(), sizeof(double) * imcc.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(set_err(MX_EHIP, "upload of the tip/tilt moments failed"));
+    }
     if (hipMemset(h->d_grad, 0, sizeof(float) * nwfs * 2 * nsa) != hipSuccess ||
         hipMemset(h->d_i0acc, 0, sizeof(double) * nwfs * nsa * npix) != hipSuccess ||
         hipMemset(h->d_mtche, 0, sizeof(float) * nwfs * nsa * 2 * npix) != hipSuccess ||
@@ -2341,7 +2456,7 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     drop_graph(h);
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
-                    h->d_lotf_t, h->d_F, h->d_F16, h->d_mwv, h->d_lf_tw, h->d_b1, h->d_b2, h->d_b3, h->d_psf, h->d_img,
+                    h->d_lotf_t, h->d_F, h->d_F16, h->d_mwv, h->d_imcc, h->d_opdadd, h->d_ptt, h->d_lf_tw, h->d_b1, h->d_b2, h->d_b3, h->d_psf, h->d_img,
                     h->d_lfw,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
     for (void *q : ptrs)
@@ -2649,9 +2764,9 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         if (g_atm[r].nps != g_atm[0].nps)
             return set_err(MX_EINVAL, "realisation %d has %d layers, realisation 0 %d", r, g_atm[r].nps, g_atm[0].nps);
     }
-    for (int w = 0; w < h->nwvl; w++)
+    for (int w = 0; w < h->nwvl && !h->geom; w++)
         if (!h->dtf_set[w]) return set_err(MX_ESTATE, "dtf of wavelength %d not set", w);
-    if (h->cfg.llt_nx > 0)
+    if (h->cfg.llt_nx > 0 && !h->geom)
         for (int w = 0; w < h->nwvl; w++)
             if (!h->d_etf[w]) return set_err(MX_ESTATE, "etf of wavelength %d not set", w);
     // the staged screen tile holds a grid whose step is <= 1 screen pixel;
@@ -2661,11 +2776,12 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         for (int l = 0; l < g_atm[r].nps; l++)
             if (h->cfg.dx * g_atm[r].h_layers[l].dx1 > 1. + 1e-9 || h->cfg.dx * g_atm[r].h_layers[l].dy1 > 1. + 1e-9)
                 direct = true;
-    if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche) return set_err(MX_ESTATE, "matched filter not set");
-    if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum)
+    if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche && !h->geom)
+        return set_err(MX_ESTATE, "matched filter not set");
+    if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum && !h->geom)
         return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=1)");
     const int variant = effective_variant(h);
-    if (variant < 0)
+    if (variant < 0 && !h->geom)
         return set_err(MX_EINVAL, "the MFMA form needs notf=64, nxsa=32, <= %d sampled image rows and "
                        "columns and no second ETF", MX_MFMA_NS);
     HIPCHK(hipSetDevice(h->device));
@@ -2687,7 +2803,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         }
     }
     // Argument blocks live in the handle (a graph's kernel nodes copy them).
-    const bool llt = h->cfg.llt_nx > 0;
+    const bool llt = h->cfg.llt_nx > 0 && !h->geom; // geometric gradients need no uplink
     OpdArgs &o = h->arg_opd;
     o = OpdArgs{};
     o.nsa = (int)h->nsa;
@@ -2824,6 +2940,54 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     const void *kopd = kopd_t[opd_nx == 32][direct][o.tiles > 1];
     L[nl++] = {kopd,
                dim3((unsigned)(h->nsa * h->nwfs * o.tiles * o.tiles + o.nllt)), dim3(MX_OPD_THREADS), &o};
+    // static surface and tip/tilt inputs of the WFS OPD
+    const bool opdin = h->opdadd_set || h->ptt_set;
+    if (opdin) {
+        if (h->ptt_dirty) {
+            HIPCHK(hipMemcpyAsync(h->d_ptt, h->h_ptt.data(), sizeof(double) * h->h_ptt.size(), hipMemcpyHostToDevice,
+                                  h->stream));
+            h->ptt_dirty = false;
+        }
+        OpdAddArgs &q = h->arg_opdadd;
+        q = OpdAddArgs{};
+        q.nsa = (int)h->nsa;
+        q.nxsa = (int)h->nxsa;
+        q.nwfs = h->nwfs;
+        q.dx = h->cfg.dx;
+        q.ptsx = h->d_ptsx;
+        q.ptsy = h->d_ptsy;
+        q.opdadd = h->opdadd_set ? h->d_opdadd : nullptr;
+        q.ptt = h->ptt_set ? h->d_ptt : nullptr;
+        q.opd = h->d_opd;
+        const long tot = h->nsa * h->nwfs * h->nxsa * h->nxsa;
+        L[nl++] = {reinterpret_cast<const void *>(&k_opd_add), dim3((unsigned)std::min<long>((tot + 255) / 256, 65536)),
+                   dim3(256), &q};
+    }
+    sp.slot_opd_last = nl - 1;
+    if (h->geom) {
+        ZtiltArgs &z = h->arg_ztilt;
+        z = ZtiltArgs{};
+        z.nsa = (int)h->nsa;
+        z.nxsa = (int)h->nxsa;
+        z.nwfs = h->nwfs;
+        z.dx = h->cfg.dx;
+        z.ptsx = h->d_ptsx;
+        z.ptsy = h->d_ptsy;
+        z.amp = h->d_amp;
+        z.imcc = h->d_imcc;
+        z.opd = h->d_opd;
+        z.saa = h->d_saa;
+        z.saat = (float)h->cfg.saat;
+        z.grad = h->d_grad;
+        sp.slot_img = nl;
+        L[nl++] = {reinterpret_cast<const void *>(&k_ztilt), dim3((unsigned)(h->nsa * h->nwfs)), dim3(64), &z};
+        sp.nl = nl;
+        sp.key = (1 << 30) | (1 << 29) | (opdin ? 1 << 28 : 0) | (direct ? 8 : 0) | (o.tiles > 1 ? 1 << 7 : 0) |
+                 (opd_nx << 8);
+        sp.slot_llt = -1;
+        sp.llt = false;
+        return MX_OK;
+    }
     if (h->lf) {
         LfWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
@@ -2968,8 +3132,9 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
                    dim3((unsigned)(ngroup * h->nwfs)), dim3(64), &a};
     }
     sp.nl = nl;
-    sp.key = (1 << 30) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) | (direct ? 8 : 0) | (variant << 4) |
-             (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) | (h->lf ? h->lf_threads << 14 : 0);
+    sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
+             (direct ? 8 : 0) | (variant << 4) | (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) |
+             (h->lf ? h->lf_threads << 14 : 0);
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;
@@ -3030,7 +3195,10 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
         }
         for (int i = 0; i < nl; i++) {
             hipEvent_t eb = i == 0 ? ev[2] : i == slot_llt ? ev[0] : i == slot_img ? ev[4] : nullptr;
-            hipEvent_t ee = i == 0 ? ev[3] : (slot_llt >= 0 && i == slot_img - 1) ? ev[1] : i == nl - 1 ? ev[5] : nullptr;
+            hipEvent_t ee = i == sp.slot_opd_last                     ? ev[3]
+                            : (slot_llt >= 0 && i == slot_img - 1) ? ev[1]
+                            : i == nl - 1                          ? ev[5]
+                                                                   : nullptr;
             if (h->timing && eb) HIPCHK(hipEventRecord(eb, h->stream));
             void *argv[1] = {L[i].arg};
             HIPCHK(hipLaunchKernel(L[i].f, L[i].grid, L[i].block, argv, 0, h->stream));
@@ -3273,6 +3441,48 @@ int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant) {
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant) {
     if (!h || !variant) return set_err(MX_EINVAL, "null argument");
     *variant = effective_variant(h);
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_geom(mx_wfsgrad_t *h, int mode) {
+    if (!h || mode < 0 || mode > 1) return set_err(MX_EINVAL, "mx_wfsgrad_set_geom: bad handle or mode %d", mode);
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->geom = mode;
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_opdadd(mx_wfsgrad_t *h, int iwfs, const double *opd) {
+    if (!h || iwfs < -1 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_set_opdadd: bad handle or iwfs");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long n = h->nsa * h->nxsa * h->nxsa;
+    int rc;
+    if ((rc = dev_alloc(&h->d_opdadd, (size_t)h->nwfs * n))) return rc;
+    if (!h->opdadd_set) HIPCHK(hipMemset(h->d_opdadd, 0, sizeof(float) * h->nwfs * n));
+    std::vector<float> t(n, 0.f);
+    if (opd)
+        for (long i = 0; i < n; i++) t[i] = (float)opd[i];
+    const int w0 = iwfs < 0 ? 0 : iwfs, w1 = iwfs < 0 ? h->nwfs : iwfs + 1;
+    for (int w = w0; w < w1; w++)
+        HIPCHK(hipMemcpy(h->d_opdadd + (long)w * n, t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    h->opdadd_set = true;
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_ptt(mx_wfsgrad_t *h, int iwfs, const double *ptt) {
+    if (!h || iwfs < -1 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_set_ptt: bad handle or iwfs");
+    int rc;
+    if (!h->d_ptt) {
+        HIPCHK(hipSetDevice(h->device));
+        if ((rc = dev_alloc(&h->d_ptt, (size_t)3 * h->nwfs))) return rc;
+        h->h_ptt.assign(3 * (size_t)h->nwfs, 0.);
+    }
+    const int w0 = iwfs < 0 ? 0 : iwfs, w1 = iwfs < 0 ? h->nwfs : iwfs + 1;
+    for (int w = w0; w < w1; w++)
+        for (int k = 0; k < 3; k++) h->h_ptt[3 * w + k] = ptt ? ptt[k] : 0.;
+    h->ptt_set = true;
+    h->ptt_dirty = true;
     return MX_OK;
 }
 

@@ -133,6 +133,20 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_get_variant(self.h, C.byref(v)))
         return v.value
 
+    def set_opdadd(self, opd, iwfs=-1):
+        """Static surface OPD [nsa*nxsa^2] added to the WFS OPD (None: zero)."""
+        a = None if opd is None else np.ascontiguousarray(opd, dtype=np.float64).reshape(-1)
+        check(_lib.hip().mx_wfsgrad_set_opdadd(self.h, iwfs, dptr(a)))
+
+    def set_ptt(self, ptt, iwfs=-1):
+        """Piston, tip, tilt added to the WFS OPD from the next step on."""
+        a = None if ptt is None else np.ascontiguousarray(ptt, dtype=np.float64)
+        check(_lib.hip().mx_wfsgrad_set_ptt(self.h, iwfs, dptr(a)))
+
+    def set_geom(self, mode=1):
+        """1: geometric Zernike-tilt gradients from the OPD; 0: physical optics."""
+        check(_lib.hip().mx_wfsgrad_set_geom(self.h, mode))
+
     def set_noise(self, enable=True, rne=0.0, bkgrnd=0.0, bkgrndc=0.0, seed=1):
         """Photon and read noise on every later step's intensities."""
         check(_lib.hip().mx_wfsgrad_set_noise(self.h, int(enable), rne, bkgrnd, bkgrndc, seed))

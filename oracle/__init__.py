@@ -86,6 +86,7 @@ def lib():
         L.orc_wfsgrad_iwfs.argtypes = [dp, dp, C.POINTER(OrcPowfs), C.POINTER(OrcWfs),
                                        C.POINTER(OrcAtm), C.c_long, C.c_double, C.c_double,
                                        C.c_int, dp, dp]
+        L.orc_ztilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double]
         L.orc_ccpcorner.argtypes = [dp, C.c_long, C.c_long, dp, C.c_long, C.c_long]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint, C.c_uint]
         L.orc_randp.restype = C.c_double
@@ -212,6 +213,33 @@ class Oracle:
         a = np.ascontiguousarray(ints, dtype=np.float64).reshape(-1)
         g = np.zeros(2 * p.nsa)
         lib().orc_shwfs_grad(_d(g), _d(a), C.byref(self.s), 0.0)
+        return g
+
+    def _wfs(self, wfs):
+        w = OrcWfs(wfs.thetax, wfs.thetay, 0.0 if np.isinf(wfs.hs) else wfs.hs, wfs.sigsim,
+                   wfs.misregx, wfs.misregy, _d(wfs.wvlwts))
+        if np.isinf(wfs.hs):
+            w.hs = float("inf")
+        self._keep_w = wfs.wvlwts
+        return w
+
+    def image(self, wfs, opd, lotf=None):
+        """wfsints and shwfs_grad from a given OPD [nsa*nxsa^2] (no LLT, or
+        the uplink OTF lotf): returns (ints [nsa, npix], grad)."""
+        p = self.p
+        w = self._wfs(wfs)
+        ints = np.zeros(p.nsa * p.npix)
+        o = np.ascontiguousarray(opd, dtype=np.float64).reshape(-1)
+        lib().orc_wfsints(_d(ints), _d(o), _d(lotf), C.byref(self.s), C.byref(w), 0, p.nsa)
+        return ints.reshape(p.nsa, p.npix), self.grad(ints)
+
+    def ztilt(self, opd):
+        """Geometric Zernike-tilt gradients of an OPD [nsa*nxsa^2]."""
+        p = self.p
+        g = np.zeros(2 * p.nsa)
+        o = np.ascontiguousarray(opd, dtype=np.float64).reshape(-1)
+        lib().orc_ztilt(_d(g), _d(o), _d(np.ascontiguousarray(p.amp)), _d(np.ascontiguousarray(p.ptsx)),
+                        _d(np.ascontiguousarray(p.ptsy)), p.nsa, p.nxsa, p.dx)
         return g
 
     def step(self, wfs, isim, dt, atmscale=1.0, nthread=0, want_opd=False, want_lotf=False):

@@ -656,3 +656,52 @@ void orc_addnoise(double *ints, long npix, double bkgrnd, double bkgrndc, double
         ints[p] = c + rne * st_normal(&s) - bkgrndc;
     }
 }
+
+/* ---------------------------------------------------------------------
+ * Geometric Zernike-tilt gradients: pts_mcc_ptt (math/loc.c:612-650), its
+ * inverse (dcellinvspd_each), pts_ztilt (math/loc.c:839-880).
+ * ------------------------------------------------------------------- */
+void orc_ztilt(double *grad, const double *opd, const double *amp, const double *ptsx,
+               const double *ptsy, long nsa, long nxsa, double dx) {
+    for (long isa = 0; isa < nsa; isa++) {
+        const double *a = amp + isa * nxsa * nxsa, *o = opd + isa * nxsa * nxsa;
+        double m[3][3] = {{0}}, c[3] = {0, 0, 0};
+        for (long iy = 0; iy < nxsa; iy++)
+            for (long ix = 0; ix < nxsa; ix++) {
+                const double x = ptsx[isa] + ix * dx, y = ptsy[isa] + iy * dx, w = a[iy * nxsa + ix];
+                const double v[3] = {1., x, y};
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++) m[i][j] += w * v[i] * v[j];
+                if (w != 0)
+                    for (int i = 0; i < 3; i++) c[i] += w * o[iy * nxsa + ix] * v[i];
+            }
+        /* Gauss-Jordan on the symmetric 3x3 system m g = c */
+        double aug[3][4];
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) aug[i][j] = m[i][j];
+            aug[i][3] = c[i];
+        }
+        int ok = 1;
+        for (int k = 0; k < 3 && ok; k++) {
+            int piv = k;
+            for (int i = k + 1; i < 3; i++)
+                if (fabs(aug[i][k]) > fabs(aug[piv][k])) piv = i;
+            if (aug[piv][k] == 0) {
+                ok = 0;
+                break;
+            }
+            for (int j = 0; j < 4; j++) {
+                const double t = aug[k][j];
+                aug[k][j] = aug[piv][j];
+                aug[piv][j] = t;
+            }
+            for (int i = 0; i < 3; i++)
+                if (i != k) {
+                    const double f = aug[i][k] / aug[k][k];
+                    for (int j = 0; j < 4; j++) aug[i][j] -= f * aug[k][j];
+                }
+        }
+        grad[isa] = ok ? aug[1][3] / aug[1][1] : 0;
+        grad[isa + nsa] = ok ? aug[2][3] / aug[2][2] : 0;
+    }
+}

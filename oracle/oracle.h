@@ -121,6 +121,13 @@ void orc_wfsgrad_iwfs(double *ints, double *grad, const orc_powfs_t *p,
  * nx x ny array out, zero elsewhere (padding or cropping). */
 void orc_ccpcorner(double *out, long nx, long ny, const double *in, long ninx, long niny);
 
+/* Geometric Zernike-tilt gradients (pts_ztilt, math/loc.c:839-880, with
+ * the moments of pts_mcc_ptt, math/loc.c:612-650): the tip and tilt of the
+ * amplitude-weighted least-squares fit of [1, x, y] to each subaperture's
+ * OPD.  grad is 2*nsa (x block then y block), overwritten. */
+void orc_ztilt(double *grad, const double *opd, const double *amp, const double *ptsx,
+               const double *ptsy, long nsa, long nxsa, double dx);
+
 /* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
  * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ctr is replaced
  * by the four output words. */

@@ -469,6 +469,78 @@ def test_recorder_writes_maos_telemetry(ngs_small, tmp_path):
     eng.close()
 
 
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16"])
+def test_opd_inputs_surface_and_tip_tilt(ngs_two_wvl, variant):
+    """A static surface OPD per WFS and a per-step piston/tip/tilt enter the
+    WFS OPD (debug_opd equals the oracle's traced OPD plus both), and the
+    intensities and gradients equal the oracle's from that OPD."""
+    cfg, p, layers = ngs_two_wvl
+    print("case: opdadd + ptt, ngs two wavelengths, %s" % variant, flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    eng.set_variant(VARIANTS[variant])
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    n = p.nxsa
+    rs = np.random.RandomState(17)
+    surf = [5e-8 * rs.standard_normal(p.nsa * n * n) for _ in wfs]
+    ptt = [np.array([1e-8, 2e-7, -1.5e-7]), np.array([0.0, -3e-7, 1e-7])]
+    for iw in range(len(wfs)):
+        eng.set_opdadd(surf[iw], iwfs=iw)
+        eng.set_ptt(ptt[iw], iwfs=iw)
+    g = np.arange(n) * p.dx
+    X = (p.ptsx[:, None, None] + g[None, None, :] + 0 * g[None, :, None]).reshape(-1)
+    Y = (p.ptsy[:, None, None] + g[None, :, None] + 0 * g[None, None, :]).reshape(-1)
+    isim = 21
+    eng.queue(isim, dt)
+    for iw in range(len(wfs)):
+        gg, ints = eng.sync(iw)
+        opd = eng.debug_opd(iw)
+        _, _, ropd, _ = o.step(wfs[iw], isim, dt, want_opd=True)
+        ropd = ropd + surf[iw] + ptt[iw][0] + ptt[iw][1] * X + ptt[iw][2] * Y
+        ri, rg = o.image(wfs[iw], ropd)
+        e_opd = np.abs(opd - ropd).max() / np.abs(ropd).max()
+        e_int = rel_l2(ints, ri)
+        e_g = np.abs(gg - rg).max() * RAD2MAS
+        print("wfs=%d opd=%.2e ints=%.2e grad=%.3e mas" % (iw, e_opd, e_int, e_g))
+        assert e_opd < 1e-5 and e_int <= INTS_TOL and e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
+@pytest.mark.parametrize("case", ["lgs_small", "ngs_small"])
+def test_geometric_ztilt(case, request):
+    """Geometric Zernike-tilt gradients (set_geom(1)) against the oracle's
+    pts_ztilt of the oracle's OPD; a tip/tilt input moves them by exactly
+    that tilt."""
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: ztilt %s" % case, flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    eng.set_geom(1)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    isim = 33
+    eng.queue(isim, dt)
+    g0 = []
+    for iw in range(len(wfs)):
+        gg, _ = eng.sync(iw, want_ints=False)
+        _, _, ropd, _ = o.step(wfs[iw], isim, dt, want_opd=True)
+        rg = o.ztilt(ropd)
+        vign = np.concatenate([p.saa < p.cfg.saat] * 2)
+        e_g = np.abs(gg - rg)[~vign].max() * RAD2MAS
+        print("wfs=%d ztilt err %.3e mas (|g| max %.1f mas)" % (iw, e_g, np.abs(rg).max() * RAD2MAS))
+        assert e_g < 1e-3
+        g0.append(gg)
+    tilt = np.array([0.0, 2e-6, -1e-6])
+    eng.set_ptt(tilt)
+    eng.queue(isim, dt)
+    for iw in range(len(wfs)):
+        gg, _ = eng.sync(iw, want_ints=False)
+        d = gg - g0[iw]
+        ok = ~(p.saa < p.cfg.saat)
+        assert np.allclose(d[:p.nsa][ok], 2e-6, atol=2e-6 * 1e-4)
+        assert np.allclose(d[p.nsa:][ok], -1e-6, atol=1e-6 * 1e-4)
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C

@@ -363,3 +363,31 @@ def test_ccpcorner_padding_keeps_the_transform(n, m):
     assert np.allclose(big[::r, ::r], small, rtol=0, atol=1e-9 * abs(small).max())
     # cropping back recovers the original
     assert np.array_equal(oracle.ccpcorner(pad, n, n), psf)
+
+
+def test_ztilt_recovers_a_tilt_exactly(lgs_small):
+    """An OPD c + tx x + ty y over every subaperture gives gradients (tx, ty)
+    exactly, whatever the amplitude (the fit is exact for a plane); adding
+    a pattern orthogonal to the plane under the weights changes nothing."""
+    import oracle
+    cfg, p, layers = lgs_small
+    o = oracle.Oracle(p)
+    n = p.nxsa
+    g = np.arange(n) * p.dx
+    x = np.broadcast_to(p.ptsx[:, None, None] + g[None, None, :], (p.nsa, n, n))
+    y = np.broadcast_to(p.ptsy[:, None, None] + g[None, :, None], (p.nsa, n, n))
+    tx, ty = 3e-6, -1.7e-6
+    opd = 4e-7 + tx * x + ty * y
+    gr = o.ztilt(opd)
+    assert np.allclose(gr[:p.nsa], tx, rtol=1e-9) and np.allclose(gr[p.nsa:], ty, rtol=1e-9)
+    # a weighted-orthogonal addition: r - projection of r onto [1, x, y]
+    amp = p.amp.reshape(p.nsa, n, n)
+    rs = np.random.RandomState(5)
+    r = rs.standard_normal((p.nsa, n, n))
+    for isa in range(p.nsa):
+        A = np.stack([np.ones(n * n), x[isa].ravel(), y[isa].ravel()], 1)
+        W = np.maximum(amp[isa].ravel(), 0.0)  # the annulus edge carries -2e-16 values
+        coef = np.linalg.lstsq(A * np.sqrt(W)[:, None], r[isa].ravel() * np.sqrt(W), rcond=None)[0]
+        r[isa] -= (A @ coef).reshape(n, n)
+    gr2 = o.ztilt(opd + 1e-7 * r)
+    assert np.allclose(gr2, gr, rtol=1e-7, atol=1e-15)
