@@ -143,6 +143,19 @@ int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale);
 int mx_atm2gpu_real(int ireal, const mx_atm_layer_t *layers, int nps,
                     double atmscale);
 
+/* The deformable-mirror surfaces seen by every WFS of later steps
+ * (maos/wfsgrad.c:236-242: prop from sim.cachedm or dmreal with alpha=-1
+ * and no wrap, maos/sim_utils.c:1035-1052): each map (p, nx, ny, dx, dy, ox,
+ * oy and the conjugate height ht; vx, vy unused) is traced onto the
+ * subaperture grids after the atmosphere and subtracted; points beyond a
+ * map's last row or column are left alone.  Replaces the previous set
+ * (ndm=0 removes them); waits for queued steps first.  At most 8 DMs;
+ * mx_dm2gpu_real sets realisation ireal's.  No counterpart call in the
+ * reference's GPU plugin, which keeps the DM on the GPU with its own
+ * reconstructor. */
+int mx_dm2gpu(const mx_atm_layer_t *dms, int ndm);
+int mx_dm2gpu_real(int ireal, const mx_atm_layer_t *dms, int ndm);
+
 /* Enqueue step isim for every WFS of the engine (gpu_wfsgrad_queue,
  * cuda/sim/wfsgrad.cu:483): ray trace, LLT uplink, imaging, gradients and
  * (when enabled) FP64 i0 accumulation.  Asynchronous. */

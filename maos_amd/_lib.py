@@ -82,6 +82,8 @@ HIP_API = {
     "mx_wfsgrad_update_ref": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_atm2gpu": (C.c_int, [C.POINTER(MxAtmLayer), C.c_int, C.c_double]),
     "mx_atm2gpu_real": (C.c_int, [C.c_int, C.POINTER(MxAtmLayer), C.c_int, C.c_double]),
+    "mx_dm2gpu": (C.c_int, [C.POINTER(MxAtmLayer), C.c_int]),
+    "mx_dm2gpu_real": (C.c_int, [C.c_int, C.POINTER(MxAtmLayer), C.c_int]),
     "mx_wfsgrad_queue": (C.c_int, [C.c_void_p, C.c_long, C.c_double]),
     "mx_wfsgrad_sync": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_wfsgrad_i0_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),

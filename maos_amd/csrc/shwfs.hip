@@ -757,6 +757,60 @@ __global__ __launch_bounds__(256) void k_opd_add(OpdAddArgs a) {
     }
 }
 
+// ------------------------------------------------------------ DM surfaces
+// The deformable mirrors seen by each WFS (maos/wfsgrad.c:236-242 with the
+// propdata of maos/sim_utils.c:1035-1052): every DM surface map is traced
+// onto the subaperture grids like a screen at the DM's conjugate height
+// (cone scale 1 - ht/hs, displacement ht theta + scale misreg, no wind) with
+// alpha = -1 and without wrapping: a point is used when its row lies in
+// [irows, rowdiv0) and its column in [icols, coldiv), the ranges of
+// prop_grid_block's unequal-sampling form (lib/accphi_grid_do.c:147-302),
+// so points beyond the map's last column or row are left alone.
+constexpr int MX_MAXDM = 8;
+struct DmArgs {
+    int nsa, nxsa, nwfs, ndm;
+    double ptsdx;
+    const double *ptsx, *ptsy;
+    const DevLayer *dms; // [MX_MAXREAL][MX_MAXDM]
+    const DevWfs *wfs;
+    float *opd;
+};
+__global__ __launch_bounds__(64) void k_dm_opd(DmArgs a) {
+    const int unit = xcd_unit((int)blockIdx.x, a.nsa * a.nwfs);
+    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    const DevWfs &W = a.wfs[iwfs];
+    const int n = a.nxsa;
+    float *op = a.opd + ((long)iwfs * a.nsa + isa) * n * n;
+    const double ihs = W.hs != 0. ? 1. / W.hs : 0.;
+    for (int d = 0; d < a.ndm; d++) {
+        const DevLayer L = a.dms[W.ireal * MX_MAXDM + d];
+        if (W.hs != 0. && L.ht > W.hs) continue;
+        const double scale = 1. - L.ht * ihs;
+        const double dxout = a.ptsdx * L.dx1 * scale, dyout = a.ptsdx * L.dy1 * scale;
+        const double oxout = a.ptsx[isa] * L.dx1 * scale + (L.ht * W.thetax + scale * W.misregx - L.ox) * L.dx1;
+        const double oyout = a.ptsy[isa] * L.dy1 * scale + (L.ht * W.thetay + scale * W.misregy - L.oy) * L.dy1;
+        const double dxout1 = 1. / dxout, dyout1 = 1. / dyout;
+        const int wrapx = L.nx - 1, wrapy = L.ny - 1;
+        const long icols = oyout < 0 ? (long)ceil(-oyout * dyout1) : 0;
+        const long coldiv = (long)floor((wrapy - oyout) * dyout1) + 1;
+        const long irows = oxout < 0 ? (long)ceil(-oxout * dxout1) : 0;
+        const long rowdiv0 = (long)ceil((wrapx - oxout) * dxout1);
+        for (int pt = threadIdx.x; pt < n * n; pt += 64) {
+            const int ix = pt % n, iy = pt / n;
+            if (iy < icols || iy >= coldiv || ix < irows || ix >= rowdiv0) continue;
+            const double x = fma((double)ix, dxout, oxout), y = fma((double)iy, dyout, oyout);
+            // the ranges keep x in [0, nx-1) and y in [0, ny-1]; the clamps
+            // only catch rounding at those edges
+            const int nx0 = min(max((int)floor(x), 0), wrapx - 1), ny0 = min(max((int)floor(y), 0), wrapy);
+            const float fx = (float)(x - nx0), fy = (float)(y - ny0);
+            const float *r0 = L.p + (long)ny0 * L.nx, *r1 = L.p + (long)(ny0 == wrapy ? 0 : ny0 + 1) * L.nx;
+            const float v0 = fmaf(fx, r0[nx0 + 1] - r0[nx0], r0[nx0]);
+            const float v1 = fmaf(fx, r1[nx0 + 1] - r1[nx0], r1[nx0]);
+            op[pt] -= fmaf(fy, v1 - v0, v0);
+        }
+    }
+}
+
 // ------------------------------------------------------------ geometric
 // Zernike-tilt gradients of each (WFS, subaperture) from its OPD (pts_ztilt,
 // math/loc.c:839-880, used at maos/wfsgrad.c:303-306): moments
@@ -1992,6 +2046,7 @@ struct mx_wfsgrad_s {
     bool ptt_set = false, ptt_dirty = false;
     OpdAddArgs arg_opdadd{};
     ZtiltArgs arg_ztilt{};
+    DmArgs arg_dm{};
     bool queued = false;
     // timing
     bool timing = false;
@@ -2014,6 +2069,15 @@ struct AtmState {
 };
 AtmState g_atm[MX_MAXREAL];
 DevLayer *g_dlayers = nullptr; // device [MX_MAXREAL][MX_MAXLAYER]
+// DM surfaces of one realisation
+struct DmState {
+    int ndm = 0;
+    DevLayer h_layers[MX_MAXDM];
+    float *d_data = nullptr;
+    size_t bytes = 0;
+};
+DmState g_dm[MX_MAXREAL];
+DevLayer *g_ddms = nullptr; // device [MX_MAXREAL][MX_MAXDM]
 double *g_datmscale = nullptr; // device [MX_MAXREAL]
 int g_device = 0;
 
@@ -2752,6 +2816,57 @@ int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale) {
     return mx_atm2gpu_real(0, layers, nps, atmscale);
 }
 
+int mx_dm2gpu_real(int ireal, const mx_atm_layer_t *dms, int ndm) {
+    if (ireal < 0 || ireal >= MX_MAXREAL) return set_err(MX_EINVAL, "mx_dm2gpu_real: ireal=%d", ireal);
+    if (ndm < 0 || ndm > MX_MAXDM || (ndm && !dms)) return set_err(MX_EINVAL, "mx_dm2gpu: ndm=%d (at most %d)", ndm,
+                                                                   MX_MAXDM);
+    HIPCHK(hipSetDevice(g_device));
+    // no queued step may read the surfaces replaced here
+    HIPCHK(hipDeviceSynchronize());
+    DmState &D = g_dm[ireal];
+    size_t total = 0;
+    for (int i = 0; i < ndm; i++) {
+        if (dms[i].nx < 2 || dms[i].ny < 2 || !dms[i].p || dms[i].dx <= 0 || dms[i].dy <= 0)
+            return set_err(MX_EINVAL, "DM %d is invalid", i);
+        total += (size_t)dms[i].nx * dms[i].ny;
+    }
+    if (D.d_data && D.bytes < total * sizeof(float)) {
+        HIPCHK(hipFree(D.d_data));
+        D.d_data = nullptr;
+    }
+    if (!D.d_data && total) {
+        HIPCHK(hipMalloc(&D.d_data, total * sizeof(float)));
+        D.bytes = total * sizeof(float);
+    }
+    if (!g_ddms) HIPCHK(hipMalloc(&g_ddms, sizeof(DevLayer) * MX_MAXREAL * MX_MAXDM));
+    size_t off = 0;
+    std::vector<float> tmp;
+    for (int i = 0; i < ndm; i++) {
+        const mx_atm_layer_t &L = dms[i];
+        const size_t n = (size_t)L.nx * L.ny;
+        tmp.resize(n);
+        for (size_t k = 0; k < n; k++) tmp[k] = (float)L.p[k];
+        HIPCHK(hipMemcpy(D.d_data + off, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+        DevLayer &d = D.h_layers[i];
+        d = DevLayer{};
+        d.p = D.d_data + off;
+        d.nx = (int)L.nx;
+        d.ny = (int)L.ny;
+        d.dx1 = 1. / L.dx;
+        d.dy1 = 1. / L.dy;
+        d.ox = L.ox;
+        d.oy = L.oy;
+        d.ht = L.ht;
+        off += n;
+    }
+    HIPCHK(hipMemcpy(g_ddms + (size_t)ireal * MX_MAXDM, D.h_layers, sizeof(DevLayer) * MX_MAXDM,
+                     hipMemcpyHostToDevice));
+    D.ndm = ndm;
+    return MX_OK;
+}
+
+int mx_dm2gpu(const mx_atm_layer_t *dms, int ndm) { return mx_dm2gpu_real(0, dms, ndm); }
+
 } // extern "C"
 
 namespace {
@@ -2940,6 +3055,26 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     const void *kopd = kopd_t[opd_nx == 32][direct][o.tiles > 1];
     L[nl++] = {kopd,
                dim3((unsigned)(h->nsa * h->nwfs * o.tiles * o.tiles + o.nllt)), dim3(MX_OPD_THREADS), &o};
+    // the DM surfaces (every realisation of the engine has the same count)
+    const int ndm = g_dm[0].ndm;
+    for (int r = 1; r < h->nreal; r++)
+        if (g_dm[r].ndm != ndm)
+            return set_err(MX_EINVAL, "realisation %d has %d DMs, realisation 0 %d", r, g_dm[r].ndm, ndm);
+    if (ndm > 0) {
+        DmArgs &dm = h->arg_dm;
+        dm = DmArgs{};
+        dm.nsa = (int)h->nsa;
+        dm.nxsa = (int)h->nxsa;
+        dm.nwfs = h->nwfs;
+        dm.ndm = ndm;
+        dm.ptsdx = h->cfg.dx;
+        dm.ptsx = h->d_ptsx;
+        dm.ptsy = h->d_ptsy;
+        dm.dms = g_ddms;
+        dm.wfs = h->d_wfs;
+        dm.opd = h->d_opd;
+        L[nl++] = {reinterpret_cast<const void *>(&k_dm_opd), dim3((unsigned)(h->nsa * h->nwfs)), dim3(64), &dm};
+    }
     // static surface and tip/tilt inputs of the WFS OPD
     const bool opdin = h->opdadd_set || h->ptt_set;
     if (opdin) {
@@ -2982,8 +3117,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         sp.slot_img = nl;
         L[nl++] = {reinterpret_cast<const void *>(&k_ztilt), dim3((unsigned)(h->nsa * h->nwfs)), dim3(64), &z};
         sp.nl = nl;
-        sp.key = (1 << 30) | (1 << 29) | (opdin ? 1 << 28 : 0) | (direct ? 8 : 0) | (o.tiles > 1 ? 1 << 7 : 0) |
-                 (opd_nx << 8);
+        sp.key = (1 << 30) | (1 << 29) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (direct ? 8 : 0) |
+                 (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8);
         sp.slot_llt = -1;
         sp.llt = false;
         return MX_OK;
@@ -3132,7 +3267,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
                    dim3((unsigned)(ngroup * h->nwfs)), dim3(64), &a};
     }
     sp.nl = nl;
-    sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
+    sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
              (direct ? 8 : 0) | (variant << 4) | (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) |
              (h->lf ? h->lf_threads << 14 : 0);
     sp.slot_llt = slot_llt;

@@ -45,6 +45,19 @@ def atm2gpu(layers, atmscale=1.0, ireal=0):
     check(_lib.hip().mx_atm2gpu_real(ireal, arr, len(layers), atmscale))
 
 
+def dm2gpu(dms, ireal=0):
+    """Upload DM surface maps: a list of (p [ny, nx] FP64, dx, ox, oy, ht)."""
+    if not _initialised:
+        init(0)
+    arr = (MxAtmLayer * max(1, len(dms)))()
+    keep = []
+    for i, (p, dx, ox, oy, ht) in enumerate(dms):
+        a = np.ascontiguousarray(p, dtype=np.float64)
+        keep.append(a)
+        arr[i] = MxAtmLayer(dptr(a), a.shape[1], a.shape[0], dx, dx, ox, oy, ht, 0.0, 0.0)
+    check(_lib.hip().mx_dm2gpu_real(ireal, arr, len(dms)))
+
+
 class WfsGrad:
     """One powfs and its WFS.  With nreal > 1 the engine advances nreal
     realisations together; WFS ireal * len(wfs_list) + i is realisation

@@ -273,8 +273,51 @@ long orc_prop_grid_pts(const double *phiin, long nxin, long nyin, double dxin,
                        const double *ptsy, long nsa, long nxsa, double ptsdx,
                        double *phiout, double alpha, double displacex,
                        double displacey, double scale, int wrap, long start, long end) {
-    if (!wrap) return -1; /* the WFS path always wraps (maos/sim_utils.c:1017) */
     const double dx_in1 = 1. / dxin, dy_in1 = 1. / dyin;
+    if (!wrap) {
+        /* the DM path (wrap=0, maos/sim_utils.c:1042): prop_grid_block's
+         * unequal-sampling rules (lib/accphi_grid_do.c:147-302, the cached
+         * form used for grids of more than 10 rows): a point is used when its
+         * row lies in [irows, rowdiv0) (x in [0, nxin-1)) and its column in
+         * [icols, coldiv) (y in [0, nyin-1]); others are left alone */
+        const double dxout = ptsdx * dx_in1 * scale, dyout = ptsdx * dy_in1 * scale;
+        const double dxout1 = 1. / dxout, dyout1 = 1. / dyout;
+        const double dispx = (displacex - oxin) * dx_in1, dispy = (displacey - oyin) * dy_in1;
+        const long wrapx = nxin - 1, wrapy = nyin - 1;
+        long missing = 0;
+        if (!end) end = nsa;
+        for (long isa = start; isa < end; isa++) {
+            const double oxout = ptsx[isa] * dx_in1 * scale + dispx;
+            const double oyout = ptsy[isa] * dy_in1 * scale + dispy;
+            const long icols = oyout < 0 ? (long)ceil(-oyout * dyout1) : 0;
+            long nyout = nxsa;
+            const long coldiv = (long)floor((wrapy - oyout) * dyout1) + 1;
+            if (coldiv < nyout) nyout = coldiv;
+            const long irows = oxout < 0 ? (long)ceil(-oxout * dxout1) : 0;
+            long rowdiv0 = (long)ceil((wrapx - oxout) * dxout1);
+            if (rowdiv0 > nxsa) rowdiv0 = nxsa;
+            double *out = phiout + isa * nxsa * nxsa;
+            for (long icol = icols; icol < nyout; icol++) {
+                const double y = fma((double)icol, dyout, oyout);
+                long ny0 = (long)floor(y); /* clamps: rounding at the range edges */
+                ny0 = ny0 < 0 ? 0 : (ny0 > wrapy ? wrapy : ny0);
+                const double fy = y - ny0;
+                const double *r0 = phiin + ny0 * nxin, *r1 = phiin + (ny0 == wrapy ? 0 : ny0 + 1) * nxin;
+                for (long irow = irows; irow < rowdiv0; irow++) {
+                    const double x = fma((double)irow, dxout, oxout);
+                    long nx0 = (long)floor(x);
+                    nx0 = nx0 < 0 ? 0 : (nx0 > wrapx - 1 ? wrapx - 1 : nx0);
+                    const double fx = x - nx0;
+                    out[icol * nxsa + irow] +=
+                        alpha * ((1 - fy) * ((1 - fx) * r0[nx0] + fx * r0[nx0 + 1]) +
+                                 fy * ((1 - fx) * r1[nx0] + fx * r1[nx0 + 1]));
+                }
+            }
+            const long used = (nyout > icols ? nyout - icols : 0) * (rowdiv0 > irows ? rowdiv0 - irows : 0);
+            missing += nxsa * nxsa - used;
+        }
+        return missing;
+    }
     const double dxout = ptsdx * dx_in1 * scale;
     const double dyout = ptsdx * dy_in1 * scale;
     displacex = (displacex - oxin) * dx_in1;

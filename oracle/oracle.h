@@ -21,8 +21,10 @@ void orc_fft2(double *c, long nx, long ny, int dir);
 
 /* prop_grid_pts + prop_grid_block (lib/accphi_grid_do.c:49-352, 408-446),
  * non-transposed, with both the equal-sampling and the unequal-sampling
- * code paths.  phiout (nsa*nxsa*nxsa) is accumulated.  Returns the number of
- * points that fell outside the screen (0 with wrap). */
+ * code paths for wrap=1 (screens), and the unequal-sampling rules for
+ * wrap=0 (DM maps: points outside [0, nxin-1) x [0, nyin-1] are left
+ * alone).  phiout (nsa*nxsa*nxsa) is accumulated.  Returns the number of
+ * points that fell outside the map (0 with wrap). */
 long orc_prop_grid_pts(const double *phiin, long nxin, long nyin, double dxin,
                        double dyin, double oxin, double oyin, const double *ptsx,
                        const double *ptsy, long nsa, long nxsa, double ptsdx,

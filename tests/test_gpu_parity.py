@@ -541,6 +541,69 @@ def test_geometric_ztilt(case, request):
     eng.close()
 
 
+def _dm_maps(rs):
+    """two DM surfaces: 30 m at 1/64 m on the ground and a 3.5 m map at 1/32 m
+    conjugated to 8 km (smaller than the footprints there, so that some
+    points fall outside and are left alone)"""
+    out = []
+    for n, dx, ht in ((1920, 1.0 / 64, 0.0), (112, 1.0 / 32, 8000.0)):
+        y, x = np.mgrid[0:n, 0:n] * dx
+        p = 1e-7 * (np.sin(2 * np.pi * x / 3.1 + rs.uniform(0, 6)) * np.cos(2 * np.pi * y / 2.3)
+                    + 0.3 * rs.standard_normal((n, n)))
+        out.append((p, dx, -(n // 2) * dx, -(n // 2) * dx, ht))
+    return out
+
+
+@pytest.mark.parametrize("case", ["ngs_two_wvl", "lgs_small"])
+def test_dm_surfaces(case, request):
+    """DM surfaces traced onto the WFS (no wrap, alpha = -1): the OPD equals
+    the oracle's traced OPD minus the oracle's non-wrapping prop of every DM,
+    and the images and gradients follow from it."""
+    import ctypes as C
+    import oracle
+    from maos_amd import engine
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: DM surfaces %s" % case, flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    if p.llt_nx:
+        mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+        o.set_ref(mtche, i0sum)
+        eng.update_ref(mtche, i0sum)
+    eng.set_variant(VARIANTS["fft32"])
+    dms = _dm_maps(np.random.RandomState(4))
+    engine.dm2gpu(dms)
+    try:
+        isim = 12
+        eng.queue(isim, dt)
+        L = oracle.lib()
+        n = p.nxsa
+        for iw in range(len(wfs)):
+            gg, ints = eng.sync(iw)
+            opd = eng.debug_opd(iw)
+            _, _, ropd, rlotf = o.step(wfs[iw], isim, dt, want_opd=True, want_lotf=True)
+            missing = 0
+            for (m, dx, ox, oy, ht) in dms:
+                m = np.ascontiguousarray(m)
+                missing += L.orc_prop(m.ctypes.data_as(oracle.dp), m.shape[1], m.shape[0], dx, ox, oy, ht,
+                                      np.ascontiguousarray(p.ptsx).ctypes.data_as(oracle.dp),
+                                      np.ascontiguousarray(p.ptsy).ctypes.data_as(oracle.dp), p.nsa, n, p.dx,
+                                      ropd.ctypes.data_as(oracle.dp), -1.0, wfs[iw].hs, wfs[iw].thetax,
+                                      wfs[iw].thetay, wfs[iw].misregx, wfs[iw].misregy, 0.0, 0.0, 0)
+            ri, rg = o.image(wfs[iw], ropd, rlotf)
+            e_opd = np.abs(opd - ropd).max() / np.abs(ropd).max()
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(gg - rg).max() * RAD2MAS
+            print("wfs=%d points outside the DM maps %d, opd=%.2e ints=%.2e grad=%.3e mas"
+                  % (iw, missing, e_opd, e_int, e_g))
+            assert missing > 0  # the edge rule is exercised
+            assert e_opd < 1e-5 and e_int <= INTS_TOL and e_g <= GRAD_TOL_MAS
+    finally:
+        engine.dm2gpu([])
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C

@@ -391,3 +391,36 @@ def test_ztilt_recovers_a_tilt_exactly(lgs_small):
         r[isa] -= (A @ coef).reshape(n, n)
     gr2 = o.ztilt(opd + 1e-7 * r)
     assert np.allclose(gr2, gr, rtol=1e-7, atol=1e-15)
+
+
+def test_prop_nowrap_matches_wrap_inside_and_skips_outside():
+    """Non-wrapping prop (the DM path): where every point's four neighbours
+    lie inside the map it equals the wrapping prop (pinned above); points
+    beyond the map's last column or row are left alone; a plane is
+    reproduced exactly where used."""
+    n, dxin = 96, 1.0 / 16
+    scr = np.ascontiguousarray(_plane_screen(n, 0.3, 1.0, -0.5))
+    ox = oy = -(n // 2) * dxin
+    L = oracle.lib()
+    nxsa = 16
+    px = np.array([-1.0, 0.5, 2.0])  # the last grid runs past the map edge (x < 3.0)
+    py = np.array([-0.5, 0.25, 1.0])
+    kw = dict()
+    for scale_ht in (0.0, 20e3):
+        a = np.zeros(3 * nxsa * nxsa)
+        b = np.zeros(3 * nxsa * nxsa)
+        args = lambda out, wrap: L.orc_prop(scr.ctypes.data_as(oracle.dp), n, n, dxin, ox, oy, scale_ht,
+                                            px.ctypes.data_as(oracle.dp), py.ctypes.data_as(oracle.dp), 3,
+                                            nxsa, dxin * 1.5, out.ctypes.data_as(oracle.dp), -1.0, 90e3,
+                                            1e-6, 2e-6, 0, 0, 0, 0, wrap)
+        miss = args(a, 0)
+        args(b, 1)
+        a3 = a.reshape(3, nxsa, nxsa)
+        b3 = b.reshape(3, nxsa, nxsa)
+        assert np.allclose(a3[:2], b3[:2], rtol=0, atol=1e-12)  # inside
+        scale = 1 - scale_ht / 90e3
+        x = px[2] * scale + np.arange(nxsa) * dxin * 1.5 * scale + scale_ht * 1e-6
+        xi = (x - ox) / dxin
+        inside = xi < n - 1
+        assert miss == nxsa * np.sum(~inside) and np.all(a3[2][:, ~inside] == 0)
+        assert np.allclose(a3[2][:, inside], b3[2][:, inside], rtol=0, atol=1e-12)
