@@ -340,15 +340,38 @@ __device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i,
     ff = (float)fr;
 }
 
+// four floats at 4-byte (a screen row) and 16-byte (an LDS tile row)
+// alignment
+typedef float float4_a4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef float float4_a16 __attribute__((ext_vector_type(4), aligned(16)));
+
+// row stride (floats) of the staged screen tile of an NXSA x NXSA grid: its
+// NXSA + 2 columns rounded up to 16 bytes
+constexpr int rt_tile_stride(int nxsa) { return (nxsa + 2 + 3) / 4 * 4; }
+// rows of that tile: its NXSA + 2 rows rounded up to whole loads of
+// 64 / (stride / 4) rows
+constexpr int rt_tile_rows(int nxsa) {
+    return (nxsa + 2 + 64 / (rt_tile_stride(nxsa) / 4) - 1) / (64 / (rt_tile_stride(nxsa) / 4)) *
+           (64 / (rt_tile_stride(nxsa) / 4));
+}
+
 // OPD of one NXSA x NXSA grid (origin origx, origy, step ptsdx) summed over
-// the layers, by the N lanes of one wave.  Per layer, lanes [0, 2 NXSA) find
-// the column and row coordinates (cidx, cfrac) in FP64, the (NXSA+2)^2 screen
-// tile that the grid touches (its step on the screen is <= 1 pixel) is
-// staged in LDS with coalesced loads, and the bilinear interpolation runs
-// from LDS.  A tile that crosses the screen edge wraps periodically; the
-// common case that does not (a uniform test) skips the wrap arithmetic.
+// the layers, by the 64 lanes of one wave (one block).  Per layer, lanes
+// [0, 2 NXSA) find the column and row coordinates in FP64; the (NXSA+2)^2
+// screen tile that the grid touches (its step on the screen is <= 1 pixel)
+// is staged in LDS with coalesced loads, and the bilinear interpolation runs
+// from LDS.  The layers are software-pipelined: the tile of the next layer
+// is fetched into registers while the current one is interpolated, and the
+// coordinates of the next layer are computed while the current tile is
+// still in flight.  The tile origin (column of the first point, row of the
+// first row) is wave-uniform, so the screen base address is scalar and each
+// lane addresses the tile with a 32-bit offset; the coordinate lanes store
+// tile-relative offsets (column, row * tile width) with the periodic wrap
+// already applied.  A tile that crosses the screen edge wraps periodically;
+// the common case that does not (a uniform test) skips the wrap arithmetic.
 // DIRECT (a grid coarser than some screen, whose footprint exceeds the
 // tile) reads the four neighbours of each point from the screen itself.
+// cidx, cfrac hold 4 NXSA entries (two layers' coordinates).
 template <int N, int NXSA, bool DIRECT>
 __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, double hs,
                                           double thetax, double thetay, double misregx,
@@ -356,104 +379,171 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
                                           long isim, double dt, double atmscale, double ptsdx,
                                           double origx, double origy, int line, float *opd,
                                           int *cidx, float *cfrac, float *tile, int ostride = NXSA) {
-    static_assert(2 * NXSA <= N, "coordinate lanes");
+    static_assert(N == 64 && 2 * NXSA <= N, "one wave holds the coordinate lanes");
     constexpr int PPT = NXSA * NXSA / N;
-    constexpr int TW = NXSA + 2;
-    constexpr int NFILL = (TW * TW + N - 1) / N;
+    constexpr int TW = NXSA + 2, TS = rt_tile_stride(NXSA);
     static_assert(PPT * N == NXSA * NXSA, "points per lane");
-    const bool act = line < N;
     const double ihs = hs != 0. ? 1. / hs : 0.;
     float acc[PPT];
 #pragma unroll
     for (int j = 0; j < PPT; j++) acc[j] = 0.f;
-    for (int l = 0; l < nps; l++) {
-        const DevLayer L = layers[l];
-        if (hs != 0. && L.ht > hs) continue; // layer above the source (uniform)
-        __syncthreads(); // the previous layer's coordinates and tile fully read
-        if (line < 2 * NXSA) {
-            const bool isx = line < NXSA;
+    // layers above the source are skipped (uniform)
+    auto next_layer = [&](int l) {
+        for (l++; l < nps; l++)
+            if (!(hs != 0. && layers[l].ht > hs)) break;
+        return l;
+    };
+    const bool isx = line < NXSA;
+    const int ci = isx ? line : line - NXSA;
+    // FP64 coordinates of layer L, as (ii, ff) on the lanes [0, 2 NXSA)
+    auto coords = [&](const DevLayer &L, int &ii, float &ff) {
+        ii = 0;
+        ff = 0.f;
+        if (line < 2 * NXSA)
+            screen_coord(L, isx, ci, ihs, isx ? thetax : thetay, isx ? misregx : misregy,
+                         isx ? shift0x : shift0y, isim, dt, ptsdx, isx ? origx : origy, ii, ff);
+    };
+    if constexpr (DIRECT) {
+        for (int l = next_layer(-1); l < nps; l = next_layer(l)) {
+            const DevLayer L = layers[l];
+            __syncthreads(); // the previous layer's coordinates fully read
             int ii;
             float ff;
-            screen_coord(L, isx, isx ? line : line - NXSA, ihs, isx ? thetax : thetay, isx ? misregx : misregy,
-                         isx ? shift0x : shift0y, isim, dt, ptsdx, isx ? origx : origy, ii, ff);
-            cidx[line] = ii;
-            cfrac[line] = ff;
-        }
-        __syncthreads();
-        const int nx = L.nx, ny = L.ny;
-        if constexpr (DIRECT) {
-            if (act)
-#pragma unroll
-                for (int j = 0; j < PPT; j++) {
-                    const int pt = line + j * N;
-                    const int ix = pt % NXSA, iy = pt / NXSA;
-                    const int c0 = cidx[ix], r0i = cidx[NXSA + iy];
-                    const int c1 = c0 + 1 == nx ? 0 : c0 + 1, r1i = r0i + 1 == ny ? 0 : r0i + 1;
-                    const float *p0 = L.p + (long)r0i * nx, *p1 = L.p + (long)r1i * nx;
-                    const float fx = cfrac[ix], fy = cfrac[NXSA + iy];
-                    const float r0 = fmaf(fx, p0[c1] - p0[c0], p0[c0]);
-                    const float r1 = fmaf(fx, p1[c1] - p1[c0], p1[c0]);
-                    acc[j] += fmaf(fy, r1 - r0, r0);
-                }
-            continue;
-        }
-        const int tx0 = cidx[0], ty0 = cidx[NXSA];
-        const bool wrap = tx0 > nx - TW || ty0 > ny - TW; // uniform
-        if (act) {
-            if (!wrap) {
-                const float *base = L.p + (long)ty0 * nx + tx0;
-#pragma unroll
-                for (int k = 0; k < NFILL; k++) {
-                    const int e = line + k * N;
-                    if (e < TW * TW) {
-                        const int r = e / TW, c = e - r * TW;
-                        tile[e] = base[r * nx + c];
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < NFILL; k++) {
-                    const int e = line + k * N;
-                    if (e < TW * TW) {
-                        const int r = e / TW, c = e - r * TW;
-                        int gx = tx0 + c, gy = ty0 + r;
-                        gx -= gx >= nx ? nx : 0;
-                        gy -= gy >= ny ? ny : 0;
-                        tile[e] = L.p[(long)gy * nx + gx];
-                    }
-                }
+            coords(L, ii, ff);
+            if (line < 2 * NXSA) {
+                cidx[line] = ii;
+                cfrac[line] = ff;
             }
-        }
-        __syncthreads();
-        // bilinear interpolation as two lerps, a + f (b - a)
-        auto interp = [&](auto wrapped) {
+            __syncthreads();
+            const int nx = L.nx, ny = L.ny;
 #pragma unroll
             for (int j = 0; j < PPT; j++) {
                 const int pt = line + j * N;
                 const int ix = pt % NXSA, iy = pt / NXSA;
-                int lc = cidx[ix] - tx0, lr = cidx[NXSA + iy] - ty0;
-                if (decltype(wrapped)::value) {
-                    lc += lc < 0 ? nx : 0;
-                    lr += lr < 0 ? ny : 0;
-                }
+                const int c0 = cidx[ix], r0i = cidx[NXSA + iy];
+                const int c1 = c0 + 1 == nx ? 0 : c0 + 1, r1i = r0i + 1 == ny ? 0 : r0i + 1;
+                const float *p0 = L.p + (long)r0i * nx, *p1 = L.p + (long)r1i * nx;
                 const float fx = cfrac[ix], fy = cfrac[NXSA + iy];
-                const float *t0 = tile + lr * TW + lc;
-                const float r0 = fmaf(fx, t0[1] - t0[0], t0[0]);
-                const float r1 = fmaf(fx, t0[TW + 1] - t0[TW], t0[TW]);
+                const float r0 = fmaf(fx, p0[c1] - p0[c0], p0[c0]);
+                const float r1 = fmaf(fx, p1[c1] - p1[c0], p1[c0]);
                 acc[j] += fmaf(fy, r1 - r0, r0);
             }
+        }
+    } else {
+        // The tile is TH = KM RPI >= TW rows of TS = TW rounded up to 4
+        // floats (the padding rows and columns are loaded, never read).
+        // Lane line loads 16 bytes: row k RPI + line / LPR, columns
+        // 4 (line % LPR) .. +3, LPR lanes a row; the row step lives in the
+        // 32-bit offset and the LDS offset immediates.  Lanes past RPI LPR
+        // repeat lane 0's slot (the same value to the same address), so no
+        // load is masked.  The screen rows are 4-byte aligned only: the
+        // 16-byte loads are unaligned, which this target performs (the
+        // compiler splits them where it cannot).
+        constexpr int LPR = TS / 4, RPI = N / LPR, KM = (TW + RPI - 1) / RPI, TH = KM * RPI;
+        static_assert(TH <= rt_tile_rows(NXSA), "tile rows");
+        const bool lane_on = line < RPI * LPR;
+        const int mrow = lane_on ? line / LPR : 0, mcol = lane_on ? 4 * (line % LPR) : 0;
+        float4_a4 v[KM];
+        // Screen and tile origin of a staged layer.  A tile inside the screen
+        // is prefetched into v (its loads stay in flight until the tile is
+        // stored); one that crosses the screen edge (rare) is loaded with
+        // the periodic wrap when it is stored, so that v has one producer
+        // and the compiler need not wait for the prefetch to merge values.
+        struct Staged {
+            const float *p;
+            int nx, ny, tx0, ty0;
+            bool inside;
         };
-        if (act) {
-            if (wrap) interp(std::true_type{});
-            else interp(std::false_type{});
+        auto fetch = [&](const DevLayer &L, int tx0, int ty0) {
+            Staged t{L.p, L.nx, L.ny, tx0, ty0, tx0 <= L.nx - TS && ty0 <= L.ny - TH};
+            if (t.inside) {
+                const char *base = (const char *)(L.p + (long)ty0 * L.nx + tx0);
+                const unsigned moff = (unsigned)(mrow * L.nx + mcol) * 4u, rstep = (unsigned)(RPI * L.nx) * 4u;
+#pragma unroll
+                for (int k = 0; k < KM; k++)
+                    v[k] = *(const float4_a4 *)(base + (size_t)(moff + (unsigned)k * rstep));
+            }
+            return t;
+        };
+        auto store = [&](const Staged &t) {
+            if (t.inside) {
+#pragma unroll
+                for (int k = 0; k < KM; k++) *(float4_a16 *)(tile + (k * RPI + mrow) * TS + mcol) = v[k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < KM; k++)
+                    if (k * RPI + mrow < TW) {
+                        int gy = t.ty0 + k * RPI + mrow;
+                        gy -= gy >= t.ny ? t.ny : 0;
+#pragma unroll
+                        for (int q = 0; q < 4; q++)
+                            if (mcol + q < TW) {
+                                int gx = t.tx0 + mcol + q;
+                                gx -= gx >= t.nx ? t.nx : 0;
+                                tile[(k * RPI + mrow) * TS + mcol + q] = t.p[(long)gy * t.nx + gx];
+                            }
+                    }
+            }
+        };
+        // tile-relative coordinates of layer L into slot s; returns its
+        // tile origin through tx0, ty0
+        auto publish = [&](const DevLayer &L, int s, int &tx0, int &ty0) {
+            int ii;
+            float ff;
+            coords(L, ii, ff);
+            tx0 = __builtin_amdgcn_readlane(ii, 0);
+            ty0 = __builtin_amdgcn_readlane(ii, NXSA);
+            if (line < 2 * NXSA) {
+                int rel = ii - (isx ? tx0 : ty0);
+                rel += rel < 0 ? (isx ? L.nx : L.ny) : 0;
+                cidx[s * 2 * NXSA + line] = isx ? rel : rel * TS;
+                cfrac[s * 2 * NXSA + line] = ff;
+            }
+        };
+        int l = next_layer(-1), s = 0;
+        Staged cur{};
+        if (l < nps) {
+            int tx0, ty0;
+            const DevLayer L0 = layers[l];
+            publish(L0, 0, tx0, ty0);
+            cur = fetch(L0, tx0, ty0);
+        }
+        // (also keeps the layer table loads in the loop scalar: the compiler
+        // proves them unclobbered across a barrier, not across the stores
+        // above)
+        __syncthreads();
+        const int ix = line % NXSA, iy0 = line / NXSA;
+        while (l < nps) {
+            const int ln = next_layer(l);
+            const DevLayer L1 = layers[ln < nps ? ln : l];
+            int tx1 = 0, ty1 = 0;
+            if (ln < nps) publish(L1, s ^ 1, tx1, ty1); // overlaps the loads of layer l
+            __syncthreads(); // layer l-1's interpolation done with the tile
+            store(cur);
+            __syncthreads();
+            if (ln < nps) cur = fetch(L1, tx1, ty1); // in flight during the interpolation
+            const int *cx = cidx + s * 2 * NXSA;
+            const float *cf = cfrac + s * 2 * NXSA;
+            const int cofs = cx[ix];
+            const float fx = cf[ix];
+#pragma unroll
+            for (int j = 0; j < PPT; j++) {
+                const int iy = iy0 + j * (N / NXSA);
+                const float *t0 = tile + cx[NXSA + iy] + cofs;
+                const float fy = cf[NXSA + iy];
+                const float r0 = fmaf(fx, t0[1] - t0[0], t0[0]);
+                const float r1 = fmaf(fx, t0[TS + 1] - t0[TS], t0[TS]);
+                acc[j] += fmaf(fy, r1 - r0, r0);
+            }
+            l = ln;
+            s ^= 1;
         }
     }
-    if (act)
 #pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            const int pt = line + j * N;
-            opd[(pt / NXSA) * ostride + pt % NXSA] = (float)atmscale * acc[j];
-        }
+    for (int j = 0; j < PPT; j++) {
+        const int pt = line + j * N;
+        opd[(pt / NXSA) * ostride + pt % NXSA] = (float)atmscale * acc[j];
+    }
 }
 
 // XCD-aware unit index: consecutive units (the WFS of one subaperture
@@ -674,21 +764,25 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
 
 // ------------------------------------------------------------ ray trace
 // OPD of every (WFS, subaperture) at step isim (the atmosphere loop of
-// wfsgrad_iwfs, maos/wfsgrad.c:175-194).  Small register and LDS footprint,
-// so many waves per SIMD hide the screen-fetch latency; the FP32 OPD goes to
-// HBM for the imaging kernel.
+// wfsgrad_iwfs, maos/wfsgrad.c:175-194).  Small register and LDS footprint
+// (6 waves per SIMD for the one-tile 32 x 32 form), and the next layer's
+// tile in flight during each interpolation, hide the screen-fetch latency;
+// the FP32 OPD goes to HBM for the imaging kernel.
 // threads of a ray-trace block (one grid per block, one wave)
 constexpr int MX_OPD_THREADS = 64;
+// smallest screen side that the staged tile (<= 36 pixels wide) wraps at
+// most once
+constexpr int MX_TILE_MINSCREEN = 40;
 #ifndef MX_OPD_WAVES
-#define MX_OPD_WAVES 4
+#define MX_OPD_WAVES 6
 #endif
 // TILED: a subaperture grid is tiles x tiles blocks (else one block, with
 // the register budget of the one-tile case untouched).
 template <int NXSA, bool DIRECT, bool TILED>
-__global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(MX_OPD_WAVES))) void k_wfs_opd(OpdArgs a) {
-    __shared__ int cidx[2 * NXSA];
-    __shared__ float cfrac[2 * NXSA];
-    __shared__ float tile[(NXSA + 2) * (NXSA + 2)];
+__global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(DIRECT || TILED ? MX_OPD_WAVES - 1 : MX_OPD_WAVES))) void k_wfs_opd(OpdArgs a) {
+    __shared__ int cidx[4 * NXSA];
+    __shared__ float cfrac[4 * NXSA];
+    __shared__ __align__(16) float tile[rt_tile_rows(NXSA) * rt_tile_stride(NXSA)];
     const int b = (int)blockIdx.x;
     if (b < a.nllt) {
         // LLT uplink grid of WFS b (maos/wfsgrad.c:329-386): theta - o/hs,
@@ -2884,13 +2978,18 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     if (h->cfg.llt_nx > 0 && !h->geom)
         for (int w = 0; w < h->nwvl; w++)
             if (!h->d_etf[w]) return set_err(MX_ESTATE, "etf of wavelength %d not set", w);
-    // the staged screen tile holds a grid whose step is <= 1 screen pixel;
-    // a coarser grid gathers from the screens directly
+    // the staged screen tile holds a grid whose step is <= 1 screen pixel
+    // and wraps its tile once at most (a screen of >= MX_TILE_MINSCREEN
+    // pixels); a coarser grid or a smaller screen gathers from the screens
+    // directly
     bool direct = false;
     for (int r = 0; r < h->nreal; r++)
-        for (int l = 0; l < g_atm[r].nps; l++)
-            if (h->cfg.dx * g_atm[r].h_layers[l].dx1 > 1. + 1e-9 || h->cfg.dx * g_atm[r].h_layers[l].dy1 > 1. + 1e-9)
+        for (int l = 0; l < g_atm[r].nps; l++) {
+            const DevLayer &hl = g_atm[r].h_layers[l];
+            if (h->cfg.dx * hl.dx1 > 1. + 1e-9 || h->cfg.dx * hl.dy1 > 1. + 1e-9 || hl.nx < MX_TILE_MINSCREEN ||
+                hl.ny < MX_TILE_MINSCREEN)
                 direct = true;
+        }
     if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche && !h->geom)
         return set_err(MX_ESTATE, "matched filter not set");
     if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum && !h->geom)
