@@ -265,6 +265,7 @@ struct OpdArgs {
     double llt_origx, llt_origy, llt_ox, llt_oy;
     float *llt_opd; // [nwfs][nxsa^2]
     int tiles;      // a subaperture grid is tiles x tiles blocks of the kernel's NXSA
+    int tworow;     // interpolate every layer from two tile rows a point (verification)
 };
 
 // Transfer-function tables of the MFMA imaging kernel (N = 64) cover the half
@@ -313,16 +314,22 @@ __device__ __forceinline__ void seg_sum(float &v) {
 // ihs = 1/hs (0 for an NGS).  The wrap count floor(X/n) comes from X * (1/n)
 // except within 1e-9 of an integer quotient, where the division decides (the
 // wrapped coordinate then matches the division's rounding exactly).
-__device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i, double ihs, double theta,
-                                             double misreg, double shift0, long isim, double dt,
-                                             double ptsdx, double orig, int &ii, float &ff) {
+// The map is split into its grid-uniform part, point i of the axis at
+// screen pixel X = i dout + oout (axis_map), and the per-point wrap
+// (screen_wrap).
+__device__ __forceinline__ void axis_map(const DevLayer &L, bool isx, double ihs, double theta, double misreg,
+                                         double shift0, long isim, double dt, double ptsdx, double orig,
+                                         double &dout, double &oout) {
     const double scale = 1. - L.ht * ihs;
     const double disp = L.ht * theta + scale * misreg + shift0 - (isx ? L.vx : L.vy) * dt * (double)isim;
     const double d1 = isx ? L.dx1 : L.dy1;
-    const double nn = (double)(isx ? L.nx : L.ny);
     const double o0 = isx ? L.ox : L.oy;
-    const double dout = ptsdx * d1 * scale;
-    const double oout = orig * d1 * scale + (disp - o0) * d1;
+    dout = ptsdx * d1 * scale;
+    oout = orig * d1 * scale + (disp - o0) * d1;
+}
+__device__ __forceinline__ void screen_wrap(const DevLayer &L, bool isx, int i, double dout, double oout, int &ii,
+                                            float &ff) {
+    const double nn = (double)(isx ? L.nx : L.ny);
     double X = fma((double)i, dout, oout);
     const double q = X * (isx ? L.inx : L.iny);
     double fq = floor(q);
@@ -338,6 +345,13 @@ __device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i,
         fr = 0.;
     }
     ff = (float)fr;
+}
+__device__ __forceinline__ void screen_coord(const DevLayer &L, bool isx, int i, double ihs, double theta,
+                                             double misreg, double shift0, long isim, double dt,
+                                             double ptsdx, double orig, int &ii, float &ff) {
+    double dout, oout;
+    axis_map(L, isx, ihs, theta, misreg, shift0, isim, dt, ptsdx, orig, dout, oout);
+    screen_wrap(L, isx, i, dout, oout, ii, ff);
 }
 
 // four floats at 4-byte (a screen row) and 16-byte (an LDS tile row)
@@ -378,7 +392,8 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
                                           double misregy, double shift0x, double shift0y,
                                           long isim, double dt, double atmscale, double ptsdx,
                                           double origx, double origy, int line, float *opd,
-                                          int *cidx, float *cfrac, float *tile, int ostride = NXSA) {
+                                          int *cidx, float *cfrac, float *tile, double *ctab,
+                                          bool tworow, int ostride = NXSA) {
     static_assert(N == 64 && 2 * NXSA <= N, "one wave holds the coordinate lanes");
     constexpr int PPT = NXSA * NXSA / N;
     constexpr int TW = NXSA + 2, TS = rt_tile_stride(NXSA);
@@ -395,6 +410,8 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
     };
     const bool isx = line < NXSA;
     const int ci = isx ? line : line - NXSA;
+    // this lane's points: column ix of the PPT consecutive rows from jb
+    const int ix = line % NXSA, jb = line / NXSA * PPT;
     // FP64 coordinates of layer L, as (ii, ff) on the lanes [0, 2 NXSA)
     auto coords = [&](const DevLayer &L, int &ii, float &ff) {
         ii = 0;
@@ -418,8 +435,7 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
             const int nx = L.nx, ny = L.ny;
 #pragma unroll
             for (int j = 0; j < PPT; j++) {
-                const int pt = line + j * N;
-                const int ix = pt % NXSA, iy = pt / NXSA;
+                const int iy = jb + j;
                 const int c0 = cidx[ix], r0i = cidx[NXSA + iy];
                 const int c1 = c0 + 1 == nx ? 0 : c0 + 1, r1i = r0i + 1 == ny ? 0 : r0i + 1;
                 const float *p0 = L.p + (long)r0i * nx, *p1 = L.p + (long)r1i * nx;
@@ -487,63 +503,106 @@ __device__ __forceinline__ void ray_trace(const DevLayer *layers, int nps, doubl
         };
         // tile-relative coordinates of layer L into slot s; returns its
         // tile origin through tx0, ty0
-        auto publish = [&](const DevLayer &L, int s, int &tx0, int &ty0) {
-            int ii;
-            float ff;
-            coords(L, ii, ff);
+        // jump: some row step of the grid is not 0 or 1 tile row (a step
+        // of one pixel rounding over a row): that layer interpolates each
+        // point from two tile rows instead of carrying the upper one
+        auto publish = [&](const DevLayer &L, int lidx, int s, int &tx0, int &ty0, bool &jump) {
+            int ii = 0;
+            float ff = 0.f;
+            if (line < 2 * NXSA) {
+                const double *m = ctab + 2 * (2 * lidx + (isx ? 0 : 1));
+                screen_wrap(L, isx, ci, m[0], m[1], ii, ff);
+            }
             tx0 = __builtin_amdgcn_readlane(ii, 0);
             ty0 = __builtin_amdgcn_readlane(ii, NXSA);
+            int rel = ii - (isx ? tx0 : ty0);
+            rel += rel < 0 ? (isx ? L.nx : L.ny) : 0;
+            const int prev = __shfl_up(rel, 1);
+            jump = tworow || __any(line > NXSA && line < 2 * NXSA && (rel - prev > 1 || rel < prev));
             if (line < 2 * NXSA) {
-                int rel = ii - (isx ? tx0 : ty0);
-                rel += rel < 0 ? (isx ? L.nx : L.ny) : 0;
                 cidx[s * 2 * NXSA + line] = isx ? rel : rel * TS;
                 cfrac[s * 2 * NXSA + line] = ff;
             }
         };
+        // grid-uniform axis maps of every layer, lane 2 l + (y axis), once
+        if (line < 2 * nps) {
+            const bool ax = (line & 1) == 0;
+            double dout, oout;
+            axis_map(layers[line >> 1], ax, ihs, ax ? thetax : thetay, ax ? misregx : misregy,
+                     ax ? shift0x : shift0y, isim, dt, ptsdx, ax ? origx : origy, dout, oout);
+            ctab[2 * line] = dout;
+            ctab[2 * line + 1] = oout;
+        }
+        __syncthreads();
         int l = next_layer(-1), s = 0;
         Staged cur{};
+        bool jcur = false;
         if (l < nps) {
             int tx0, ty0;
             const DevLayer L0 = layers[l];
-            publish(L0, 0, tx0, ty0);
+            publish(L0, l, 0, tx0, ty0, jcur);
             cur = fetch(L0, tx0, ty0);
         }
         // (also keeps the layer table loads in the loop scalar: the compiler
         // proves them unclobbered across a barrier, not across the stores
         // above)
         __syncthreads();
-        const int ix = line % NXSA, iy0 = line / NXSA;
         while (l < nps) {
             const int ln = next_layer(l);
             const DevLayer L1 = layers[ln < nps ? ln : l];
             int tx1 = 0, ty1 = 0;
-            if (ln < nps) publish(L1, s ^ 1, tx1, ty1); // overlaps the loads of layer l
+            bool jnext = false;
+            if (ln < nps) publish(L1, ln, s ^ 1, tx1, ty1, jnext); // overlaps the loads of layer l
             __syncthreads(); // layer l-1's interpolation done with the tile
             store(cur);
             __syncthreads();
             if (ln < nps) cur = fetch(L1, tx1, ty1); // in flight during the interpolation
+            // The lane's rows jb .. jb + PPT - 1 are consecutive and their
+            // tile rows advance by 0 or 1 (a step <= 1 pixel; else jump):
+            // the x-interpolated upper row of one point is the lower row of
+            // the next one that advances, so each point reads one tile row.
+            // Scheduling barriers every 4 points bound the registers that
+            // preloaded coordinates take.
             const int *cx = cidx + s * 2 * NXSA;
             const float *cf = cfrac + s * 2 * NXSA;
-            const int cofs = cx[ix];
+            const float *tc = tile + cx[ix];
             const float fx = cf[ix];
+            auto xlerp = [&](int roff) {
+                const float *t = tc + roff;
+                return fmaf(fx, t[1] - t[0], t[0]);
+            };
+            if (!jcur) {
+                int rp = cx[NXSA + jb];
+                float lo = xlerp(rp), up = xlerp(rp + TS);
+                acc[0] += fmaf(cf[NXSA + jb], up - lo, lo);
 #pragma unroll
-            for (int j = 0; j < PPT; j++) {
-                const int iy = iy0 + j * (N / NXSA);
-                const float *t0 = tile + cx[NXSA + iy] + cofs;
-                const float fy = cf[NXSA + iy];
-                const float r0 = fmaf(fx, t0[1] - t0[0], t0[0]);
-                const float r1 = fmaf(fx, t0[TS + 1] - t0[TS], t0[TS]);
-                acc[j] += fmaf(fy, r1 - r0, r0);
+                for (int j = 1; j < PPT; j++) {
+                    if (j % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+                    const int r = cx[NXSA + jb + j];
+                    const float nu = xlerp(r + TS);
+                    if (r != rp) {
+                        lo = up;
+                        up = nu;
+                    }
+                    rp = r;
+                    acc[j] += fmaf(cf[NXSA + jb + j], up - lo, lo);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < PPT; j++) {
+                    if (j % 4 == 0) __builtin_amdgcn_sched_barrier(0);
+                    const int r = cx[NXSA + jb + j];
+                    const float lo = xlerp(r), up = xlerp(r + TS);
+                    acc[j] += fmaf(cf[NXSA + jb + j], up - lo, lo);
+                }
             }
             l = ln;
             s ^= 1;
+            jcur = jnext;
         }
     }
 #pragma unroll
-    for (int j = 0; j < PPT; j++) {
-        const int pt = line + j * N;
-        opd[(pt / NXSA) * ostride + pt % NXSA] = (float)atmscale * acc[j];
-    }
+    for (int j = 0; j < PPT; j++) opd[(jb + j) * ostride + ix] = (float)atmscale * acc[j];
 }
 
 // XCD-aware unit index: consecutive units (the WFS of one subaperture
@@ -782,6 +841,7 @@ template <int NXSA, bool DIRECT, bool TILED>
 __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(DIRECT || TILED ? MX_OPD_WAVES - 1 : MX_OPD_WAVES))) void k_wfs_opd(OpdArgs a) {
     __shared__ int cidx[4 * NXSA];
     __shared__ float cfrac[4 * NXSA];
+    __shared__ double ctab[4 * MX_MAXLAYER];
     __shared__ __align__(16) float tile[rt_tile_rows(NXSA) * rt_tile_stride(NXSA)];
     const int b = (int)blockIdx.x;
     if (b < a.nllt) {
@@ -792,7 +852,7 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
         ray_trace<MX_OPD_THREADS, NXSA, DIRECT>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax - a.llt_ox / W.hs,
                                         W.thetay - a.llt_oy / W.hs, 0., 0., a.llt_ox, a.llt_oy, a.isim, a.dt,
                                         a.atmscale[W.ireal], a.ptsdx, a.llt_origx, a.llt_origy,
-                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile);
+                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile, ctab, a.tworow);
         return;
     }
     if constexpr (!TILED) {
@@ -802,7 +862,7 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
         ray_trace<MX_OPD_THREADS, NXSA, DIRECT>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax, W.thetay,
                                                 W.misregx, W.misregy, 0., 0., a.isim, a.dt, a.atmscale[W.ireal],
                                                 a.ptsdx, a.ptsx[isa], a.ptsy[isa], threadIdx.x,
-                                                a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac, tile);
+                                                a.opd + ((long)iwfs * a.nsa + isa) * NXSA * NXSA, cidx, cfrac, tile, ctab, a.tworow);
         return;
     }
     // unit: (WFS, subaperture, tile); a subaperture grid of side NXSA * tiles
@@ -818,7 +878,7 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
                                     a.ptsx[isa] + a.ptsdx * (double)(NXSA * tx), a.ptsy[isa] + a.ptsdx * (double)(NXSA * ty),
                                     threadIdx.x,
                                     a.opd + ((long)iwfs * a.nsa + isa) * side * side + (long)ty * NXSA * side + tx * NXSA,
-                                    cidx, cfrac, tile, side);
+                                    cidx, cfrac, tile, ctab, a.tworow, side);
 }
 
 // ------------------------------------------------------------ OPD inputs
@@ -3038,6 +3098,12 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     o.llt_ox = h->cfg.llt_ox;
     o.llt_oy = h->cfg.llt_oy;
     o.llt_opd = h->d_llt_opd;
+    // MX_RT_TWOROW=1 in the environment forces the two-row interpolation that
+    // layers with a row step over one tile row take (tests compare the forms)
+    {
+        const char *e = getenv("MX_RT_TWOROW");
+        o.tworow = e && atoi(e) != 0;
+    }
     // the ray trace covers a subaperture grid with tiles of 32 or 16 points
     // a side (an LLT grid is one tile of nxsa)
     const int opd_nx = h->opd_tile ? h->opd_tile : ((h->lf && h->nxsa % 32 == 0) || h->nxsa == 32 ? 32 : 16);

@@ -604,6 +604,66 @@ def test_dm_surfaces(case, request):
     eng.close()
 
 
+@pytest.mark.parametrize("case", ["lgs_small", "ngs_two_wvl", "lgs_pad128"])
+def test_raytrace_two_row_form(case, request, monkeypatch):
+    """The ray trace carries each lane's upper x-interpolated tile row to the
+    next output row; a layer whose row step passes over a tile row takes the
+    plain two-row interpolation instead.  MX_RT_TWOROW=1 forces that form on
+    every layer: both forms do the same arithmetic on the same values, so
+    the traced OPD is bit-identical, and it matches the oracle's."""
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: %s nsa=%d nxsa=%d, both interpolation forms" % (case, p.nsa, p.nxsa), flush=True)
+    from maos_amd.powfs import PTYPE_MF
+    eng, wfs = _engine(cfg, p, layers)
+    if cfg["powfs"].phytype == PTYPE_MF:  # the step needs a filter; the OPD does not use it
+        eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    for isim in (0, 77):
+        monkeypatch.delenv("MX_RT_TWOROW", raising=False)
+        eng.queue(isim, dt)
+        eng.wait()
+        carried = [eng.debug_opd(iw) for iw in range(len(wfs))]
+        monkeypatch.setenv("MX_RT_TWOROW", "1")
+        eng.queue(isim, dt)
+        eng.wait()
+        for iw in range(len(wfs)):
+            two = eng.debug_opd(iw)
+            _, _, ropd, _ = o.step(wfs[iw], isim, dt, want_opd=True)
+            e_opd = np.abs(two - ropd).max() / np.abs(ropd).max()
+            print("isim=%d wfs=%d forms equal=%s opd vs oracle %.2e"
+                  % (isim, iw, np.array_equal(two, carried[iw]), e_opd))
+            assert np.array_equal(two, carried[iw])
+            assert e_opd < 1e-5
+    monkeypatch.delenv("MX_RT_TWOROW", raising=False)
+    eng.close()
+
+
+def test_screens_smaller_than_tile(ngs_two_wvl):
+    """Screens of 32 x 32 pixels (narrower than the staged 34-pixel tile of a
+    32-point grid, so a tile would wrap more than once) are traced by the
+    direct gather; the OPD matches the oracle's periodic trace."""
+    import dataclasses
+    from maos_amd.powfs import genatm
+    cfg, p, _ = ngs_two_wvl
+    atm = dataclasses.replace(cfg["atm"], nx=32)
+    layers = genatm(atm)
+    print("case: ngs_two_wvl geometry, %d layers of %s" % (len(layers), layers[0].p.shape), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    for isim in (0, 31):
+        eng.queue(isim, dt)
+        eng.wait()
+        for iw in range(len(wfs)):
+            opd = eng.debug_opd(iw)
+            _, _, ropd, _ = o.step(wfs[iw], isim, dt, want_opd=True)
+            e_opd = np.abs(opd - ropd).max() / np.abs(ropd).max()
+            print("isim=%d wfs=%d opd vs oracle %.2e" % (isim, iw, e_opd))
+            assert e_opd < 1e-5
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C
