@@ -113,6 +113,45 @@ int mxs_mtch(double *mtche, double *sanea, double *i0sum, double *i0sumsum,
              double rne, double pixthetax, double pixthetay, int mtchcr,
              double sigratio);
 
+
+/* Model i0 of a powfs from the turbulence-averaged short-exposure OTF
+ * (genseotf -> gensepsf -> gensei, lib/gensei.c:42-419 and
+ * lib/genotf.c:50-300, as maos/powfs.c:1545-1622 builds it for the matched
+ * filter).  Complex arrays are interleaved (re, im). */
+
+/* OPD covariance (m^2) at separations r of von Karman turbulence (L0
+ * finite) or Kolmogorov (L0 = inf) (turbcov, lib/turbulence.c:400-431). */
+int mxs_turbcov(const double *r, long n, double r0, double L0, double *cov);
+
+/* Turbulence OTF of each subaperture of an nxsa x nxsa grid of step dx
+ * (genotf, lib/genotf.c:230-300): npsf x npsf per subaperture, peak in the
+ * centre, OTF(0) = 1.  amp is nsa*nxsa^2 (NULL: 1); area nsa (NULL: all
+ * computed), subapertures above thres share the largest one's OTF, those
+ * at or below 0.01 are zero.  cov (nxsa^2 x nxsa^2) replaces the von Karman
+ * covariance of r0, L0 when not NULL.  pttr removes the amp-weighted
+ * piston/tip/tilt from both sides of the covariance. */
+int mxs_genotf(double *otf, long nxsa, double dx, const double *amp, const double *area, long nsa,
+               double thres, double wvl, const double *cov, double r0, double L0, long npsf, int pttr);
+
+/* Short-exposure PSF of each subaperture (gensepsf, lib/gensei.c:136-206):
+ * otf nsa*npsf^2 (peak in the centre) times conj(lotf) (nlotf^2, centred
+ * embed; NULL: none), inverse FFT at npsf, times saa/npsf^2 (saa NULL: 1),
+ * then cropped or zero-padded, centred, to psf nsa*notf^2 (real, peak in
+ * the centre). */
+int mxs_gensepsf(double *psf, const double *otf, long npsf, long nsa, const double *lotf, long nlotf,
+                 const double *saa, long notf);
+
+/* Adds one wavelength's model i0, gx, gy (nsa*npix each; any may be NULL)
+ * for sepsf psf (gensei, lib/gensei.c:213-419): OTF of the PSF, optionally
+ * re-centred by its thresholded CoG (shift2center, powfs.mtchstc), times the
+ * ETF (etf nsa*notf^2 complex, nominal fused) or the nominal (notf^2 real),
+ * inverse FFT, sampled by si (CSC, as mxs_si) with weight wvlsig; gx, gy
+ * use the derivative factors -2 pi i f/(dtheta notf).  i0scale rescales
+ * each subaperture's sums to saa. */
+int mxs_gensei(double *i0, double *gx, double *gy, const double *psf, long nsa, long notf,
+               const double *etf, const double *nominal, long npix, const long *pp, const long *pi,
+               const double *px, double dtheta, double wvlsig, int shift2center, int i0scale,
+               const double *saa);
 /* MAOS .bin files (sys/bin.c:693-800, sys/bin.h:39-84): header = optional
  * comment block, M_SKIP, magic, nx, ny (64-bit), then nx*ny elements, x
  * fastest; ".gz" names are gzip-compressed (zfopen, sys/bin.c:301).  Every

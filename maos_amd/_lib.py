@@ -151,6 +151,15 @@ SETUP_API = {
                            c_double_p, c_double_p, C.c_long, C.c_long, C.c_long,
                            C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_int, C.c_double]),
+    "mxs_turbcov": (C.c_int, [c_double_p, C.c_long, C.c_double, C.c_double, c_double_p]),
+    "mxs_genotf": (C.c_int, [c_double_p, C.c_long, C.c_double, c_double_p, c_double_p, C.c_long,
+                             C.c_double, C.c_double, c_double_p, C.c_double, C.c_double, C.c_long,
+                             C.c_int]),
+    "mxs_gensepsf": (C.c_int, [c_double_p, c_double_p, C.c_long, C.c_long, c_double_p, C.c_long,
+                               c_double_p, C.c_long]),
+    "mxs_gensei": (C.c_int, [c_double_p, c_double_p, c_double_p, c_double_p, C.c_long, C.c_long,
+                             c_double_p, c_double_p, C.c_long, c_long_p, c_long_p, c_double_p,
+                             C.c_double, C.c_double, C.c_int, C.c_int, c_double_p]),
 }
 
 

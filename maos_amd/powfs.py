@@ -212,6 +212,55 @@ class Powfs:
             self.llt_origin = org.value
         self.sigsim = cfg.siglev * (sim.dt / sim.dtref) * math.cos(sim.za)
 
+    # -------------------------------------------------------------- i0 model
+    def i0_model(self, r0, L0, shift2center=None, i0scale=False):
+        """Model i0, gx, gy [nsa, npix] from the turbulence-averaged
+        short-exposure OTF, as MAOS builds them for the matched filter at
+        setup (maos/powfs.c:1545-1622): genseotf of the subapertures and of
+        the LLT pupil (piston/tip/tilt removed, sampled at nxsa*embfac),
+        gensepsf with the uplink OTF, gensei with the ETF (nominal fused) or
+        the nominal.  r0, L0: the powfs's (MAOS defaults them to the
+        atmosphere's).  shift2center (powfs.mtchstc) defaults to on without
+        an LLT and off with one, as MAOS's SH NGS and LGS configurations do.
+        The images are in the units of the simulated ones (wvlwts * sigsim
+        photons)."""
+        S = _lib.setup()
+        cfg = self.cfg
+        N, nsa, npix = self.notf, self.nsa, self.npix
+        if shift2center is None:
+            shift2center = self.llt is None
+        npsf = self.nxsa * cfg.embfac
+        thres = 1.0 - 1e-10
+        i0 = np.zeros(nsa * npix)
+        gx = np.zeros(nsa * npix)
+        gy = np.zeros(nsa * npix)
+        for iw in range(self.nwvl):
+            otf = np.zeros(nsa * npsf * npsf * 2)
+            rc = S.mxs_genotf(dptr(otf), self.nxsa, self.dx, dptr(self.amp), dptr(self.saa), nsa, thres,
+                              self.wvl[iw], None, r0, L0, npsf, 1)
+            if rc != 0:
+                raise RuntimeError("genotf failed (%d)" % rc)
+            lotf, nl = None, 0
+            if self.llt is not None:
+                nl = self.llt_nx * cfg.embfac
+                lotf = np.zeros(nl * nl * 2)
+                rc = S.mxs_genotf(dptr(lotf), self.llt_nx, self.dx, dptr(self.llt_amp), None, 1, thres,
+                                  self.wvl[iw], None, r0, L0, nl, 1)
+                if rc != 0:
+                    raise RuntimeError("genotf (LLT) failed (%d)" % rc)
+            psf = np.zeros(nsa * N * N)
+            rc = S.mxs_gensepsf(dptr(psf), dptr(otf), npsf, nsa, dptr(lotf), nl, dptr(self.saa), N)
+            if rc != 0:
+                raise RuntimeError("gensepsf failed (%d)" % rc)
+            etf = None if self.etf is None else self.etf[iw]
+            rc = S.mxs_gensei(dptr(i0), dptr(gx), dptr(gy), dptr(psf), nsa, N, dptr(etf),
+                              dptr(self.nominal[iw]), npix, lptr(self.si_pp[iw]), lptr(self.si_pi[iw]),
+                              dptr(self.si_px[iw]), self.dtheta[iw], self.wvlwts[iw] * self.sigsim,
+                              int(shift2center), int(i0scale), dptr(self.saa))
+            if rc != 0:
+                raise RuntimeError("gensei failed (%d)" % rc)
+        return i0.reshape(nsa, npix), gx.reshape(nsa, npix), gy.reshape(nsa, npix)
+
     # -------------------------------------------------------------- MF
     def mtch(self, i0, gx=None, gy=None):
         """Matched filter from averaged images i0 [nsa, npix] (mtch_cell,
