@@ -174,6 +174,44 @@ int mx_wfsgrad_i0_control(mx_wfsgrad_t *h, int enable, int reset);
 int mx_wfsgrad_i0_get(mx_wfsgrad_t *h, int iwfs, double *i0sum_out,
                       long *nframe);
 
+/* TT-dither statistics for the matched-filter update (maos/wfsgrad.c:
+ * 449-466).  With enable=1 every later step adds, per WFS and pixel, ints,
+ * cs*ints and ss*ints (FP64) to the sums imb, imx, imy; (cs, ss) is the
+ * WFS's dither phase for the step (dither_position, maos/utils.c:884-899;
+ * mxs_dither_position computes it), set by mx_wfsgrad_set_dither before
+ * the step is queued.  The first call allocates the sums and the model
+ * i0/gx/gy (zero); reset=1 clears the sums and the folded sums. */
+int mx_wfsgrad_dither_control(mx_wfsgrad_t *h, int enable, int reset);
+int mx_wfsgrad_set_dither(mx_wfsgrad_t *h, int iwfs, double cs, double ss);
+/* At a PLL output (maos/wfsgrad.c:709-731): folded i0 += s0*imb, folded
+ * gx += sg*imx, folded gy += sg*imy (MAOS: s0 = 1/pllrat, sg =
+ * 2/(pllrat*a2m)), then imb, imx, imy are cleared.  iwfs=-1: all WFS.
+ * Queued on the engine's stream. */
+int mx_wfsgrad_dither_fold(mx_wfsgrad_t *h, int iwfs, double s0, double sg);
+/* Model i0, gx, gy (nsa*npix each, NULL: zero) that the update blends
+ * into, e.g. the setup-time model of mxs_gensei (intstat->i0/gx/gy). */
+int mx_wfsgrad_set_i0model(mx_wfsgrad_t *h, int iwfs, const double *i0, const double *gx,
+                           const double *gy);
+/* Matched-filter update on the GPU (maos/wfsgrad.c:1036-1047 then
+ * mtch_cell, lib/mtch.c:223-335): model = keep*model + acc*folded (MAOS:
+ * keep = 1-g2, acc = scale1*g2) for i0, gx, gy, the folded sums cleared,
+ * then mtche and i0sum rebuilt in FP64 for the MF stage, with bkgrnd,
+ * bkgrndc, rne, mtchcr and sigratio as mxs_mtch takes them.  No host round
+ * trip; queued on the engine's stream.  iwfs=-1: all WFS.  This entry
+ * point has no single counterpart in the reference: it replaces the host
+ * update and the gpu_wfsgrad_update_ref upload (cuda/sim/wfsgrad.cu). */
+int mx_wfsgrad_mtch_update(mx_wfsgrad_t *h, int iwfs, double keep, double acc, double bkgrnd,
+                           double bkgrndc, double rne, int mtchcr, double sigratio);
+/* Copies of the per-step dither sums (nsa*npix each, any may be NULL) and
+ * the number of frames since the last reset. */
+int mx_wfsgrad_dither_get(mx_wfsgrad_t *h, int iwfs, double *imb, double *imx, double *imy,
+                          long *nframe);
+/* The MF stage's filter (nsa*2*npix, as mx_wfsgrad_update_ref takes it)
+ * and, after an update, its i0sum (nsa), NEA^2 (nsa*3: xx, yy, xy, as
+ * mxs_mtch's sanea) and the model i0, gx, gy; any may be NULL. */
+int mx_wfsgrad_mtch_get(mx_wfsgrad_t *h, int iwfs, double *mtche, double *i0sum, double *sanea,
+                        double *i0, double *gx, double *gy);
+
 /* Debug/parity outputs of the last step: the WFS OPD (nsa*nxsa*nxsa, m)
  * and the uplink OTF lotfc of wavelength iwvl (notf*notf complex, as
  * wfsints.c:149-199 leaves it before conjugation). */

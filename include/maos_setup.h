@@ -114,6 +114,12 @@ int mxs_mtch(double *mtche, double *sanea, double *i0sum, double *i0sumsum,
              double sigratio);
 
 
+/* Dither phase of a WFS for step isim, averaged over the integration
+ * (dither_position, maos/utils.c:884-899): alfsm the FSM delay, dtrat the
+ * WFS frame ratio, npoint steps per dither period, deltam the PLL phase. */
+void mxs_dither_position(double *cs, double *ss, int alfsm, int dtrat, int npoint, long isim,
+                         double deltam);
+
 /* Model i0 of a powfs from the turbulence-averaged short-exposure OTF
  * (genseotf -> gensepsf -> gensei, lib/gensei.c:42-419 and
  * lib/genotf.c:50-300, as maos/powfs.c:1545-1622 builds it for the matched

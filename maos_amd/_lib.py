@@ -88,6 +88,16 @@ HIP_API = {
     "mx_wfsgrad_sync": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_wfsgrad_i0_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mx_wfsgrad_i0_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_long_p]),
+    "mx_wfsgrad_dither_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "mx_wfsgrad_set_dither": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
+    "mx_wfsgrad_dither_fold": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
+    "mx_wfsgrad_set_i0model": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_double_p]),
+    "mx_wfsgrad_mtch_update": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_int, C.c_double]),
+    "mx_wfsgrad_dither_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_double_p,
+                                        c_long_p]),
+    "mx_wfsgrad_mtch_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, c_double_p,
+                                      c_double_p, c_double_p, c_double_p]),
     "mx_wfsgrad_debug_opd": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "mx_wfsgrad_debug_lotf": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p]),
     "mx_wfsgrad_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
@@ -152,6 +162,8 @@ SETUP_API = {
                            C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_int, C.c_double]),
     "mxs_turbcov": (C.c_int, [c_double_p, C.c_long, C.c_double, C.c_double, c_double_p]),
+    "mxs_dither_position": (None, [c_double_p, c_double_p, C.c_int, C.c_int, C.c_int, C.c_long,
+                                   C.c_double]),
     "mxs_genotf": (C.c_int, [c_double_p, C.c_long, C.c_double, c_double_p, c_double_p, C.c_long,
                              C.c_double, C.c_double, c_double_p, C.c_double, C.c_double, C.c_long,
                              C.c_int]),

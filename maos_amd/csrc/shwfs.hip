@@ -155,6 +155,22 @@ struct DevWvl {
 // WFS*nsa + subaperture, step) and the 64-bit seed as key: one stream per
 // pixel and step, independent of launch order.  Normals are Box-Muller (one
 // per pair of uniforms).  Poisson arithmetic is FP64.
+// Dither statistics of the TT-dithered matched-filter update
+// (maos/wfsgrad.c:449-466): each step adds ints, cs ints and ss ints (FP64)
+// to imb, imx, imy, with (cs, ss) of the WFS's dither phase for the step
+// (dither_position, maos/utils.c:884-899, on the host).
+struct DitherAcc {
+    double *imb, *imx, *imy; // [nwfs][nsa][npix], or imb null when off
+    const double *cs;        // [nwfs][2]: cs, ss
+};
+__device__ __forceinline__ void dither_add(const DitherAcc &d, int iwfs, long i, float v) {
+    if (!d.imb) return;
+    const double x = (double)v;
+    d.imb[i] += x;
+    d.imx[i] += d.cs[2 * iwfs] * x;
+    d.imy[i] += d.cs[2 * iwfs + 1] * x;
+}
+
 struct NoiseArgs {
     int on;
     float bkgrnd, bkgrndc, rne;
@@ -246,6 +262,7 @@ struct KArgs {
     float *ints;     // [nwfs][nsa][npix]
     float *grad;     // [nwfs][2*nsa]
     double *i0acc;   // [nwfs][nsa][npix] or null
+    DitherAcc dith;
     NoiseArgs noise;
 };
 
@@ -778,6 +795,7 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
             if (valid) {
                 a.ints[ibase + p] = pv[j];
                 if (a.i0acc) a.i0acc[ibase + p] += (double)pv[j];
+                dither_add(a.dith, iwfs, ibase + p, pv[j]);
             }
             if (a.phytype == MX_PTYPE_MF) {
                 const float *m = a.mtche + ibase * 2 + 2 * p;
@@ -908,6 +926,259 @@ __global__ __launch_bounds__(256) void k_opd_add(OpdAddArgs a) {
             v += (float)(t[0] + t[1] * x + t[2] * y);
         }
         a.opd[i] = v;
+    }
+}
+
+// ------------------------------------------------------------ dither fold
+// At each PLL output of the TT dither (maos/wfsgrad.c:709-731): the folded
+// sums take fi0 += s0 imb, fgx += sg imx, fgy += sg imy, and the per-step
+// sums are cleared.
+struct FoldArgs {
+    long n;
+    double s0, sg;
+    double *imb, *imx, *imy, *fi0, *fgx, *fgy;
+};
+__global__ __launch_bounds__(256) void k_dither_fold(FoldArgs a) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += (long)gridDim.x * blockDim.x) {
+        a.fi0[i] += a.s0 * a.imb[i];
+        a.fgx[i] += a.sg * a.imx[i];
+        a.fgy[i] += a.sg * a.imy[i];
+        a.imb[i] = 0.;
+        a.imx[i] = 0.;
+        a.imy[i] = 0.;
+    }
+}
+
+// ------------------------------------------------------------ MF update
+// Matched filter rebuilt on the GPU (FP64) from the model i0, gx, gy after
+// the low-pass blend of maos/wfsgrad.c:1036-1047, i0 = keep i0 + acc fi0
+// (gx, gy likewise; the folded sums are cleared), following mtch
+// (lib/mtch.c:93-211): columns [gx, gy, i0/pixtheta + 2 bkgrnd_res] and,
+// with the constraints, i0 shifted by one pixel either way in x and/or y
+// (+ bkgrnd_res); weights 1/(rne^2 + bkgrnd + i0) (1/rne^2 where i0 < 0);
+// mtche = i0m pinv(A, W) with the eigenvalue thresholds of dpinv (1e-7 of
+// the largest, and 1e-3 of the previous kept); NEA^2 = mtche W^-1 mtche^T.
+// One block per (WFS, subaperture); the per-WFS thresholds of mtch_cell
+// (lib/mtch.c:300-318) follow in k_mtch_fin.
+constexpr int MX_MTCH_THREADS = 256;
+struct MtchArgs {
+    int nsa, npix, pixpsax, pixpsay, iwfs0;
+    int crx, cry; // column of the x, y constraint pairs, or 0
+    int nmod;
+    double keep, acc;
+    double bkres, bkgrnd, rne2, kpx, kpy, sigratio, nea2thres, rne;
+    double *i0, *gx, *gy;    // model [nwfs][nsa][npix], updated in place
+    double *fi0, *fgx, *fgy; // folded sums, cleared
+    double *nea;             // [nwfs][nsa][4]
+    double *i0sum;           // [nwfs][nsa]
+    float *mtche;            // [nwfs][nsa][npix][2]
+    float *i0sumf;           // [nwfs][nsa]
+};
+
+// column k of the mtch design matrix at pixel p (mki0sh, lib/mtch.c:23-33)
+__device__ __forceinline__ double mtch_col(const MtchArgs &a, const double *i0, const double *gx,
+                                          const double *gy, int p, int k) {
+    if (k == 0) return gx[p];
+    if (k == 1) return gy[p];
+    if (k == 2) return i0[p] * a.kpx + 2. * a.bkres;
+    const int ix = p % a.pixpsax, iy = p / a.pixpsax;
+    if (a.crx && (k == a.crx || k == a.crx + 1)) {
+        // i0x1 = i0 shifted +1 in x, i0x2 = shifted -1 (zero where outside)
+        if (k == a.crx) return (ix >= 1 ? i0[p - 1] * a.kpx : 0.) + a.bkres;
+        return (ix + 1 < a.pixpsax ? i0[p + 1] * a.kpx : 0.) + a.bkres;
+    }
+    if (k == a.cry) return (iy >= 1 ? i0[p - a.pixpsax] * a.kpy : 0.) + a.bkres;
+    return (iy + 1 < a.pixpsay ? i0[p + a.pixpsax] * a.kpy : 0.) + a.bkres;
+}
+
+// symmetric eigen-decomposition by cyclic Jacobi rotations (n <= 7)
+__device__ void jacobi_eig_dev(double *A, int n, double *d, double *V) {
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) V[i + j * n] = (i == j);
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) off += A[p + q * n] * A[p + q * n];
+        if (off < 1e-300) break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[p + q * n];
+                if (fabs(apq) < 1e-300) continue;
+                const double app = A[p + p * n], aqq = A[q + q * n];
+                const double tau = (aqq - app) / (2 * apq);
+                const double t = (tau >= 0 ? 1. : -1.) / (fabs(tau) + sqrt(1 + tau * tau));
+                const double c = 1. / sqrt(1 + t * t), s = t * c;
+                for (int k = 0; k < n; k++) {
+                    const double akp = A[k + p * n], akq = A[k + q * n];
+                    A[k + p * n] = c * akp - s * akq;
+                    A[k + q * n] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double apk = A[p + k * n], aqk = A[q + k * n];
+                    A[p + k * n] = c * apk - s * aqk;
+                    A[q + k * n] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {
+                    const double vkp = V[k + p * n], vkq = V[k + q * n];
+                    V[k + p * n] = c * vkp - s * vkq;
+                    V[k + q * n] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) d[i] = A[i + i * n];
+}
+
+// block sum of v (all threads get it)
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int w = threadIdx.x / 64;
+    __syncthreads(); // red free
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < NT / 64; i++) s += red[i];
+    return s;
+}
+
+__global__ __launch_bounds__(MX_MTCH_THREADS) void k_mtch(MtchArgs a) {
+    __shared__ double red[MX_MTCH_THREADS / 64];
+    __shared__ double inv[49];
+    const int isa = (int)(blockIdx.x % (unsigned)a.nsa), iwfs = a.iwfs0 + (int)(blockIdx.x / (unsigned)a.nsa);
+    const long base = ((long)iwfs * a.nsa + isa) * a.npix;
+    double *i0 = a.i0 + base, *gx = a.gx + base, *gy = a.gy + base;
+    // the low-pass blend of the folded sums into the model
+    for (int p = threadIdx.x; p < a.npix; p += MX_MTCH_THREADS) {
+        i0[p] = a.keep * i0[p] + a.acc * a.fi0[base + p];
+        gx[p] = a.keep * gx[p] + a.acc * a.fgx[base + p];
+        gy[p] = a.keep * gy[p] + a.acc * a.fgy[base + p];
+        a.fi0[base + p] = 0.;
+        a.fgx[base + p] = 0.;
+        a.fgy[base + p] = 0.;
+    }
+    __syncthreads();
+    const int nm = a.nmod;
+    // normal matrix A^T W A (upper triangle) and the i0 sum
+    double cc[28];
+    for (int k = 0; k < 28; k++) cc[k] = 0.;
+    double s0 = 0.;
+    for (int p = threadIdx.x; p < a.npix; p += MX_MTCH_THREADS) {
+        const double w = i0[p] < 0 ? 1. / a.rne2 : 1. / (a.rne2 + a.bkgrnd + i0[p]);
+        double col[7];
+        for (int k = 0; k < nm; k++) col[k] = mtch_col(a, i0, gx, gy, p, k);
+        int t = 0;
+        for (int i = 0; i < nm; i++)
+            for (int j = i; j < nm; j++) cc[t++] += col[i] * w * col[j];
+        s0 += i0[p];
+    }
+    const int ncc = nm * (nm + 1) / 2;
+    for (int k = 0; k < ncc; k++) cc[k] = block_sum<MX_MTCH_THREADS>(cc[k], red);
+    s0 = block_sum<MX_MTCH_THREADS>(s0, red);
+    if (threadIdx.x == 0) {
+        double A[49], d[7], V[49];
+        int t = 0;
+        for (int i = 0; i < nm; i++)
+            for (int j = i; j < nm; j++) {
+                A[i + j * nm] = cc[t];
+                A[j + i * nm] = cc[t];
+                t++;
+            }
+        jacobi_eig_dev(A, nm, d, V);
+        int idx[7];
+        for (int i = 0; i < nm; i++) idx[i] = i;
+        for (int i = 0; i < nm; i++)
+            for (int j = i + 1; j < nm; j++)
+                if (d[idx[j]] > d[idx[i]]) {
+                    const int tt = idx[i];
+                    idx[i] = idx[j];
+                    idx[j] = tt;
+                }
+        double sv[7];
+        const double maxeig = d[idx[0]];
+        const double thres10 = 1e-7 * maxeig;
+        double thres20 = 1e-3 * maxeig;
+        bool cut = false;
+        for (int i = 0; i < nm; i++) {
+            const double ev = d[idx[i]];
+            if (!cut && ev > thres10 && ev > thres20) {
+                thres20 = ev * 1e-3;
+                sv[i] = 1. / ev;
+            } else {
+                cut = true;
+                sv[i] = 0.;
+            }
+        }
+        for (int i = 0; i < nm; i++)
+            for (int j = 0; j < nm; j++) {
+                double acc = 0;
+                for (int k = 0; k < nm; k++) acc += V[i + idx[k] * nm] * sv[k] * V[j + idx[k] * nm];
+                inv[i + j * nm] = acc;
+            }
+    }
+    __syncthreads();
+    // mtche = i0m inv A^T W; NEA^2 = mtche W^-1 mtche^T
+    double n00 = 0, n01 = 0, n11 = 0;
+    for (int p = threadIdx.x; p < a.npix; p += MX_MTCH_THREADS) {
+        const double w = i0[p] < 0 ? 1. / a.rne2 : 1. / (a.rne2 + a.bkgrnd + i0[p]);
+        double col[7];
+        for (int k = 0; k < nm; k++) col[k] = mtch_col(a, i0, gx, gy, p, k);
+        double m[2] = {0., 0.};
+        for (int i = 0; i < nm; i++) {
+            double ti = 0;
+            for (int j = 0; j < nm; j++) ti += inv[i + j * nm] * col[j];
+            ti *= w;
+            // i0m: rows (gx | +x, -x), (gy | +y, -y)
+            if (i == 0 || (a.crx && i == a.crx)) m[0] += ti;
+            if (a.crx && i == a.crx + 1) m[0] -= ti;
+            if (i == 1 || (a.cry && i == a.cry)) m[1] += ti;
+            if (a.cry && i == a.cry + 1) m[1] -= ti;
+        }
+        a.mtche[(base + p) * 2] = (float)m[0];
+        a.mtche[(base + p) * 2 + 1] = (float)m[1];
+        n00 += m[0] * m[0] / w;
+        n01 += m[0] * m[1] / w;
+        n11 += m[1] * m[1] / w;
+    }
+    n00 = block_sum<MX_MTCH_THREADS>(n00, red);
+    n01 = block_sum<MX_MTCH_THREADS>(n01, red);
+    n11 = block_sum<MX_MTCH_THREADS>(n11, red);
+    if (threadIdx.x == 0) {
+        const long u = (long)iwfs * a.nsa + isa;
+        const double r = fabs(a.sigratio - 1.) > 1e-5 ? 1. / a.sigratio : 1.;
+        a.nea[4 * u] = n00 * r;
+        a.nea[4 * u + 1] = n01 * r;
+        a.nea[4 * u + 2] = n01 * r;
+        a.nea[4 * u + 3] = n11 * r;
+        a.i0sum[u] = s0;
+        a.i0sumf[u] = (float)s0;
+    }
+}
+
+// per WFS: subapertures with too little signal (i0sum below max(0.1 of the
+// largest, 10 rne)) or too large an NEA (above 100 pixtheta^2) get a zero
+// filter and NEA^2 = nea2thres (mtch_cell, lib/mtch.c:300-318)
+__global__ __launch_bounds__(MX_MTCH_THREADS) void k_mtch_fin(MtchArgs a) {
+    __shared__ double red[MX_MTCH_THREADS / 64];
+    const int iwfs = a.iwfs0 + (int)blockIdx.x;
+    const long u0 = (long)iwfs * a.nsa;
+    double mx = -INFINITY;
+    for (int isa = threadIdx.x; isa < a.nsa; isa += MX_MTCH_THREADS) mx = fmax(mx, a.i0sum[u0 + isa]);
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = mx;
+    __syncthreads();
+    mx = red[0];
+    for (int i = 1; i < MX_MTCH_THREADS / 64; i++) mx = fmax(mx, red[i]);
+    const double i0thres = mx * 0.1 > 10 * a.rne ? mx * 0.1 : 10 * a.rne;
+    for (int isa = 0; isa < a.nsa; isa++) {
+        const long u = u0 + isa;
+        const bool zero = a.i0sum[u] < i0thres || a.nea[4 * u] > a.nea2thres || a.nea[4 * u + 3] > a.nea2thres;
+        if (!zero) continue; // uniform: every thread reads the same values
+        for (int p = threadIdx.x; p < 2 * a.npix; p += MX_MTCH_THREADS) a.mtche[u * 2 * a.npix + p] = 0.f;
+        __syncthreads(); // all threads have read nea before thread 0 changes it
+        if (threadIdx.x == 0) {
+            a.nea[4 * u] = a.nea[4 * u + 3] = a.nea2thres;
+            a.nea[4 * u + 1] = a.nea[4 * u + 2] = 0.;
+        }
     }
 }
 
@@ -1190,6 +1461,7 @@ struct MfmaArgs {
     float saat;
     float *ints, *grad;
     double *i0acc;
+    DitherAcc dith;
     NoiseArgs noise;
     // uplink mode (one workgroup per WFS and wavelength): LLT OPD
     // [nwfs][32*32] and amplitude, 1/nlwvf^2, and the two outputs conj(lotfc)
@@ -1552,6 +1824,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
             a.ints[ibase + p] = pv;
             if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
+            dither_add(a.dith, iwfs, ibase + p, pv);
             if (a.phytype == MX_PTYPE_MF) {
                 const float *m = a.mtche + ibase * 2 + 2 * p;
                 s0 = m[0] * pv;
@@ -2020,6 +2293,7 @@ struct LfSampleArgs {
     float saat;
     float *ints, *grad;
     double *i0acc;
+    DitherAcc dith;
     NoiseArgs noise;
 };
 __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
@@ -2039,6 +2313,7 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
         if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
         a.ints[ibase + p] = pv;
         if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
+        dither_add(a.dith, iwfs, ibase + p, pv);
         if (a.phytype == MX_PTYPE_MF) {
             const float *mt = a.mtche + ibase * 2 + 2 * p;
             s0 += mt[0] * pv;
@@ -2201,6 +2476,16 @@ struct mx_wfsgrad_s {
     OpdAddArgs arg_opdadd{};
     ZtiltArgs arg_ztilt{};
     DmArgs arg_dm{};
+    // TT-dither statistics and the on-GPU matched-filter update
+    double *d_imb = nullptr, *d_imx = nullptr, *d_imy = nullptr; // per-step sums
+    double *d_fi0 = nullptr, *d_fgx = nullptr, *d_fgy = nullptr; // folded at PLL outputs
+    double *d_mi0 = nullptr, *d_mgx = nullptr, *d_mgy = nullptr; // model i0, gx, gy
+    double *d_nea = nullptr, *d_i0s64 = nullptr, *d_dcs = nullptr;
+    std::vector<double> h_dcs;
+    bool dith_on = false, dith_dirty = false;
+    long dith_nframe = 0;
+    FoldArgs arg_fold{};
+    MtchArgs arg_mtch{};
     bool queued = false;
     // timing
     bool timing = false;
@@ -2676,7 +2961,9 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
                     h->d_lotf_t, h->d_F, h->d_F16, h->d_mwv, h->d_imcc, h->d_opdadd, h->d_ptt, h->d_lf_tw, h->d_b1, h->d_b2, h->d_b3, h->d_psf, h->d_img,
                     h->d_lfw,
-                    h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc};
+                    h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc,
+                    h->d_imb, h->d_imx, h->d_imy, h->d_fi0, h->d_fgx, h->d_fgy, h->d_mi0, h->d_mgx, h->d_mgy,
+                    h->d_nea, h->d_i0s64, h->d_dcs};
     for (void *q : ptrs)
         if (q) hipFree(q);
     for (int w = 0; w < MX_MAXWVL; w++) {
@@ -3136,6 +3423,12 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     a.ints = h->d_ints;
     a.grad = h->d_grad;
     a.i0acc = h->i0_enable ? h->d_i0acc : nullptr;
+    a.dith = h->dith_on ? DitherAcc{h->d_imb, h->d_imx, h->d_imy, h->d_dcs} : DitherAcc{};
+    if (h->dith_on && h->dith_dirty) {
+        HIPCHK(hipMemcpyAsync(h->d_dcs, h->h_dcs.data(), sizeof(double) * h->h_dcs.size(), hipMemcpyHostToDevice,
+                              h->stream));
+        h->dith_dirty = false;
+    }
     a.noise = h->noise;
     a.noise.isim = (unsigned)isim;
     a.opd = h->d_opd;
@@ -3184,6 +3477,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         m.ints = a.ints;
         m.grad = a.grad;
         m.i0acc = a.i0acc;
+        m.dith = a.dith;
         m.noise = a.noise;
         m.llt_opd = h->d_llt_opd;
         m.llt_amp = h->d_llt_amp;
@@ -3417,6 +3711,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         g.ints = a.ints;
         g.grad = a.grad;
         g.i0acc = a.i0acc;
+        g.dith = a.dith;
         g.noise = a.noise;
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)),
                    dim3((unsigned)std::min(h->lf_threads, 256)), &g};
@@ -3522,6 +3817,7 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
 // bookkeeping after a step was queued
 void step_queued(mx_wfsgrad_t *h) {
     if (h->i0_enable) h->i0_nframe++;
+    if (h->dith_on) h->dith_nframe++;
     h->queued = true;
 }
 } // namespace
@@ -3695,6 +3991,184 @@ int mx_wfsgrad_i0_get(mx_wfsgrad_t *h, int iwfs, double *out, long *nframe) {
     return MX_OK;
 }
 
+int mx_wfsgrad_dither_control(mx_wfsgrad_t *h, int enable, int reset) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(h->device));
+    const size_t n = (size_t)h->nwfs * h->nsa * h->npix;
+    double **bufs[] = {&h->d_imb, &h->d_imx, &h->d_imy, &h->d_fi0, &h->d_fgx, &h->d_fgy,
+                       &h->d_mi0, &h->d_mgx, &h->d_mgy};
+    if (!h->d_imb) {
+        int rc;
+        for (double **b : bufs)
+            if ((rc = dev_alloc(b, n))) return rc;
+        if ((rc = dev_alloc(&h->d_nea, (size_t)4 * h->nwfs * h->nsa))) return rc;
+        if ((rc = dev_alloc(&h->d_i0s64, (size_t)h->nwfs * h->nsa))) return rc;
+        if ((rc = dev_alloc(&h->d_dcs, (size_t)2 * h->nwfs))) return rc;
+        h->h_dcs.assign(2 * (size_t)h->nwfs, 0.);
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (double **b : bufs) HIPCHK(hipMemset(*b, 0, sizeof(double) * n));
+        HIPCHK(hipMemset(h->d_dcs, 0, sizeof(double) * 2 * h->nwfs));
+        h->dith_nframe = 0;
+    } else if (reset) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int k = 0; k < 6; k++) HIPCHK(hipMemset(*bufs[k], 0, sizeof(double) * n));
+        h->dith_nframe = 0;
+    }
+    h->dith_on = enable != 0;
+    if (h->dith_on) h->dith_dirty = true;
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_dither(mx_wfsgrad_t *h, int iwfs, double cs, double ss) {
+    if (!h || iwfs < -1 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_set_dither: bad handle or iwfs");
+    if (!h->d_imb) return set_err(MX_ESTATE, "dither statistics not set up (mx_wfsgrad_dither_control)");
+    if (!std::isfinite(cs) || !std::isfinite(ss)) return set_err(MX_EINVAL, "dither weights %g %g", cs, ss);
+    const int w0 = iwfs < 0 ? 0 : iwfs, w1 = iwfs < 0 ? h->nwfs : iwfs + 1;
+    for (int w = w0; w < w1; w++) {
+        h->h_dcs[2 * w] = cs;
+        h->h_dcs[2 * w + 1] = ss;
+    }
+    h->dith_dirty = true;
+    return MX_OK;
+}
+
+int mx_wfsgrad_dither_fold(mx_wfsgrad_t *h, int iwfs, double s0, double sg) {
+    if (!h || iwfs < -1 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_dither_fold: bad handle or iwfs");
+    if (!h->d_imb) return set_err(MX_ESTATE, "dither statistics not set up (mx_wfsgrad_dither_control)");
+    HIPCHK(hipSetDevice(h->device));
+    const long per = h->nsa * h->npix, off = iwfs < 0 ? 0 : (long)iwfs * per;
+    FoldArgs &f = h->arg_fold;
+    f = FoldArgs{};
+    f.n = iwfs < 0 ? per * h->nwfs : per;
+    f.s0 = s0;
+    f.sg = sg;
+    f.imb = h->d_imb + off;
+    f.imx = h->d_imx + off;
+    f.imy = h->d_imy + off;
+    f.fi0 = h->d_fi0 + off;
+    f.fgx = h->d_fgx + off;
+    f.fgy = h->d_fgy + off;
+    const unsigned nb = (unsigned)std::min<long>((f.n + 255) / 256, 4096);
+    k_dither_fold<<<nb, 256, 0, h->stream>>>(f);
+    HIPCHK(hipGetLastError());
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_i0model(mx_wfsgrad_t *h, int iwfs, const double *i0, const double *gx, const double *gy) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_set_i0model: bad handle or iwfs");
+    if (!h->d_imb) return set_err(MX_ESTATE, "dither statistics not set up (mx_wfsgrad_dither_control)");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long n = h->nsa * h->npix;
+    const double *src[3] = {i0, gx, gy};
+    double *dst[3] = {h->d_mi0, h->d_mgx, h->d_mgy};
+    for (int k = 0; k < 3; k++) {
+        if (src[k])
+            HIPCHK(hipMemcpy(dst[k] + (long)iwfs * n, src[k], sizeof(double) * n, hipMemcpyHostToDevice));
+        else
+            HIPCHK(hipMemset(dst[k] + (long)iwfs * n, 0, sizeof(double) * n));
+    }
+    return MX_OK;
+}
+
+int mx_wfsgrad_mtch_update(mx_wfsgrad_t *h, int iwfs, double keep, double acc, double bkgrnd, double bkgrndc,
+                           double rne, int mtchcr, double sigratio) {
+    if (!h || iwfs < -1 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_mtch_update: bad handle or iwfs");
+    if (!h->d_imb) return set_err(MX_ESTATE, "dither statistics not set up (mx_wfsgrad_dither_control)");
+    if (mtchcr < 0 || mtchcr > 3 || !(rne >= 0.) || !(bkgrnd >= 0.) || !std::isfinite(keep) || !std::isfinite(acc))
+        return set_err(MX_EINVAL, "mtch_update: mtchcr=%d rne=%g bkgrnd=%g", mtchcr, rne, bkgrnd);
+    if (h->npix > 64 * MX_MTCH_THREADS) return set_err(MX_EINVAL, "mtch_update: npix=%ld", h->npix);
+    HIPCHK(hipSetDevice(h->device));
+    MtchArgs &m = h->arg_mtch;
+    m = MtchArgs{};
+    m.nsa = (int)h->nsa;
+    m.npix = (int)h->npix;
+    m.pixpsax = (int)h->cfg.pixpsax;
+    m.pixpsay = (int)h->cfg.pixpsay;
+    m.iwfs0 = iwfs < 0 ? 0 : iwfs;
+    m.nmod = 3;
+    if (mtchcr == 1 || mtchcr == 2) {
+        m.crx = m.nmod;
+        m.nmod += 2;
+    }
+    if (mtchcr == 1 || mtchcr == 3) {
+        m.cry = m.nmod;
+        m.nmod += 2;
+    }
+    m.keep = keep;
+    m.acc = acc;
+    m.bkres = bkgrnd - bkgrndc;
+    m.bkgrnd = bkgrnd;
+    m.rne = rne;
+    m.rne2 = rne * rne;
+    m.kpx = 1. / h->cfg.pixthetax;
+    m.kpy = 1. / h->cfg.pixthetay;
+    m.sigratio = sigratio ? sigratio : 1.;
+    m.nea2thres = h->cfg.pixthetax * h->cfg.pixthetay * 100.;
+    m.i0 = h->d_mi0;
+    m.gx = h->d_mgx;
+    m.gy = h->d_mgy;
+    m.fi0 = h->d_fi0;
+    m.fgx = h->d_fgx;
+    m.fgy = h->d_fgy;
+    m.nea = h->d_nea;
+    m.i0sum = h->d_i0s64;
+    m.mtche = h->d_mtche;
+    m.i0sumf = h->d_i0sum;
+    const unsigned nw = iwfs < 0 ? (unsigned)h->nwfs : 1u;
+    k_mtch<<<nw * (unsigned)h->nsa, MX_MTCH_THREADS, 0, h->stream>>>(m);
+    HIPCHK(hipGetLastError());
+    k_mtch_fin<<<nw, MX_MTCH_THREADS, 0, h->stream>>>(m);
+    HIPCHK(hipGetLastError());
+    h->have_mtche = h->have_i0sum = true;
+    return MX_OK;
+}
+
+int mx_wfsgrad_dither_get(mx_wfsgrad_t *h, int iwfs, double *imb, double *imx, double *imy, long *nframe) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_dither_get: bad handle or iwfs");
+    if (!h->d_imb) return set_err(MX_ESTATE, "dither statistics not set up (mx_wfsgrad_dither_control)");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long n = h->nsa * h->npix;
+    double *out[3] = {imb, imx, imy};
+    const double *src[3] = {h->d_imb, h->d_imx, h->d_imy};
+    for (int k = 0; k < 3; k++)
+        if (out[k]) HIPCHK(hipMemcpy(out[k], src[k] + (long)iwfs * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (nframe) *nframe = h->dith_nframe;
+    return MX_OK;
+}
+
+int mx_wfsgrad_mtch_get(mx_wfsgrad_t *h, int iwfs, double *mtche, double *i0sum, double *sanea, double *i0,
+                        double *gx, double *gy) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_mtch_get: bad handle or iwfs");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const long n = h->nsa * h->npix;
+    if (mtche) {
+        std::vector<float> t(2 * n);
+        HIPCHK(hipMemcpy(t.data(), h->d_mtche + (long)iwfs * 2 * n, sizeof(float) * 2 * n, hipMemcpyDeviceToHost));
+        for (long i = 0; i < 2 * n; i++) mtche[i] = t[i];
+    }
+    if ((i0sum || sanea || i0 || gx || gy) && !h->d_imb)
+        return set_err(MX_ESTATE, "no GPU matched-filter update yet (mx_wfsgrad_dither_control)");
+    if (i0sum) HIPCHK(hipMemcpy(i0sum, h->d_i0s64 + (long)iwfs * h->nsa, sizeof(double) * h->nsa, hipMemcpyDeviceToHost));
+    if (sanea) {
+        std::vector<double> t(4 * h->nsa);
+        HIPCHK(hipMemcpy(t.data(), h->d_nea + (long)iwfs * 4 * h->nsa, sizeof(double) * 4 * h->nsa,
+                         hipMemcpyDeviceToHost));
+        for (long isa = 0; isa < h->nsa; isa++) {
+            sanea[isa] = t[4 * isa];
+            sanea[isa + h->nsa] = t[4 * isa + 3];
+            sanea[isa + 2 * h->nsa] = t[4 * isa + 1];
+        }
+    }
+    double *out[3] = {i0, gx, gy};
+    const double *src[3] = {h->d_mi0, h->d_mgx, h->d_mgy};
+    for (int k = 0; k < 3; k++)
+        if (out[k]) HIPCHK(hipMemcpy(out[k], src[k] + (long)iwfs * n, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return MX_OK;
+}
+
 int mx_wfsgrad_debug_opd(mx_wfsgrad_t *h, int iwfs, double *opd) {
     if (!h || iwfs < 0 || iwfs >= h->nwfs || !opd) return set_err(MX_EINVAL, "mx_wfsgrad_debug_opd: bad argument");
     HIPCHK(hipSetDevice(h->device));
@@ -3847,9 +4321,10 @@ int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep, char *r
         for (int t : tiles)
             for (int th : threads) cand.push_back({v, t, th, 0.});
     const int sv = h->variant, st = h->opd_tile, sth = h->lf_threads;
-    const bool si0 = h->i0_enable, stim = h->timing;
-    const long sn = h->i0_nframe;
-    h->i0_enable = false; // tuning steps leave the i0 sums alone
+    const bool si0 = h->i0_enable, stim = h->timing, sdi = h->dith_on;
+    const long sn = h->i0_nframe, sdn = h->dith_nframe;
+    h->i0_enable = false; // tuning steps leave the i0 and dither sums alone
+    h->dith_on = false;
     int rc = MX_OK;
     for (Cand &c : cand) {
         h->variant = c.variant;
@@ -3868,6 +4343,8 @@ int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep, char *r
     h->t_llt = h->t_opd = h->t_ints = 0;
     h->i0_enable = si0;
     h->i0_nframe = sn;
+    h->dith_on = sdi;
+    h->dith_nframe = sdn;
     if (rc) {
         h->variant = sv;
         h->opd_tile = st;

@@ -127,6 +127,59 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_i0_get(self.h, iwfs, dptr(out), C.cast(C.byref(n), _lib.c_long_p)))
         return out.reshape(self.nsa, self.npix), n.value
 
+    # TT-dither statistics and the on-GPU matched-filter update
+    def dither_control(self, enable=True, reset=False):
+        check(_lib.hip().mx_wfsgrad_dither_control(self.h, int(enable), int(reset)))
+
+    def set_dither(self, cs, ss, iwfs=-1):
+        """Dither weights of the next queued step (dither_position)."""
+        check(_lib.hip().mx_wfsgrad_set_dither(self.h, iwfs, float(cs), float(ss)))
+
+    def dither_fold(self, s0, sg, iwfs=-1):
+        check(_lib.hip().mx_wfsgrad_dither_fold(self.h, iwfs, float(s0), float(sg)))
+
+    def set_i0model(self, iwfs, i0=None, gx=None, gy=None):
+        def c(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        check(_lib.hip().mx_wfsgrad_set_i0model(self.h, iwfs, dptr(c(i0)), dptr(c(gx)), dptr(c(gy))))
+
+    def mtch_update(self, keep, acc, iwfs=-1, bkgrnd=None, bkgrndc=0.0, rne=None, mtchcr=None,
+                    sigratio=1.0):
+        """Blend the folded sums into the model i0/gx/gy and rebuild the
+        matched filter on the GPU; bkgrnd, rne and mtchcr default to the
+        powfs's (bkgrnd per frame)."""
+        cfg = self.powfs.cfg
+        bk = cfg.bkgrnd * cfg.dtrat if bkgrnd is None else bkgrnd
+        check(_lib.hip().mx_wfsgrad_mtch_update(self.h, iwfs, float(keep), float(acc), float(bk),
+                                                 float(bkgrndc), float(cfg.rne if rne is None else rne),
+                                                 int(cfg.mtchcr if mtchcr is None else mtchcr),
+                                                 float(sigratio)))
+
+    def dither_get(self, iwfs):
+        n = self.nsa * self.npix
+        imb, imx, imy = np.zeros(n), np.zeros(n), np.zeros(n)
+        nf = C.c_long(0)
+        check(_lib.hip().mx_wfsgrad_dither_get(self.h, iwfs, dptr(imb), dptr(imx), dptr(imy),
+                                                C.cast(C.byref(nf), _lib.c_long_p)))
+        s = (self.nsa, self.npix)
+        return imb.reshape(s), imx.reshape(s), imy.reshape(s), nf.value
+
+    def mtch_get(self, iwfs, model=True):
+        """(mtche [nsa, 2*npix], i0sum, sanea [3, nsa], i0, gx, gy) of the MF
+        stage; the last five only after a GPU update (model=True)."""
+        n = self.nsa * self.npix
+        m = np.zeros(2 * n)
+        if not model:
+            check(_lib.hip().mx_wfsgrad_mtch_get(self.h, iwfs, dptr(m), None, None, None, None, None))
+            return m.reshape(self.nsa, 2 * self.npix)
+        i0sum, sanea = np.zeros(self.nsa), np.zeros(3 * self.nsa)
+        i0, gx, gy = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(_lib.hip().mx_wfsgrad_mtch_get(self.h, iwfs, dptr(m), dptr(i0sum), dptr(sanea), dptr(i0),
+                                              dptr(gx), dptr(gy)))
+        s = (self.nsa, self.npix)
+        return (m.reshape(self.nsa, 2 * self.npix), i0sum, sanea.reshape(3, self.nsa), i0.reshape(s),
+                gx.reshape(s), gy.reshape(s))
+
     def debug_opd(self, iwfs):
         out = np.zeros(self.nsa * self.powfs.nxsa ** 2)
         check(_lib.hip().mx_wfsgrad_debug_opd(self.h, iwfs, dptr(out)))

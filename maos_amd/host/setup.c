@@ -846,3 +846,20 @@ int mxs_mtch(double *mtche, double *sanea, double *i0sum, double *i0sumsum,
     if (i0sumsum) *i0sumsum = sumsum;
     return 0;
 }
+
+/* dither_position (maos/utils.c:884-899): the dither phase at the start of
+ * the integration, shifted by the FSM delay (alfsm + 1 - dtrat frames),
+ * and the next one, blended by the fractional delay beta; the frame counts
+ * are integer divisions as there. */
+void mxs_dither_position(double *cs, double *ss, int alfsm, int dtrat, int npoint, long isim,
+                         double deltam) {
+    const long adjust = alfsm + 1 - dtrat, adjust2 = dtrat - 1;
+    const double dphi = 2. * M_PI / npoint, phi0 = 0.5 * M_PI;
+    const double a1 = phi0 + (double)((isim - adjust - adjust2) / dtrat) * dphi + deltam;
+    const double a2 = phi0 + (double)((isim - adjust) / dtrat) * dphi + deltam;
+    const double delay = (double)adjust / dtrat;
+    const double beta = 1 + delay + floor(-delay);
+    const double sc = 1. / (beta * beta + (1 - beta) * (1 - beta));
+    *cs = (beta * cos(a1) + (1 - beta) * cos(a2)) * sc;
+    *ss = (beta * sin(a1) + (1 - beta) * sin(a2)) * sc;
+}

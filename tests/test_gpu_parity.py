@@ -664,6 +664,89 @@ def test_screens_smaller_than_tile(ngs_two_wvl):
     eng.close()
 
 
+def test_dither_statistics_and_gpu_matched_filter(lgs_small):
+    """TT-dither statistics and the on-GPU matched-filter update:
+    - each step adds ints, cs ints and ss ints (FP64) per WFS; the sums
+      equal the host's FP64 sums of the synced images;
+    - a fold moves s0 imb, sg imx, sg imy into the folded sums and clears
+      the step sums;
+    - the update blends keep*model + acc*folded and rebuilds mtche, i0sum
+      and NEA^2 on the GPU, equal to the host mtch of the same blended
+      model (mxs_mtch, FP64) to FP32 rounding of the filter;
+    - the next step's gradients with that filter equal those with the host
+      filter uploaded by mx_wfsgrad_update_ref."""
+    cfg, p, layers = lgs_small
+    print("case: lgs order20 nsa=%d nwfs=%d, dither + GPU MF update" % (p.nsa, len(cfg["wfs"])), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    nw = len(wfs)
+    i0m, gxm, gym = p.i0_model(cfg["atm"].r0, cfg["atm"].L0)
+    mt0, is0, _ = p.mtch(i0m, gxm, gym)
+    eng.update_ref(mt0, is0)
+    eng.dither_control(True, True)
+    for iw in range(nw):
+        eng.set_i0model(iw, i0m, gxm, gym)
+    dt = cfg["sim"].dt
+    rng = np.random.default_rng(5)
+    n = p.nsa * p.npix
+    imb = np.zeros((nw, n))
+    imx = np.zeros((nw, n))
+    imy = np.zeros((nw, n))
+    for isim in range(3):
+        cs = rng.uniform(-1, 1, nw)
+        ss = rng.uniform(-1, 1, nw)
+        for iw in range(nw):
+            eng.set_dither(cs[iw], ss[iw], iwfs=iw)
+        eng.queue(isim, dt)
+        for iw in range(nw):
+            _, ints = eng.sync(iw)
+            x = ints.reshape(-1)
+            imb[iw] += x
+            imx[iw] += cs[iw] * x
+            imy[iw] += ss[iw] * x
+    for iw in range(nw):
+        b, gx_, gy_, nf = eng.dither_get(iw)
+        assert nf == 3
+        np.testing.assert_allclose(b.reshape(-1), imb[iw], rtol=1e-14)
+        np.testing.assert_allclose(gx_.reshape(-1), imx[iw], rtol=1e-13, atol=1e-12 * np.abs(imx[iw]).max())
+        np.testing.assert_allclose(gy_.reshape(-1), imy[iw], rtol=1e-13, atol=1e-12 * np.abs(imy[iw]).max())
+    s0, sg = 1.0 / 3, 2.0 / (3 * 0.05)
+    eng.dither_fold(s0, sg)
+    b, gx_, gy_, _ = eng.dither_get(0)
+    assert not b.any() and not gx_.any() and not gy_.any()
+    keep, acc = 0.6, 0.4
+    eng.mtch_update(keep, acc)
+    for iw in range(nw):
+        m, i0sum, sanea, i0, gx, gy = eng.mtch_get(iw)
+        want_i0 = keep * i0m.reshape(-1) + acc * s0 * imb[iw]
+        want_gx = keep * gxm.reshape(-1) + acc * sg * imx[iw]
+        want_gy = keep * gym.reshape(-1) + acc * sg * imy[iw]
+        np.testing.assert_allclose(i0.reshape(-1), want_i0, rtol=1e-12, atol=1e-12 * np.abs(want_i0).max())
+        np.testing.assert_allclose(gx.reshape(-1), want_gx, rtol=1e-12, atol=1e-12 * np.abs(want_gx).max())
+        np.testing.assert_allclose(gy.reshape(-1), want_gy, rtol=1e-12, atol=1e-12 * np.abs(want_gy).max())
+        hm, hs, hsanea = p.mtch(i0, gx, gy)
+        hm = hm.reshape(p.nsa, 2 * p.npix)
+        err = np.abs(m - hm).max() / np.abs(hm).max()
+        print("wfs=%d mtche GPU vs host max rel %.2e, zeroed %d/%d" % (iw, err, int((~m.any(1)).sum()), p.nsa))
+        assert err < 1e-6
+        np.testing.assert_array_equal(~m.any(1), ~hm.any(1))
+        np.testing.assert_allclose(i0sum, hs, rtol=1e-12)
+        np.testing.assert_allclose(sanea, hsanea, rtol=1e-7, atol=1e-9 * np.abs(hsanea).max())
+    # the next step with the GPU filter equals the host filter uploaded
+    eng.queue(3, dt)
+    g_gpu = [eng.sync(iw, want_ints=False)[0] for iw in range(nw)]
+    for iw in range(nw):
+        _, _, _, i0, gx, gy = eng.mtch_get(iw)
+        hm, hs, _ = p.mtch(i0, gx, gy)
+        eng.update_ref(hm, hs, iwfs=iw)
+    eng.queue(3, dt)
+    for iw in range(nw):
+        g_host = eng.sync(iw, want_ints=False)[0]
+        e = np.abs(g_gpu[iw] - g_host).max() * RAD2MAS
+        print("wfs=%d gradients GPU-filter vs host-filter %.2e mas" % (iw, e))
+        assert e < 1e-3
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C

@@ -14,6 +14,8 @@ pinned by closed forms and independent computations:
 - gx, gy against finite differences of images of shifted PSFs;
 - the matched filter of an LGS model image inverting its own gradients.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 import scipy.special as sp
@@ -291,3 +293,36 @@ def test_lgs_model_i0_flux(lgs_small):
     k = np.argmax(p.saa)
     np.testing.assert_allclose(m[k] @ gx[k], [1, 0], atol=1e-6)
     np.testing.assert_allclose(m[k] @ gy[k], [0, 1], atol=1e-6)
+
+
+def _dpos(alfsm, dtrat, npoint, isim, deltam):
+    cs, ss = C.c_double(0), C.c_double(0)
+    _S().mxs_dither_position(C.cast(C.byref(cs), _lib.c_double_p), C.cast(C.byref(ss), _lib.c_double_p),
+                             alfsm, dtrat, npoint, isim, deltam)
+    return cs.value, ss.value
+
+
+def test_dither_position_properties():
+    """With an integer delay the averaged dither position is the unit phasor
+    of the frame's phase: unit amplitude, one period per npoint frames, a
+    step of 2 pi/npoint per frame, lagging by the delay (alfsm + 1 - dtrat
+    frames) from phase pi/2 + deltam at frame 0.  A fractional delay (dtrat
+    2, alfsm 0) blends two phases with weights beta, 1-beta normalised by
+    beta^2 + (1-beta)^2."""
+    npoint, dm = 4, 0.3
+    for alfsm in (0, 1, 2):
+        for isim in range(-3, 9):
+            cs, ss = _dpos(alfsm, 1, npoint, isim, dm)
+            assert abs(cs * cs + ss * ss - 1) < 1e-14
+            want = np.pi / 2 + (isim - alfsm) * 2 * np.pi / npoint + dm
+            assert abs(cs - np.cos(want)) < 1e-14 and abs(ss - np.sin(want)) < 1e-14
+            c2, s2 = _dpos(alfsm, 1, npoint, isim + npoint, dm)
+            assert abs(c2 - cs) < 1e-13 and abs(s2 - ss) < 1e-13
+    # dtrat 2, alfsm 0: delay -1/2 frame, beta 1/2; frame counts by C's
+    # truncating division
+    for isim in range(-4, 8):
+        cs, ss = _dpos(0, 2, 8, isim, 0.0)
+        a1 = np.pi / 2 + int((isim + 1 - 1) / 2) * 2 * np.pi / 8
+        a2 = np.pi / 2 + int((isim + 1) / 2) * 2 * np.pi / 8
+        assert abs(cs - (np.cos(a1) + np.cos(a2))) < 1e-13
+        assert abs(ss - (np.sin(a1) + np.sin(a2))) < 1e-13
