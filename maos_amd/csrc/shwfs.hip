@@ -817,7 +817,10 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
     seg_sum<N>(s2);
     if (line == 0 && valid && !(a.saa[isa] < a.saat)) {
         float gx = 0.f, gy = 0.f;
-        if (a.phytype == MX_PTYPE_MF) {
+        if (a.sigmatch == 2) { // raw moments; k_sigmatch2 forms the gradients
+            gx = s0;
+            gy = s1;
+        } else if (a.phytype == MX_PTYPE_MF) {
             float scale = 1.f;
             if (a.sigmatch == 1) {
                 scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
@@ -1844,7 +1847,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         seg_sum<64>(s2);
         if (lane == 0 && !(a.saa[isa] < a.saat)) {
             float gx = 0.f, gy = 0.f;
-            if (a.phytype == MX_PTYPE_MF) {
+            if (a.sigmatch == 2) { // raw moments; k_sigmatch2 forms the gradients
+                gx = s0;
+                gy = s1;
+            } else if (a.phytype == MX_PTYPE_MF) {
                 float scale = 1.f;
                 if (a.sigmatch == 1) {
                     scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
@@ -2347,7 +2353,10 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
         }
         if (!(a.saa[isa] < a.saat)) {
             float gx = 0.f, gy = 0.f;
-            if (a.phytype == MX_PTYPE_MF) {
+            if (a.sigmatch == 2) { // raw moments; k_sigmatch2 forms the gradients
+                gx = s0;
+                gy = s1;
+            } else if (a.phytype == MX_PTYPE_MF) {
                 float scale = 1.f;
                 if (a.sigmatch == 1) {
                     scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
@@ -2368,12 +2377,65 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
         }
     }
 }
+// ------------------------------------------------------------ sigmatch 2
+// Global flux normalisation (shwfs_grad with powfs.sigmatch = 2,
+// maos/utils.c:1019-1080): i1sum = sum of every subaperture's pixels of the
+// WFS; the matched filter's gradients are scaled by i0sumsum/i1sum (i0sumsum
+// the sum of i0sum), and the CoG divides its moments by the flux
+// i1sum/saasum * saa[isa] (dcog's flux override).  The imaging epilogues
+// left the raw moments in grad; subapertures with saa < saat are skipped
+// there and here.  One block per WFS.
+struct Sig2Args {
+    int nsa, npix, phytype, pixpsax, pixpsay;
+    float cogthres, pixthetax, pixthetay, saat;
+    double saasum;
+    const float *ints, *i0sum, *saa;
+    float *grad;
+};
+__global__ __launch_bounds__(256) void k_sigmatch2(Sig2Args a) {
+    __shared__ double red[2][4];
+    const int iwfs = (int)blockIdx.x;
+    const long n = (long)a.nsa * a.npix;
+    const float *im = a.ints + (long)iwfs * n;
+    double s1 = 0., s0 = 0.;
+    for (long i = threadIdx.x; i < n; i += 256) s1 += (double)im[i];
+    if (a.phytype == MX_PTYPE_MF)
+        for (int isa = threadIdx.x; isa < a.nsa; isa += 256) s0 += (double)a.i0sum[(long)iwfs * a.nsa + isa];
+    for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o);
+        s0 += __shfl_xor(s0, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x / 64] = s1;
+        red[1][threadIdx.x / 64] = s0;
+    }
+    __syncthreads();
+    const double i1sum = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    const double i0sumg = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    float *gx = a.grad + (long)iwfs * 2 * a.nsa, *gy = gx + a.nsa;
+    for (int isa = threadIdx.x; isa < a.nsa; isa += 256) {
+        if (a.saa[isa] < a.saat) continue;
+        if (a.phytype == MX_PTYPE_MF) {
+            const float sc = (float)(i0sumg / i1sum);
+            gx[isa] *= sc;
+            gy[isa] *= sc;
+        } else {
+            const float flux = (float)(i1sum / a.saasum * a.saa[isa]);
+            if (flux > a.cogthres) {
+                gx[isa] = (gx[isa] / flux - (float)(a.pixpsax - 1) * 0.5f) * a.pixthetax;
+                gy[isa] = (gy[isa] / flux - (float)(a.pixpsay - 1) * 0.5f) * a.pixthetay;
+            } else {
+                gx[isa] = gy[isa] = 0.f;
+            }
+        }
+    }
+}
 } // namespace
 
 // ======================================================================
 // C ABI
 // ======================================================================
-constexpr int MX_MAXLAUNCH = 12; // kernels of one step
+constexpr int MX_MAXLAUNCH = 16; // kernels of one step
 
 // The kernels of one queued step in stream order; arg points into the
 // handle's argument blocks (a graph's kernel nodes copy them).
@@ -2486,6 +2548,7 @@ struct mx_wfsgrad_s {
     long dith_nframe = 0;
     FoldArgs arg_fold{};
     MtchArgs arg_mtch{};
+    Sig2Args arg_sig2{};
     bool queued = false;
     // timing
     bool timing = false;
@@ -2783,8 +2846,10 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
         return set_err(MX_EINVAL, "npix=%ld too large for notf=%ld", npix, p->notf);
     if (p->phytype != MX_PTYPE_MF && p->phytype != MX_PTYPE_COG)
         return set_err(MX_EINVAL, "phytype %d unsupported", p->phytype);
-    if (p->sigmatch < 0 || p->sigmatch > 1)
-        return set_err(MX_EINVAL, "sigmatch %d unsupported (0 or 1)", p->sigmatch);
+    if (p->sigmatch < 0 || p->sigmatch > 2)
+        return set_err(MX_EINVAL, "sigmatch %d unsupported (0, 1 or 2)", p->sigmatch);
+    if (p->sigmatch == 2 && !(p->saasum > 0))
+        return set_err(MX_EINVAL, "sigmatch 2 needs saasum > 0 (saasum=%g)", p->saasum);
     if (p->llt_nx > 0 && !(p->llt_nx * p->embfac == p->notf && p->llt_nx == p->nxsa))
         return set_err(MX_EINVAL, "LLT grid %ld must equal nxsa with nlwvf == notf", p->llt_nx);
     if (p->nsa < 1) return set_err(MX_EINVAL, "nsa=%ld", p->nsa);
@@ -3339,8 +3404,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         }
     if (h->cfg.phytype == MX_PTYPE_MF && !h->have_mtche && !h->geom)
         return set_err(MX_ESTATE, "matched filter not set");
-    if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch == 1 && !h->have_i0sum && !h->geom)
-        return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=1)");
+    if (h->cfg.phytype == MX_PTYPE_MF && h->cfg.sigmatch >= 1 && !h->have_i0sum && !h->geom)
+        return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=%d)", h->cfg.sigmatch);
     const int variant = effective_variant(h);
     if (variant < 0 && !h->geom)
         return set_err(MX_EINVAL, "the MFMA form needs notf=64, nxsa=32, <= %d sampled image rows and "
@@ -3726,10 +3791,30 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
                            : reinterpret_cast<const void *>(&k_shwfs_step<32, 16, 2>),
                    dim3((unsigned)(ngroup * h->nwfs)), dim3(64), &a};
     }
+    if (h->cfg.sigmatch == 2) {
+        Sig2Args &q = h->arg_sig2;
+        q = Sig2Args{};
+        q.nsa = (int)h->nsa;
+        q.npix = (int)h->npix;
+        q.phytype = h->cfg.phytype;
+        q.pixpsax = (int)h->cfg.pixpsax;
+        q.pixpsay = (int)h->cfg.pixpsay;
+        q.cogthres = (float)h->cfg.cogthres;
+        q.pixthetax = (float)h->cfg.pixthetax;
+        q.pixthetay = (float)h->cfg.pixthetay;
+        q.saat = (float)h->cfg.saat;
+        q.saasum = h->cfg.saasum;
+        q.ints = h->d_ints;
+        q.i0sum = h->d_i0sum;
+        q.saa = h->d_saa;
+        q.grad = h->d_grad;
+        L[nl++] = {reinterpret_cast<const void *>(&k_sigmatch2), dim3((unsigned)h->nwfs), dim3(256), &q};
+    }
+    if (nl > MX_MAXLAUNCH) return set_err(MX_EINVAL, "internal: %d kernels in a step", nl);
     sp.nl = nl;
     sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
              (direct ? 8 : 0) | (variant << 4) | (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) |
-             (h->lf ? h->lf_threads << 14 : 0);
+             (h->lf ? h->lf_threads << 14 : 0) | (h->cfg.sigmatch == 2 ? 1 << 25 : 0);
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;

@@ -747,6 +747,44 @@ def test_dither_statistics_and_gpu_matched_filter(lgs_small):
     eng.close()
 
 
+@pytest.mark.parametrize("case,variant", [("lgs_small", "mfma_bf16"), ("lgs_small", "fft32"),
+                                          ("ngs_two_wvl", "fft32"), ("lf_mixed", "fft32")])
+def test_sigmatch_global(case, variant, request):
+    """powfs.sigmatch = 2 (flux normalised over the whole WFS): the matched
+    filter's gradients scaled by i0sumsum / sum(ints), the CoG's moments
+    divided by sum(ints)/saasum * saa (imaging epilogues of every form leave
+    the raw moments, one pass per WFS finishes them); parity with the
+    oracle's shwfs_grad."""
+    import dataclasses
+    from maos_amd.powfs import PTYPE_MF, Powfs
+    cfg0, _, layers = request.getfixturevalue(case)
+    pc = dataclasses.replace(cfg0["powfs"], sigmatch=2)
+    cfg = dict(cfg0, powfs=pc)
+    p = Powfs(cfg["aper"], pc, cfg["sim"])
+    print("case: %s sigmatch=2 %s phytype=%d nsa=%d" % (case, variant, pc.phytype, p.nsa), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    if not eng.variant_ok(VARIANTS[variant]):
+        pytest.skip("%s does not apply to %s" % (variant, case))
+    eng.set_variant(VARIANTS[variant])
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    if pc.phytype == PTYPE_MF:
+        mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+        o.set_ref(mtche, i0sum)
+        eng.update_ref(mtche, i0sum)
+    for isim in (0, 41):
+        eng.queue(isim, dt)
+        for iw in range(len(wfs)):
+            g, ints = eng.sync(iw)
+            ri, rg, _, _ = o.step(wfs[iw], isim, dt)
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(g - rg).max() * RAD2MAS
+            print("isim=%d wfs=%d ints=%.2e grad=%.3e mas" % (isim, iw, e_int, e_g))
+            assert e_int <= INTS_TOL
+            assert e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C
