@@ -124,6 +124,12 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal,
  * etf2wt = (isim%dtrat)/dtrat (maos/wfsints.c:215-223). */
 int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf,
                           const double *etf2, double etf2wt);
+/* Per-step weight of the second sodium profile's ETF, w2 = (isim %
+ * coldtrat)/coldtrat (maos/wfsints.c:211-220; the first gets 1 - w2),
+ * without re-uploading the tables; needs a second ETF from
+ * mx_wfsgrad_update_etf.  The MFMA forms blend the two half-plane tables
+ * in a kernel of the step. */
+int mx_wfsgrad_set_etf_weight(mx_wfsgrad_t *h, double etf2wt);
 
 /* Matched filter and reference flux of WFS iwfs (gpu_wfsgrad_update_ref,
  * cuda/sim/wfsinit.cu:107; maos/wfsgrad.c:1207): mtche is nsa matrices of

@@ -88,6 +88,7 @@ HIP_API = {
     "mx_wfsgrad_sync": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_wfsgrad_i0_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mx_wfsgrad_i0_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_long_p]),
+    "mx_wfsgrad_set_etf_weight": (C.c_int, [C.c_void_p, C.c_double]),
     "mx_wfsgrad_dither_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mx_wfsgrad_set_dither": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
     "mx_wfsgrad_dither_fold": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),

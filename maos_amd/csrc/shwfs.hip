@@ -2377,6 +2377,27 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
         }
     }
 }
+// ------------------------------------------------------------ ETF blend
+// The two sodium-profile ETFs of the MFMA forms blended for the step,
+// w1 etf + w2 etf2 with w2 = (isim % coldtrat)/coldtrat (maos/wfsints.c:
+// 211-220), over the half-plane tables of every wavelength.
+struct EtfBlendArgs {
+    long n; // elements per wavelength (nsa * half plane)
+    int nwvl;
+    float w1, w2;
+    const float2 *e1[MX_MAXWVL], *e2[MX_MAXWVL];
+    float2 *out[MX_MAXWVL];
+};
+__global__ __launch_bounds__(256) void k_etf_blend(EtfBlendArgs a) {
+    const long tot = a.n * a.nwvl;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (long)gridDim.x * blockDim.x) {
+        const int w = (int)(i / a.n);
+        const long k = i - (long)w * a.n;
+        const float2 x = a.e1[w][k], y = a.e2[w][k];
+        a.out[w][k] = make_float2(x.x * a.w1 + y.x * a.w2, x.y * a.w1 + y.y * a.w2);
+    }
+}
+
 // ------------------------------------------------------------ sigmatch 2
 // Global flux normalisation (shwfs_grad with powfs.sigmatch = 2,
 // maos/utils.c:1019-1080): i1sum = sum of every subaperture's pixels of the
@@ -2500,6 +2521,9 @@ struct mx_wfsgrad_s {
     uint16_t *d_gx16[MX_MAXWVL] = {}, *d_gy16[MX_MAXWVL] = {}; // FP16
     int4 *d_sic[MX_MAXWVL] = {};
     float2 *d_etf_t[MX_MAXWVL] = {};
+    // second ETF profile of the MFMA forms (half-plane order) and the blend
+    // w1 etf + w2 etf2 that the imaging kernel reads
+    float2 *d_etf2_t[MX_MAXWVL] = {}, *d_etfb_t[MX_MAXWVL] = {};
     float *d_nominal_t[MX_MAXWVL] = {};
     float2 *d_lotf_t = nullptr;
     MfmaWvl *d_mwv = nullptr;
@@ -2549,6 +2573,7 @@ struct mx_wfsgrad_s {
     FoldArgs arg_fold{};
     MtchArgs arg_mtch{};
     Sig2Args arg_sig2{};
+    EtfBlendArgs arg_etfb{};
     bool queued = false;
     // timing
     bool timing = false;
@@ -2796,7 +2821,7 @@ std::vector<uint16_t> build_Gh(const std::vector<int> &rows) {
 // whenever its size and sampling limits hold; -1 if the request cannot run
 int effective_variant(const mx_wfsgrad_t *h) {
     bool ok = h->mfma_size;
-    for (int w = 0; w < h->nwvl; w++) ok = ok && h->mfma_wvl_ok[w] && !h->d_etf2[w];
+    for (int w = 0; w < h->nwvl; w++) ok = ok && h->mfma_wvl_ok[w] && (!h->d_etf2[w] || h->d_etf2_t[w]);
     if (h->variant == MX_VARIANT_FFT32) return MX_VARIANT_FFT32;
     if (h->variant == MX_VARIANT_MFMA_BF16 || h->variant == MX_VARIANT_MFMA_FP16) return ok ? h->variant : -1;
     return ok ? MX_VARIANT_MFMA_BF16 : MX_VARIANT_FFT32;
@@ -3039,6 +3064,8 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
         if (h->d_gy16[w]) hipFree(h->d_gy16[w]);
         if (h->d_sic[w]) hipFree(h->d_sic[w]);
         if (h->d_etf_t[w]) hipFree(h->d_etf_t[w]);
+        if (h->d_etf2_t[w]) hipFree(h->d_etf2_t[w]);
+        if (h->d_etfb_t[w]) hipFree(h->d_etfb_t[w]);
         if (h->d_nominal_t[w]) hipFree(h->d_nominal_t[w]);
         if (h->d_etf[w]) hipFree(h->d_etf[w]);
         if (h->d_etf2[w]) hipFree(h->d_etf2[w]);
@@ -3218,8 +3245,12 @@ int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const do
         const double *src = which ? etf2 : etf;
         if (!src) {
             if (which && h->d_etf2[iwvl]) {
+                HIPCHK(hipStreamSynchronize(h->stream));
                 hipFree(h->d_etf2[iwvl]);
                 h->d_etf2[iwvl] = nullptr;
+                if (h->d_etf2_t[iwvl]) hipFree(h->d_etf2_t[iwvl]);
+                if (h->d_etfb_t[iwvl]) hipFree(h->d_etfb_t[iwvl]);
+                h->d_etf2_t[iwvl] = h->d_etfb_t[iwvl] = nullptr;
             }
             continue;
         }
@@ -3227,18 +3258,29 @@ int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const do
         float2 **dst = which ? &h->d_etf2[iwvl] : &h->d_etf[iwvl];
         if ((rc = dev_alloc(dst, nsa * N * N))) return rc;
         HIPCHK(hipMemcpy(*dst, t.data(), sizeof(float2) * nsa * N * N, hipMemcpyHostToDevice));
-        if (h->mfma_size && !which) {
+        if (h->mfma_size) {
             for (long isa = 0; isa < nsa; isa++)
                 for (long uy = 0; uy < N / 2; uy++)
                     for (long ux = 0; ux < N; ux++) {
                         const double *e = src + 2 * (isa * N * N + ux + uy * N);
                         t[isa * MX_HALF + half_slot((int)ux, (int)uy)] = make_float2((float)e[0], (float)e[1]);
                     }
-            if ((rc = dev_alloc(&h->d_etf_t[iwvl], nsa * MX_HALF))) return rc;
-            HIPCHK(hipMemcpy(h->d_etf_t[iwvl], t.data(), sizeof(float2) * nsa * MX_HALF, hipMemcpyHostToDevice));
+            float2 **dt = which ? &h->d_etf2_t[iwvl] : &h->d_etf_t[iwvl];
+            if ((rc = dev_alloc(dt, nsa * MX_HALF))) return rc;
+            HIPCHK(hipMemcpy(*dt, t.data(), sizeof(float2) * nsa * MX_HALF, hipMemcpyHostToDevice));
+            if (which && (rc = dev_alloc(&h->d_etfb_t[iwvl], nsa * MX_HALF))) return rc;
         }
     }
     h->etf2wt = etf2 ? (float)etf2wt : 0.f;
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_etf_weight(mx_wfsgrad_t *h, double etf2wt) {
+    if (!h || !(etf2wt >= 0.) || !(etf2wt <= 1.)) return set_err(MX_EINVAL, "mx_wfsgrad_set_etf_weight: bad argument");
+    bool two = false;
+    for (int w = 0; w < h->nwvl; w++) two = two || h->d_etf2[w];
+    if (!two) return set_err(MX_ESTATE, "no second ETF (mx_wfsgrad_update_etf with etf2)");
+    h->etf2wt = (float)etf2wt;
     return MX_OK;
 }
 
@@ -3503,7 +3545,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     if (variant == MX_VARIANT_MFMA_BF16 || f16) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
-            hw[w].etf_t = h->d_etf_t[w];
+            hw[w].etf_t = h->d_etf2_t[w] ? h->d_etfb_t[w] : h->d_etf_t[w];
             hw[w].nominal_t = h->d_nominal_t[w];
             hw[w].gx = f16 ? h->d_gx16[w] : h->d_gx[w];
             hw[w].gy = f16 ? h->d_gy16[w] : h->d_gy[w];
@@ -3781,6 +3823,23 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)),
                    dim3((unsigned)std::min(h->lf_threads, 256)), &g};
     } else if (mf) {
+        bool two = false;
+        for (int w = 0; w < h->nwvl; w++) two = two || h->d_etf2_t[w];
+        if (two) {
+            EtfBlendArgs &b = h->arg_etfb;
+            b = EtfBlendArgs{};
+            b.n = h->nsa * MX_HALF;
+            b.nwvl = h->nwvl;
+            b.w1 = 1.f - h->etf2wt;
+            b.w2 = h->etf2wt;
+            for (int w = 0; w < h->nwvl; w++) {
+                b.e1[w] = h->d_etf_t[w];
+                b.e2[w] = h->d_etf2_t[w];
+                b.out[w] = h->d_etfb_t[w];
+            }
+            L[nl++] = {reinterpret_cast<const void *>(&k_etf_blend),
+                       dim3((unsigned)std::min<long>((b.n * h->nwvl + 255) / 256, 8192)), dim3(256), &b};
+        }
         L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16>)
                        : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16>),
                    dim3((unsigned)(h->nsa * h->nwfs)),
@@ -3814,7 +3873,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     sp.nl = nl;
     sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
              (direct ? 8 : 0) | (variant << 4) | (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) |
-             (h->lf ? h->lf_threads << 14 : 0) | (h->cfg.sigmatch == 2 ? 1 << 25 : 0);
+             (h->lf ? h->lf_threads << 14 : 0) | (h->cfg.sigmatch == 2 ? 1 << 25 : 0) |
+             (mf && h->d_etf2_t[0] ? 1 << 24 : 0);
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;

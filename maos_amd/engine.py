@@ -106,6 +106,10 @@ class WfsGrad:
     def update_etf(self, iwvl, etf, etf2=None, etf2wt=0.0):
         check(_lib.hip().mx_wfsgrad_update_etf(self.h, iwvl, dptr(etf), dptr(etf2), etf2wt))
 
+    def set_etf_weight(self, etf2wt):
+        """Weight of the second sodium profile's ETF for the next steps."""
+        check(_lib.hip().mx_wfsgrad_set_etf_weight(self.h, float(etf2wt)))
+
     def queue(self, isim, dt):
         check(_lib.hip().mx_wfsgrad_queue(self.h, isim, dt))
 

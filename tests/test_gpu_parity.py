@@ -785,6 +785,47 @@ def test_sigmatch_global(case, variant, request):
     eng.close()
 
 
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16"])
+def test_two_sodium_profiles(lgs_small, variant):
+    """Two sodium-profile ETFs blended per step (w1 etf + w2 etf2,
+    maos/wfsints.c:211-220): the weight changes between steps through
+    mx_wfsgrad_set_etf_weight; every form matches the oracle's blend."""
+    import ctypes as C
+    import dataclasses
+    from maos_amd.powfs import Powfs, SodiumCfg
+    cfg, p, layers = lgs_small
+    pc2 = dataclasses.replace(cfg["powfs"], sodium=SodiumCfg(center=92e3, fwhm=6e3, hmin=70e3, hmax=110e3,
+                                                             dh=20.0))
+    p2 = Powfs(cfg["aper"], pc2, cfg["sim"])
+    print("case: lgs order20 nsa=%d, two sodium profiles, %s" % (p.nsa, variant), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+    o.set_ref(mtche, i0sum)
+    eng.update_ref(mtche, i0sum)
+    etf2 = np.ascontiguousarray(p2.etf.reshape(-1))
+    o.keep.append(etf2)
+    o.s.etf2 = etf2.ctypes.data_as(C.POINTER(C.c_double))
+    for iw in range(p.nwvl):
+        eng.update_etf(iw, p.etf[iw], p2.etf[iw], 0.3)
+    eng.set_variant(VARIANTS[variant])
+    assert eng.variant() == VARIANTS[variant]
+    for isim, w2 in ((0, 0.3), (17, 0.75)):
+        eng.set_etf_weight(w2)
+        o.s.etf2wt = w2
+        eng.queue(isim, dt)
+        for iw in range(len(wfs)):
+            g, ints = eng.sync(iw)
+            ri, rg, _, _ = o.step(wfs[iw], isim, dt)
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(g - rg).max() * RAD2MAS
+            print("isim=%d w2=%.2f wfs=%d ints=%.2e grad=%.3e mas" % (isim, w2, iw, e_int, e_g))
+            assert e_int <= INTS_TOL
+            assert e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
 def test_errors_reported():
     """Invalid calls return error codes with a message and launch nothing."""
     import ctypes as C
