@@ -254,10 +254,17 @@ int mx_wfsgrad_set_ptt(mx_wfsgrad_t *h, int iwfs, const double *ptt);
 
 /* Gradient type: mode 0 (the default) physical optics; mode 1 geometric
  * Zernike-tilt gradients from the OPD (pts_ztilt, math/loc.c:839-880, as
- * maos/wfsgrad.c:303-306 computes them for gtype_sim = GTYPE_Z): the step
- * runs the ray trace and one reduction per subaperture, leaves ints
- * untouched, and needs no DTF, ETF or matched filter. */
+ * maos/wfsgrad.c:303-306 computes them for gtype_sim = GTYPE_Z); mode 2
+ * geometric average-gradient (G-tilt) gradients GS0 * opd (gtype_sim =
+ * GTYPE_G, maos/wfsgrad.c:307-308), with GS0 from mx_wfsgrad_set_gs0.  The
+ * geometric steps run the ray trace and one reduction per subaperture (or
+ * row), leave ints untouched, and need no DTF, ETF or matched filter. */
 int mx_wfsgrad_set_geom(mx_wfsgrad_t *h, int mode);
+/* The G-tilt operator GS0 (powfs.GS0 of setup_shwfs_grad, maos/powfs.c:
+ * 603-630; mxs_mkg builds it): CSR with 2*nsa rows (x gradients, then y)
+ * over the nsa*nxsa^2 OPD points, shared by all WFS.  The structure is
+ * checked (rowp monotone from 0 to nnz, columns in range) before upload. */
+int mx_wfsgrad_set_gs0(mx_wfsgrad_t *h, const long *rowp, const long *coli, const double *val, long nnz);
 
 /* Photon and read noise on the pixel intensities of every later step
  * (addnoise, lib/libmisc.c:31-57, as applied at maos/wfsgrad.c:423-442;

@@ -114,6 +114,20 @@ int mxs_mtch(double *mtche, double *sanea, double *i0sum, double *i0sumsum,
              double sigratio);
 
 
+/* Average-gradient (G-tilt) operator GS0 (mkg, lib/mkg.c:108-468, as
+ * setup_shwfs_grad calls it, maos/powfs.c:603-630): CSR with 2*nsa rows
+ * (x gradients, then y) over the nsa*nxsa^2 OPD points (subaperture isa's
+ * point (ix, iy) at ptsx[isa] + ix dx, ptsy[isa] + iy dx; the grids form
+ * one common grid, so a subaperture's edge cells use its neighbours'
+ * points).  Row r: the amplitude-weighted average over the subaperture
+ * square of the gradient of the bilinear OPD, over cells whose four corners
+ * exist with amplitude > 1e-14; zero if the amplitude integral is at most 1%
+ * of the area; empty if saa < saat (saa may be NULL).  rowp has 2*nsa+1
+ * entries.  Returns nnz, or -2 if more than nzmax entries are needed
+ * ((nxsa+2)^2 per row suffice). */
+long mxs_mkg(long *rowp, long *coli, double *val, long nzmax, const double *ptsx, const double *ptsy,
+             long nsa, long nxsa, double dx, const double *amp, const double *saa, double saat);
+
 /* Dither phase of a WFS for step isim, averaged over the integration
  * (dither_position, maos/utils.c:884-899): alfsm the FSM delay, dtrat the
  * WFS frame ratio, npoint steps per dither period, deltam the PLL phase. */

@@ -89,6 +89,7 @@ HIP_API = {
     "mx_wfsgrad_i0_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mx_wfsgrad_i0_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_long_p]),
     "mx_wfsgrad_set_etf_weight": (C.c_int, [C.c_void_p, C.c_double]),
+    "mx_wfsgrad_set_gs0": (C.c_int, [C.c_void_p, c_long_p, c_long_p, c_double_p, C.c_long]),
     "mx_wfsgrad_dither_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mx_wfsgrad_set_dither": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
     "mx_wfsgrad_dither_fold": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double]),
@@ -163,6 +164,8 @@ SETUP_API = {
                            C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                            C.c_int, C.c_double]),
     "mxs_turbcov": (C.c_int, [c_double_p, C.c_long, C.c_double, C.c_double, c_double_p]),
+    "mxs_mkg": (C.c_long, [c_long_p, c_long_p, c_double_p, C.c_long, c_double_p, c_double_p, C.c_long,
+                           C.c_long, C.c_double, c_double_p, c_double_p, C.c_double]),
     "mxs_dither_position": (None, [c_double_p, c_double_p, C.c_int, C.c_int, C.c_int, C.c_long,
                                    C.c_double]),
     "mxs_genotf": (C.c_int, [c_double_p, C.c_long, C.c_double, c_double_p, c_double_p, C.c_long,

@@ -1245,6 +1245,43 @@ __global__ __launch_bounds__(64) void k_dm_opd(DmArgs a) {
 // a = sum amp opd [1, x, y] in FP64, times the inverse of
 // M = sum amp [1, x, y]^T [1, x, y] (pts_mcc_ptt, math/loc.c:612-650,
 // inverted on the host); the gradients are the x and y terms.
+// Geometric average-gradient (G-tilt) gradients: grad = GS0 opd
+// (dspmm at maos/wfsgrad.c:308) with GS0 from mkg (lib/mkg.c:108-468), a
+// CSR of 2 nsa rows (x, then y) over the nsa nxsa^2 OPD points, shared by
+// all WFS.  One wave per row computes the row for every WFS (the row's
+// entries are read once), FP64 sums; an empty row gives 0.
+constexpr int MX_GT_WFS = 8; // WFS per pass of a row
+struct GtiltArgs {
+    int nsa, nwfs;
+    long npts; // OPD points per WFS
+    const int *rowp, *coli;
+    const double *val;
+    const float *opd; // [nwfs][npts]
+    float *grad;      // [nwfs][2*nsa]
+};
+__global__ __launch_bounds__(64) void k_gtilt(GtiltArgs a) {
+    const int r = (int)blockIdx.x, lane = (int)threadIdx.x;
+    const int s = a.rowp[r], e = a.rowp[r + 1];
+    for (int w0 = 0; w0 < a.nwfs; w0 += MX_GT_WFS) {
+        double acc[MX_GT_WFS];
+#pragma unroll
+        for (int j = 0; j < MX_GT_WFS; j++) acc[j] = 0.;
+        for (int k = s + lane; k < e; k += 64) {
+            const double v = a.val[k];
+            const long c = a.coli[k];
+#pragma unroll
+            for (int j = 0; j < MX_GT_WFS; j++)
+                if (w0 + j < a.nwfs) acc[j] += v * (double)a.opd[(long)(w0 + j) * a.npts + c];
+        }
+#pragma unroll
+        for (int j = 0; j < MX_GT_WFS; j++) {
+            double t = acc[j];
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+            if (lane == 0 && w0 + j < a.nwfs) a.grad[(long)(w0 + j) * 2 * a.nsa + r] = (float)t;
+        }
+    }
+}
+
 struct ZtiltArgs {
     int nsa, nxsa, nwfs;
     double dx;
@@ -2554,6 +2591,10 @@ struct mx_wfsgrad_s {
     // closed-loop OPD inputs and geometric gradients
     int geom = 0;                     // 1: Zernike-tilt gradients from the OPD, no imaging
     double *d_imcc = nullptr;         // [nsa][9] inverse tip/tilt moment matrices
+    int *d_gs_rowp = nullptr, *d_gs_coli = nullptr; // G-tilt operator GS0 (CSR, 2 nsa rows)
+    double *d_gs_val = nullptr;
+    long gs_nnz = -1;
+    GtiltArgs arg_gtilt{};
     float *d_opdadd = nullptr;        // [nwfs][nsa][nxsa^2] static surface OPD
     bool opdadd_set = false;
     double *d_ptt = nullptr;          // [nwfs][3] piston, tip, tilt per step
@@ -3053,7 +3094,7 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
                     h->d_lfw,
                     h->d_mtche, h->d_i0sum, h->d_ints, h->d_grad, h->d_opd, h->d_i0acc,
                     h->d_imb, h->d_imx, h->d_imy, h->d_fi0, h->d_fgx, h->d_fgy, h->d_mi0, h->d_mgx, h->d_mgy,
-                    h->d_nea, h->d_i0s64, h->d_dcs};
+                    h->d_nea, h->d_i0s64, h->d_dcs, h->d_gs_rowp, h->d_gs_coli, h->d_gs_val};
     for (void *q : ptrs)
         if (q) hipFree(q);
     for (int w = 0; w < MX_MAXWVL; w++) {
@@ -3665,6 +3706,27 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
                    dim3(256), &q};
     }
     sp.slot_opd_last = nl - 1;
+    if (h->geom == 2) {
+        if (h->gs_nnz < 0) return set_err(MX_ESTATE, "G-tilt gradients need GS0 (mx_wfsgrad_set_gs0)");
+        GtiltArgs &q = h->arg_gtilt;
+        q = GtiltArgs{};
+        q.nsa = (int)h->nsa;
+        q.nwfs = h->nwfs;
+        q.npts = h->nsa * h->nxsa * h->nxsa;
+        q.rowp = h->d_gs_rowp;
+        q.coli = h->d_gs_coli;
+        q.val = h->d_gs_val;
+        q.opd = h->d_opd;
+        q.grad = h->d_grad;
+        sp.slot_img = nl;
+        L[nl++] = {reinterpret_cast<const void *>(&k_gtilt), dim3((unsigned)(2 * h->nsa)), dim3(64), &q};
+        sp.nl = nl;
+        sp.key = (1 << 30) | (1 << 29) | (1 << 26) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (direct ? 8 : 0) |
+                 (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8);
+        sp.slot_llt = -1;
+        sp.llt = false;
+        return MX_OK;
+    }
     if (h->geom) {
         ZtiltArgs &z = h->arg_ztilt;
         z = ZtiltArgs{};
@@ -4363,8 +4425,43 @@ int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant) {
     return MX_OK;
 }
 
+int mx_wfsgrad_set_gs0(mx_wfsgrad_t *h, const long *rowp, const long *coli, const double *val, long nnz) {
+    if (!h || !rowp || nnz < 0 || (nnz && (!coli || !val)) || nnz > (1L << 30))
+        return set_err(MX_EINVAL, "mx_wfsgrad_set_gs0: bad argument");
+    const long nrow = 2 * h->nsa, npts = h->nsa * h->nxsa * h->nxsa;
+    // the kernel trusts the structure: check every row and column here
+    if (rowp[0] != 0 || rowp[nrow] != nnz) return set_err(MX_EINVAL, "GS0: rowp[0]=%ld rowp[%ld]=%ld nnz=%ld",
+                                                           rowp[0], nrow, rowp[nrow], nnz);
+    for (long r = 0; r < nrow; r++)
+        if (rowp[r + 1] < rowp[r]) return set_err(MX_EINVAL, "GS0: rowp decreases at row %ld", r);
+    for (long k = 0; k < nnz; k++)
+        if (coli[k] < 0 || coli[k] >= npts) return set_err(MX_EINVAL, "GS0: column %ld out of range", coli[k]);
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->d_gs_coli) {
+        hipFree(h->d_gs_coli);
+        hipFree(h->d_gs_val);
+        h->d_gs_coli = nullptr;
+        h->d_gs_val = nullptr;
+    }
+    int rc;
+    if ((rc = dev_alloc(&h->d_gs_rowp, (size_t)nrow + 1))) return rc;
+    if ((rc = dev_alloc(&h->d_gs_coli, (size_t)std::max<long>(nnz, 1)))) return rc;
+    if ((rc = dev_alloc(&h->d_gs_val, (size_t)std::max<long>(nnz, 1)))) return rc;
+    std::vector<int> ti(std::max<long>(nnz, nrow + 1));
+    for (long r = 0; r <= nrow; r++) ti[r] = (int)rowp[r];
+    HIPCHK(hipMemcpy(h->d_gs_rowp, ti.data(), sizeof(int) * (nrow + 1), hipMemcpyHostToDevice));
+    for (long k = 0; k < nnz; k++) ti[k] = (int)coli[k];
+    if (nnz) {
+        HIPCHK(hipMemcpy(h->d_gs_coli, ti.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_gs_val, val, sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+    h->gs_nnz = nnz;
+    return MX_OK;
+}
+
 int mx_wfsgrad_set_geom(mx_wfsgrad_t *h, int mode) {
-    if (!h || mode < 0 || mode > 1) return set_err(MX_EINVAL, "mx_wfsgrad_set_geom: bad handle or mode %d", mode);
+    if (!h || mode < 0 || mode > 2) return set_err(MX_EINVAL, "mx_wfsgrad_set_geom: bad handle or mode %d", mode);
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->geom = mode;

@@ -213,8 +213,16 @@ class WfsGrad:
         a = None if ptt is None else np.ascontiguousarray(ptt, dtype=np.float64)
         check(_lib.hip().mx_wfsgrad_set_ptt(self.h, iwfs, dptr(a)))
 
+    def set_gs0(self, rowp, coli, val):
+        """G-tilt operator (CSR, as Powfs.gs0 returns it) for set_geom(2)."""
+        r = np.ascontiguousarray(rowp, dtype=np.int64)
+        c = np.ascontiguousarray(coli, dtype=np.int64)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        check(_lib.hip().mx_wfsgrad_set_gs0(self.h, lptr(r), lptr(c), dptr(v), len(v)))
+
     def set_geom(self, mode=1):
-        """1: geometric Zernike-tilt gradients from the OPD; 0: physical optics."""
+        """1: geometric Zernike-tilt gradients from the OPD; 2: G-tilt (GS0,
+        set_gs0); 0: physical optics."""
         check(_lib.hip().mx_wfsgrad_set_geom(self.h, mode))
 
     def set_noise(self, enable=True, rne=0.0, bkgrnd=0.0, bkgrndc=0.0, seed=1):

@@ -261,6 +261,21 @@ class Powfs:
                 raise RuntimeError("gensei failed (%d)" % rc)
         return i0.reshape(nsa, npix), gx.reshape(nsa, npix), gy.reshape(nsa, npix)
 
+    # -------------------------------------------------------------- G-tilt
+    def gs0(self):
+        """Average-gradient operator GS0 (mkg, lib/mkg.c:108-468) as CSR
+        (rowp, coli, val): 2*nsa rows (x then y) over the OPD points."""
+        S = _lib.setup()
+        nz = 2 * self.nsa * (self.nxsa + 2) ** 2
+        rowp = np.zeros(2 * self.nsa + 1, dtype=np.int64)
+        coli = np.zeros(nz, dtype=np.int64)
+        val = np.zeros(nz)
+        n = S.mxs_mkg(lptr(rowp), lptr(coli), dptr(val), nz, dptr(self.ptsx), dptr(self.ptsy), self.nsa,
+                      self.nxsa, self.dx, dptr(self.amp), dptr(self.saa), self.cfg.saat)
+        if n < 0:
+            raise RuntimeError("mkg failed (%d)" % n)
+        return rowp, coli[:n].copy(), val[:n].copy()
+
     # -------------------------------------------------------------- MF
     def mtch(self, i0, gx=None, gy=None):
         """Matched filter from averaged images i0 [nsa, npix] (mtch_cell,

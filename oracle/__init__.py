@@ -87,6 +87,7 @@ def lib():
                                        C.POINTER(OrcAtm), C.c_long, C.c_double, C.c_double,
                                        C.c_int, dp, dp]
         L.orc_ztilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double]
+        L.orc_gtilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double, dp, C.c_double]
         L.orc_ccpcorner.argtypes = [dp, C.c_long, C.c_long, dp, C.c_long, C.c_long]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint, C.c_uint]
         L.orc_randp.restype = C.c_double
@@ -240,6 +241,16 @@ class Oracle:
         o = np.ascontiguousarray(opd, dtype=np.float64).reshape(-1)
         lib().orc_ztilt(_d(g), _d(o), _d(np.ascontiguousarray(p.amp)), _d(np.ascontiguousarray(p.ptsx)),
                         _d(np.ascontiguousarray(p.ptsy)), p.nsa, p.nxsa, p.dx)
+        return g
+
+    def gtilt(self, opd):
+        """Geometric average-gradient (G-tilt) gradients of an OPD [nsa*nxsa^2]."""
+        p = self.p
+        g = np.zeros(2 * p.nsa)
+        o = np.ascontiguousarray(opd, dtype=np.float64).reshape(-1)
+        lib().orc_gtilt(_d(g), _d(o), _d(np.ascontiguousarray(p.amp)), _d(np.ascontiguousarray(p.ptsx)),
+                        _d(np.ascontiguousarray(p.ptsy)), p.nsa, p.nxsa, p.dx, _d(np.ascontiguousarray(p.saa)),
+                        p.cfg.saat)
         return g
 
     def step(self, wfs, isim, dt, atmscale=1.0, nthread=0, want_opd=False, want_lotf=False):

@@ -748,3 +748,86 @@ void orc_ztilt(double *grad, const double *opd, const double *amp, const double 
         grad[isa + nsa] = ok ? aug[2][3] / aug[2][2] : 0;
     }
 }
+
+/* ---------------------------------------------------------------------
+ * Geometric average-gradient (G-tilt) gradients, GS0 of mkg
+ * (lib/mkg.c:108-468, maos/powfs.c:603-630, do_partial = 1) applied to an
+ * OPD: per subaperture, g = (1/dx) int A dPhi / int A over the cells of the
+ * common point grid that meet the subaperture square and whose four corners
+ * exist with amplitude > 1e-14; A and Phi are bilinear.  Each cell's part
+ * is integrated by 2x2 Gauss-Legendre quadrature (exact: the integrand is
+ * at most quadratic in each direction).  A subaperture whose amplitude
+ * integral is at most 1% of its area, or with saa < saat (an empty row,
+ * applied by dspmm at maos/wfsgrad.c:308), gives 0.
+ * ------------------------------------------------------------------- */
+void orc_gtilt(double *grad, const double *opd, const double *amp, const double *ptsx, const double *ptsy,
+               long nsa, long nxsa, double dx, const double *saa, double saat) {
+    const long npsa = nxsa * nxsa;
+    /* points keyed by integer grid coordinates relative to subaperture 0 */
+    long minx = 0, miny = 0, maxx = 0, maxy = 0;
+    long *gx = (long *)malloc(sizeof(long) * nsa), *gy = (long *)malloc(sizeof(long) * nsa);
+    for (long isa = 0; isa < nsa; isa++) {
+        gx[isa] = (long)floor((ptsx[isa] - ptsx[0]) / dx + 0.5);
+        gy[isa] = (long)floor((ptsy[isa] - ptsy[0]) / dx + 0.5);
+        if (gx[isa] < minx) minx = gx[isa];
+        if (gy[isa] < miny) miny = gy[isa];
+        if (gx[isa] > maxx) maxx = gx[isa];
+        if (gy[isa] > maxy) maxy = gy[isa];
+    }
+    const long w = maxx - minx + nxsa + 4, hgt = maxy - miny + nxsa + 4;
+    long *idx = (long *)malloc(sizeof(long) * w * hgt);
+    for (long i = 0; i < w * hgt; i++) idx[i] = -1;
+#define ORC_AT(X, Y) idx[((X) - minx + 2) + ((Y) - miny + 2) * w]
+    for (long isa = 0; isa < nsa; isa++)
+        for (long iy = 0; iy < nxsa; iy++)
+            for (long ix = 0; ix < nxsa; ix++) ORC_AT(gx[isa] + ix, gy[isa] + iy) = isa * npsa + iy * nxsa + ix;
+    const double gq = 0.5 / sqrt(3.);
+    for (long isa = 0; isa < nsa; isa++) {
+        if (saa && saat > 0 && saa[isa] < saat) { /* an empty row of GS0 */
+            grad[isa] = grad[isa + nsa] = 0;
+            continue;
+        }
+        double sa = 0, sx = 0, sy = 0;
+        const double lo_x = gx[isa] - 0.5, hi_x = gx[isa] + nxsa - 0.5;
+        const double lo_y = gy[isa] - 0.5, hi_y = gy[isa] + nxsa - 0.5;
+        for (long cy = gy[isa] - 1; cy <= gy[isa] + nxsa - 1; cy++)
+            for (long cx = gx[isa] - 1; cx <= gx[isa] + nxsa - 1; cx++) {
+                const double u1 = lo_x - cx > 0 ? lo_x - cx : 0, u2 = hi_x - cx < 1 ? hi_x - cx : 1;
+                const double v1 = lo_y - cy > 0 ? lo_y - cy : 0, v2 = hi_y - cy < 1 ? hi_y - cy : 1;
+                if (u2 <= u1 || v2 <= v1) continue;
+                double av[2][2], ph[2][2];
+                int ok = 1;
+                for (int j = 0; j < 2; j++)
+                    for (int i = 0; i < 2; i++) {
+                        const long k = ORC_AT(cx + i, cy + j);
+                        av[j][i] = k < 0 ? 0 : (amp ? amp[k] : 1.);
+                        ph[j][i] = k < 0 ? 0 : opd[k];
+                        if (!(fabs(av[j][i]) > 1e-14)) ok = 0;
+                    }
+                if (!ok) continue;
+                const double um = 0.5 * (u1 + u2), ur = 0.5 * (u2 - u1), vm = 0.5 * (v1 + v2), vr = 0.5 * (v2 - v1);
+                for (int b = 0; b < 2; b++)
+                    for (int a = 0; a < 2; a++) {
+                        const double u = um + (a ? gq : -gq) * 2 * ur, v = vm + (b ? gq : -gq) * 2 * vr;
+                        const double wq = ur * vr; /* each of the four nodes weighs a quarter of the area */
+                        const double A = av[0][0] * (1 - u) * (1 - v) + av[0][1] * u * (1 - v) +
+                                         av[1][0] * (1 - u) * v + av[1][1] * u * v;
+                        const double du = (ph[0][1] - ph[0][0]) * (1 - v) + (ph[1][1] - ph[1][0]) * v;
+                        const double dv = (ph[1][0] - ph[0][0]) * (1 - u) + (ph[1][1] - ph[0][1]) * u;
+                        sa += wq * A;
+                        sx += wq * A * du;
+                        sy += wq * A * dv;
+                    }
+            }
+        if (sa > 0.01 * (double)npsa) {
+            grad[isa] = sx / (sa * dx);
+            grad[isa + nsa] = sy / (sa * dx);
+        } else {
+            grad[isa] = grad[isa + nsa] = 0;
+        }
+    }
+#undef ORC_AT
+    free(idx);
+    free(gx);
+    free(gy);
+}

@@ -130,6 +130,15 @@ void orc_ccpcorner(double *out, long nx, long ny, const double *in, long ninx, l
 void orc_ztilt(double *grad, const double *opd, const double *amp, const double *ptsx,
                const double *ptsy, long nsa, long nxsa, double dx);
 
+/* Geometric average-gradient (G-tilt) gradients: GS0 of mkg
+ * (lib/mkg.c:108-468, maos/powfs.c:603-630) applied to opd (nsa*nxsa^2,
+ * the subaperture grids forming one common grid): the amplitude-weighted
+ * average of the bilinear OPD's gradient over each subaperture square, by
+ * Gauss-Legendre quadrature per cell.  grad is 2*nsa; subapertures with
+ * saa < saat get 0 (an empty row). */
+void orc_gtilt(double *grad, const double *opd, const double *amp, const double *ptsx, const double *ptsy,
+               long nsa, long nxsa, double dx, const double *saa, double saat);
+
 /* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
  * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ctr is replaced
  * by the four output words. */

@@ -541,6 +541,50 @@ def test_geometric_ztilt(case, request):
     eng.close()
 
 
+@pytest.mark.parametrize("case", ["lgs_small", "ngs_small"])
+def test_geometric_gtilt(case, request):
+    """Geometric average-gradient gradients (set_geom(2), GS0 from mkg)
+    against the oracle's cell-by-cell evaluation on the GPU's own traced OPD
+    (FP32) and on the oracle's OPD; a tip/tilt input moves every
+    illuminated subaperture by exactly that tilt."""
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: gtilt %s nsa=%d" % (case, p.nsa), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    rowp, coli, val = p.gs0()
+    eng.set_gs0(rowp, coli, val)
+    eng.set_geom(2)
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    isim = 21
+    eng.queue(isim, dt)
+    g0 = []
+    for iw in range(len(wfs)):
+        gg, _ = eng.sync(iw, want_ints=False)
+        own = o.gtilt(eng.debug_opd(iw))
+        _, _, ropd, _ = o.step(wfs[iw], isim, dt, want_opd=True)
+        rg = o.gtilt(ropd)
+        scale = np.abs(rg).max()
+        e_own = np.abs(gg - own).max() / scale
+        e_ref = np.abs(gg - rg).max() * RAD2MAS
+        print("wfs=%d gtilt vs own OPD %.2e rel, vs oracle OPD %.3e mas (|g| max %.1f mas)"
+              % (iw, e_own, e_ref, scale * RAD2MAS))
+        assert e_own < 1e-6
+        assert e_ref < 1e-3
+        g0.append(gg)
+    tilt = np.array([0.0, 2e-6, -1e-6])
+    eng.set_ptt(tilt)
+    eng.queue(isim, dt)
+    full = np.abs(val).sum() > 0
+    for iw in range(len(wfs)):
+        gg, _ = eng.sync(iw, want_ints=False)
+        d = gg - g0[iw]
+        lit = np.diff(rowp) > 0
+        assert full
+        assert np.allclose(d[lit[:p.nsa].nonzero()[0]], 2e-6, atol=2e-6 * 1e-4)
+        assert np.allclose(d[p.nsa + lit[p.nsa:].nonzero()[0]], -1e-6, atol=1e-6 * 1e-4)
+    eng.close()
+
+
 def _dm_maps(rs):
     """two DM surfaces: 30 m at 1/64 m on the ground and a 3.5 m map at 1/32 m
     conjugated to 8 km (smaller than the footprints there, so that some
