@@ -319,6 +319,66 @@ int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep,
 /* Wait for all queued work of the engine. */
 int mx_wfsgrad_wait(mx_wfsgrad_t *h);
 
+/* ---------------------------------------------------------------------
+ * Science-field evaluation: OPD on the aperture points and long-exposure
+ * PSF per (direction, wavelength).
+ * ------------------------------------------------------------------- */
+typedef struct mx_evl_s mx_evl_t;
+
+/* Replaces gpu_perfevl_init_sim's PSF tables (cuda/sim/perfinit.cu:28-90)
+ * built from locfft_init (lib/locfft.c:32-68).  nloc aperture points with
+ * amplitude amp (not normalised here: the mean PSF is divided by
+ * (sum amp)^2).  locx/locy (may be NULL when only the PSF of a given OPD is
+ * wanted) are the point coordinates for mx_evl_trace.  ndir (1..64)
+ * directions thetax/thetay (rad; NULL: 0) at height hs (m; 0 or NULL: at
+ * infinity).  For each of nwvl (1..4) wavelengths wvl (m): the FFT side
+ * nembed (a power of two from 32 to 4096; mxs_loc_create_embed computes
+ * it), psfsize (1 for the Strehl-only form, else even and at most nembed),
+ * and embed[iwvl * nloc + iloc], the index ix + iy * nembed of each point
+ * in the FFT grid (loc_create_embed, math/loc.c:147-179).  Two points at
+ * one index: the later one is used. */
+int mx_evl_init(mx_evl_t **out, long nloc, const double *locx, const double *locy,
+                const double *amp, int ndir, const double *thetax, const double *thetay,
+                const double *hs, int nwvl, const double *wvl, const long *nembed,
+                const long *psfsize, const long *embed);
+int mx_evl_free(mx_evl_t *h);
+
+/* Replaces psfcomp_r (cuda/sim/perfevl.cu:255-290) and the psfmean branch
+ * of perfevl_psfcl (maos/perfevl.c:64-78): adds, for every direction and
+ * wavelength, |FFT(amp exp(2 pi i opd / wvl))|^2 over the centred
+ * psfsize^2 window (psfsize 1: |sum amp exp(2 pi i opd / wvl)|^2) to the
+ * accumulators.  opd is [ndir][nloc] in metres (host, FP64; converted to
+ * FP32).  Queued on the handle's stream. */
+int mx_evl_psf_add(mx_evl_t *h, const double *opd);
+/* The same from an FP32 OPD [ndir][nloc] already in device memory, which
+ * must stay unchanged until the handle's work is done (mx_evl_wait). */
+int mx_evl_psf_add_device(mx_evl_t *h, const float *d_opd);
+
+/* Replaces the atmosphere and DM ray trace onto the evaluation points
+ * (maos/perfevl.c:275-300, gpu_atm2loc in cuda/sim/accphi.cu): the screens
+ * of realisation 0 (mx_atm2gpu) at step isim, frozen flow v dt isim,
+ * bilinear with periodic wrap, times atmscale; minus the DM surfaces
+ * (mx_dm2gpu; no wrap, points outside a surface take nothing).  add_psf=1
+ * then adds the PSF of that OPD (mx_evl_psf_add).  Queued. */
+int mx_evl_trace(mx_evl_t *h, long isim, double dt, int add_psf);
+/* The last OPD traced or added from the host, [ndir][nloc] (waits). */
+int mx_evl_opd_get(mx_evl_t *h, double *opd);
+
+/* The mean PSF of direction idir and wavelength iwvl, psfsize^2 values with
+ * x fastest, centred as corner2center (cuda/math/kernel.cu:292-307):
+ * accumulated sum / (nacc (sum amp)^2), so a diffraction-limited peak is
+ * the Strehl ratio 1 (gpu_perfevl_save, cuda/sim/perfevl.cu:618-660).
+ * nacc (may be NULL) receives the number of PSFs added.  Waits. */
+int mx_evl_psf_get(mx_evl_t *h, int idir, int iwvl, double *psf, long *nacc);
+/* Clear the accumulators (queued). */
+int mx_evl_psf_reset(mx_evl_t *h);
+int mx_evl_wait(mx_evl_t *h);
+/* Kernel time, summed over timed calls, of the trace and of the PSF's row
+ * and column FFT passes (all wavelengths), from HIP events on the handle's
+ * stream; every timed call waits for its kernels.  n counts traces. */
+int mx_evl_timing_enable(mx_evl_t *h, int enable);
+int mx_evl_timing(mx_evl_t *h, long *n, double *ms_trace, double *ms_rows, double *ms_cols);
+
 #ifdef __cplusplus
 }
 #endif

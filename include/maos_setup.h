@@ -25,6 +25,14 @@ void mxs_fft2(double *c, long nx, long ny, int dir);
 /* nextfftsize (math/mathmisc.c:109-146). */
 long mxs_nextfftsize(long n);
 
+/* loc_create_embed (math/loc.c:147-179): embed[i] = ix + iy * nembed, the
+ * index of point i in a square grid of side nembed.  *nembed > 0 gives the
+ * side (error -2 below 0.6 of the minimum ceil(max(nx, ny) * oversize));
+ * otherwise it is that minimum, raised by nextfftsize when fftpad, and is
+ * written back.  Returns the side, or < 0 on error (-3: a point outside). */
+long mxs_loc_create_embed(long *embed, long *nembed, long nloc, const double *locx, const double *locy, double dx,
+                          double dy, double oversize, int fftpad);
+
 /* Subaperture lower-left corners on the padded grid, before area pruning
  * (maos/powfs.c:302-340).  Returns the count written (<= maxn). */
 long mxs_saloc(int order, double dsa, double din, double misregrmax,

@@ -116,6 +116,19 @@ HIP_API = {
     "mx_wfsgroup_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]),
     "mx_wfsgroup_queue": (C.c_int, [C.c_void_p, C.c_long, C.c_double]),
     "mx_wfsgroup_free": (C.c_int, [C.c_void_p]),
+    "mx_evl_init": (C.c_int, [C.POINTER(C.c_void_p), C.c_long, c_double_p, c_double_p, c_double_p, C.c_int,
+                              c_double_p, c_double_p, c_double_p, C.c_int, c_double_p, c_long_p, c_long_p,
+                              c_long_p]),
+    "mx_evl_free": (C.c_int, [C.c_void_p]),
+    "mx_evl_psf_add": (C.c_int, [C.c_void_p, c_double_p]),
+    "mx_evl_psf_add_device": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mx_evl_trace": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_int]),
+    "mx_evl_opd_get": (C.c_int, [C.c_void_p, c_double_p]),
+    "mx_evl_psf_get": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, c_long_p]),
+    "mx_evl_psf_reset": (C.c_int, [C.c_void_p]),
+    "mx_evl_wait": (C.c_int, [C.c_void_p]),
+    "mx_evl_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "mx_evl_timing": (C.c_int, [C.c_void_p, c_long_p, c_double_p, c_double_p, c_double_p]),
 }
 
 VARIANT_AUTO, VARIANT_FFT32, VARIANT_MFMA_BF16, VARIANT_MFMA_FP16 = -1, 0, 1, 2
@@ -166,6 +179,8 @@ SETUP_API = {
     "mxs_turbcov": (C.c_int, [c_double_p, C.c_long, C.c_double, C.c_double, c_double_p]),
     "mxs_mkg": (C.c_long, [c_long_p, c_long_p, c_double_p, C.c_long, c_double_p, c_double_p, C.c_long,
                            C.c_long, C.c_double, c_double_p, c_double_p, C.c_double]),
+    "mxs_loc_create_embed": (C.c_long, [c_long_p, c_long_p, C.c_long, c_double_p, c_double_p, C.c_double,
+                                        C.c_double, C.c_double, C.c_int]),
     "mxs_dither_position": (None, [c_double_p, c_double_p, C.c_int, C.c_int, C.c_int, C.c_long,
                                    C.c_double]),
     "mxs_genotf": (C.c_int, [c_double_p, C.c_long, C.c_double, c_double_p, c_double_p, C.c_long,

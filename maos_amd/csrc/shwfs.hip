@@ -4634,3 +4634,6 @@ int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *n, double *ms_llt, double *ms_opd, 
 }
 
 } // extern "C"
+
+// science-field evaluation (shares this unit's screens, DM surfaces and helpers)
+#include "evl.hip"

@@ -308,3 +308,102 @@ class WfsGroup:
             self.close()
         except Exception:
             pass
+
+
+def loc_embed(locx, locy, dx, nembed=0, oversize=2.0, fftpad=True):
+    """Embedding of aperture points in a square FFT grid (loc_create_embed,
+    math/loc.c:147-179): (embed indices ix + iy * nembed, nembed)."""
+    locx = np.ascontiguousarray(locx, dtype=np.float64)
+    locy = np.ascontiguousarray(locy, dtype=np.float64)
+    emb = np.zeros(len(locx), dtype=np.int64)
+    n = C.c_long(int(nembed))
+    r = _lib.setup().mxs_loc_create_embed(lptr(emb), C.cast(C.byref(n), _lib.c_long_p), len(locx), dptr(locx),
+                                           dptr(locy), dx, dx, oversize, int(fftpad))
+    if r < 0:
+        raise ValueError("mxs_loc_create_embed failed (%d)" % r)
+    return emb, int(n.value)
+
+
+class Evl:
+    """Science-field evaluation on the GPU (mx_evl_*): the OPD of each
+    direction on the aperture points (screens and DM surfaces already
+    uploaded) and the accumulated long-exposure PSF per (direction,
+    wavelength), as MAOS's perfevl PSF path (gpu_perfevl_init_sim,
+    psfcomp_r, gpu_perfevl_save)."""
+
+    def __init__(self, amp, wvl, nembed, psfsize, embed, locx=None, locy=None, dirs=((0.0, 0.0, 0.0),)):
+        if not _initialised:
+            init(0)
+        self.amp = np.ascontiguousarray(amp, dtype=np.float64)
+        self.nloc = len(self.amp)
+        self.ndir = len(dirs)
+        self.nwvl = len(wvl)
+        self.psfsize = [int(p) for p in psfsize]
+        d = np.asarray(dirs, dtype=np.float64).reshape(self.ndir, 3)
+        keep = [np.ascontiguousarray(d[:, i]) for i in range(3)]
+        w = np.ascontiguousarray(wvl, dtype=np.float64)
+        ne = np.ascontiguousarray(nembed, dtype=np.int64)
+        ps = np.ascontiguousarray(psfsize, dtype=np.int64)
+        em = np.ascontiguousarray(np.asarray(embed, dtype=np.int64).reshape(self.nwvl, self.nloc))
+        lx = np.ascontiguousarray(locx, dtype=np.float64) if locx is not None else None
+        ly = np.ascontiguousarray(locy, dtype=np.float64) if locy is not None else None
+        h = C.c_void_p()
+        check(_lib.hip().mx_evl_init(C.byref(h), self.nloc, dptr(lx) if lx is not None else None,
+                                     dptr(ly) if ly is not None else None, dptr(self.amp), self.ndir,
+                                     dptr(keep[0]), dptr(keep[1]), dptr(keep[2]), self.nwvl, dptr(w), lptr(ne),
+                                     lptr(ps), lptr(em)))
+        self.h = h
+
+    def psf_add(self, opd):
+        """Add the PSFs of opd [ndir, nloc] (metres)."""
+        o = np.ascontiguousarray(opd, dtype=np.float64).reshape(self.ndir, self.nloc)
+        check(_lib.hip().mx_evl_psf_add(self.h, dptr(o)))
+
+    def psf_add_device(self, ptr):
+        """Add the PSFs of an FP32 [ndir, nloc] OPD at device address ptr."""
+        check(_lib.hip().mx_evl_psf_add_device(self.h, C.c_void_p(int(ptr))))
+
+    def trace(self, isim, dt, add_psf=True):
+        check(_lib.hip().mx_evl_trace(self.h, int(isim), float(dt), int(add_psf)))
+
+    def opd(self):
+        out = np.zeros((self.ndir, self.nloc))
+        check(_lib.hip().mx_evl_opd_get(self.h, dptr(out)))
+        return out
+
+    def psf(self, idir=0, iwvl=0):
+        """(mean PSF [psfsize, psfsize] with x fastest, number of PSFs added)."""
+        P = self.psfsize[iwvl]
+        out = np.zeros((P, P))
+        n = C.c_long(0)
+        check(_lib.hip().mx_evl_psf_get(self.h, idir, iwvl, dptr(out), C.cast(C.byref(n), _lib.c_long_p)))
+        return out, n.value
+
+    def reset(self):
+        check(_lib.hip().mx_evl_psf_reset(self.h))
+
+    def wait(self):
+        check(_lib.hip().mx_evl_wait(self.h))
+
+    def timing_enable(self, on=True):
+        check(_lib.hip().mx_evl_timing_enable(self.h, int(on)))
+
+    def timing(self):
+        """(timed traces, ms trace, ms row FFT pass, ms column FFT pass), summed."""
+        n = C.c_long(0)
+        a, b, c = C.c_double(0), C.c_double(0), C.c_double(0)
+        check(_lib.hip().mx_evl_timing(self.h, C.cast(C.byref(n), _lib.c_long_p),
+                                       C.cast(C.byref(a), _lib.c_double_p), C.cast(C.byref(b), _lib.c_double_p),
+                                       C.cast(C.byref(c), _lib.c_double_p)))
+        return n.value, a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.hip().mx_evl_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -117,6 +117,40 @@ long mxs_nextfftsize(long n) {
     return n3;
 }
 
+/* Embedding of a point set in a square FFT grid (loc_create_embed,
+ * math/loc.c:147-179): the grid side is ceil(max(nx, ny) * oversize), raised
+ * by nextfftsize when fftpad, unless *nembed > 0 gives it (below 0.6 of the
+ * minimum it is an error); the points are centred with the odd pad on the
+ * left. */
+long mxs_loc_create_embed(long *embed, long *nembed, long nloc, const double *locx, const double *locy, double dx,
+                          double dy, double oversize, int fftpad) {
+    if (!embed || !nembed || nloc < 1 || !locx || !locy || !(dx > 0) || !(dy > 0)) return -1;
+    double xmin = locx[0], xmax = locx[0], ymin = locy[0], ymax = locy[0];
+    for (long i = 1; i < nloc; i++) {
+        xmin = fmin(xmin, locx[i]);
+        xmax = fmax(xmax, locx[i]);
+        ymin = fmin(ymin, locy[i]);
+        ymax = fmax(ymax, locy[i]);
+    }
+    const double idx = 1. / dx, idy = 1. / dy;
+    const long nx = lround((xmax - xmin) * idx) + 1, ny = lround((ymax - ymin) * idy) + 1;
+    long nxy = (long)ceil((nx > ny ? nx : ny) * oversize);
+    if (fftpad) nxy = mxs_nextfftsize(nxy);
+    if (*nembed > 0) {
+        if (*nembed < (long)(nxy * 0.6)) return -2;
+        nxy = *nembed;
+    }
+    *nembed = nxy;
+    xmin -= (nxy - nx + 1) / 2 * dx;
+    ymin -= (nxy - ny + 1) / 2 * dy;
+    for (long i = 0; i < nloc; i++) {
+        const long ix = lround((locx[i] - xmin) * idx), iy = lround((locy[i] - ymin) * idy);
+        if (ix < 0 || iy < 0 || ix >= nxy || iy >= nxy) return -3;
+        embed[i] = ix + iy * nxy;
+    }
+    return nxy;
+}
+
 /* ------------------------------------------------------------------------
  * Geometry: maos/powfs.c:255-462
  * --------------------------------------------------------------------- */

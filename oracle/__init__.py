@@ -88,6 +88,8 @@ def lib():
                                        C.c_int, dp, dp]
         L.orc_ztilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double]
         L.orc_gtilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double, dp, C.c_double]
+        L.orc_locfft_psf.argtypes = [dp, dp, dp, C.c_long, C.POINTER(C.c_long), C.c_long, C.c_long,
+                                     C.c_double]
         L.orc_ccpcorner.argtypes = [dp, C.c_long, C.c_long, dp, C.c_long, C.c_long]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint, C.c_uint]
         L.orc_randp.restype = C.c_double
@@ -97,6 +99,19 @@ def lib():
                                    C.c_uint, C.c_uint, C.c_uint]
         _lib = L
     return _lib
+
+
+def locfft_psf(opd, amp, embed, nembed, psfsize, wvl):
+    """Unscaled |PSF|^2 over the centred psfsize^2 window, [psfsize, psfsize]
+    with x fastest (orc_locfft_psf)."""
+    opd = np.ascontiguousarray(opd, dtype=np.float64)
+    amp = np.ascontiguousarray(amp, dtype=np.float64)
+    embed = np.ascontiguousarray(embed, dtype=np.int64)
+    out = np.zeros((psfsize, psfsize))
+    rc = lib().orc_locfft_psf(out.ctypes.data_as(dp), opd.ctypes.data_as(dp), amp.ctypes.data_as(dp),
+                              len(opd), embed.ctypes.data_as(C.POINTER(C.c_long)), nembed, psfsize, wvl)
+    assert rc == 0
+    return out
 
 
 def ccpcorner(a, nx, ny):

@@ -348,6 +348,40 @@ long orc_prop(const double *phiin, long nxin, long nyin, double dxin, double oxi
 }
 
 /* ---------------------------------------------------------------------
+ * Science PSF: lib/locfft.c:90-188
+ * ------------------------------------------------------------------- */
+int orc_locfft_psf(double *psf, const double *opd, const double *amp, long nloc, const long *embed,
+                   long nembed, long psfsize, double wvl) {
+    const double wvk = 2. * M_PI / wvl;
+    if (psfsize == 1) {
+        cplx s = 0;
+        for (long i = 0; i < nloc; i++) s += amp[i] * cexp(I * wvk * opd[i]);
+        psf[0] = creal(s) * creal(s) + cimag(s) * cimag(s);
+        return 0;
+    }
+    if (psfsize < 2 || psfsize > nembed || (psfsize & 1)) return -1;
+    cplx *f = (cplx *)calloc((size_t)nembed * nembed, sizeof(cplx));
+    for (long i = 0; i < nloc; i++) {
+        if (embed[i] < 0 || embed[i] >= nembed * nembed) {
+            free(f);
+            return -1;
+        }
+        f[embed[i]] = amp[i] * cexp(I * wvk * opd[i]);
+    }
+    orc_fft2((double *)f, nembed, nembed, -1);
+    const long h = psfsize / 2;
+    for (long iy = 0; iy < psfsize; iy++) {
+        const long ky = (iy - h + nembed) % nembed;
+        for (long ix = 0; ix < psfsize; ix++) {
+            const cplx v = f[(ix - h + nembed) % nembed + ky * nembed];
+            psf[ix + iy * psfsize] = creal(v) * creal(v) + cimag(v) * cimag(v);
+        }
+    }
+    free(f);
+    return 0;
+}
+
+/* ---------------------------------------------------------------------
  * Imaging: math/matcomp.c, maos/wfsints.c
  * ------------------------------------------------------------------- */
 void orc_embed_wvf(double *wvf_, long nwvf, const double *opd, const double *amp, long nopd,

@@ -139,6 +139,18 @@ void orc_ztilt(double *grad, const double *opd, const double *amp, const double 
 void orc_gtilt(double *grad, const double *opd, const double *amp, const double *ptsx, const double *ptsy,
                long nsa, long nxsa, double dx, const double *saa, double saat);
 
+/* locfft_psf (lib/locfft.c:110-188) for one OPD and wavelength, unscaled
+ * and squared as psfcomp_r accumulates it (cuda/sim/perfevl.cu:255-290):
+ * the field amp exp(i 2 pi opd / wvl) written at embed[iloc] of a zero
+ * nembed^2 grid in point order (a later point at the same index replaces
+ * the earlier one), forward FFT, and |.|^2 of the centre psfsize^2 window
+ * (ccpcorner2center / corner2center_abs2_do, cuda/math/kernel.cu:292-307):
+ * psf[ix + iy psfsize] = |F[(ix - psfsize/2) mod nembed, (iy - psfsize/2)
+ * mod nembed]|^2.  psfsize 1: |sum amp exp(i 2 pi opd / wvl)|^2
+ * (strehlcomp, lib/locfft.c:90-100).  Returns 0, or -1 on a bad size. */
+int orc_locfft_psf(double *psf, const double *opd, const double *amp, long nloc, const long *embed,
+                   long nembed, long psfsize, double wvl);
+
 /* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
  * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ctr is replaced
  * by the four output words. */
