@@ -38,6 +38,7 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_BF16_TFLOPS = 2500.0
 HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_shwfs_mfma_lgs30m_traffic.txt")
+EVL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_psf_cols_evl30m_traffic.txt")
 KERNELS = {0: "k_shwfs_step<64,32,1>", 1: "k_shwfs_mfma<false, __bf16>", 2: "k_shwfs_mfma<false, _Float16>"}
 FORMS = {0: "fp32-radix-fft", 1: "mfma-split-bf16", 2: "mfma-split-fp16"}
 
@@ -67,11 +68,11 @@ def mfma_flops_per_subap():
     return n32 * 32 * 32 * 16 * 2 + n16 * 16 * 16 * 32 * 2
 
 
-def read_traffic():
-    """HBM bytes per launch of the MFMA imaging kernel recorded by the
-    counter runs (FETCH_SIZE + WRITE_SIZE scaled by a plain copy), or None."""
+def read_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch of a kernel recorded by the counter runs
+    (FETCH_SIZE + WRITE_SIZE scaled by a plain copy), or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             for line in f:
                 if line.startswith("bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -429,7 +430,8 @@ def evl_main(args):
                                   "4096^2 PSF accumulated" % (nloc, args.nx)},
            "kernels_us": {k: round(v, 2) for k, v in us.items()},
            "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": round(ach, 1), "peak": HBM_GBS,
-                        "unit": "GB/s", "frac": round(ach / HBM_GBS, 4), "traffic": None,
+                        "unit": "GB/s", "frac": round(ach / HBM_GBS, 4),
+                        "traffic": read_traffic(EVL_TRAFFIC_FILE) if dom[0] == "k_psf_cols<4096>" else None,
                         "fft_tflops": round(dom[2] / (us[dom[0]] * 1e-6) / 1e12, 2)},
            "algorithmic": {"k_psf_rows_bytes": by_rows, "k_psf_cols_bytes": by_cols, "k_psf_rows_flop": fl_rows,
                            "k_psf_cols_flop": fl_cols, "k_evl_opd_bytes": by_trace, "band_rows": nband}}
@@ -447,6 +449,8 @@ def evl_main(args):
                      "strehl": float(g_psf.max())}
     print(json.dumps(out), flush=True)
     ev.close()
+    if args.copy_calib > 0:
+        copy_calibration(args.copy_calib)
 
 
 def main():
