@@ -364,6 +364,18 @@ int mx_evl_trace(mx_evl_t *h, long isim, double dt, int add_psf);
 /* The last OPD traced or added from the host, [ndir][nloc] (waits). */
 int mx_evl_opd_get(mx_evl_t *h, double *opd);
 
+/* Wavefront error of the last OPD traced or added from the host, per
+ * direction (calc_ptt + calc_ptt_post, cuda/sim/perfevl.cu:60-127, with the
+ * FP64 sums of loc_calc_ptt, math/loc.c:653-704): from
+ * c = sum amp opd [1, x, y], tot = sum amp opd^2 and
+ * mcc = sum amp [1, x, y]^T [1, x, y] (loc_mcc_ptt, math/loc.c:612-650),
+ * rms[3 d + 0..2] = {tot - c0^2 / mcc00 (piston removed),
+ * c^T mcc^-1 c - c0^2 / mcc00 (tip/tilt), tot - c^T mcc^-1 c (piston, tip
+ * and tilt removed)} and coeff[3 d + 0..2] = mcc^-1 c (piston, tip, tilt).
+ * Variances in m^2 when amp sums to 1.  Either output may be NULL; needs the
+ * point coordinates.  Waits. */
+int mx_evl_wfe(mx_evl_t *h, double *rms, double *coeff);
+
 /* The mean PSF of direction idir and wavelength iwvl, psfsize^2 values with
  * x fastest, centred as corner2center (cuda/math/kernel.cu:292-307):
  * accumulated sum / (nacc (sum amp)^2), so a diffraction-limited peak is

@@ -316,6 +316,42 @@ __global__ __launch_bounds__(64) void k_strehl_fin(StrehlArgs a) {
     a.acc1[idir] += re * re + im * im;
 }
 
+// Piston/tip/tilt sums of one slice of the points, FP64 (calc_ptt,
+// cuda/sim/perfevl.cu:60-103, in the FP64 form of loc_calc_ptt_stride,
+// math/loc.c:653-704): sum amp opd, amp opd x, amp opd y, amp opd^2.
+struct PttArgs {
+    const float *opd;
+    const double *amp, *locx, *locy;
+    double *part; // [ndir][MX_STREHL_PARTS][4]
+    long nloc;
+};
+__global__ __launch_bounds__(256) void k_evl_ptt(PttArgs a) {
+    __shared__ double sh[4];
+    const int idir = blockIdx.y, part = blockIdx.x;
+    const long per = (a.nloc + MX_STREHL_PARTS - 1) / MX_STREHL_PARTS;
+    const long i0 = part * per, i1 = min(a.nloc, i0 + per);
+    const float *op = a.opd + (long)idir * a.nloc;
+    double c0 = 0, cx = 0, cy = 0, tot = 0;
+    for (long i = i0 + threadIdx.x; i < i1; i += 256) {
+        const double o = op[i], w = a.amp[i] * o;
+        c0 += w;
+        cx += w * a.locx[i];
+        cy += w * a.locy[i];
+        tot += w * o;
+    }
+    double *dst = a.part + ((long)idir * MX_STREHL_PARTS + part) * 4;
+    c0 = block_sum_d(c0, sh);
+    cx = block_sum_d(cx, sh);
+    cy = block_sum_d(cy, sh);
+    tot = block_sum_d(tot, sh);
+    if (threadIdx.x == 0) {
+        dst[0] = c0;
+        dst[1] = cx;
+        dst[2] = cy;
+        dst[3] = tot;
+    }
+}
+
 // OPD of the evaluation directions on the aperture points: the screens,
 // bilinear with periodic wrap at the frozen-flow displacement of step isim
 // (prop to aper->locs with alpha = atmscale, maos/perfevl.c:275-300 and
@@ -416,6 +452,8 @@ struct mx_evl_s {
     double thetax[MX_EVL_MAXDIR] = {}, thetay[MX_EVL_MAXDIR] = {}, hs[MX_EVL_MAXDIR] = {};
     float *d_amp = nullptr, *d_opd = nullptr;
     double *d_locx = nullptr, *d_locy = nullptr, *d_part = nullptr;
+    double *d_amp64 = nullptr, *d_ptt = nullptr; // FP64 amplitude and p/t/t partial sums (with coordinates)
+    double imcc[9] = {}, ipcc = 0;               // inverse of sum amp [1 x y]^T [1 x y], 1 / sum amp
     float2 *d_B = nullptr;
     EvlWvl w[MX_MAXWVL];
     long nacc = 0;
@@ -427,11 +465,30 @@ struct mx_evl_s {
 };
 
 namespace {
+// inverse of a symmetric 3x3 matrix by cofactors; 1 if singular
+int inv3x3(const double *m, double *out) {
+    const double c00 = m[4] * m[8] - m[5] * m[7], c01 = m[5] * m[6] - m[3] * m[8], c02 = m[3] * m[7] - m[4] * m[6];
+    const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    const double scale = fabs(m[0] * m[4] * m[8]) + 1e-300;
+    if (!(fabs(det) > 1e-12 * scale)) return 1;
+    const double id = 1. / det;
+    out[0] = c00 * id;
+    out[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    out[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    out[3] = c01 * id;
+    out[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    out[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    out[6] = c02 * id;
+    out[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    out[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+    return 0;
+}
+
 void evl_release(mx_evl_t *h) {
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    void *ptrs[] = {h->d_amp, h->d_opd, h->d_locx, h->d_locy, h->d_part, h->d_B};
+    void *ptrs[] = {h->d_amp, h->d_opd, h->d_locx, h->d_locy, h->d_part, h->d_B, h->d_amp64, h->d_ptt};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (int iw = 0; iw < MX_MAXWVL; iw++) {
@@ -550,6 +607,20 @@ int mx_evl_init(mx_evl_t **out, long nloc, const double *locx, const double *loc
         if ((rc = dev_alloc(&h->d_locx, nloc)) || (rc = dev_alloc(&h->d_locy, nloc))) return fail(rc);
         EVCHK(hipMemcpy(h->d_locx, locx, sizeof(double) * nloc, hipMemcpyHostToDevice));
         EVCHK(hipMemcpy(h->d_locy, locy, sizeof(double) * nloc, hipMemcpyHostToDevice));
+        // the p/t/t weights: mcc = sum amp [1 x y]^T [1 x y] (loc_mcc_ptt,
+        // math/loc.c:612-650), inverted in FP64
+        double m[9] = {};
+        for (long i = 0; i < nloc; i++) {
+            const double v[3] = {1., locx[i], locy[i]};
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) m[r * 3 + c] += amp[i] * v[r] * v[c];
+        }
+        if (!(m[0] > 0)) return fail(set_err(MX_EINVAL, "mx_evl_init: the amplitudes sum to %g", m[0]));
+        if (inv3x3(m, h->imcc)) return fail(set_err(MX_EINVAL, "mx_evl_init: the points span no plane"));
+        h->ipcc = 1. / m[0];
+        if ((rc = dev_alloc(&h->d_amp64, nloc)) || (rc = dev_alloc(&h->d_ptt, (size_t)ndir * MX_STREHL_PARTS * 4)))
+            return fail(rc);
+        EVCHK(hipMemcpy(h->d_amp64, amp, sizeof(double) * nloc, hipMemcpyHostToDevice));
     }
     size_t bmax = 0;
     for (int iw = 0; iw < nwvl; iw++) {
@@ -683,6 +754,40 @@ int mx_evl_opd_get(mx_evl_t *h, double *opd) {
     std::vector<float> t(n);
     HIPCHK(hipMemcpy(t.data(), h->d_opd, sizeof(float) * n, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; i++) opd[i] = t[i];
+    return MX_OK;
+}
+
+int mx_evl_wfe(mx_evl_t *h, double *rms, double *coeff) {
+    if (!h || (!rms && !coeff)) return set_err(MX_EINVAL, "mx_evl_wfe: null argument");
+    if (!h->d_ptt) return set_err(MX_ESTATE, "mx_evl_wfe: the handle was made without point coordinates");
+    HIPCHK(hipSetDevice(h->device));
+    PttArgs a{h->d_opd, h->d_amp64, h->d_locx, h->d_locy, h->d_ptt, h->nloc};
+    k_evl_ptt<<<dim3(MX_STREHL_PARTS, (unsigned)h->ndir), dim3(256), 0, h->stream>>>(a);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part((size_t)h->ndir * MX_STREHL_PARTS * 4);
+    HIPCHK(hipMemcpyAsync(part.data(), h->d_ptt, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int d = 0; d < h->ndir; d++) {
+        double c[3] = {0, 0, 0}, tot = 0;
+        for (int p = 0; p < MX_STREHL_PARTS; p++) {
+            const double *q = &part[((size_t)d * MX_STREHL_PARTS + p) * 4];
+            c[0] += q[0];
+            c[1] += q[1];
+            c[2] += q[2];
+            tot += q[3];
+        }
+        // calc_ptt_post (cuda/sim/perfevl.cu:105-127)
+        double ic[3];
+        for (int r = 0; r < 3; r++) ic[r] = h->imcc[r * 3] * c[0] + h->imcc[r * 3 + 1] * c[1] + h->imcc[r * 3 + 2] * c[2];
+        if (coeff)
+            for (int r = 0; r < 3; r++) coeff[d * 3 + r] = ic[r];
+        if (rms) {
+            const double pis = h->ipcc * c[0] * c[0], ptt = c[0] * ic[0] + c[1] * ic[1] + c[2] * ic[2];
+            rms[d * 3] = tot - pis;     // piston removed
+            rms[d * 3 + 1] = ptt - pis; // tip/tilt
+            rms[d * 3 + 2] = tot - ptt; // piston, tip and tilt removed
+        }
+    }
     return MX_OK;
 }
 

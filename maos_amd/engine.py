@@ -371,6 +371,14 @@ class Evl:
         check(_lib.hip().mx_evl_opd_get(self.h, dptr(out)))
         return out
 
+    def wfe(self):
+        """(rms [ndir, 3]: piston removed, tip/tilt, p/t/t removed; coeff
+        [ndir, 3]: piston, tip, tilt) of the last OPD (mx_evl_wfe)."""
+        rms = np.zeros((self.ndir, 3))
+        coeff = np.zeros((self.ndir, 3))
+        check(_lib.hip().mx_evl_wfe(self.h, dptr(rms), dptr(coeff)))
+        return rms, coeff
+
     def psf(self, idir=0, iwvl=0):
         """(mean PSF [psfsize, psfsize] with x fastest, number of PSFs added)."""
         P = self.psfsize[iwvl]

@@ -90,6 +90,7 @@ def lib():
         L.orc_gtilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double, dp, C.c_double]
         L.orc_locfft_psf.argtypes = [dp, dp, dp, C.c_long, C.POINTER(C.c_long), C.c_long, C.c_long,
                                      C.c_double]
+        L.orc_loc_calc_ptt.argtypes = [dp, dp, dp, dp, dp, dp, C.c_long]
         L.orc_ccpcorner.argtypes = [dp, C.c_long, C.c_long, dp, C.c_long, C.c_long]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint, C.c_uint]
         L.orc_randp.restype = C.c_double
@@ -112,6 +113,15 @@ def locfft_psf(opd, amp, embed, nembed, psfsize, wvl):
                               len(opd), embed.ctypes.data_as(C.POINTER(C.c_long)), nembed, psfsize, wvl)
     assert rc == 0
     return out
+
+
+def loc_calc_ptt(locx, locy, amp, opd):
+    """(rms [PR, TT, PTTR], coeff [piston, tip, tilt]) of orc_loc_calc_ptt."""
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (locx, locy, amp, opd)]
+    rms, coeff = np.zeros(3), np.zeros(3)
+    assert lib().orc_loc_calc_ptt(rms.ctypes.data_as(dp), coeff.ctypes.data_as(dp), *[x.ctypes.data_as(dp) for x in a],
+                                  len(a[0])) == 0
+    return rms, coeff
 
 
 def ccpcorner(a, nx, ny):

@@ -381,6 +381,48 @@ int orc_locfft_psf(double *psf, const double *opd, const double *amp, long nloc,
     return 0;
 }
 
+int orc_loc_calc_ptt(double *rms, double *coeff, const double *locx, const double *locy, const double *amp,
+                     const double *opd, long nloc) {
+    double m[3][6] = {{0}}, c[3] = {0, 0, 0}, tot = 0;
+    for (long i = 0; i < nloc; i++) {
+        const double v[3] = {1., locx[i], locy[i]};
+        for (int r = 0; r < 3; r++) {
+            for (int k = 0; k < 3; k++) m[r][k] += amp[i] * v[r] * v[k];
+            c[r] += amp[i] * opd[i] * v[r];
+        }
+        tot += amp[i] * opd[i] * opd[i];
+    }
+    const double pcc = m[0][0];
+    for (int r = 0; r < 3; r++) m[r][3 + r] = 1.;
+    for (int col = 0; col < 3; col++) { /* Gauss-Jordan with partial pivoting */
+        int piv = col;
+        for (int r = col + 1; r < 3; r++)
+            if (fabs(m[r][col]) > fabs(m[piv][col])) piv = r;
+        if (m[piv][col] == 0) return -1;
+        for (int k = 0; k < 6; k++) {
+            const double t = m[col][k];
+            m[col][k] = m[piv][k];
+            m[piv][k] = t;
+        }
+        const double d = m[col][col];
+        for (int k = 0; k < 6; k++) m[col][k] /= d;
+        for (int r = 0; r < 3; r++) {
+            if (r == col) continue;
+            const double f = m[r][col];
+            for (int k = 0; k < 6; k++) m[r][k] -= f * m[col][k];
+        }
+    }
+    double ic[3];
+    for (int r = 0; r < 3; r++) ic[r] = m[r][3] * c[0] + m[r][4] * c[1] + m[r][5] * c[2];
+    if (coeff)
+        for (int r = 0; r < 3; r++) coeff[r] = ic[r];
+    const double pis = c[0] * c[0] / pcc, ptt = c[0] * ic[0] + c[1] * ic[1] + c[2] * ic[2];
+    rms[0] = tot - pis;
+    rms[1] = ptt - pis;
+    rms[2] = tot - ptt;
+    return 0;
+}
+
 /* ---------------------------------------------------------------------
  * Imaging: math/matcomp.c, maos/wfsints.c
  * ------------------------------------------------------------------- */

@@ -151,6 +151,13 @@ void orc_gtilt(double *grad, const double *opd, const double *amp, const double 
 int orc_locfft_psf(double *psf, const double *opd, const double *amp, long nloc, const long *embed,
                    long nembed, long psfsize, double wvl);
 
+/* loc_calc_ptt (math/loc.c:653-709) with mcc from loc_mcc_ptt
+ * (math/loc.c:612-650) inverted by Gauss-Jordan elimination: rms = {piston
+ * removed, tip/tilt, piston/tip/tilt removed} variances, coeff = the fitted
+ * piston, tip and tilt.  Returns 0, or -1 if mcc is singular. */
+int orc_loc_calc_ptt(double *rms, double *coeff, const double *locx, const double *locy, const double *amp,
+                     const double *opd, long nloc);
+
 /* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
  * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ctr is replaced
  * by the four output words. */

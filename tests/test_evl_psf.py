@@ -11,7 +11,11 @@ math/loc.c:147-179): the oracle and the host setup code, pinned by
   a full-size window;
 - the Strehl-only form equals the window's centre pixel;
 - the embedding's size rule (nextfftsize of twice the extent), its centring
-  (the odd pad on the left) and its errors.
+  (the odd pad on the left) and its errors;
+- the wavefront-error decomposition (loc_calc_ptt, math/loc.c:653-709):
+  against numpy's weighted least-squares residuals (the piston-removed and
+  the piston/tip/tilt-removed variances as residual norms, the fit by
+  lstsq), and exact on planes and pistons.
 """
 import numpy as np
 import pytest
@@ -113,3 +117,28 @@ def test_oracle_strehl_is_centre_pixel():
     c = oracle.locfft_psf(opd, amp, emb, N, 8, 1.2e-6)[4, 4]
     assert s == pytest.approx(c, rel=1e-11)
     assert s == pytest.approx(abs(np.sum(amp * np.exp(2j * np.pi / 1.2e-6 * opd))) ** 2, rel=1e-12)
+
+
+def test_oracle_wfe_against_least_squares():
+    import oracle
+    locx, locy = annulus(4.0, 0.8, 1.0 / 16)
+    rng = np.random.default_rng(6)
+    amp = rng.uniform(0.3, 1.0, len(locx))
+    opd = rng.normal(size=len(locx)) * 1e-7 + 3e-7 * locx - 2e-7 * locy + 5e-7
+    rms, coeff = oracle.loc_calc_ptt(locx, locy, amp, opd)
+    w = np.sqrt(amp)
+    A = np.stack([np.ones_like(locx), locx, locy], 1)
+    fit = np.linalg.lstsq(A * w[:, None], opd * w, rcond=None)[0]
+    np.testing.assert_allclose(coeff, fit, rtol=1e-10)
+    mean = np.sum(amp * opd) / amp.sum()
+    pr = np.sum(amp * (opd - mean) ** 2)
+    pttr = np.sum(amp * (opd - A @ fit) ** 2)
+    np.testing.assert_allclose(rms, [pr, pr - pttr, pttr], rtol=1e-9)
+    # a plane has no residual; a piston has neither residual nor tip/tilt
+    rms, coeff = oracle.loc_calc_ptt(locx, locy, amp, 2e-7 + 4e-7 * locx + 1e-7 * locy)
+    np.testing.assert_allclose(coeff, [2e-7, 4e-7, 1e-7], rtol=1e-10)
+    assert abs(rms[2]) < 1e-12 * rms[1]
+    piston = np.full(len(locx), 3e-7)
+    rms, _ = oracle.loc_calc_ptt(locx, locy, amp, piston)
+    tot = np.sum(amp * piston ** 2)
+    assert np.all(np.abs(rms) < 1e-12 * tot)  # rounding of the differences only

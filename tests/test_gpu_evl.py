@@ -189,6 +189,7 @@ def test_trace_and_psf_end_to_end(lgs_small):
 
 
 def _trace_checks(engine, layers, dms, locx, locy, amp, emb, N):
+    import oracle
     dirs = [(0.0, 0.0, 0.0), (3e-5, -2e-5, 0.0), (1e-5, 1e-5, 90e3)]
     ev = engine.Evl(amp, [1.65e-6], [N], [512], emb, locx=locx, locy=locy, dirs=dirs)
     dt = 1 / 800
@@ -202,6 +203,12 @@ def _trace_checks(engine, layers, dms, locx, locy, amp, emb, N):
         print("  isim %d OPD max err %.2e of rms %.2e" % (isim, err, rms), flush=True)
         assert err <= 1e-5 * rms
         opds.append(want)
+        # wavefront error of the traced OPD against the oracle's, on the oracle OPD
+        g_rms, g_coeff = ev.wfe()
+        for d in range(len(dirs)):
+            o_rms, o_coeff = oracle.loc_calc_ptt(locx, locy, amp, want[d])
+            np.testing.assert_allclose(g_rms[d], o_rms, rtol=1e-5)
+            np.testing.assert_allclose(g_coeff[d], o_coeff, rtol=1e-5, atol=1e-6 * np.abs(o_coeff).max())
     for d in range(len(dirs)):
         got, nacc = ev.psf(d, 0)
         assert nacc == 2
