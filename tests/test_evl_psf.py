@@ -142,3 +142,31 @@ def test_oracle_wfe_against_least_squares():
     rms, _ = oracle.loc_calc_ptt(locx, locy, amp, piston)
     tot = np.sum(amp * piston ** 2)
     assert np.all(np.abs(rms) < 1e-12 * tot)  # rounding of the differences only
+
+
+def test_evl_init_rejects_unsupported_sizes():
+    """mx_evl_init checks its sizes before any GPU call: a side that is not a
+    power of two from 32 to 4096, an odd or oversized window, and direction
+    or wavelength counts out of range are errors with a message."""
+    import ctypes as C
+    from maos_amd import _lib
+    from maos_amd._lib import dptr, lptr
+    lib = _lib.hip()
+    amp = np.ones(10)
+    wvl = np.array([1e-6])
+    emb = np.arange(10, dtype=np.int64)
+
+    def init(nembed, psfsize, ndir=1, nwvl=1):
+        h = C.c_void_p()
+        ne = np.array([nembed] * max(nwvl, 1), dtype=np.int64)
+        ps = np.array([psfsize] * max(nwvl, 1), dtype=np.int64)
+        w = np.full(max(nwvl, 1), 1e-6)
+        return lib.mx_evl_init(C.byref(h), 10, None, None, dptr(amp), ndir, None, None, None, nwvl, dptr(w),
+                               lptr(ne), lptr(ps), lptr(np.tile(emb, max(nwvl, 1))))
+
+    assert init(96, 32) == -1 and b"powers of two" in lib.mx_last_error()
+    assert init(8192, 32) == -1
+    assert init(64, 7) == -1 and b"psfsize" in lib.mx_last_error()
+    assert init(64, 128) == -1
+    assert init(64, 32, ndir=0) == -1 and init(64, 32, ndir=65) == -1
+    assert init(64, 32, nwvl=5) == -1
