@@ -26,11 +26,11 @@
 //         the column is zero, FFT along y, and |.|^2 of the psfsize window
 //         rows is added to the accumulator stored transposed
 //         ([dir][x][y], contiguous along y for these writes).
-// The FFT is a Stockham radix-8 autosort (a final radix-4 or radix-2 stage
-// when log2 N is not a multiple of 3), in place: every stage reads all its
+// The FFT is a Stockham radix-16 autosort (a final radix-8, 4 or 2 stage
+// when log2 N is not a multiple of 4), in place: every stage reads all its
 // butterfly inputs into registers, waits, then writes.  One thread per
-// radix-8 butterfly (up to 1024 threads).  LDS element i lives at
-// i + i/8 (one pad per 8 complex), which spreads the strided writes of the
+// radix-16 butterfly (64 to 1024 threads).  LDS element i lives at
+// i + i/16 (one pad per 16 complex), which spreads the strided writes of the
 // first two stages over the banks.
 
 #define MX_EVL_MAXDIR 64
@@ -53,6 +53,35 @@ __device__ __forceinline__ void dft4p(float2 &a0, float2 &a1, float2 &a2, float2
     a3 = csub(t1, t3);
 }
 __device__ __forceinline__ void dft_r(float2 (&v)[4]) { dft4p(v[0], v[1], v[2], v[3]); }
+// 16 points as 4 x 4: DFT4 over x[4 n1 + n2] for each n2, twiddle
+// exp(-2 pi i n2 k1 / 16), DFT4 over n2 for each k1; output k1 + 4 k2
+__device__ __forceinline__ void dft_r(float2 (&v)[16]) {
+    constexpr float c1 = 0.92387953251128676f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
+    float2 a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        a[n2][0] = v[n2];
+        a[n2][1] = v[n2 + 4];
+        a[n2][2] = v[n2 + 8];
+        a[n2][3] = v[n2 + 12];
+        dft4p(a[n2][0], a[n2][1], a[n2][2], a[n2][3]);
+    }
+    a[1][1] = cmul(a[1][1], make_float2(c1, -s1));
+    a[1][2] = cmul(a[1][2], make_float2(h, -h));
+    a[1][3] = cmul(a[1][3], make_float2(s1, -c1));
+    a[2][1] = cmul(a[2][1], make_float2(h, -h));
+    a[2][2] = mul_negi(a[2][2]);
+    a[2][3] = cmul(a[2][3], make_float2(-h, -h));
+    a[3][1] = cmul(a[3][1], make_float2(s1, -c1));
+    a[3][2] = cmul(a[3][2], make_float2(-h, -h));
+    a[3][3] = cmul(a[3][3], make_float2(-c1, s1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        dft4p(a[0][k1], a[1][k1], a[2][k1], a[3][k1]);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = a[k2][k1];
+    }
+}
 __device__ __forceinline__ void dft_r(float2 (&v)[8]) {
     float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
     dft4p(e0, e1, e2, e3);
@@ -76,7 +105,7 @@ __device__ __forceinline__ void dft_r(float2 (&v)[8]) {
 // N).  Butterfly j of a row reads x[j + m N/R], m < R, twiddles input m by
 // w^(m k0), w = exp(-2 pi i / (NS R)), k0 = j mod NS, and writes output m to
 // (j / NS) NS R + k0 + m NS.  tw[t] = exp(-2 pi i t / N).
-__device__ __forceinline__ int lp(int i) { return i + (i >> 3); }
+__device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
 
 template <int N, int NB, int T, int R, int NS>
 __device__ __forceinline__ void fft_stage(float2 *s, const float2 *__restrict__ tw) {
@@ -116,9 +145,11 @@ __device__ __forceinline__ void fft_stage(float2 *s, const float2 *__restrict__ 
 
 template <int N, int NB, int T, int NS>
 __device__ __forceinline__ void fft_stages(float2 *s, const float2 *tw) {
-    if constexpr (NS * 8 <= N) {
+    if constexpr (NS * 16 <= N) {
+        fft_stage<N, NB, T, 16, NS>(s, tw);
+        fft_stages<N, NB, T, NS * 16>(s, tw);
+    } else if constexpr (NS * 8 == N) {
         fft_stage<N, NB, T, 8, NS>(s, tw);
-        fft_stages<N, NB, T, NS * 8>(s, tw);
     } else if constexpr (NS * 4 == N) {
         fft_stage<N, NB, T, 4, NS>(s, tw);
     } else if constexpr (NS * 2 == N) {
@@ -134,7 +165,7 @@ __device__ __forceinline__ void lds_fft(float2 *s, const float2 *tw) {
     fft_stages<N, NB, T, 1>(s, tw);
 }
 
-// rows (or columns) per workgroup, threads per workgroup (one per radix-8
+// rows (or columns) per workgroup, threads per workgroup (one per radix-16
 // butterfly, 64 to 1024) and padded LDS elements
 template <int N>
 constexpr int evl_rows() {
@@ -142,11 +173,25 @@ constexpr int evl_rows() {
 }
 template <int N>
 constexpr int evl_threads() {
-    return evl_rows<N>() * N / 8 > 1024 ? 1024 : (evl_rows<N>() * N / 8 < 64 ? 64 : evl_rows<N>() * N / 8);
+    return evl_rows<N>() * N / 16 > 1024 ? 1024 : (evl_rows<N>() * N / 16 < 64 ? 64 : evl_rows<N>() * N / 16);
 }
 template <int N>
 constexpr int evl_lds() {
-    return evl_rows<N>() * N + evl_rows<N>() * N / 8;
+    return evl_rows<N>() * N + evl_rows<N>() * N / 16;
+}
+// the column pass: half the LDS of the row pass (4096 points), so that four
+// workgroups share a CU and hide each other's HBM round trips
+template <int N>
+constexpr int evl_cols() {
+    return 4096 / N < 16 ? (4096 / N > 1 ? 4096 / N : 1) : 16;
+}
+template <int N>
+constexpr int evl_cthreads() {
+    return evl_cols<N>() * N / 16 > 1024 ? 1024 : (evl_cols<N>() * N / 16 < 64 ? 64 : evl_cols<N>() * N / 16);
+}
+template <int N>
+constexpr int evl_clds() {
+    return evl_cols<N>() * N + evl_cols<N>() * N / 16;
 }
 
 struct PsfRowArgs {
@@ -211,35 +256,42 @@ struct PsfColArgs {
     const float2 *tw; // [N]
     float *acc;       // [ndir][P (x)][P (y)]
     int ndir, nband, ylo, P, C, ncb;
+    int CK, nck; // columns per workgroup (a power of two dividing P, at most evl_cols), P / CK
 };
 
 template <int N>
-__global__ __launch_bounds__(evl_threads<N>()) void k_psf_cols(PsfColArgs a) {
-    constexpr int CM = evl_rows<N>(), T = evl_threads<N>();
-    __shared__ float2 s[evl_lds<N>()];
-    const int unit = xcd_unit((int)blockIdx.x, a.ncb * a.ndir);
-    const int idir = unit / a.ncb, cb = unit % a.ncb;
-    const int C = a.C;
-    for (int i = threadIdx.x; i < evl_lds<N>(); i += T) s[i] = make_float2(0.f, 0.f);
+__global__ __launch_bounds__(evl_cthreads<N>()) void k_psf_cols(PsfColArgs a) {
+    constexpr int CM = evl_cols<N>(), T = evl_cthreads<N>();
+    __shared__ float2 s[evl_clds<N>()];
+    const int unit = xcd_unit((int)blockIdx.x, a.nck * a.ndir);
+    const int idir = unit / a.nck, c0 = (unit % a.nck) * a.CK;
+    const int C = a.C, CK = a.CK;
+    for (int i = threadIdx.x; i < evl_clds<N>(); i += T) s[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    const float2 *src = a.B + ((long)idir * a.ncb + cb) * a.nband * C;
-    const int nin = a.nband * C;
+    // columns c0 .. c0 + CK - 1 lie in one block of C (CK <= C or CK a
+    // multiple of C); column c is at offset c % C of block c / C
+    const int nin = a.nband * CK;
     for (int i0 = threadIdx.x; i0 < nin; i0 += 4 * T) {
         float2 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (i0 + u * T < nin) v[u] = src[i0 + u * T];
+        for (int u = 0; u < 4; u++) {
+            const int i = i0 + u * T;
+            if (i < nin) {
+                const int c = c0 + i % CK, b = i / CK;
+                v[u] = a.B[(((long)idir * a.ncb + c / C) * a.nband + b) * C + c % C];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int i = i0 + u * T;
-            if (i < nin) s[lp((i % C) * N + a.ylo + i / C)] = v[u];
+            if (i < nin) s[lp((i % CK) * N + a.ylo + i / CK)] = v[u];
         }
     }
     lds_fft<N, CM, T>(s, a.tw);
     // centre window rows r < P: ky = r - P/2 (mod N)
-    float *acc = a.acc + ((long)idir * a.P + (long)cb * C) * a.P;
+    float *acc = a.acc + ((long)idir * a.P + c0) * a.P;
     // eight sums per thread per round: the accumulator loads issued together
-    const int nout = C * a.P;
+    const int nout = CK * a.P;
     for (int i0 = threadIdx.x; i0 < nout; i0 += 8 * T) {
         float old[8];
 #pragma unroll
@@ -411,7 +463,7 @@ __global__ __launch_bounds__(256) void k_evl_opd(EvlOpdArgs a) {
 }
 
 struct EvlWvl {
-    int N = 0, P = 0, C = 0, nband = 0, ylo = 0, ngroup = 0, ncb = 0;
+    int N = 0, P = 0, C = 0, nband = 0, ylo = 0, ngroup = 0, ncb = 0, CK = 0, nck = 0;
     double wvl = 0;
     int2 *d_ent = nullptr;
     int *d_rowp = nullptr;
@@ -428,7 +480,7 @@ int psf_launch(const EvlWvl &w, PsfRowArgs &ra, PsfColArgs &ca, hipStream_t st, 
     if (ev) HIPCHK(hipEventRecord(ev[0], st));
     k_psf_rows<N><<<dim3((unsigned)(w.ngroup * ra.ndir)), dim3(evl_threads<N>()), 0, st>>>(ra);
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
-    k_psf_cols<N><<<dim3((unsigned)(w.ncb * ca.ndir)), dim3(evl_threads<N>()), 0, st>>>(ca);
+    k_psf_cols<N><<<dim3((unsigned)(w.nck * ca.ndir)), dim3(evl_cthreads<N>()), 0, st>>>(ca);
     if (ev) {
         HIPCHK(hipEventRecord(ev[2], st));
         HIPCHK(hipEventSynchronize(ev[2]));
@@ -517,7 +569,7 @@ int evl_psf_queue(mx_evl_t *h, const float *d_opd) {
         }
         PsfRowArgs ra{d_opd, h->d_amp, w.d_ent, w.d_rowp, w.d_tw, h->d_B, h->nloc, h->ndir, w.nband, w.ngroup,
                       w.P, w.C, k};
-        PsfColArgs ca{h->d_B, w.d_tw, w.d_acc, h->ndir, w.nband, w.ylo, w.P, w.C, w.ncb};
+        PsfColArgs ca{h->d_B, w.d_tw, w.d_acc, h->ndir, w.nband, w.ylo, w.P, w.C, w.ncb, w.CK, w.nck};
         int rc;
         switch (w.N) {
         case 32: rc = psf_launch<32>(w, ra, ca, st, ev, h->t_pass); break;
@@ -662,12 +714,18 @@ int mx_evl_init(mx_evl_t **out, long nloc, const double *locx, const double *loc
         for (int b = 0; b < w.nband; b++) rowp[b + 1] += rowp[b];
         const int R = MX_EVL_LDS / w.N < 16 ? MX_EVL_LDS / w.N : 16;
         w.ngroup = (w.nband + R - 1) / R;
-        // column blocks: the widest power of two up to the column pass's LDS
-        // rows that divides psfsize
+        // column blocks of the scratch array: the widest power of two up to
+        // the row pass's rows that divides psfsize; columns per column-pass
+        // workgroup: the same up to its LDS columns (4096 / N, at most 16).
+        // Both are powers of two, so one divides the other.
         int C = R;
         while (w.P % C) C >>= 1;
         w.C = C;
         w.ncb = w.P / C;
+        int CK = 4096 / w.N < 16 ? (4096 / w.N > 1 ? 4096 / w.N : 1) : 16;
+        while (w.P % CK) CK >>= 1;
+        w.CK = CK;
+        w.nck = w.P / CK;
         std::vector<float2> tw(N);
         for (long t = 0; t < N; t++) {
             const double a = -2. * M_PI * (double)t / (double)N;
