@@ -217,3 +217,27 @@ def _trace_checks(engine, layers, dms, locx, locy, amp, emb, N):
         print("  dir %d PSF rel-L2 %.2e" % (d, e), flush=True)
         assert e <= PSF_TOL
     ev.close()
+
+
+def test_psf_edge_cases():
+    """One aperture point: a flat PSF, every pixel (amp / sum amp)^2 = 1.
+    The largest direction count (64) at a small FFT size, against the
+    oracle."""
+    from maos_amd import engine
+    ev = engine.Evl([0.37], [1e-6], [32], [8], [5 + 7 * 32])
+    ev.psf_add(np.array([[2.3e-7]]))
+    got, n = ev.psf(0, 0)
+    assert n == 1
+    np.testing.assert_allclose(got, 1.0, rtol=1e-6)
+    ev.close()
+    locx, locy, amp, emb = _case(1.0, 0.2, 1 / 16, 32, seed=8)
+    rng = np.random.default_rng(8)
+    ndir = 64
+    opd = rng.normal(size=(ndir, len(amp))) * 2e-7
+    ev = engine.Evl(amp, [0.8e-6], [32], [16], emb, dirs=[(0, 0, 0)] * ndir)
+    ev.psf_add(opd)
+    for d in (0, 31, 63):
+        got, _ = ev.psf(d, 0)
+        want = _oracle_mean([opd[d]], amp, emb, 32, 16, 0.8e-6)
+        assert rel_l2(got, want) <= PSF_TOL, d
+    ev.close()
