@@ -266,47 +266,42 @@ __global__ __launch_bounds__(evl_cthreads<N>()) void k_psf_cols(PsfColArgs a) {
     const int unit = xcd_unit((int)blockIdx.x, a.nck * a.ndir);
     const int idir = unit / a.nck, c0 = (unit % a.nck) * a.CK;
     const int C = a.C, CK = a.CK;
+    // Both HBM round trips are started early: the column values before the
+    // LDS is cleared, the accumulator values before the FFT.  Each lane
+    // holds at most 16 of each (nband CK <= N CK = 16 T and CK P <= 16 T).
+    const int nin = a.nband * CK, nout = CK * a.P;
+    float2 pre[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int i = (int)threadIdx.x + u * T;
+        if (i < nin) {
+            const int c = c0 + i % CK, b = i / CK;
+            pre[u] = a.B[(((long)idir * a.ncb + c / C) * a.nband + b) * C + c % C];
+        }
+    }
     for (int i = threadIdx.x; i < evl_clds<N>(); i += T) s[i] = make_float2(0.f, 0.f);
     __syncthreads();
-    // columns c0 .. c0 + CK - 1 lie in one block of C (CK <= C or CK a
-    // multiple of C); column c is at offset c % C of block c / C
-    const int nin = a.nband * CK;
-    for (int i0 = threadIdx.x; i0 < nin; i0 += 4 * T) {
-        float2 v[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int i = i0 + u * T;
-            if (i < nin) {
-                const int c = c0 + i % CK, b = i / CK;
-                v[u] = a.B[(((long)idir * a.ncb + c / C) * a.nband + b) * C + c % C];
-            }
-        }
+    for (int u = 0; u < 16; u++) {
+        const int i = (int)threadIdx.x + u * T;
+        if (i < nin) s[lp((i % CK) * N + a.ylo + i / CK)] = pre[u];
+    }
+    float *acc = a.acc + ((long)idir * a.P + c0) * a.P; // column cc = i / P, row r = i % P
+    float old[16];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int i = i0 + u * T;
-            if (i < nin) s[lp((i % CK) * N + a.ylo + i / CK)] = v[u];
-        }
+    for (int u = 0; u < 16; u++) {
+        const int i = (int)threadIdx.x + u * T;
+        if (i < nout) old[u] = acc[i];
     }
     lds_fft<N, CM, T>(s, a.tw);
     // centre window rows r < P: ky = r - P/2 (mod N)
-    float *acc = a.acc + ((long)idir * a.P + c0) * a.P;
-    // eight sums per thread per round: the accumulator loads issued together
-    const int nout = CK * a.P;
-    for (int i0 = threadIdx.x; i0 < nout; i0 += 8 * T) {
-        float old[8];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int i = i0 + u * T;
-            if (i < nout) old[u] = acc[i]; // column cc = i / P, row r = i % P
-        }
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int i = i0 + u * T;
-            if (i >= nout) continue;
-            const int cc = i / a.P, r = i % a.P;
-            const float2 v = s[lp(cc * N + ((r - a.P / 2 + N) & (N - 1)))];
-            acc[i] = old[u] + (v.x * v.x + v.y * v.y);
-        }
+    for (int u = 0; u < 16; u++) {
+        const int i = (int)threadIdx.x + u * T;
+        if (i >= nout) continue;
+        const int cc = i / a.P, r = i % a.P;
+        const float2 v = s[lp(cc * N + ((r - a.P / 2 + N) & (N - 1)))];
+        acc[i] = old[u] + (v.x * v.x + v.y * v.y);
     }
 }
 
@@ -726,6 +721,13 @@ int mx_evl_init(mx_evl_t **out, long nloc, const double *locx, const double *loc
         while (w.P % CK) CK >>= 1;
         w.CK = CK;
         w.nck = w.P / CK;
+        {
+            // the column pass keeps at most 16 values per lane of each kind
+            const int cm = 4096 / w.N < 16 ? (4096 / w.N > 1 ? 4096 / w.N : 1) : 16;
+            const int T = cm * w.N / 16 > 1024 ? 1024 : (cm * w.N / 16 < 64 ? 64 : cm * w.N / 16);
+            if ((long)w.nband * CK > 16L * T || (long)CK * w.P > 16L * T)
+                return fail(set_err(MX_EINVAL, "internal: column pass of nembed %d holds too many values", w.N));
+        }
         std::vector<float2> tw(N);
         for (long t = 0; t < N; t++) {
             const double a = -2. * M_PI * (double)t / (double)N;
