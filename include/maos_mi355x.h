@@ -361,6 +361,10 @@ int mx_evl_psf_add_device(mx_evl_t *h, const float *d_opd);
  * (mx_dm2gpu; no wrap, points outside a surface take nothing).  add_psf=1
  * then adds the PSF of that OPD (mx_evl_psf_add).  Queued. */
 int mx_evl_trace(mx_evl_t *h, long isim, double dt, int add_psf);
+/* The realisation (0 <= ireal < 32, of mx_atm2gpu_real / mx_dm2gpu_real)
+ * whose screens and DM surfaces mx_evl_trace reads; 0 by default.  No
+ * counterpart in the reference, which runs one realisation per process. */
+int mx_evl_set_real(mx_evl_t *h, int ireal);
 /* The last OPD traced or added from the host, [ndir][nloc] (waits). */
 int mx_evl_opd_get(mx_evl_t *h, double *opd);
 

@@ -124,6 +124,7 @@ HIP_API = {
     "mx_evl_psf_add_device": (C.c_int, [C.c_void_p, C.c_void_p]),
     "mx_evl_trace": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_int]),
     "mx_evl_opd_get": (C.c_int, [C.c_void_p, c_double_p]),
+    "mx_evl_set_real": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_evl_wfe": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
     "mx_evl_psf_get": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, c_long_p]),
     "mx_evl_psf_reset": (C.c_int, [C.c_void_p]),

@@ -504,6 +504,7 @@ struct mx_evl_s {
     float2 *d_B = nullptr;
     EvlWvl w[MX_MAXWVL];
     long nacc = 0;
+    int ireal = 0; // realisation whose screens and DM surfaces mx_evl_trace reads
     std::vector<float> hopd;
     bool timing = false;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -773,20 +774,21 @@ int mx_evl_psf_add_device(mx_evl_t *h, const float *d_opd) {
 int mx_evl_trace(mx_evl_t *h, long isim, double dt, int add_psf) {
     if (!h) return set_err(MX_EINVAL, "mx_evl_trace: null handle");
     if (!h->d_locx) return set_err(MX_ESTATE, "mx_evl_trace: the handle was made without point coordinates");
-    if (g_atm[0].nps < 0) return set_err(MX_ESTATE, "mx_evl_trace: no screens uploaded (mx_atm2gpu)");
+    const int r = h->ireal;
+    if (g_atm[r].nps < 0) return set_err(MX_ESTATE, "mx_evl_trace: no screens uploaded for realisation %d", r);
     HIPCHK(hipSetDevice(h->device));
     EvlOpdArgs a{};
-    a.layers = g_dlayers;
-    a.dms = g_ddms;
+    a.layers = g_dlayers + (size_t)r * MX_MAXLAYER;
+    a.dms = g_dm[r].ndm ? g_ddms + (size_t)r * MX_MAXDM : nullptr;
     a.locx = h->d_locx;
     a.locy = h->d_locy;
     a.opd = h->d_opd;
     a.nloc = h->nloc;
     a.isim = isim;
     a.dt = dt;
-    a.atmscale = g_atm[0].atmscale;
-    a.nps = g_atm[0].nps;
-    a.ndm = g_dm[0].ndm;
+    a.atmscale = g_atm[r].atmscale;
+    a.nps = g_atm[r].nps;
+    a.ndm = g_dm[r].ndm;
     for (int d = 0; d < h->ndir; d++) {
         a.thetax[d] = h->thetax[d];
         a.thetay[d] = h->thetay[d];
@@ -804,6 +806,13 @@ int mx_evl_trace(mx_evl_t *h, long isim, double dt, int add_psf) {
         h->t_n++;
     }
     return add_psf ? evl_psf_queue(h, h->d_opd) : MX_OK;
+}
+
+int mx_evl_set_real(mx_evl_t *h, int ireal) {
+    if (!h) return set_err(MX_EINVAL, "mx_evl_set_real: null handle");
+    if (ireal < 0 || ireal >= MX_MAXREAL) return set_err(MX_EINVAL, "mx_evl_set_real: ireal=%d", ireal);
+    h->ireal = ireal;
+    return MX_OK;
 }
 
 int mx_evl_opd_get(mx_evl_t *h, double *opd) {

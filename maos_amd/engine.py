@@ -366,6 +366,10 @@ class Evl:
     def trace(self, isim, dt, add_psf=True):
         check(_lib.hip().mx_evl_trace(self.h, int(isim), float(dt), int(add_psf)))
 
+    def set_real(self, ireal):
+        """Trace the screens and DM surfaces of realisation ireal."""
+        check(_lib.hip().mx_evl_set_real(self.h, int(ireal)))
+
     def opd(self):
         out = np.zeros((self.ndir, self.nloc))
         check(_lib.hip().mx_evl_opd_get(self.h, dptr(out)))

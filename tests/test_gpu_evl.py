@@ -241,3 +241,29 @@ def test_psf_edge_cases():
         want = _oracle_mean([opd[d]], amp, emb, 32, 16, 0.8e-6)
         assert rel_l2(got, want) <= PSF_TOL, d
     ev.close()
+
+
+def test_trace_other_realisation(lgs_small):
+    """mx_evl_set_real: the trace reads realisation 1's screens (here the
+    negated screens of realisation 0), then realisation 0's again."""
+    from maos_amd import engine
+    from maos_amd.powfs import Layer
+    cfg, p, layers = lgs_small
+    neg = [Layer(-L.p, L.dx, L.ht, L.vx, L.vy) for L in layers]
+    engine.init(0)
+    engine.atm2gpu(layers)
+    engine.atm2gpu(neg, ireal=1)
+    locx, locy = annulus(4.0, 0.5, 1 / 64)
+    emb, N = engine.loc_embed(locx, locy, 1 / 64, nembed=1024)
+    amp = np.full(len(locx), 1.0 / len(locx))
+    dirs = [(2e-5, 1e-5, 0.0)]
+    ev = engine.Evl(amp, [1e-6], [N], [64], emb, locx=locx, locy=locy, dirs=dirs)
+    ev.set_real(1)
+    ev.trace(3, 1 / 800, add_psf=False)
+    want = _oracle_opd(neg, [], locx, locy, dirs[0], 3, 1 / 800)
+    got = ev.opd()[0]
+    assert np.abs(got - want).max() <= 1e-5 * np.sqrt(np.mean(want ** 2))
+    ev.set_real(0)
+    ev.trace(3, 1 / 800, add_psf=False)
+    np.testing.assert_allclose(ev.opd()[0], -got, rtol=0, atol=1e-12)
+    ev.close()
