@@ -82,15 +82,23 @@ void mxs_srot_srsa(const double *salocx, const double *salocy, long nsa,
 long mxs_smooth(const double *prof, long nh, double dxnew, double *out,
                 long maxout);
 
-/* Elongation transfer function, interpolation path of mketf
- * (lib/mkdtf.c:281-378), fused with the nominal (lib/mkdtf.c:389-393).
- * prof is nh x 2.  etf is nsa*notf*notf complex, peak in corner.
- * If etf1d is not NULL it also receives the per-subaperture 1-D ETF
- * (nsa * 4*notf complex, peak in centre) from which etf is interpolated. */
+/* Elongation transfer function (mketf, lib/mkdtf.c:186-396), fused with
+ * the nominal (lib/mkdtf.c:389-393; nominal may be NULL).  prof is nh x 2
+ * (column-major: heights, then strengths).  etf is nsa*notf*notf complex,
+ * peak in corner.  Two forms, as MAOS's llt.na_interp selects them:
+ * no_interp=0 (the default there) interpolates the profile onto a uniform
+ * angle grid, FFTs it and rotates the 1-D result onto the 2-D grid
+ * (lib/mkdtf.c:281-378); it needs evenly spaced heights (returns -2
+ * otherwise, where MAOS stops with an error).  no_interp=1, or a profile of
+ * one sample, sums the Fourier phasor of every sample directly
+ * (lib/mkdtf.c:255-278): any spacing, no renormalisation.  If etf1d is not
+ * NULL the interpolating form also writes the per-subaperture 1-D ETF
+ * (nsa * 4*notf complex, peak in centre).  Returns 0, or -1 for bad sizes,
+ * -3 if the FoV exceeds the interpolation grid. */
 int mxs_mketf(double *etf, double *etf1d, const double *nominal, long notf,
               double dtheta, const double *prof, long nh, const double *srot,
               const double *srsa, long nsa, double hs, double htel,
-              double za);
+              double za, int no_interp);
 
 /* LLT pupil (setup_powfs_llt, maos/powfs.c:1182-1242): Gaussian beam of
  * waist widthp*d/2 clipped at d/2, normalised so sum(amp^2)=1 for the

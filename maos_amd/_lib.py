@@ -168,7 +168,7 @@ SETUP_API = {
     "mxs_smooth": (C.c_long, [c_double_p, C.c_long, C.c_double, c_double_p, C.c_long]),
     "mxs_mketf": (C.c_int, [c_double_p, c_double_p, c_double_p, C.c_long, C.c_double,
                             c_double_p, C.c_long, c_double_p, c_double_p, C.c_long,
-                            C.c_double, C.c_double, C.c_double]),
+                            C.c_double, C.c_double, C.c_double, C.c_int]),
     "mxs_llt_amp": (C.c_long, [C.c_double, C.c_double, C.c_double, C.c_double, c_double_p,
                                C.c_long, c_double_p]),
     "mxs_genscreen": (None, [c_double_p, C.c_long, C.c_long, C.c_double, C.c_double,

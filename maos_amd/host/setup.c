@@ -448,12 +448,59 @@ long mxs_smooth(const double *prof, long nxin, double dxnew, double *out, long m
     return nxnew;
 }
 
+/* The ETF as the Fourier transform of the sodium column projected on the
+ * sky (the direct-sum form of mketf, lib/mkdtf.c:255-278, selected there by
+ * llt.na_interp=0 or a one-sample profile).  A sample of strength w at
+ * altitude h (range r = (h - htel)/cos(za)) seen from a subaperture at
+ * distance rsa from the launch axis sits at the radial angle
+ * theta = rsa/r - rsa/hs from the focus; along the radial direction
+ * (cos srot, sin srot) it contributes w exp(-2 pi i k_r theta) at spatial
+ * frequency k_r = cos(srot) kx + sin(srot) ky, with kx, ky signed multiples
+ * of 1/(dtheta notf) in corner (FFT) order.  The phase is separable in kx
+ * and ky, so each sample costs two 1-D phasor tables and one outer product.
+ * No renormalisation: ETF(0) = sum of the strengths. */
+static void mketf_direct(cplx *etf2d_all, const double *nominal, long notf, double dtheta,
+                         const double *px, const double *pna, long nhp, const double *srot,
+                         const double *srsa, long nsa, double hs, double htel, double cosza) {
+    const double du = 1. / (dtheta * notf);
+#pragma omp parallel
+    {
+        cplx *ex = (cplx *)malloc(sizeof(cplx) * 2 * notf);
+        cplx *ey = ex + notf;
+#pragma omp for schedule(dynamic, 8)
+        for (long isa = 0; isa < nsa; isa++) {
+            cplx *e = etf2d_all + isa * notf * notf;
+            memset(e, 0, sizeof(cplx) * notf * notf);
+            const double cx = cos(srot[isa]) * du, cy = sin(srot[isa]) * du;
+            for (long ih = 0; ih < nhp; ih++) {
+                if (!(pna[ih] > 0)) continue;
+                const double theta = srsa[isa] * cosza / (px[ih] - htel) - srsa[isa] / hs;
+                for (long i = 0; i < notf; i++) {
+                    const double k = (double)(i < notf / 2 ? i : i - notf);
+                    ex[i] = cexp(-2. * M_PI * I * (theta * cx * k));
+                    ey[i] = pna[ih] * cexp(-2. * M_PI * I * (theta * cy * k));
+                }
+                for (long iy = 0; iy < notf; iy++)
+                    for (long ix = 0; ix < notf; ix++) e[ix + iy * notf] += ex[ix] * ey[iy];
+            }
+            if (nominal)
+                for (long i = 0; i < notf * notf; i++) e[i] *= nominal[i];
+        }
+        free(ex);
+    }
+}
+
 int mxs_mketf(double *etf_out, double *etf1d_out, const double *nominal, long notf,
               double dtheta, const double *prof, long nhp, const double *srot,
-              const double *srsa, long nsa, double hs, double htel, double za) {
+              const double *srsa, long nsa, double hs, double htel, double za,
+              int no_interp) {
     const double *px = prof, *pna = prof + nhp;
     const double cosza = cos(za);
-    if (nhp < 2) return -1;
+    if (nhp < 1 || notf < 2 || (notf & 1)) return -1;
+    if (no_interp || nhp == 1) {
+        mketf_direct((cplx *)etf_out, nominal, notf, dtheta, px, pna, nhp, srot, srsa, nsa, hs, htel, cosza);
+        return 0;
+    }
     double hpmin = (px[0] - htel) / cosza;
     double dhp1 = cosza / (px[1] - px[0]);
     if (fabs((px[nhp - 1] - px[0]) / ((nhp - 1) * (px[1] - px[0])) - 1.) > 1.e-3) return -2;

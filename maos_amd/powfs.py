@@ -35,6 +35,7 @@ class LltCfg:
     ox: float = 0.0         # llt.ox
     oy: float = 0.0         # llt.oy
     na_smooth: int = 1      # llt.na_smooth
+    na_interp: int = 1      # llt.na_interp: 0 sums the profile's phasors directly
 
 
 @dataclass
@@ -200,7 +201,8 @@ class Powfs:
             for iw in range(self.nwvl):
                 rc = S.mxs_mketf(dptr(self.etf[iw]), None, dptr(self.nominal[iw]), N,
                                  self.dtheta[iw], dptr(prof), nh, dptr(self.srot),
-                                 dptr(self.srsa), n, cfg.hs, sim.htel, sim.za)
+                                 dptr(self.srsa), n, cfg.hs, sim.htel, sim.za,
+                                 0 if cfg.llt.na_interp else 1)
                 if rc != 0:
                     raise RuntimeError("mketf failed (%d)" % rc)
             amp = np.zeros(4096)

@@ -112,7 +112,7 @@ def test_etf_limits():
     srsa = np.array([0.0, 10.0])
     etf = np.zeros(2 * 2 * N * N)
     rc = S().mxs_mketf(dptr(etf), None, dptr(nominal), N, dth, dptr(prof), h.size, dptr(srot),
-                       dptr(srsa), 2, 90e3, 0.0, 0.0)
+                       dptr(srsa), 2, 90e3, 0.0, 0.0, 0)
     assert rc == 0
     e = (etf[0::2] + 1j * etf[1::2]).reshape(2, N, N)
     assert np.allclose(e[0], 1.0)
