@@ -559,12 +559,18 @@ def main():
     achieved = flop_launch / t_ints / 1e12
     traffic = read_traffic() if variant == _lib.VARIANT_MFMA_BF16 else None
     is_mfma = variant in (_lib.VARIANT_MFMA_BF16, _lib.VARIANT_MFMA_FP16)
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+    # The unit that does the transforms sets the roof: the BF16/FP16 dense
+    # matrix-core peak for the MFMA forms, the FP32 vector peak for the
+    # radix-FFT form.  achieved is the algorithmic (FFT-convention) rate.
+    peak = PEAK_BF16_TFLOPS if is_mfma else PEAK_FP32_TFLOPS
+    roof = {"bound": "mfma" if is_mfma else "valu", "achieved": round(achieved, 3), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": traffic,
             "kernel": KERNELS[variant], "avg_launch_us": round(t_ints * 1e6, 2),
             "llt_kernel_us": round(t_llt * 1e6, 2), "raytrace_kernel_us": round(t_opd * 1e6, 2),
             "flop_per_launch": flop_launch,
-            "peak_kind": "fp32 dense (MI355X published): the path's results are FP32-accurate"}
+            "peak_kind": ("bf16/fp16 dense MFMA (MI355X published)" if is_mfma
+                          else "fp32 vector (MI355X published)"),
+            "frac_of_fp32_vector_peak": round(achieved / PEAK_FP32_TFLOPS, 4)}
     if is_mfma:
         ex = mfma_flops_per_subap() * p.nsa * nwfs
         roof["mfma_executed_tflops"] = round(ex / t_ints / 1e12, 2)

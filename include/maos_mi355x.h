@@ -121,7 +121,8 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal,
  * (gpu_wfsgrad_update_etf, cuda/sim/wfsinit.cu:32; maos/sim_utils.c:388):
  * etf is powfs.etfsim[iwvl].etf, nsa*notf*notf complex with the nominal
  * fused; etf2 (or NULL) is etfsim2 for profile interpolation with weight
- * etf2wt = (isim%dtrat)/dtrat (maos/wfsints.c:215-223). */
+ * etf2wt = (isim%dtrat)/dtrat (maos/wfsints.c:215-223).  Waits for the
+ * engine's queued steps before replacing the tables. */
 int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf,
                           const double *etf2, double etf2wt);
 /* Per-step weight of the second sodium profile's ETF, w2 = (isim %
@@ -135,13 +136,16 @@ int mx_wfsgrad_set_etf_weight(mx_wfsgrad_t *h, double etf2wt);
  * cuda/sim/wfsinit.cu:107; maos/wfsgrad.c:1207): mtche is nsa matrices of
  * 2 x npix (intstat->mtche column of this WFS), i0sum is nsa values
  * (intstat->i0sum).  iwfs = -1 loads the same tables into every WFS.
- * Either pointer may be NULL to leave that table unchanged. */
+ * Either pointer may be NULL to leave that table unchanged.  Waits for the
+ * engine's queued steps before replacing the tables. */
 int mx_wfsgrad_update_ref(mx_wfsgrad_t *h, int iwfs, const double *mtche,
                           const double *i0sum);
 
 /* Upload the turbulence screens (gpu_atm2gpu, cuda/sim/atm.cu:106; called
  * at maos/sim.c:79).  Values are stored as FP32; geometry stays FP64.
- * atmscale multiplies every layer (simu->atmscale). */
+ * atmscale multiplies every layer (simu->atmscale).  The call first waits
+ * for all queued GPU work (every engine and evaluation handle), so a step
+ * or trace queued before it reads the previous screens. */
 int mx_atm2gpu(const mx_atm_layer_t *layers, int nps, double atmscale);
 /* The screens of realisation ireal (0 <= ireal < 32) for batched engines;
  * mx_atm2gpu uploads realisation 0.  All realisations of an engine must have
@@ -235,9 +239,9 @@ int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *nlaunch, double *ms_llt,
 
 /* Select how the per-subaperture transforms are computed (no counterpart in
  * the reference, whose GPU code always uses cuFFT).  MX_VARIANT_MFMA_BF16
- * and MX_VARIANT_MFMA_FP16 need notf=64, nxsa=32, at most 16 sampled image
- * rows and columns per wavelength and no second ETF; AUTO (the default) uses
- * the BF16 form when those hold.  Other sizes run the FP32 FFT forms.
+ * and MX_VARIANT_MFMA_FP16 need notf=64, nxsa=32 and at most 16 sampled
+ * image rows and columns per wavelength (a second ETF is blended in a kernel
+ * of the step); AUTO (the default) uses the BF16 form when those hold.  Other sizes run the FP32 FFT forms.
  * get_variant reports the form the next step will use. */
 int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant);
 int mx_wfsgrad_get_variant(mx_wfsgrad_t *h, int *variant);

@@ -3129,6 +3129,7 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
     if (si_nx != h->npix || si_ny != N * N || !si_pp || !si_pi || !si_px)
         return set_err(MX_EINVAL, "si must be %ld x %ld (got %ld x %ld)", h->npix, N * N, si_nx, si_ny);
     HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream)); // a queued step may still read the tables replaced here
     // per-pixel list of up to 4 (index, weight) pairs
     std::vector<int4> idx(h->npix);
     std::vector<float4> wt(h->npix);
@@ -3279,6 +3280,7 @@ int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const do
     if (!h || iwvl < 0 || iwvl >= h->nwvl || !etf) return set_err(MX_EINVAL, "mx_wfsgrad_update_etf: bad argument");
     if (h->cfg.llt_nx == 0) return set_err(MX_EINVAL, "ETF given for a powfs without LLT");
     HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream)); // a queued step may still read the tables replaced here
     const long N = h->notf, nsa = h->nsa;
     std::vector<float2> t(nsa * N * N);
     int rc;
@@ -3286,7 +3288,6 @@ int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const do
         const double *src = which ? etf2 : etf;
         if (!src) {
             if (which && h->d_etf2[iwvl]) {
-                HIPCHK(hipStreamSynchronize(h->stream));
                 hipFree(h->d_etf2[iwvl]);
                 h->d_etf2[iwvl] = nullptr;
                 if (h->d_etf2_t[iwvl]) hipFree(h->d_etf2_t[iwvl]);
@@ -3328,6 +3329,7 @@ int mx_wfsgrad_set_etf_weight(mx_wfsgrad_t *h, double etf2wt) {
 int mx_wfsgrad_update_ref(mx_wfsgrad_t *h, int iwfs, const double *mtche, const double *i0sum) {
     if (!h || iwfs < -1 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_update_ref: bad handle or iwfs");
     HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream)); // a queued step may still read the filter replaced here
     const long n1 = h->nsa * 2 * h->npix;
     const int w0 = iwfs < 0 ? 0 : iwfs, w1 = iwfs < 0 ? h->nwfs : iwfs + 1;
     std::vector<float> t(n1);
@@ -3357,8 +3359,11 @@ int mx_atm2gpu_real(int ireal, const mx_atm_layer_t *layers, int nps, double atm
             return set_err(MX_EINVAL, "layer %d is invalid", i);
         total += (size_t)layers[i].nx * layers[i].ny;
     }
+    // The screens and the layer table are overwritten in place with
+    // null-stream copies, which do not wait for the engines' and evaluation
+    // handles' non-blocking streams: wait for every queued kernel first.
+    HIPCHK(hipDeviceSynchronize());
     if (A.d_data && A.bytes < total * sizeof(float)) {
-        HIPCHK(hipDeviceSynchronize()); // no queued step reads the old screens any more
         HIPCHK(hipFree(A.d_data));
         A.d_data = nullptr;
     }
