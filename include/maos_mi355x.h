@@ -399,6 +399,68 @@ int mx_evl_wait(mx_evl_t *h);
 int mx_evl_timing_enable(mx_evl_t *h, int enable);
 int mx_evl_timing(mx_evl_t *h, long *n, double *ms_trace, double *ms_rows, double *ms_cols);
 
+/* ---------------------------------------------------------------------
+ * Pyramid WFS (lib/pywfs.c; the reference's GPU form cuda/sim/pywfs.cu).
+ * ------------------------------------------------------------------- */
+typedef struct mx_pywfs_s mx_pywfs_t;
+
+/* Static description of one pyramid WFS: the pywfs_t tables pywfs_new
+ * builds (lib/pywfs.c:110-370; mxs_pywfs_* in maos_setup.h make them) and
+ * the pywfs_cfg_t keys the step reads. */
+typedef struct mx_pywfs_cfg_t {
+    long nloc;                   /* pupil points (powfs.loc) */
+    const double *locx, *locy;   /* [nloc] (m), NULL if only mx_pywfs_ints_opd is used */
+    const double *amp;           /* [nloc] */
+    const long *embed;           /* [nloc] ix + iy nembed in the FFT grid (locfft->embed) */
+    long nembed;                 /* FFT grid side (locfft->nembed), even, factors 2, 3, 5, 7, <= 4096 */
+    double dx;                   /* pupil sampling (loc->dx) */
+    int nwvl;
+    const double *wvl, *wvlwts;  /* [nwvl] */
+    const double *mask;          /* [nwvl][nembed^2] complex pyramid mask (pywfs->pyramid) */
+    const double *nominal;       /* [nembed^2] detector transfer function (pywfs->nominal, real) */
+    long nsa;
+    const double *salocx, *salocy; /* [nsa] detector subaperture centres (pywfs->saloc) */
+    int nside;                   /* pupil images: 4 (pyramid), 3, or 2 (roof) */
+    const double *sioff;         /* [nside][2] pupil-image centres in units of nembed/2 (pywfs->sioff) */
+    double modulate;             /* modulation radius (rad) */
+    int modulpos, modulring;
+    int sigmatch, raw;           /* gradient normalisation (pywfs_cfg_t) */
+    double siglev;               /* pywfs_cfg_t siglev (sigmatch 0 and raw) */
+    double thetax, thetay, hs;   /* direction (rad) and range (m, INFINITY: NGS) for the trace */
+} mx_pywfs_cfg_t;
+
+/* Create the pyramid WFS (the GPU tables of cuda/sim/pywfs.cu, set up at
+ * gpu_wfsgrad_init).  The gain is 1, saa 1 and no gradient offset until
+ * mx_pywfs_set_calib. */
+int mx_pywfs_init(mx_pywfs_t **out, const mx_pywfs_cfg_t *cfg);
+int mx_pywfs_free(mx_pywfs_t *h);
+/* The calibration pywfs_new derives from the imaging itself
+ * (lib/pywfs.c:298-368): the inverse optical gain, the normalised
+ * subaperture areas saa [nsa] and the gradients of a flat wavefront
+ * gradoff [ng nsa] (NULL: none).  Waits for queued steps. */
+int mx_pywfs_set_calib(mx_pywfs_t *h, double gain, const double *saa, const double *gradoff);
+/* pywfs_ints + pywfs_grad (lib/pywfs.c:370-522) of a given pupil OPD
+ * [nloc] (m): the intensities at siglev photons per subaperture.  Queued. */
+int mx_pywfs_ints_opd(mx_pywfs_t *h, const double *opd, double siglev);
+/* One step (the pyramid branch of wfsgrad_iwfs, maos/wfsgrad.c:400-410, as
+ * gpu_wfsgrad_queue runs it): the screens of mx_atm2gpu (and DM surfaces of
+ * mx_dm2gpu) traced onto the pupil points at step isim in the WFS
+ * direction, then the intensities and gradients.  Queued. */
+int mx_pywfs_queue(mx_pywfs_t *h, long isim, double dt, double siglev);
+/* Wait and copy the gradients [ng nsa] (x block then y block; ng = 2 for
+ * three or four sides, else nside) and the intensities [nside][nsa]; either
+ * may be NULL. */
+int mx_pywfs_sync(mx_pywfs_t *h, double *grad, double *ints);
+/* The last traced or given OPD [nloc] (waits). */
+int mx_pywfs_opd_get(mx_pywfs_t *h, double *opd);
+/* The realisation whose screens mx_pywfs_queue reads (0 by default). */
+int mx_pywfs_set_real(mx_pywfs_t *h, int ireal);
+/* Time later steps with HIP events (each timed step waits): the summed
+ * milliseconds from the first kernel of the imaging to the gradients. */
+int mx_pywfs_timing_enable(mx_pywfs_t *h, int enable);
+int mx_pywfs_timing(mx_pywfs_t *h, long *nstep, double *ms);
+int mx_pywfs_wait(mx_pywfs_t *h);
+
 #ifdef __cplusplus
 }
 #endif

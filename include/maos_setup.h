@@ -213,6 +213,76 @@ int mxs_bin_read_header(mxs_bin_t *f, uint32_t *magic, uint64_t *nx, uint64_t *n
                         char *keywords, uint64_t klen);
 int mxs_bin_read_data(mxs_bin_t *f, void *data, uint64_t bytes);
 
+/* The LGS matched-filter loop (host, FP64; maos_amd/host/loop.c).
+ *
+ * mxs_calc_dither_amp: the amplitude of a dither of npoint frames per cycle
+ * in a signal (nmod x nsim, a column per step, a WFS frame every dtrat
+ * columns) by synchronous detection, optionally detrended; combine=1 for a
+ * circular tip/tilt dither (nmod = 2) (calc_dither_amp, lib/libmisc.c:
+ * 347-414).  res (nmod, or NULL) receives each mode's amplitude; returns
+ * the first, or -1 on bad arguments. */
+double mxs_calc_dither_amp(double *res, const double *signal, long nmod, long nsim, long dtrat, long npoint,
+                           int detrend, int combine);
+/* State and keys of one WFS's tip/tilt dither loop (dither_t, maos/types.h:
+ * 470-492; powfs.dither_* keys).  Fill the keys, zero the state. */
+typedef struct mxs_dither_t {
+    int npoint, pllrat, pllskip, dtrat, alfsm, detrend; /* detrend: 1 for NGS, 0 with an LLT */
+    double amp, gpll, gdrift;                           /* dither_amp (rad), dither_gpll, dither_gdrift */
+    double delta, deltam, deltao, a2m, a2me;            /* state */
+} mxs_dither_t;
+/* Per step (maos/wfsgrad.c:584-606): the step's dither phase (cs, ss) from
+ * the last PLL output, and, from step pllskip on, the PLL update from the
+ * residual tip/tilt fsmerr[2].  Returns 0 or -1. */
+int mxs_dither_step(mxs_dither_t *d, long isim, const double *fsmerr, double *cs, double *ss);
+/* At a PLL output (maos/wfsgrad.c:638-653, 707-731): deltam, a2m and a2me
+ * from the FSM command and error histories (2 x ncol, ncol = (pllrat - 1)
+ * dtrat + 1 steps ending now), and the fold scales s0, sg of
+ * mx_wfsgrad_dither_fold.  Returns 0 or -1. */
+int mxs_dither_output(mxs_dither_t *d, const double *fsmcmds, const double *fsmerrs, long ncol, double *s0,
+                      double *sg);
+/* The dither tip/tilt tt[2] to add to the FSM error and, through the TT
+ * operator, to the gradients (maos/wfsgrad.c:734-751). */
+void mxs_dither_tt(const mxs_dither_t *d, double cs, double ss, double *tt);
+/* Sodium profile fit of accumulated LGS images (sodium_fit, one iteration,
+ * matched-filter path, maos/powfs_utils.c:146-360; no gradient offsets):
+ * model images of one-sample profiles every dh from 80 to 105 km (mketf's
+ * direct sum, then gensei), least squares with the eigenvalue thresholds of
+ * svd_pow2 (svdthres and 1e-3 of the previous), the profile scaled to sum
+ * 1 into prof (nh x 2); i0, gx, gy (any may be NULL) replaced by the model
+ * of the fitted profile.  i0in, i0, gx, gy: [ni0][nsa][npix]; sepsf
+ * [nwvl][nsa][notf^2]; nominal [nwvl][notf^2]; si per wavelength as mxs_si
+ * ([nwvl][notf^2 + 1], [nwvl][si_nzmax]); siglev [ni0].  Returns nh, -1 on
+ * bad arguments, -2 if maxnh is short. */
+long mxs_sodium_fit(double *prof, long maxnh, double *i0, double *gx, double *gy, const double *i0in, long ni0,
+                    long nsa, long notf, long npix, long nwvl, const double *sepsf, const double *nominal,
+                    const long *si_pp, const long *si_pi, const double *si_px, long si_nzmax,
+                    const double *dtheta, const double *wvlwts, const double *siglev, const double *saa,
+                    const double *srot, const double *srsa, double dh, double hs, double htel, double za,
+                    double svdthres);
+
+/* Pyramid WFS tables (pywfs_new, lib/pywfs.c:110-370).
+ *
+ * mxs_pywfs_pyramid: the focal-plane phase mask of one wavelength, notf x
+ * notf complex (x fastest), apex at (notf/2, notf/2): exp(i pi/2 h) with h
+ * the facet height in focal-plane samples, |x| + |y| for nside 4, |x| for a
+ * roof (nside 2), the three-facet form of nside 3, a pi dot for nside 1;
+ * zero outside the clear aperture of diameter nstop samples (notf without a
+ * field stop, ceil(fieldstop/dtheta/2)*2 with one, lib/pywfs.c:176-186);
+ * eskip/vskip are the half-widths (samples) of flat edges and vertex
+ * (flate/dtheta/2, flatv/dtheta/2).  Returns 0, or -1 for bad sizes. */
+int mxs_pywfs_pyramid(double *mask, long notf, int nside, long nstop, double eskip,
+                      double vskip);
+/* The detector transfer function of the pupil images, notf x notf real in
+ * FFT (corner) order: pixel box of side dsa and Gaussian blur of rms
+ * pixblur*dsa, for pupil-image sampling dx2 (lib/pywfs.c:236-262). */
+void mxs_pywfs_nominal(double *nominal, long notf, double dsa, double dx2, double pixblur);
+/* Detector subaperture centres: order x order grid of pitch dsa centred on
+ * the axis (lib/pywfs.c:281-296); returns order^2 or -1 if maxn is short. */
+long mxs_pywfs_saloc(double *x, double *y, long maxn, long order, double dsa);
+/* Centre of each of the nside pupil images relative to the grid centre, in
+ * units of notf/2 samples, [nside][2] (pywfs_mksi, lib/pywfs.c:34-107). */
+int mxs_pywfs_sioff(double *sioff, int nside);
+
 /* zfarr (math/zfarr.c:33-110): a cell file of nstep x 1 elements written one
  * per step, as MAOS saves telemetry (wfs%d_gradcl_%d.bin and the like,
  * maos/sim_utils.c:955-998).  Steps not pushed are written as empty

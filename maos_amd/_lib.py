@@ -42,6 +42,24 @@ class MxWfs(C.Structure):
                 ("wvlwts", c_double_p)]
 
 
+class MxsDither(C.Structure):
+    """mxs_dither_t: keys, then state (zero at the start)."""
+    _fields_ = [("npoint", C.c_int), ("pllrat", C.c_int), ("pllskip", C.c_int), ("dtrat", C.c_int),
+                ("alfsm", C.c_int), ("detrend", C.c_int), ("amp", C.c_double), ("gpll", C.c_double),
+                ("gdrift", C.c_double), ("delta", C.c_double), ("deltam", C.c_double), ("deltao", C.c_double),
+                ("a2m", C.c_double), ("a2me", C.c_double)]
+
+
+class MxPywfsCfg(C.Structure):
+    _fields_ = [("nloc", C.c_long), ("locx", c_double_p), ("locy", c_double_p), ("amp", c_double_p),
+                ("embed", c_long_p), ("nembed", C.c_long), ("dx", C.c_double), ("nwvl", C.c_int),
+                ("wvl", c_double_p), ("wvlwts", c_double_p), ("mask", c_double_p), ("nominal", c_double_p),
+                ("nsa", C.c_long), ("salocx", c_double_p), ("salocy", c_double_p), ("nside", C.c_int),
+                ("sioff", c_double_p), ("modulate", C.c_double), ("modulpos", C.c_int), ("modulring", C.c_int),
+                ("sigmatch", C.c_int), ("raw", C.c_int), ("siglev", C.c_double), ("thetax", C.c_double),
+                ("thetay", C.c_double), ("hs", C.c_double)]
+
+
 def dptr(a):
     """Pointer to a C-contiguous float64 numpy array (or NULL for None)."""
     if a is None:
@@ -116,6 +134,17 @@ HIP_API = {
     "mx_wfsgroup_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]),
     "mx_wfsgroup_queue": (C.c_int, [C.c_void_p, C.c_long, C.c_double]),
     "mx_wfsgroup_free": (C.c_int, [C.c_void_p]),
+    "mx_pywfs_init": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(MxPywfsCfg)]),
+    "mx_pywfs_free": (C.c_int, [C.c_void_p]),
+    "mx_pywfs_set_calib": (C.c_int, [C.c_void_p, C.c_double, c_double_p, c_double_p]),
+    "mx_pywfs_ints_opd": (C.c_int, [C.c_void_p, c_double_p, C.c_double]),
+    "mx_pywfs_queue": (C.c_int, [C.c_void_p, C.c_long, C.c_double, C.c_double]),
+    "mx_pywfs_sync": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    "mx_pywfs_opd_get": (C.c_int, [C.c_void_p, c_double_p]),
+    "mx_pywfs_set_real": (C.c_int, [C.c_void_p, C.c_int]),
+    "mx_pywfs_timing_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "mx_pywfs_timing": (C.c_int, [C.c_void_p, c_long_p, c_double_p]),
+    "mx_pywfs_wait": (C.c_int, [C.c_void_p]),
     "mx_evl_init": (C.c_int, [C.POINTER(C.c_void_p), C.c_long, c_double_p, c_double_p, c_double_p, C.c_int,
                               c_double_p, c_double_p, c_double_p, C.c_int, c_double_p, c_long_p, c_long_p,
                               c_long_p]),
@@ -183,6 +212,19 @@ SETUP_API = {
                            C.c_long, C.c_double, c_double_p, c_double_p, C.c_double]),
     "mxs_loc_create_embed": (C.c_long, [c_long_p, c_long_p, C.c_long, c_double_p, c_double_p, C.c_double,
                                         C.c_double, C.c_double, C.c_int]),
+    "mxs_calc_dither_amp": (C.c_double, [c_double_p, c_double_p, C.c_long, C.c_long, C.c_long, C.c_long, C.c_int,
+                                         C.c_int]),
+    "mxs_dither_step": (C.c_int, [C.c_void_p, C.c_long, c_double_p, c_double_p, c_double_p]),
+    "mxs_dither_output": (C.c_int, [C.c_void_p, c_double_p, c_double_p, C.c_long, c_double_p, c_double_p]),
+    "mxs_dither_tt": (None, [C.c_void_p, C.c_double, C.c_double, c_double_p]),
+    "mxs_sodium_fit": (C.c_long, [c_double_p, C.c_long, c_double_p, c_double_p, c_double_p, c_double_p, C.c_long,
+                                  C.c_long, C.c_long, C.c_long, C.c_long, c_double_p, c_double_p, c_long_p, c_long_p,
+                                  c_double_p, C.c_long, c_double_p, c_double_p, c_double_p, c_double_p, c_double_p,
+                                  c_double_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "mxs_pywfs_pyramid": (C.c_int, [c_double_p, C.c_long, C.c_int, C.c_long, C.c_double, C.c_double]),
+    "mxs_pywfs_nominal": (None, [c_double_p, C.c_long, C.c_double, C.c_double, C.c_double]),
+    "mxs_pywfs_saloc": (C.c_long, [c_double_p, c_double_p, C.c_long, C.c_long, C.c_double]),
+    "mxs_pywfs_sioff": (C.c_int, [c_double_p, C.c_int]),
     "mxs_dither_position": (None, [c_double_p, c_double_p, C.c_int, C.c_int, C.c_int, C.c_long,
                                    C.c_double]),
     "mxs_genotf": (C.c_int, [c_double_p, C.c_long, C.c_double, c_double_p, c_double_p, C.c_long,

@@ -41,7 +41,7 @@ def build_hip():
 
 def build_setup():
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(PKG, "host", f) for f in ("setup.c", "binio.c", "gensei.c", "mkg.c")]
+    srcs = [os.path.join(PKG, "host", f) for f in ("setup.c", "binio.c", "gensei.c", "mkg.c", "pywfs.c", "loop.c")]
     tmp = SETUP_LIB + ".tmp"
     _run(["gcc", "-O2", "-std=gnu11", "-fopenmp", "-fPIC", "-shared", "-Wall",
           "-o", tmp] + srcs + ["-lz", "-lm"])

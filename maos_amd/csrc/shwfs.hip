@@ -1912,7 +1912,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 
 // ------------------------------------------------------------ large fields
 // Fields larger than the one-workgroup subaperture forms hold: nwvf and notf
-// up to 1024 with prime factors 2, 3 and 5 (the 2x2 and 1x1 NGS of 15 and
+// up to 1024 with prime factors 2, 3, 5 and 7 (the 2x2 and 1x1 NGS of 15 and
 // 30 m, a 128^2 padding of an LGS subaperture).  Each (WFS, subaperture,
 // wavelength) chain c = (iwfs nsa + isa) nwvl + iwvl runs five batched
 // line-transform passes; a block transforms L lines held in LDS (rows, or L
@@ -1947,12 +1947,12 @@ __host__ __device__ constexpr int lf_lines(int n) {
     while (L > 1 && L * lf_stride(n) > MX_LF_LDS) L /= 2;
     return L;
 }
-// radix of stage s in the plan of n (radix 4 first, then 2, 3, 5); 0 past
-// the last stage or if n has another prime factor
+// radix of stage s in the plan of n (radix 4 first, then 2, 3, 5, 7); 0
+// past the last stage or if n has another prime factor
 __host__ __device__ constexpr int lf_radix(int n, int s) {
-    const int rad[4] = {4, 2, 3, 5};
+    const int rad[5] = {4, 2, 3, 5, 7};
     int r = n, c = 0;
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < 5; k++)
         while (r > 1 && r % rad[k] == 0) {
             if (c == s) return rad[k];
             c++;
@@ -2027,6 +2027,33 @@ __device__ __forceinline__ void small_dft(float2 (&v)[R], float t) {
         v[0] = cadd(v[0], s);
         v[1] = cadd(m, r);
         v[2] = csub(m, r);
+    } else if constexpr (R == 7) {
+        // pairs r, 7 - r: v[k] = v0 + sum_r cos(2 pi r k / 7) (v[r] + v[7-r])
+        //                      + t i sum_r sin(2 pi r k / 7) (v[r] - v[7-r])
+        constexpr float c[4] = {1.f, 0.62348980185873353f, -0.22252093395631440f, -0.90096886790241913f};
+        constexpr float sn[4] = {0.f, 0.78183148246802981f, 0.97492791218182361f, 0.43388373911755812f};
+        float2 a[4], b[4];
+#pragma unroll
+        for (int r = 1; r <= 3; r++) {
+            a[r] = cadd(v[r], v[7 - r]);
+            b[r] = csub(v[r], v[7 - r]);
+        }
+        const float2 v0 = v[0];
+#pragma unroll
+        for (int k = 1; k <= 3; k++) {
+            float2 p = v0, q = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int r = 1; r <= 3; r++) {
+                const int m = (r * k) % 7; // cos(2 pi m / 7), sin of it with the sign of m <= 3
+                const float cm = c[m <= 3 ? m : 7 - m], sm = m <= 3 ? sn[m] : -sn[7 - m];
+                p = make_float2(p.x + cm * a[r].x, p.y + cm * a[r].y);
+                q = make_float2(q.x + sm * b[r].x, q.y + sm * b[r].y);
+            }
+            q = mul_ti(q, t);
+            v[k] = cadd(p, q);
+            v[7 - k] = csub(p, q);
+        }
+        v[0] = cadd(v0, cadd(a[1], cadd(a[2], a[3])));
     } else { // 5: cos and sin of 2 pi/5 and 4 pi/5
         const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
         const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;
@@ -2086,7 +2113,8 @@ __device__ float2 *lds_fft_rt(float2 *src, float2 *dst, const LfPlan &P, int L, 
         if (R == 4) lds_stage<4, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
         else if (R == 2) lds_stage<2, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
         else if (R == 3) lds_stage<3, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
-        else lds_stage<5, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
+        else if (R == 5) lds_stage<5, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
+        else lds_stage<7, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
         __syncthreads();
         float2 *t = src;
         src = dst;
@@ -2671,7 +2699,7 @@ bool supported(long notf, long nxsa, long embfac) {
 }
 
 // Stockham plan of a line length n <= MX_LF_MAXN whose prime factors are 2,
-// 3 and 5: radix 4 first, then 2, 3, 5.  False if n does not factor so.
+// 3, 5 and 7: radix 4 first, then 2, 3, 5, 7.  False if n does not factor so.
 bool lf_plan(long n, LfPlan &P) {
     P = LfPlan{};
     if (n < 2 || n > MX_LF_MAXN) return false;
@@ -2899,7 +2927,7 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     if (lf && !lf_supported(p->notf, p->nxsa, p->embfac))
         return set_err(MX_EINVAL,
                        "unsupported size: notf=%ld nxsa=%ld embfac=%ld (supported: 64/32/2, 32/16/2, or nxsa a "
-                       "multiple of 16 with nxsa*embfac <= notf <= %d, both even with prime factors 2, 3 and 5)",
+                       "multiple of 16 with nxsa*embfac <= notf <= %d, both even with prime factors 2, 3, 5 and 7)",
                        p->notf, p->nxsa, p->embfac, MX_LF_MAXN);
     if (lf && p->llt_nx > 0 &&
         !(p->llt_nx == p->nxsa && (p->nxsa == 16 || p->nxsa == 32) && p->llt_nx * p->embfac == p->notf))
@@ -4642,3 +4670,4 @@ int mx_wfsgrad_timing(mx_wfsgrad_t *h, long *n, double *ms_llt, double *ms_opd, 
 
 // science-field evaluation (shares this unit's screens, DM surfaces and helpers)
 #include "evl.hip"
+#include "pywfs.hip"

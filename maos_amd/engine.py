@@ -419,3 +419,67 @@ class Evl:
             self.close()
         except Exception:
             pass
+
+
+class PywfsEngine:
+    """A pyramid WFS on the GPU (mx_pywfs_*) built from a maos_amd.pywfs.Pywfs:
+    mx_pywfs_init with its tables, then set_calib with its calibration."""
+
+    def __init__(self, pw):
+        if not _initialised:
+            init(0)
+        c = pw.cfg
+        self.pw = pw
+        self.nsa, self.nside, self.ng = pw.nsa, c.nside, pw.ng
+        self._keep = [np.ascontiguousarray(a) for a in (pw.locx, pw.locy, pw.amp, pw.embed, pw.wvl, pw.wvlwts,
+                                                         pw.mask.ravel(), pw.nominal, pw.salocx, pw.salocy,
+                                                         pw.sioff)]
+        k = self._keep
+        cfg = _lib.MxPywfsCfg(nloc=pw.nloc, locx=dptr(k[0]), locy=dptr(k[1]), amp=dptr(k[2]), embed=lptr(k[3]),
+                              nembed=pw.N, dx=c.dx, nwvl=pw.nwvl, wvl=dptr(k[4]), wvlwts=dptr(k[5]),
+                              mask=dptr(k[6]), nominal=dptr(k[7]), nsa=pw.nsa, salocx=dptr(k[8]),
+                              salocy=dptr(k[9]), nside=c.nside, sioff=dptr(k[10]), modulate=pw.modulate,
+                              modulpos=c.modulpos, modulring=c.modulring, sigmatch=c.sigmatch, raw=c.raw,
+                              siglev=c.siglev, thetax=c.thetax, thetay=c.thetay, hs=c.hs)
+        h = C.c_void_p()
+        check(_lib.hip().mx_pywfs_init(C.byref(h), C.byref(cfg)))
+        self.h = h
+        saa = np.ascontiguousarray(pw.saa, dtype=np.float64)
+        goff = None if pw.gradoff is None else np.ascontiguousarray(pw.gradoff, dtype=np.float64)
+        check(_lib.hip().mx_pywfs_set_calib(h, pw.gain, dptr(saa), dptr(goff)))
+
+    def image(self, opd, siglev=None):
+        """(ints [nside, nsa], grad [ng nsa]) of a pupil OPD."""
+        o = np.ascontiguousarray(opd, dtype=np.float64)
+        check(_lib.hip().mx_pywfs_ints_opd(self.h, dptr(o), self.pw.cfg.siglev if siglev is None else siglev))
+        return self.sync()
+
+    def queue(self, isim, dt, siglev=None):
+        check(_lib.hip().mx_pywfs_queue(self.h, isim, dt, self.pw.cfg.siglev if siglev is None else siglev))
+
+    def sync(self):
+        g = np.zeros(self.ng * self.nsa)
+        i = np.zeros(self.nside * self.nsa)
+        check(_lib.hip().mx_pywfs_sync(self.h, dptr(g), dptr(i)))
+        return i.reshape(self.nside, self.nsa), g
+
+    def opd(self):
+        o = np.zeros(self.pw.nloc)
+        check(_lib.hip().mx_pywfs_opd_get(self.h, dptr(o)))
+        return o
+
+    def timing_enable(self, enable=True):
+        check(_lib.hip().mx_pywfs_timing_enable(self.h, int(enable)))
+
+    def timing(self):
+        n, ms = C.c_long(0), C.c_double(0)
+        check(_lib.hip().mx_pywfs_timing(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def wait(self):
+        check(_lib.hip().mx_pywfs_wait(self.h))
+
+    def close(self):
+        if self.h:
+            check(_lib.hip().mx_pywfs_free(self.h))
+            self.h = None

@@ -6,6 +6,7 @@
  * MAOS: column-major, x fastest; complex arrays interleaved (re, im).
  */
 #include "../../include/maos_setup.h"
+#include "internal.h"
 
 #include <complex.h>
 #include <math.h>
@@ -730,6 +731,8 @@ static void jacobi_eig(double *A, int n, double *d, double *V) {
     }
     for (int i = 0; i < n; i++) d[i] = A[i + i * n];
 }
+
+void mxs_sym_eig(double *A, int n, double *d, double *V) { jacobi_eig(A, n, d, V); }
 
 /* (A^T W A)^-1 A^T W with the SVD thresholds of dpinv (math/blas.c:170-206,
  * svd_pow2 :441-485 with thres1=1e-7, thres2=1e-3).  A is m x n. */

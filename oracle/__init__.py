@@ -36,6 +36,15 @@ class OrcWfs(C.Structure):
                 ("wvlwts", dp)]
 
 
+class OrcPywfs(C.Structure):
+    _fields_ = [("nloc", C.c_long), ("amp", dp), ("embed", lp), ("npsf", C.c_long), ("dx", C.c_double),
+                ("nwvl", C.c_int), ("wvl", dp), ("wvlwts", dp), ("mask", dp), ("nominal", dp),
+                ("nsa", C.c_long), ("salocx", dp), ("salocy", dp), ("nside", C.c_int), ("sioff", dp),
+                ("modulate", C.c_double), ("modulpos", C.c_int), ("modulring", C.c_int),
+                ("sigmatch", C.c_int), ("raw", C.c_int), ("siglev", C.c_double), ("gain", C.c_double),
+                ("saa", dp), ("gradoff", dp)]
+
+
 class OrcAtm(C.Structure):
     _fields_ = [("nps", C.c_long), ("nx", C.c_long), ("ny", C.c_long), ("dx", C.c_double),
                 ("ox", C.c_double), ("oy", C.c_double), ("screens", C.POINTER(dp)),
@@ -92,6 +101,8 @@ def lib():
                                      C.c_double]
         L.orc_loc_calc_ptt.argtypes = [dp, dp, dp, dp, dp, dp, C.c_long]
         L.orc_ccpcorner.argtypes = [dp, C.c_long, C.c_long, dp, C.c_long, C.c_long]
+        L.orc_pywfs_ints.argtypes = [dp, dp, C.POINTER(OrcPywfs), C.c_double]
+        L.orc_pywfs_grad.argtypes = [dp, dp, C.POINTER(OrcPywfs)]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint), C.c_uint, C.c_uint]
         L.orc_randp.restype = C.c_double
         L.orc_randp.argtypes = [C.c_double, C.c_uint, C.c_uint, C.c_uint, C.c_uint, C.c_uint,
@@ -122,6 +133,20 @@ def loc_calc_ptt(locx, locy, amp, opd):
     assert lib().orc_loc_calc_ptt(rms.ctypes.data_as(dp), coeff.ctypes.data_as(dp), *[x.ctypes.data_as(dp) for x in a],
                                   len(a[0])) == 0
     return rms, coeff
+
+
+def pywfs_image(pw, opd, siglev=None):
+    """orc_pywfs_ints and orc_pywfs_grad for a maos_amd.pywfs.Pywfs (its
+    current tables and calibration): (ints [nside, nsa], grad [ng nsa])."""
+    keep = []
+    o = pw.oracle_struct(keep)
+    siglev = pw.cfg.siglev if siglev is None else siglev
+    opd = np.ascontiguousarray(opd, dtype=np.float64)
+    ints = np.zeros(pw.cfg.nside * pw.nsa)
+    lib().orc_pywfs_ints(_d(ints), _d(opd), C.byref(o), siglev)
+    grad = np.zeros(pw.ng * pw.nsa)
+    lib().orc_pywfs_grad(_d(grad), _d(ints), C.byref(o))
+    return ints.reshape(pw.cfg.nside, pw.nsa), grad
 
 
 def ccpcorner(a, nx, ny):

@@ -907,3 +907,117 @@ void orc_gtilt(double *grad, const double *opd, const double *amp, const double 
     free(gx);
     free(gy);
 }
+
+/* ---------------------------------------------------------------------
+ * Pyramid WFS: lib/pywfs.c
+ * ------------------------------------------------------------------- */
+/* bilinear weight of the image at grid coordinate (fx, fy), with mkht's
+ * rules (lib/mkh.c:62-150): weights <= EPS and points outside the grid are
+ * dropped, the rest are not renormalised */
+static double py_sample(const double *img, long n, double fx, double fy) {
+    const long ix = (long)floor(fx), iy = (long)floor(fy);
+    const double wx[2] = {1. - (fx - ix), fx - ix}, wy[2] = {1. - (fy - iy), fy - iy};
+    double s = 0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const double w = wx[b] * wy[a];
+            const long x = ix + b, y = iy + a;
+            if (w > 1e-15 && x >= 0 && x < n && y >= 0 && y < n) s += w * img[x + y * n];
+        }
+    return s;
+}
+
+void orc_pywfs_ints(double *ints, const double *opd, const orc_pywfs_t *c, double siglev) {
+    const long n = c->npsf, n2 = n / 2, nn = n * n;
+    cplx *psf = (cplx *)malloc(sizeof(cplx) * nn * c->nwvl);
+    cplx *otf = (cplx *)malloc(sizeof(cplx) * nn);
+    double *pup = (double *)calloc(nn, sizeof(double));
+    double ampnorm = 0;
+    for (long i = 0; i < c->nloc; i++) ampnorm += c->amp[i] * c->amp[i];
+    /* locfft_psf with sum2one (lib/locfft.c:110-188): centred complex PSF */
+    for (int iw = 0; iw < c->nwvl; iw++) {
+        cplx *p = psf + iw * nn;
+        memset(p, 0, sizeof(cplx) * nn);
+        const double wvk = 2. * M_PI / c->wvl[iw];
+        for (long i = 0; i < c->nloc; i++) p[c->embed[i]] = c->amp[i] * cexp(I * (wvk * opd[i]));
+        orc_fft2((double *)p, n, n, -1);
+        const double norm = 1. / (sqrt(ampnorm) * (double)n);
+        /* cfftshift: corner sample (x, y) to ((x + n2) % n, (y + n2) % n) */
+        memcpy(otf, p, sizeof(cplx) * nn);
+        for (long y = 0; y < n; y++)
+            for (long x = 0; x < n; x++) p[(x + n2) % n + ((y + n2) % n) * n] = otf[x + y * n] * norm;
+    }
+    for (int ir = 0; ir < c->modulring; ir++) {
+        const double ri = c->modulate * (ir + 1) / c->modulring;
+        const int npos = c->modulpos * (ir + 1) / c->modulring;
+        for (int ip = 0; ip < npos; ip++) {
+            const double th = 2. * M_PI * ((double)ip / npos);
+            const double px = cos(th) * ri, py = sin(th) * ri;
+            for (int iw = 0; iw < c->nwvl; iw++) {
+                const double dtheta = c->wvl[iw] / (c->dx * (double)n);
+                const long ox = (long)round(px / dtheta), oy = (long)round(py / dtheta);
+                const cplx *mask = (const cplx *)c->mask + iw * nn;
+                const cplx *p = psf + iw * nn;
+                for (long y = 0; y < n; y++)
+                    for (long x = 0; x < n; x++) {
+                        const long sx = x + ox, sy = y + oy;
+                        otf[x + y * n] = (sx >= 0 && sx < n && sy >= 0 && sy < n) ? p[sx + sy * n] * mask[x + y * n] : 0;
+                    }
+                orc_fft2((double *)otf, n, n, 1);
+                const double a = c->wvlwts[iw] / ((double)nn * npos * c->modulring);
+                for (long i = 0; i < nn; i++) pup[i] += a * (creal(otf[i]) * creal(otf[i]) + cimag(otf[i]) * cimag(otf[i]));
+            }
+        }
+    }
+    /* detector transfer function: FFT, times nominal, inverse FFT */
+    for (long i = 0; i < nn; i++) otf[i] = pup[i];
+    orc_fft2((double *)otf, n, n, -1);
+    for (long i = 0; i < nn; i++) otf[i] *= c->nominal[i];
+    orc_fft2((double *)otf, n, n, 1);
+    for (long i = 0; i < nn; i++) pup[i] = creal(otf[i]);
+    /* sampling at the subaperture centres of each pupil image
+       (pywfs_mksi, lib/pywfs.c:34-107, on the grid of lib/pywfs.c:279) */
+    const double alpha = (double)c->nsa * siglev / (double)nn;
+    for (int is = 0; is < c->nside; is++)
+        for (long isa = 0; isa < c->nsa; isa++) {
+            const double fx = c->salocx[isa] / c->dx + c->sioff[2 * is] * n2 + n2 - 0.5;
+            const double fy = c->salocy[isa] / c->dx + c->sioff[2 * is + 1] * n2 + n2 - 0.5;
+            ints[isa + is * c->nsa] = alpha * py_sample(pup, n, fx, fy);
+        }
+    free(pup);
+    free(otf);
+    free(psf);
+}
+
+void orc_pywfs_grad(double *grad, const double *ints, const orc_pywfs_t *c) {
+    const long nsa = c->nsa;
+    const int raw = c->raw || c->nside < 3;
+    const int ng = raw ? c->nside : 2;
+    double imean = 0;
+    if (c->sigmatch == 2) {
+        for (long i = 0; i < nsa * c->nside; i++) imean += ints[i];
+        imean /= (double)nsa;
+    }
+    for (long isa = 0; isa < nsa; isa++) {
+        const double *I0 = ints + isa;
+        if (raw) {
+            for (int s = 0; s < c->nside; s++) grad[isa + nsa * s] = I0[s * nsa] * c->gain / c->siglev;
+            continue;
+        }
+        double isum = 0;
+        if (c->sigmatch == 0) isum = c->siglev * c->saa[isa];
+        else if (c->sigmatch == 1)
+            for (int s = 0; s < c->nside; s++) isum += I0[s * nsa];
+        else isum = imean * c->saa[isa];
+        const double a = c->gain / isum;
+        if (c->nside == 3) {
+            grad[isa] = (I0[nsa] - I0[2 * nsa]) * a * sqrt(3.) * 0.5;
+            grad[isa + nsa] = (I0[0] - 0.5 * (I0[nsa] + I0[2 * nsa])) * a;
+        } else {
+            grad[isa] = (I0[0] - I0[nsa] + I0[2 * nsa] - I0[3 * nsa]) * a;
+            grad[isa + nsa] = (I0[0] + I0[nsa] - I0[2 * nsa] - I0[3 * nsa]) * a;
+        }
+    }
+    if (c->gradoff)
+        for (long i = 0; i < ng * nsa; i++) grad[i] -= c->gradoff[i];
+}

@@ -158,6 +158,44 @@ int orc_locfft_psf(double *psf, const double *opd, const double *amp, long nloc,
 int orc_loc_calc_ptt(double *rms, double *coeff, const double *locx, const double *locy, const double *amp,
                      const double *opd, long nloc);
 
+/* Pyramid WFS (lib/pywfs.c): the per-powfs tables and configuration. */
+typedef struct orc_pywfs_t {
+    long nloc;               /* pupil points */
+    const double *amp;       /* [nloc] */
+    const long *embed;       /* [nloc] index ix + iy npsf of each point in the FFT grid */
+    long npsf;               /* FFT grid side (= notf) */
+    double dx;               /* pupil sampling (m) */
+    int nwvl;
+    const double *wvl, *wvlwts;
+    const double *mask;      /* [nwvl][npsf^2] complex pyramid phase mask (apex at npsf/2) */
+    const double *nominal;   /* [npsf^2] detector transfer function, corner order */
+    long nsa;
+    const double *salocx, *salocy; /* [nsa] detector subaperture centres (m) */
+    int nside;
+    const double *sioff;     /* [nside][2] pupil-image centres in units of npsf/2 */
+    double modulate;         /* modulation radius (rad) */
+    int modulpos, modulring;
+    /* gradients (pywfs_grad) */
+    int sigmatch, raw;
+    double siglev, gain;     /* cfg siglev, gain (inverse optical gain) */
+    const double *saa;       /* [nsa], mean 1 */
+    const double *gradoff;   /* [ng nsa] or NULL */
+} orc_pywfs_t;
+
+/* pywfs_ints (lib/pywfs.c:370-452): ints [nside][nsa] (overwritten).  The
+ * complex PSF of each wavelength, centred and scaled so that |psf|^2 sums
+ * to 1, is shifted to each modulation position (rounded to a sample),
+ * multiplied by the pyramid mask and inverse transformed; |.|^2 summed over
+ * positions and wavelengths with weight wvlwts / (npsf^2 npos nring) is
+ * the pupil intensity, which is convolved with the detector transfer
+ * function and sampled bilinearly at the subaperture centres of each pupil
+ * image, times nsa siglev / npsf^2.  The shifted window is the full valid
+ * overlap (as cuda/sim/pywfs.cu:213-222 copies it). */
+void orc_pywfs_ints(double *ints, const double *opd, const orc_pywfs_t *c, double siglev);
+/* pywfs_grad (lib/pywfs.c:457-522): grad [ng nsa] (ng = 2 for three or
+ * four sides unless raw; else nside) minus gradoff. */
+void orc_pywfs_grad(double *grad, const double *ints, const orc_pywfs_t *c);
+
 /* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw,
  * "Parallel random numbers: as easy as 1, 2, 3", SC 2011): ctr is replaced
  * by the four output words. */

@@ -1,0 +1,165 @@
+"""GPU parity at the workloads' own scale and at long run times.
+
+- Long runs: the frozen-flow shift -v dt isim is formed and wrapped in FP64
+  on the GPU (maos/wfsgrad.c:187-188; the reference GPU code forms it in
+  FP32, cuda/sim/accphi.cu:214-215).  At isim ~ 1e6 - 1e7 a 20 m/s wind has
+  moved 25 - 250 km, thousands of screen periods, 1e6 - 1e7 screen pixels,
+  where an FP32 coordinate cannot hold the sub-pixel fraction.
+- Full-size 30 m LGS step (6 WFS x 2820 subapertures, 7 layers 4096^2),
+  every imaging form, two steps, every WFS, against the FP64 oracle.
+- Sixteen realisations of that step in one engine (7 x 4096^2 screens each,
+  7.5 GB), the first and last realisation against the oracle.
+
+Bars as everywhere: intensities relative L2 <= 2e-5, gradients <= 0.1 mas,
+OPD max relative error <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+from conftest import RAD2MAS, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+INTS_TOL = 2e-5
+GRAD_TOL_MAS = 0.1
+VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2}
+
+
+def _oracle(p, layers, mtche=None, i0sum=None):
+    import oracle
+    o = oracle.Oracle(p, mtche, i0sum)
+    o.atm(layers)
+    return o
+
+
+def _check(tag, eng, o, wfs, iwfs_eng, iwfs_cfg, isim, dt, opd=False):
+    g, ints = eng.sync(iwfs_eng)
+    ri, rg, ropd, _ = o.step(wfs[iwfs_cfg], isim, dt, want_opd=opd)
+    e_int = rel_l2(ints, ri)
+    e_g = np.abs(g - rg).max() * RAD2MAS
+    msg = "%s isim=%d wfs=%d ints=%.2e grad=%.3e mas" % (tag, isim, iwfs_eng, e_int, e_g)
+    if opd:
+        e_opd = np.abs(eng.debug_opd(iwfs_eng) - ropd).max() / np.abs(ropd).max()
+        msg += " opd=%.2e" % e_opd
+        assert e_opd <= 1e-5, msg
+    print(msg, flush=True)
+    assert e_int <= INTS_TOL, msg
+    assert e_g <= GRAD_TOL_MAS, msg
+    return ints
+
+
+def test_long_run_screen_coordinates(lgs_small):
+    """Steps near 1e6 and 1e7 (wind 5-20 m/s, 1024^2 screens of 16 m): every
+    WFS's OPD, intensities and gradients against the oracle."""
+    from maos_amd import configs, engine
+    cfg, p, layers = lgs_small
+    dt = cfg["sim"].dt
+    # what FP32 would do: at isim = 9,999,991 the fastest layer has moved
+    # 1.3e7 screen pixels; FP32 spaces such numbers a whole pixel apart, so
+    # the interpolation fraction would be lost
+    v = max(np.hypot(L.vx, L.vy) for L in layers)
+    pix = v * dt * 9_999_991 / layers[0].dx
+    assert np.spacing(np.float32(pix)) >= 1.0
+    engine.init(0)
+    engine.atm2gpu(layers)
+    wfs = configs.wfs_specs(cfg, p)
+    eng = engine.WfsGrad(p, wfs)
+    o = _oracle(p, layers)
+    ri, _, _, _ = o.step(wfs[0], 3, dt)
+    mtche, i0sum, _ = p.mtch(ri)
+    o.set_ref(mtche, i0sum)
+    eng.update_ref(mtche, i0sum)
+    print("case: lgs order20, long runs, wind up to %.1f m/s" % v, flush=True)
+    for isim in (1_000_003, 4_194_304, 9_999_991):
+        eng.queue(isim, dt)
+        for iw in range(len(wfs)):
+            _check("long-run", eng, o, wfs, iw, iw, isim, dt, opd=True)
+    eng.close()
+
+
+@pytest.fixture(scope="module")
+def lgs30m_full():
+    from maos_amd import configs
+    from maos_amd.powfs import Powfs, genatm
+    cfg = configs.lgs30m(nx=4096, order=60)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    layers = genatm(cfg["atm"])
+    return cfg, p, layers
+
+
+@pytest.mark.timeout(300)
+def test_full_size_lgs_step_all_forms(lgs30m_full):
+    """The 30 m 6 x 60 x 60 LGS step at full size (nsa = 2820, 4096^2
+    screens): every WFS at two steps, every imaging form."""
+    from maos_amd import configs, engine
+    cfg, p, layers = lgs30m_full
+    dt = cfg["sim"].dt
+    engine.init(0)
+    engine.atm2gpu(layers)
+    wfs = configs.wfs_specs(cfg, p)
+    eng = engine.WfsGrad(p, wfs)
+    o = _oracle(p, layers)
+    ri, _, _, _ = o.step(wfs[0], 3, dt)
+    mtche, i0sum, _ = p.mtch(ri)
+    o.set_ref(mtche, i0sum)
+    eng.update_ref(mtche, i0sum)
+    print("case: lgs30m full nsa=%d, 6 WFS, steps 880 and 881" % p.nsa, flush=True)
+    ref = {}
+    for isim in (880, 881):
+        for iw in range(len(wfs)):
+            ref[isim, iw] = o.step(wfs[iw], isim, dt)[:2]
+    for name, var in VARIANTS.items():
+        eng.set_variant(var)
+        for isim in (880, 881):
+            eng.queue(isim, dt)
+            for iw in range(len(wfs)):
+                g, ints = eng.sync(iw)
+                ri, rg = ref[isim, iw]
+                e_int = rel_l2(ints, ri)
+                e_g = np.abs(g - rg).max() * RAD2MAS
+                msg = "%s isim=%d wfs=%d ints=%.2e grad=%.3e mas" % (name, isim, iw, e_int, e_g)
+                print(msg, flush=True)
+                assert e_int <= INTS_TOL, msg
+                assert e_g <= GRAD_TOL_MAS, msg
+    eng.close()
+
+
+@pytest.mark.timeout(400)
+def test_sixteen_realisations_full_scale(lgs30m_full):
+    """Sixteen realisations of the full LGS step in one engine, each with 7
+    screens of 4096^2 (7.5 GB in all) and its own r0 ~ U[0.12, 0.25] m.
+    Realisation k's screens are the base screens rolled by a k-dependent
+    offset and scaled by (r0_k / r0)^(-5/6), which keeps von Karman
+    statistics at r0_k without generating 112 screens.  Realisations 0 and
+    15, WFS 0 and 5, at two steps against the oracle."""
+    from maos_amd import configs, engine
+    from maos_amd.powfs import Layer
+    cfg, p, base = lgs30m_full
+    dt = cfg["sim"].dt
+    nreal = 16
+    r0s = np.random.RandomState(2025).uniform(0.12, 0.25, nreal)
+    engine.init(0)
+    kept = {}
+    for k in range(nreal):
+        s = (r0s[k] / cfg["atm"].r0) ** (-5.0 / 6.0)
+        layers = [Layer(np.roll(L.p, (k * 613, k * 1031), axis=(0, 1)) * s, L.dx, L.ht, L.vx, L.vy) for L in base]
+        engine.atm2gpu(layers, ireal=k)
+        if k in (0, nreal - 1):
+            kept[k] = layers
+    print("case: 16 realisations x 7 x 4096^2 uploaded", flush=True)
+    wfs = configs.wfs_specs(cfg, p)
+    nw = len(wfs)
+    eng = engine.WfsGrad(p, wfs, nreal=nreal)
+    o0 = _oracle(p, kept[0])
+    ri, _, _, _ = o0.step(wfs[0], 3, dt)
+    mtche, i0sum, _ = p.mtch(ri)
+    eng.update_ref(mtche, i0sum)
+    oracles = {k: _oracle(p, kept[k], mtche, i0sum) for k in kept}
+    for isim in (40, 41):
+        eng.queue(isim, dt)
+        got = {}
+        for k, o in oracles.items():
+            for iw in (0, 5):
+                got[k, iw] = _check("real %d" % k, eng, o, wfs, k * nw + iw, iw, isim, dt)
+        assert rel_l2(got[0, 0], got[nreal - 1, 0]) > 1e-2   # the realisations differ
+    eng.close()
