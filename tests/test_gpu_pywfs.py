@@ -83,6 +83,9 @@ def test_pywfs_radix7_grid():
     from maos_amd import engine
     pw = _pw(D=8.0, din=1.2, order=16, dx=1.0 / 32, modulpos=8, saat=0.0)
     assert pw.N == 784
+    # gain and offsets calibrated (through the oracle's images), so that the
+    # gradients are angles and the 0.1 mas bar applies
+    pw.calibrate(lambda p: (lambda opd: oracle.pywfs_image(p, opd)))
     print("case: pyramid 8 m, N=784, nsa=%d" % pw.nsa, flush=True)
     eng = engine.PywfsEngine(pw)
     opd = _opd(pw, 5, rms=2e-7)
