@@ -383,6 +383,17 @@ int mx_evl_opd_get(mx_evl_t *h, double *opd);
  * Variances in m^2 when amp sums to 1.  Either output may be NULL; needs the
  * point coordinates.  Waits. */
 int mx_evl_wfe(mx_evl_t *h, double *rms, double *coeff);
+/* mx_evl_wfe of an FP32 OPD [ndir][nloc] already in device memory
+ * (d_opd), allocating nothing: no hipMalloc, hipFree or allocating library
+ * call.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  The
+ * partial sums go to a static device array of the library (130 KiB per GPU,
+ * held until the process ends; room for 64 directions, the most a handle
+ * has), whose address is looked up at mx_init (or at the first call on a
+ * GPU).  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (from several threads or streams) race.  Same results as mx_evl_wfe
+ * (the same sums in the same order).  Waits for its kernels. */
+int mx_evl_wfe_noalloc(mx_evl_t *h, const float *d_opd, double *rms, double *coeff);
 
 /* The mean PSF of direction idir and wavelength iwvl, psfsize^2 values with
  * x fastest, centred as corner2center (cuda/math/kernel.cu:292-307):

@@ -155,6 +155,7 @@ HIP_API = {
     "mx_evl_opd_get": (C.c_int, [C.c_void_p, c_double_p]),
     "mx_evl_set_real": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_evl_wfe": (C.c_int, [C.c_void_p, c_double_p, c_double_p]),
+    "mx_evl_wfe_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, c_double_p, c_double_p]),
     "mx_evl_psf_get": (C.c_int, [C.c_void_p, C.c_int, C.c_int, c_double_p, c_long_p]),
     "mx_evl_psf_reset": (C.c_int, [C.c_void_p]),
     "mx_evl_wait": (C.c_int, [C.c_void_p]),

@@ -399,6 +399,18 @@ __global__ __launch_bounds__(256) void k_evl_ptt(PttArgs a) {
     }
 }
 
+// The four p/t/t sums of each direction from its MX_STREHL_PARTS partial
+// sums, added in part order (the order mx_evl_wfe adds them on the host, so
+// the totals are the same numbers): one thread per (direction, sum).
+__global__ __launch_bounds__(256) void k_evl_ptt_fin(const double *part, double *tot, int ndir) {
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (i >= ndir * 4) return;
+    const int d = i / 4, k = i % 4;
+    double s = 0;
+    for (int p = 0; p < MX_STREHL_PARTS; p++) s += part[((long)d * MX_STREHL_PARTS + p) * 4 + k];
+    tot[i] = s;
+}
+
 // OPD of the evaluation directions on the aperture points: the screens,
 // bilinear with periodic wrap at the frozen-flow displacement of step isim
 // (prop to aper->locs with alpha = atmscale, maos/perfevl.c:275-300 and
@@ -530,6 +542,24 @@ int inv3x3(const double *m, double *out) {
     out[7] = (m[1] * m[6] - m[0] * m[7]) * id;
     out[8] = (m[0] * m[4] - m[1] * m[3]) * id;
     return 0;
+}
+
+// calc_ptt_post (cuda/sim/perfevl.cu:105-127) of the sums [ndir][4] =
+// (sum amp opd, ... x, ... y, sum amp opd^2)
+void wfe_post(const mx_evl_t *h, const double *sums, double *rms, double *coeff) {
+    for (int d = 0; d < h->ndir; d++) {
+        const double *c = sums + 4 * d, tot = c[3];
+        double ic[3];
+        for (int r = 0; r < 3; r++) ic[r] = h->imcc[r * 3] * c[0] + h->imcc[r * 3 + 1] * c[1] + h->imcc[r * 3 + 2] * c[2];
+        if (coeff)
+            for (int r = 0; r < 3; r++) coeff[d * 3 + r] = ic[r];
+        if (rms) {
+            const double pis = h->ipcc * c[0] * c[0], ptt = c[0] * ic[0] + c[1] * ic[1] + c[2] * ic[2];
+            rms[d * 3] = tot - pis;     // piston removed
+            rms[d * 3 + 1] = ptt - pis; // tip/tilt
+            rms[d * 3 + 2] = tot - ptt; // piston, tip and tilt removed
+        }
+    }
 }
 
 void evl_release(mx_evl_t *h) {
@@ -836,27 +866,35 @@ int mx_evl_wfe(mx_evl_t *h, double *rms, double *coeff) {
     std::vector<double> part((size_t)h->ndir * MX_STREHL_PARTS * 4);
     HIPCHK(hipMemcpyAsync(part.data(), h->d_ptt, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    for (int d = 0; d < h->ndir; d++) {
-        double c[3] = {0, 0, 0}, tot = 0;
-        for (int p = 0; p < MX_STREHL_PARTS; p++) {
-            const double *q = &part[((size_t)d * MX_STREHL_PARTS + p) * 4];
-            c[0] += q[0];
-            c[1] += q[1];
-            c[2] += q[2];
-            tot += q[3];
-        }
-        // calc_ptt_post (cuda/sim/perfevl.cu:105-127)
-        double ic[3];
-        for (int r = 0; r < 3; r++) ic[r] = h->imcc[r * 3] * c[0] + h->imcc[r * 3 + 1] * c[1] + h->imcc[r * 3 + 2] * c[2];
-        if (coeff)
-            for (int r = 0; r < 3; r++) coeff[d * 3 + r] = ic[r];
-        if (rms) {
-            const double pis = h->ipcc * c[0] * c[0], ptt = c[0] * ic[0] + c[1] * ic[1] + c[2] * ic[2];
-            rms[d * 3] = tot - pis;     // piston removed
-            rms[d * 3 + 1] = ptt - pis; // tip/tilt
-            rms[d * 3 + 2] = tot - ptt; // piston, tip and tilt removed
-        }
-    }
+    std::vector<double> sums((size_t)h->ndir * 4, 0.);
+    for (int d = 0; d < h->ndir; d++)
+        for (int p = 0; p < MX_STREHL_PARTS; p++)
+            for (int k = 0; k < 4; k++) sums[d * 4 + k] += part[((size_t)d * MX_STREHL_PARTS + p) * 4 + k];
+    wfe_post(h, sums.data(), rms, coeff);
+    return MX_OK;
+}
+
+int mx_evl_wfe_noalloc(mx_evl_t *h, const float *d_opd, double *rms, double *coeff) {
+    if (!h || !d_opd || (!rms && !coeff)) return set_err(MX_EINVAL, "mx_evl_wfe_noalloc: null argument");
+    if (!h->d_amp64) return set_err(MX_ESTATE, "mx_evl_wfe_noalloc: the handle was made without point coordinates");
+    if ((long)h->ndir * (MX_STREHL_PARTS + 1) * 4 > MX_SCRATCH_DOUBLES)
+        return set_err(MX_EINVAL, "mx_evl_wfe_noalloc: %d directions need more than the %d-double scratch array",
+                       h->ndir, MX_SCRATCH_DOUBLES);
+    HIPCHK(hipSetDevice(h->device));
+    double *sc = nullptr;
+    if (const int rc = scratch_on(h->device, &sc)) return rc;
+    // scratch: partial sums [ndir][MX_STREHL_PARTS][4], then totals [ndir][4];
+    // every entry read is written first by this call
+    double *part = sc, *tot = sc + (size_t)h->ndir * MX_STREHL_PARTS * 4;
+    PttArgs a{d_opd, h->d_amp64, h->d_locx, h->d_locy, part, h->nloc};
+    k_evl_ptt<<<dim3(MX_STREHL_PARTS, (unsigned)h->ndir), dim3(256), 0, h->stream>>>(a);
+    HIPCHK(hipGetLastError());
+    k_evl_ptt_fin<<<dim3((unsigned)((h->ndir * 4 + 255) / 256)), dim3(256), 0, h->stream>>>(part, tot, h->ndir);
+    HIPCHK(hipGetLastError());
+    double sums[MX_EVL_MAXDIR * 4];
+    HIPCHK(hipMemcpyAsync(sums, tot, sizeof(double) * 4 * h->ndir, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    wfe_post(h, sums, rms, coeff);
     return MX_OK;
 }
 

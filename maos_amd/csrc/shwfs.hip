@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2677,6 +2678,36 @@ DevLayer *g_ddms = nullptr; // device [MX_MAXREAL][MX_MAXDM]
 double *g_datmscale = nullptr; // device [MX_MAXREAL]
 int g_device = 0;
 
+// Working memory of the entry points that allocate nothing
+// (mx_evl_wfe_noalloc): a fixed array in the code object, one copy per GPU,
+// held until the process ends.  On the host, `scratch` names a host-side
+// stand-in, not device memory: kernels and copies take the device address
+// that hipGetSymbolAddress returns for the current GPU (g_scratch_dev).
+constexpr int MX_SCRATCH_DOUBLES = 16640; // 130 KiB: 64 directions x (64 partial + 1 total) x 4 sums
+constexpr int MX_SCRATCH_MAXDEV = 64;
+static __device__ __align__(256) double scratch[MX_SCRATCH_DOUBLES];
+double *g_scratch_dev[MX_SCRATCH_MAXDEV] = {};
+std::mutex g_scratch_mu;
+
+// device address of `scratch` on GPU `device` (which must be current),
+// looked up once per GPU; MX_OK or an error code with the message set
+int scratch_on(int device, double **out) {
+    *out = nullptr;
+    if (device < 0 || device >= MX_SCRATCH_MAXDEV)
+        return set_err(MX_EINVAL, "device %d beyond the %d GPUs the scratch table covers", device, MX_SCRATCH_MAXDEV);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(MX_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                           hipGetErrorString(e));
+        g_scratch_dev[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_dev[device];
+    return MX_OK;
+}
+
 template <typename T>
 int dev_alloc(T **p, size_t n) {
     if (*p) return 0;
@@ -2907,6 +2938,8 @@ int mx_init(int device) {
     if (device < 0 || device >= n) return set_err(MX_EINVAL, "device %d out of range (%d GPUs)", device, n);
     HIPCHK(hipSetDevice(device));
     g_device = device;
+    double *sp = nullptr;
+    if (const int rc = scratch_on(device, &sp)) return rc; // the address of this GPU's copy of `scratch`
     return MX_OK;
 }
 

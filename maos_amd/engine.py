@@ -383,6 +383,14 @@ class Evl:
         check(_lib.hip().mx_evl_wfe(self.h, dptr(rms), dptr(coeff)))
         return rms, coeff
 
+    def wfe_noalloc(self, ptr):
+        """wfe() of an FP32 [ndir, nloc] OPD at device address ptr, with no
+        allocation (mx_evl_wfe_noalloc)."""
+        rms = np.zeros((self.ndir, 3))
+        coeff = np.zeros((self.ndir, 3))
+        check(_lib.hip().mx_evl_wfe_noalloc(self.h, C.c_void_p(int(ptr)), dptr(rms), dptr(coeff)))
+        return rms, coeff
+
     def psf(self, idir=0, iwvl=0):
         """(mean PSF [psfsize, psfsize] with x fastest, number of PSFs added)."""
         P = self.psfsize[iwvl]
