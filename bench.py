@@ -39,8 +39,8 @@ PEAK_BF16_TFLOPS = 2500.0
 HBM_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_shwfs_mfma_lgs30m_traffic.txt")
 EVL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_psf_cols_evl30m_traffic.txt")
-KERNELS = {0: "k_shwfs_step<64,32,1>", 1: "k_shwfs_mfma<false, __bf16>", 2: "k_shwfs_mfma<false, _Float16>"}
-FORMS = {0: "fp32-radix-fft", 1: "mfma-split-bf16", 2: "mfma-split-fp16"}
+KERNELS = {0: "k_shwfs_fft", 1: "k_shwfs_mfma<false, __bf16>", 2: "k_shwfs_mfma<false, _Float16>"}
+FORMS = {0: "fp32-radix8-fft", 1: "mfma-split-bf16", 2: "mfma-split-fp16"}
 
 
 def flops_per_subap(N, nxsa, nps, npix, llt=True, etf=True):
@@ -580,10 +580,23 @@ def main():
     if args.copy_calib > 0:
         copy_calibration(args.copy_calib)
     other = []
-    for v in (0, 1, 2):
-        if v != variant and eng.variant_ok(v):
-            fps_other, isim = time_variant(eng, v, isim, dt, 100)
-            other.append({"form": FORMS[v], "kernel": KERNELS[v], "frames_per_s": round(fps_other * nwfs, 2)})
+    # every other imaging form, and the FP32 register-FFT kernel that the
+    # FP32 form falls back to: frames/s and the imaging kernel's time
+    forms = [(v, FORMS[v], KERNELS[v], False) for v in (0, 1, 2) if v != variant and eng.variant_ok(v)]
+    forms.append((0, "fp32-register-fft", "k_shwfs_step<64,32,1>", True))
+    for v, form, kname, reg in forms:
+        if reg:
+            os.environ["MX_FFT_REG"] = "1"
+        fps_other, isim = time_variant(eng, v, isim, dt, 100)
+        eng.timing_enable(True)
+        for _ in range(20):
+            eng.queue(isim, dt)
+            isim += 1
+        nl_o, _, _, ms_o = eng.timing()
+        eng.timing_enable(False)
+        os.environ.pop("MX_FFT_REG", None)
+        other.append({"form": form, "kernel": kname, "frames_per_s": round(fps_other * nwfs, 2),
+                      "imaging_kernel_us": round(ms_o / nl_o * 1e3, 2)})
     eng.set_variant(variant)
 
     out = {"metric": METRIC, "value": round(fps, 2), "unit": "WFS frames/s", "n_gpus": 1,

@@ -1486,6 +1486,10 @@ struct MfmaWvl {
     const int4 *si_c;        // [npix] compact indices (iy*16 + ix) into the 16x16 image
     const float4 *si_w;      // [npix]
     float inv_wvl;
+    // FP32 radix-8 form: w(uy) exp(+2 pi i uy ys / 64) [uy < 32][16 sampled
+    // rows ys], and the 16 sampled columns xs (the list padded with its first)
+    const float2 *fy;
+    const int *fxs;
 };
 
 struct MfmaArgs {
@@ -1588,6 +1592,64 @@ __device__ __forceinline__ void cis_rev(float x, float am, float &re, float &im)
     const float r = x - rintf(x);
     re = am * __builtin_amdgcn_cosf(r);
     im = am * __builtin_amdgcn_sinf(r);
+}
+
+// Outputs and pixel processing of one subaperture (shwfs_grad,
+// maos/utils.c:983-1122) by one wave: lane p < npix holds pixel p's
+// intensity pv (npix <= 64).  Noise, the intensities, the FP64 i0 and dither
+// sums, then the matched-filter or CoG gradient (raw moments for sigmatch 2,
+// finished by k_sigmatch2).
+__device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int isa, int lane, float pv) {
+    const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    const int p = lane;
+    if (p < a.npix) {
+        if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
+        a.ints[ibase + p] = pv;
+        if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
+        dither_add(a.dith, iwfs, ibase + p, pv);
+        if (a.phytype == MX_PTYPE_MF) {
+            const float *m = a.mtche + ibase * 2 + 2 * p;
+            s0 = m[0] * pv;
+            s1 = m[1] * pv;
+            s2 = pv;
+        } else {
+            const float iI = pv - a.cogoff;
+            if (iI > a.cogthres) {
+                s0 = iI * (float)(p % a.pixpsax);
+                s1 = iI * (float)(p / a.pixpsax);
+                s2 = iI;
+            }
+        }
+    }
+    seg_sum<64>(s0);
+    seg_sum<64>(s1);
+    seg_sum<64>(s2);
+    if (lane == 0 && !(a.saa[isa] < a.saat)) {
+        float gx = 0.f, gy = 0.f;
+        if (a.sigmatch == 2) { // raw moments; k_sigmatch2 forms the gradients
+            gx = s0;
+            gy = s1;
+        } else if (a.phytype == MX_PTYPE_MF) {
+            float scale = 1.f;
+            if (a.sigmatch == 1) {
+                scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
+                if (scale <= 0.f || scale > 10.f || !isfinite(scale)) scale = 1.f;
+            }
+            gx = s0 * scale;
+            gy = s1 * scale;
+        } else {
+            float sum = s2;
+            if (a.sigmatch == 0)
+                sum = a.i0sum ? a.i0sum[(long)iwfs * a.nsa + isa] : a.sigtot * a.saa[isa];
+            if (sum > a.cogthres) {
+                gx = (s0 / sum - (float)(a.pixpsax - 1) * 0.5f) * a.pixthetax;
+                gy = (s1 / sum - (float)(a.pixpsay - 1) * 0.5f) * a.pixthetay;
+            }
+        }
+        a.grad[(long)iwfs * 2 * a.nsa + isa] = gx;
+        a.grad[(long)iwfs * 2 * a.nsa + a.nsa + isa] = gy;
+    }
 }
 
 // waves per SIMD the register budget is sized for
@@ -1856,60 +1918,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         __syncthreads();
     }
     if constexpr (UPLINK) return;
-    // outputs and pixel processing (shwfs_grad, maos/utils.c:983-1122), wave 0
-    if (wave == 0) {
-        const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
-        float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-        const int p = lane;
-        if (p < a.npix) {
-            if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
-            a.ints[ibase + p] = pv;
-            if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
-            dither_add(a.dith, iwfs, ibase + p, pv);
-            if (a.phytype == MX_PTYPE_MF) {
-                const float *m = a.mtche + ibase * 2 + 2 * p;
-                s0 = m[0] * pv;
-                s1 = m[1] * pv;
-                s2 = pv;
-            } else {
-                const float iI = pv - a.cogoff;
-                if (iI > a.cogthres) {
-                    s0 = iI * (float)(p % a.pixpsax);
-                    s1 = iI * (float)(p / a.pixpsax);
-                    s2 = iI;
-                }
-            }
-        }
-        seg_sum<64>(s0);
-        seg_sum<64>(s1);
-        seg_sum<64>(s2);
-        if (lane == 0 && !(a.saa[isa] < a.saat)) {
-            float gx = 0.f, gy = 0.f;
-            if (a.sigmatch == 2) { // raw moments; k_sigmatch2 forms the gradients
-                gx = s0;
-                gy = s1;
-            } else if (a.phytype == MX_PTYPE_MF) {
-                float scale = 1.f;
-                if (a.sigmatch == 1) {
-                    scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
-                    if (scale <= 0.f || scale > 10.f || !isfinite(scale)) scale = 1.f;
-                }
-                gx = s0 * scale;
-                gy = s1 * scale;
-            } else {
-                float sum = s2;
-                if (a.sigmatch == 0)
-                    sum = a.i0sum ? a.i0sum[(long)iwfs * a.nsa + isa] : a.sigtot * a.saa[isa];
-                if (sum > a.cogthres) {
-                    gx = (s0 / sum - (float)(a.pixpsax - 1) * 0.5f) * a.pixthetax;
-                    gy = (s1 / sum - (float)(a.pixpsay - 1) * 0.5f) * a.pixthetay;
-                }
-            }
-            a.grad[(long)iwfs * 2 * a.nsa + isa] = gx;
-            a.grad[(long)iwfs * 2 * a.nsa + a.nsa + isa] = gy;
-        }
-    }
+    // outputs and pixel processing, wave 0
+    if (wave == 0) shwfs_pixels(a, iwfs, isa, lane, pv);
 }
+
+#include "shwfs_fft.hip"
 
 // ------------------------------------------------------------ large fields
 // Fields larger than the one-workgroup subaperture forms hold: nwvf and notf
@@ -2586,6 +2599,8 @@ struct mx_wfsgrad_s {
     uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};     // BF16
     uint16_t *d_gx16[MX_MAXWVL] = {}, *d_gy16[MX_MAXWVL] = {}; // FP16
     int4 *d_sic[MX_MAXWVL] = {};
+    float2 *d_fy[MX_MAXWVL] = {};  // FP32 radix-8 form: pass F table
+    int *d_fxs[MX_MAXWVL] = {};    // and the sampled columns
     float2 *d_etf_t[MX_MAXWVL] = {};
     // second ETF profile of the MFMA forms (half-plane order) and the blend
     // w1 etf + w2 etf2 that the imaging kernel reads
@@ -2919,9 +2934,16 @@ std::vector<uint16_t> build_Gh(const std::vector<int> &rows) {
 }
 // form used by the next step: the requested one, or for AUTO the MFMA form
 // whenever its size and sampling limits hold; -1 if the request cannot run
-int effective_variant(const mx_wfsgrad_t *h) {
+// the half-plane tables (N = 64, nxsa = 32, <= 16 sampled rows and columns
+// per wavelength, a second ETF in half-plane form) that the MFMA forms and
+// the FP32 radix-8 form read
+bool half_tables_ok(const mx_wfsgrad_t *h) {
     bool ok = h->mfma_size;
     for (int w = 0; w < h->nwvl; w++) ok = ok && h->mfma_wvl_ok[w] && (!h->d_etf2[w] || h->d_etf2_t[w]);
+    return ok;
+}
+int effective_variant(const mx_wfsgrad_t *h) {
+    const bool ok = half_tables_ok(h);
     if (h->variant == MX_VARIANT_FFT32) return MX_VARIANT_FFT32;
     if (h->variant == MX_VARIANT_MFMA_BF16 || h->variant == MX_VARIANT_MFMA_FP16) return ok ? h->variant : -1;
     return ok ? MX_VARIANT_MFMA_BF16 : MX_VARIANT_FFT32;
@@ -3165,6 +3187,8 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
         if (h->d_gx16[w]) hipFree(h->d_gx16[w]);
         if (h->d_gy16[w]) hipFree(h->d_gy16[w]);
         if (h->d_sic[w]) hipFree(h->d_sic[w]);
+        if (h->d_fy[w]) hipFree(h->d_fy[w]);
+        if (h->d_fxs[w]) hipFree(h->d_fxs[w]);
         if (h->d_etf_t[w]) hipFree(h->d_etf_t[w]);
         if (h->d_etf2_t[w]) hipFree(h->d_etf2_t[w]);
         if (h->d_etfb_t[w]) hipFree(h->d_etfb_t[w]);
@@ -3264,6 +3288,21 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
             HIPCHK(hipMemcpy(h->d_gx[iwvl], gx.data(), gx.size() * 2, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(h->d_gy[iwvl], gy.data(), gy.size() * 2, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(h->d_sic[iwvl], sic.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
+            // the FP32 radix-8 form's last pass: w(uy) exp(+2 pi i uy ys / 64)
+            // for uy < 32 at the sampled rows (FP64 angles reduced mod 64),
+            // and the sampled columns
+            std::vector<float2> fy(32 * MX_MFMA_NS, make_float2(0.f, 0.f));
+            for (int uy = 0; uy < 32; uy++)
+                for (size_t r = 0; r < ys.size(); r++) {
+                    const double w = uy == 0 ? 1. : 2., ang = 2. * M_PI * (double)((uy * ys[r]) % 64) / 64.;
+                    fy[uy * MX_MFMA_NS + r] = make_float2((float)(w * std::cos(ang)), (float)(w * std::sin(ang)));
+                }
+            std::vector<int> fxs(MX_MFMA_NS, xs.empty() ? 0 : xs[0]);
+            std::copy(xs.begin(), xs.end(), fxs.begin());
+            if ((rc = dev_alloc(&h->d_fy[iwvl], fy.size())) || (rc = dev_alloc(&h->d_fxs[iwvl], fxs.size())))
+                return rc;
+            HIPCHK(hipMemcpy(h->d_fy[iwvl], fy.data(), sizeof(float2) * fy.size(), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->d_fxs[iwvl], fxs.data(), sizeof(int) * fxs.size(), hipMemcpyHostToDevice));
             if (nominal && h->cfg.llt_nx == 0) {
                 std::vector<float> nt(MX_HALF);
                 for (long uy = 0; uy < N / 2; uy++)
@@ -3649,7 +3688,16 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     MfmaArgs &m = h->arg_mfma;
     m = MfmaArgs{};
     const bool f16 = variant == MX_VARIANT_MFMA_FP16;
-    if (variant == MX_VARIANT_MFMA_BF16 || f16) {
+    // the FP32 form runs the radix-8 kernel on the half-plane tables where
+    // they exist, else the register-FFT kernel on the full tables
+    // (MX_FFT_REG=1 in the environment keeps the register-FFT kernel, so
+    // that tests and the bench can run both FP32 kernels)
+    bool fftform = variant == MX_VARIANT_FFT32 && half_tables_ok(h);
+    {
+        const char *e = getenv("MX_FFT_REG");
+        if (e && atoi(e) != 0) fftform = false;
+    }
+    if (variant == MX_VARIANT_MFMA_BF16 || f16 || fftform) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
             hw[w].etf_t = h->d_etf2_t[w] ? h->d_etfb_t[w] : h->d_etf_t[w];
@@ -3659,6 +3707,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             hw[w].si_c = h->d_sic[w];
             hw[w].si_w = h->d_si_w[w];
             hw[w].inv_wvl = (float)(1. / h->wvl[w]);
+            hw[w].fy = h->d_fy[w];
+            hw[w].fxs = h->d_fxs[w];
         }
         if (std::memcmp(hw, h->h_mwv, sizeof(hw)) != 0) {
             HIPCHK(hipMemcpyAsync(h->d_mwv, hw, sizeof(hw), hipMemcpyHostToDevice, h->stream));
@@ -3950,7 +4000,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         g.noise = a.noise;
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)),
                    dim3((unsigned)std::min(h->lf_threads, 256)), &g};
-    } else if (mf) {
+    } else if (mf || fftform) {
         bool two = false;
         for (int w = 0; w < h->nwvl; w++) two = two || h->d_etf2_t[w];
         if (two) {
@@ -3968,10 +4018,10 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             L[nl++] = {reinterpret_cast<const void *>(&k_etf_blend),
                        dim3((unsigned)std::min<long>((b.n * h->nwvl + 255) / 256, 8192)), dim3(256), &b};
         }
-        L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16>)
-                       : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16>),
-                   dim3((unsigned)(h->nsa * h->nwfs)),
-                   dim3(256), &m};
+        const void *kimg = fftform ? reinterpret_cast<const void *>(&k_shwfs_fft)
+                           : f16   ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16>)
+                                   : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16>);
+        L[nl++] = {kimg, dim3((unsigned)(h->nsa * h->nwfs)), dim3(256), &m};
     } else {
         const long spw = 64 / N, ngroup = (h->nsa + spw - 1) / spw;
         L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_shwfs_step<64, 32, 1>)
@@ -4002,7 +4052,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
              (direct ? 8 : 0) | (variant << 4) | (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) |
              (h->lf ? h->lf_threads << 14 : 0) | (h->cfg.sigmatch == 2 ? 1 << 25 : 0) |
-             (mf && h->d_etf2_t[0] ? 1 << 24 : 0);
+             ((mf || fftform) && h->d_etf2_t[0] ? 1 << 24 : 0) | (fftform ? 1 << 6 : 0);
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;

@@ -62,12 +62,16 @@ def _lgs_ref(p, o, wfs, dt, isim=3):
     return mtche, i0sum
 
 
-VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2}  # _lib.VARIANT_*
+# _lib.VARIANT_*; fft32_reg is the FP32 form with its register-FFT kernel
+# (MX_FFT_REG=1), which it also runs where the half-plane tables do not apply
+VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2, "fft32_reg": 0}
 
 
-@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16"])
-def test_lgs_step_parity(lgs_small, variant):
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16", "fft32_reg"])
+def test_lgs_step_parity(lgs_small, variant, monkeypatch):
     cfg, p, layers = lgs_small
+    if variant == "fft32_reg":
+        monkeypatch.setenv("MX_FFT_REG", "1")
     print("case: lgs order20 nsa=%d notf=%d nwfs=%d 1024^2 screens, %s"
           % (p.nsa, p.notf, len(cfg["wfs"]), variant), flush=True)
     eng, wfs = _engine(cfg, p, layers)
@@ -95,20 +99,22 @@ def test_lgs_step_parity(lgs_small, variant):
             assert e_opd < 1e-5
             # the MFMA forms compute the uplink OTF with the same split
             # products as the image, so they get the image's bar
-            assert e_lotf < (1e-5 if variant == "fft32" else INTS_TOL)
+            assert e_lotf < (1e-5 if variant.startswith("fft32") else INTS_TOL)
             assert e_int <= INTS_TOL
             assert e_g <= GRAD_TOL_MAS
     eng.close()
 
 
-@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16"])
-def test_ngs_two_wavelengths(ngs_two_wvl, variant):
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16", "fft32_reg"])
+def test_ngs_two_wavelengths(ngs_two_wvl, variant, monkeypatch):
     """N=64 without LLT or ETF (nominal DTF only), two wavelengths with their
     own si and weights, two WFS off axis: both imaging forms against the
     oracle."""
     from maos_amd import _lib
     cfg, p, layers = ngs_two_wvl
     print("case: ngs two wavelengths nsa=%d notf=%d, %s" % (p.nsa, p.notf, variant), flush=True)
+    if variant == "fft32_reg":
+        monkeypatch.setenv("MX_FFT_REG", "1")
     eng, wfs = _engine(cfg, p, layers)
     assert eng.variant() == _lib.VARIANT_MFMA_BF16  # AUTO picks the MFMA form here
     want = VARIANTS[variant]

@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
 
 INTS_TOL = 2e-5
 GRAD_TOL_MAS = 0.1
-VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2}
+# fft32_reg: the FP32 form with its register-FFT kernel (MX_FFT_REG=1)
+VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2, "fft32_reg": 0}
 
 
 def _oracle(p, layers, mtche=None, i0sum=None):
@@ -88,7 +89,7 @@ def lgs30m_full():
 
 
 @pytest.mark.timeout(300)
-def test_full_size_lgs_step_all_forms(lgs30m_full):
+def test_full_size_lgs_step_all_forms(lgs30m_full, monkeypatch):
     """The 30 m 6 x 60 x 60 LGS step at full size (nsa = 2820, 4096^2
     screens): every WFS at two steps, every imaging form."""
     from maos_amd import configs, engine
@@ -109,6 +110,8 @@ def test_full_size_lgs_step_all_forms(lgs30m_full):
         for iw in range(len(wfs)):
             ref[isim, iw] = o.step(wfs[iw], isim, dt)[:2]
     for name, var in VARIANTS.items():
+        if name == "fft32_reg":
+            monkeypatch.setenv("MX_FFT_REG", "1")
         eng.set_variant(var)
         for isim in (880, 881):
             eng.queue(isim, dt)
