@@ -8,7 +8,8 @@ before the timed region.  One frame is one WFS x one step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
                     [--variant auto|fft32|mfma] [--copy-calib BYTES]
-                    [--nreal N] [--hetero] [--evl] [--no-graph]
+                    [--nreal N] [--hetero] [--evl] [--pywfs [--pywfs-parity]]
+                    [--no-graph]
 
 Prints one JSON line.  The CPU baseline is the FP64 oracle (a C restatement
 of MAOS's CPU path) timed on this host on a bounded sample of frames.
@@ -453,6 +454,93 @@ def evl_main(args):
         copy_calibration(args.copy_calib)
 
 
+def pywfs_main(args):
+    """Pyramid WFS frames/s: the single-conjugate 30 m pyramid of MAOS's
+    scao_pywfs.conf (powfs_pyngs.conf and pywfs.conf defaults): 60 x 60
+    subapertures, pupil sampled at 1/32 m (715,336 points), FFT grid
+    3 x the pupil (2916^2 = 2^2 3^6), 0.7 um, four sides, modulation radius
+    5 lambda/D at 32 positions, sigmatch 2, traced through the seven 4096^2
+    screens each step.  The GPU calibrates the sensor first (pywfs_new's
+    sequence).  --pywfs-parity also compares one frame with the FP64 oracle
+    (about a minute of host time)."""
+    from maos_amd import configs, engine
+    from maos_amd.powfs import genatm
+    from maos_amd.pywfs import Pywfs, PywfsCfg
+
+    t0 = time.time()
+    cfg = configs.lgs30m(nx=args.nx, order=args.order)
+    layers = genatm(cfg["atm"])
+    dt = cfg["sim"].dt
+    pw = Pywfs(PywfsCfg(D=30.0, din=3.6, order=60, dx=1.0 / 32))
+    engine.init(0)
+    engine.atm2gpu(layers)
+
+    def gpu(p):
+        def f(opd):
+            e = engine.PywfsEngine(p)
+            out = e.image(opd)
+            e.close()
+            return out
+        return f
+    pw.calibrate(gpu)
+    eng = engine.PywfsEngine(pw)
+    log("pywfs: %d points, %d^2 FFT, nsa %d, %d modulation positions (%.1fs)"
+        % (pw.nloc, pw.N, pw.nsa, pw.cfg.modulpos, time.time() - t0))
+    isim = 0
+    for _ in range(args.warmup):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.queue(isim, dt)
+        isim += 1
+    eng.wait()
+    sec = time.perf_counter() - t1
+    fps = args.steps / sec
+    eng.timing_enable(True)
+    nt = max(3, min(20, args.steps // 5))
+    for _ in range(nt):
+        eng.queue(isim, dt)
+        isim += 1
+    n, ms = eng.timing()
+    eng.timing_enable(False)
+    N = pw.N
+    # algorithmic work of one frame: the field's 2-D FFT, per modulation
+    # position a 2-D inverse FFT, and the detector's forward and inverse
+    # 2-D FFTs (5 M log2 M each, M = N^2); HBM: every line pass reads and
+    # writes the N^2 complex grid once, a 2-D transform is two line passes
+    # and a transpose
+    M = float(N) * N
+    npos = pw.cfg.modulpos * pw.cfg.modulring
+    nfft = 1 + npos + 2
+    flop = nfft * 5 * M * math.log2(M)
+    out = {"metric": "pyramid WFS frames/s (30 m, 60x60, 2916^2 FFT, 32 modulation positions)",
+           "value": round(fps, 2), "unit": "WFS frames/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(sec / args.steps * 1e3, 4), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+           "data": "synthetic (seeded von Karman screens)",
+           "config": {"workload": "scao pyramid: 30 m pupil at 1/32 m, 60x60 subapertures, 0.7 um, 4 sides, "
+                      "modulation 5 lambda/D x 32, 7 layers 4096^2", "nloc": pw.nloc, "nembed": N, "nsa": pw.nsa},
+           "kernel_ms_per_frame": round(ms / n, 4),
+           "fft_tflops": round(flop / (ms / n * 1e-3) / 1e12, 2)}
+    if args.pywfs_parity:
+        import oracle
+        eng.queue(isim, dt)
+        gi, gg = eng.sync()
+        opd = eng.opd()
+        tc = time.perf_counter()
+        ri, rg = oracle.pywfs_image(pw, opd)
+        tcpu = time.perf_counter() - tc
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+        out["parity"] = {"ints_rel_l2": float(np.linalg.norm(gi - ri) / np.linalg.norm(ri)),
+                         "grad_max_abs_mas": float(np.abs(gg - rg).max() * 180 / math.pi * 3600e3)}
+        out["cpu_baseline"] = {"value": round(1.0 / tcpu, 5), "unit": "WFS frames/s", "cores": threads,
+                               "kind": "port", "sample": "one frame of the same workload, FP64 oracle"}
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -471,6 +559,10 @@ def main():
     ap.add_argument("--latency-steps", type=int, default=100)
     ap.add_argument("--evl", action="store_true",
                     help="science PSF steps/s (30 m aperture, 3 wavelengths, 4096^2 FFT) instead of the WFS step")
+    ap.add_argument("--pywfs", action="store_true",
+                    help="pyramid WFS frames/s (30 m, 60x60, 32 modulation positions) instead of the LGS step")
+    ap.add_argument("--pywfs-parity", action="store_true",
+                    help="with --pywfs: one frame against the FP64 oracle (about a minute of host time)")
     ap.add_argument("--hetero", action="store_true",
                     help="LGS + low-order NGS + TT NGS stepped as one group graph per step")
     args = ap.parse_args()
@@ -486,6 +578,9 @@ def main():
         return
     if args.evl:
         evl_main(args)
+        return
+    if args.pywfs:
+        pywfs_main(args)
         return
 
     from maos_amd import _lib, configs, engine

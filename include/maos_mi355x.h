@@ -285,6 +285,16 @@ int mx_wfsgrad_set_gs0(mx_wfsgrad_t *h, const long *rowp, const long *coli, cons
 int mx_wfsgrad_set_noise(mx_wfsgrad_t *h, int enable, double rne, double bkgrnd,
                          double bkgrndc, unsigned long long seed);
 
+/* Exposure of dtrat steps (powfs.dtrat; maos/wfsgrad.c:164-167 and the
+ * gradout branch :411-495): the frames of steps isim with isim % dtrat == 0
+ * up to (isim + 1) % dtrat == 0 add up on the GPU; only the last step of an
+ * exposure adds noise (give mx_wfsgrad_set_noise bkgrnd*dtrat), writes the
+ * intensities, adds to the i0 and dither sums (which count exposures) and
+ * forms the gradients; the other steps leave them as they were.  Physical
+ * optics only (geometric gradients are per step).  dtrat 1 (the default)
+ * is one frame per exposure.  Waits for queued steps. */
+int mx_wfsgrad_set_dtrat(mx_wfsgrad_t *h, int dtrat);
+
 /* Launch each step's kernels as one HIP graph (enable=1) or one by one
  * (enable=0, the default).  No counterpart in the reference, whose GPU code
  * launches kernels directly; the graph is built on the next queued step and

@@ -125,6 +125,7 @@ HIP_API = {
     "mx_wfsgrad_wait": (C.c_int, [C.c_void_p]),
     "mx_wfsgrad_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_wfsgrad_get_variant": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "mx_wfsgrad_set_dtrat": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_wfsgrad_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mx_wfsgrad_set_opdadd": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "mx_wfsgrad_set_ptt": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),

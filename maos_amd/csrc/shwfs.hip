@@ -172,6 +172,27 @@ __device__ __forceinline__ void dither_add(const DitherAcc &d, int iwfs, long i,
     d.imy[i] += d.cs[2 * iwfs + 1] * x;
 }
 
+// A WFS exposure of dtrat steps (maos/wfsgrad.c:164-167, 411-495): the
+// frames of one exposure add up in acc [nwfs][nsa][npix]; only the last
+// frame (MX_DT_OUT) goes on to the noise, intensities, i0 and dither sums
+// and gradients; the first (MX_DT_FIRST) starts the sum.  dtrat 1: both.
+constexpr int MX_DT_FIRST = 1, MX_DT_OUT = 2;
+struct DtArgs {
+    float *acc;
+    int phase;
+};
+// frame value pv of pixel i: accumulated (returns false: not the last
+// frame), or the whole exposure (returns true)
+__device__ __forceinline__ bool dt_frame(const DtArgs &d, long i, float &pv) {
+    if (d.phase == (MX_DT_FIRST | MX_DT_OUT)) return true;
+    if (!(d.phase & MX_DT_OUT)) {
+        d.acc[i] = (d.phase & MX_DT_FIRST) ? pv : d.acc[i] + pv;
+        return false;
+    }
+    if (!(d.phase & MX_DT_FIRST)) pv += d.acc[i];
+    return true;
+}
+
 struct NoiseArgs {
     int on;
     float bkgrnd, bkgrndc, rne;
@@ -265,6 +286,7 @@ struct KArgs {
     double *i0acc;   // [nwfs][nsa][npix] or null
     DitherAcc dith;
     NoiseArgs noise;
+    DtArgs dt;
 };
 
 struct OpdArgs {
@@ -783,6 +805,12 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
     // outputs and pixel processing (shwfs_grad, maos/utils.c:983-1122)
     const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+#pragma unroll
+    for (int j = 0; j < PPL; j++) {
+        const int p = line + j * N;
+        if (p < a.npix && valid) dt_frame(a.dt, ibase + p, pv[j]);
+    }
+    if (!(a.dt.phase & MX_DT_OUT)) return; // uniform: a frame inside a longer exposure
     if (a.noise.on)
 #pragma unroll
         for (int j = 0; j < PPL; j++) {
@@ -1508,6 +1536,7 @@ struct MfmaArgs {
     double *i0acc;
     DitherAcc dith;
     NoiseArgs noise;
+    DtArgs dt;
     // uplink mode (one workgroup per WFS and wavelength): LLT OPD
     // [nwfs][32*32] and amplitude, 1/nlwvf^2, and the two outputs conj(lotfc)
     // [nwfs][nwvl][64*64] ([ux][uy]) and its half plane [nwfs][nwvl][MX_HALF]
@@ -1603,7 +1632,12 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
     const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     const int p = lane;
+    if (!(a.dt.phase & MX_DT_OUT)) { // uniform: a frame inside a longer exposure
+        if (p < a.npix) dt_frame(a.dt, ibase + p, pv);
+        return;
+    }
     if (p < a.npix) {
+        dt_frame(a.dt, ibase + p, pv);
         if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
         a.ints[ibase + p] = pv;
         if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
@@ -2380,6 +2414,7 @@ struct LfSampleArgs {
     double *i0acc;
     DitherAcc dith;
     NoiseArgs noise;
+    DtArgs dt;
 };
 __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
     __shared__ float red[3][MX_LF_THREADS / 64];
@@ -2395,6 +2430,7 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
             const float4 w = a.wv[iwvl].si_w[p];
             pv += W.alpha[iwvl] * (w.x * im[id.x] + w.y * im[id.y] + w.z * im[id.z] + w.w * im[id.w]);
         }
+        if (!dt_frame(a.dt, ibase + p, pv)) continue; // uniform: not the exposure's last frame
         if (a.noise.on) pv = add_noise(a.noise, pv, p, iwfs, a.nsa, isa);
         a.ints[ibase + p] = pv;
         if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
@@ -2413,6 +2449,7 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
             }
         }
     }
+    if (!(a.dt.phase & MX_DT_OUT)) return; // uniform
     seg_sum<64>(s0);
     seg_sum<64>(s1);
     seg_sum<64>(s2);
@@ -2491,9 +2528,11 @@ struct Sig2Args {
     double saasum;
     const float *ints, *i0sum, *saa;
     float *grad;
+    int phase; // DtArgs phase: the gradients are formed at an exposure's last frame only
 };
 __global__ __launch_bounds__(256) void k_sigmatch2(Sig2Args a) {
     __shared__ double red[2][4];
+    if (!(a.phase & MX_DT_OUT)) return;
     const int iwfs = (int)blockIdx.x;
     const long n = (long)a.nsa * a.npix;
     const float *im = a.ints + (long)iwfs * n;
@@ -2655,6 +2694,11 @@ struct mx_wfsgrad_s {
     std::vector<double> h_dcs;
     bool dith_on = false, dith_dirty = false;
     long dith_nframe = 0;
+    // exposure of dtrat steps (mx_wfsgrad_set_dtrat): frame sums, and whether
+    // the step planned last ends an exposure
+    int dtrat = 1;
+    float *d_dtacc = nullptr;
+    bool last_out = true;
     FoldArgs arg_fold{};
     MtchArgs arg_mtch{};
     Sig2Args arg_sig2{};
@@ -3180,6 +3224,7 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
                     h->d_nea, h->d_i0s64, h->d_dcs, h->d_gs_rowp, h->d_gs_coli, h->d_gs_val};
     for (void *q : ptrs)
         if (q) hipFree(q);
+    if (h->d_dtacc) hipFree(h->d_dtacc);
     for (int w = 0; w < MX_MAXWVL; w++) {
         if (h->d_nominal[w]) hipFree(h->d_nominal[w]);
         if (h->d_gx[w]) hipFree(h->d_gx[w]);
@@ -3684,6 +3729,11 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     }
     a.noise = h->noise;
     a.noise.isim = (unsigned)isim;
+    // exposure of dtrat steps: isim % dtrat == 0 starts one, (isim + 1) %
+    // dtrat == 0 ends it (maos/wfsgrad.c:164, gradout of wfsflags)
+    a.dt.acc = h->d_dtacc;
+    a.dt.phase = (isim % h->dtrat == 0 ? MX_DT_FIRST : 0) | ((isim + 1) % h->dtrat == 0 ? MX_DT_OUT : 0);
+    h->last_out = (a.dt.phase & MX_DT_OUT) != 0;
     a.opd = h->d_opd;
     MfmaArgs &m = h->arg_mfma;
     m = MfmaArgs{};
@@ -3743,6 +3793,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         m.i0acc = a.i0acc;
         m.dith = a.dith;
         m.noise = a.noise;
+        m.dt = a.dt;
         m.llt_opd = h->d_llt_opd;
         m.llt_amp = h->d_llt_amp;
         m.llt_norm = (float)(1. / ((double)h->cfg.llt_nx * h->embfac * (double)h->cfg.llt_nx * h->embfac));
@@ -3945,6 +3996,9 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
                            : reinterpret_cast<const void *>(&k_shwfs_mfma<true, __bf16>),
                        dim3((unsigned)(h->nwfs * h->nwvl)),
                        dim3(256), &m};
+        else if (fftform)
+            L[nl++] = {reinterpret_cast<const void *>(&k_shwfs_fft<true>), dim3((unsigned)(h->nwfs * h->nwvl)),
+                       dim3(MX_FFT_T), &m};
         else
             L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_llt_otf<64, 32>)
                                : reinterpret_cast<const void *>(&k_llt_otf<32, 16>),
@@ -3998,6 +4052,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         g.i0acc = a.i0acc;
         g.dith = a.dith;
         g.noise = a.noise;
+        g.dt = a.dt;
         L[nl++] = {reinterpret_cast<const void *>(&k_lf_sample), dim3((unsigned)(h->nwfs * h->nsa)),
                    dim3((unsigned)std::min(h->lf_threads, 256)), &g};
     } else if (mf || fftform) {
@@ -4018,7 +4073,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             L[nl++] = {reinterpret_cast<const void *>(&k_etf_blend),
                        dim3((unsigned)std::min<long>((b.n * h->nwvl + 255) / 256, 8192)), dim3(256), &b};
         }
-        const void *kimg = fftform ? reinterpret_cast<const void *>(&k_shwfs_fft)
+        const void *kimg = fftform ? reinterpret_cast<const void *>(&k_shwfs_fft<false>)
                            : f16   ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16>)
                                    : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16>);
         L[nl++] = {kimg, dim3((unsigned)(h->nsa * h->nwfs)), dim3(256), &m};
@@ -4045,6 +4100,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         q.i0sum = h->d_i0sum;
         q.saa = h->d_saa;
         q.grad = h->d_grad;
+        q.phase = a.dt.phase;
         L[nl++] = {reinterpret_cast<const void *>(&k_sigmatch2), dim3((unsigned)h->nwfs), dim3(256), &q};
     }
     if (nl > MX_MAXLAUNCH) return set_err(MX_EINVAL, "internal: %d kernels in a step", nl);
@@ -4139,8 +4195,9 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
 
 // bookkeeping after a step was queued
 void step_queued(mx_wfsgrad_t *h) {
-    if (h->i0_enable) h->i0_nframe++;
-    if (h->dith_on) h->dith_nframe++;
+    // the i0 and dither sums take whole exposures
+    if (h->i0_enable && h->last_out) h->i0_nframe++;
+    if (h->dith_on && h->last_out) h->dith_nframe++;
     h->queued = true;
 }
 } // namespace
@@ -4578,6 +4635,8 @@ int mx_wfsgrad_set_gs0(mx_wfsgrad_t *h, const long *rowp, const long *coli, cons
 
 int mx_wfsgrad_set_geom(mx_wfsgrad_t *h, int mode) {
     if (!h || mode < 0 || mode > 2) return set_err(MX_EINVAL, "mx_wfsgrad_set_geom: bad handle or mode %d", mode);
+    if (mode && h->dtrat > 1)
+        return set_err(MX_EINVAL, "mx_wfsgrad_set_geom: geometric gradients are per step (set dtrat 1 first)");
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->geom = mode;
@@ -4632,6 +4691,20 @@ int mx_wfsgrad_set_noise(mx_wfsgrad_t *h, int enable, double rne, double bkgrnd,
     h->noise.bkgrndc = (float)bkgrndc;
     h->noise.key0 = (unsigned)(seed & 0xffffffffu);
     h->noise.key1 = (unsigned)(seed >> 32);
+    return MX_OK;
+}
+
+int mx_wfsgrad_set_dtrat(mx_wfsgrad_t *h, int dtrat) {
+    if (!h || dtrat < 1) return set_err(MX_EINVAL, "mx_wfsgrad_set_dtrat: bad handle or dtrat=%d", dtrat);
+    if (h->geom && dtrat > 1)
+        return set_err(MX_EINVAL, "mx_wfsgrad_set_dtrat: geometric gradients are per step (dtrat 1)");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream)); // a queued step may still use the frame sums
+    if (dtrat > 1 && !h->d_dtacc) {
+        int rc;
+        if ((rc = dev_alloc(&h->d_dtacc, (size_t)h->nwfs * h->nsa * h->npix))) return rc;
+    }
+    h->dtrat = dtrat;
     return MX_OK;
 }
 

@@ -91,6 +91,29 @@ __device__ __forceinline__ void dft8(float2 (&v)[8]) {
     v[7] = csub(e3, o3);
 }
 
+// forward 8-point DFT of samples that are zero but for v[2..5] (the band
+// of a zero-padded line): dft8<false> with the zero terms left out
+__device__ __forceinline__ void dft8_mid(float2 (&v)[8]) {
+    constexpr float r = 0.70710678118654752f;
+    const float2 a = v[2], b = v[3], c = v[4], d = v[5];
+    // DFT4 of (0, a, c, 0) and of (0, b, d, 0)
+    const float2 e0 = cadd(c, a), e2 = csub(c, a);
+    const float2 e1 = make_float2(a.y - c.x, -a.x - c.y), e3 = make_float2(-c.x - a.y, a.x - c.y);
+    float2 o0 = cadd(d, b), o2 = csub(d, b);
+    float2 o1 = make_float2(b.y - d.x, -b.x - d.y), o3 = make_float2(-d.x - b.y, b.x - d.y);
+    o1 = make_float2((o1.x + o1.y) * r, (o1.y - o1.x) * r);
+    o2 = make_float2(o2.y, -o2.x);
+    o3 = make_float2((o3.y - o3.x) * r, -(o3.x + o3.y) * r);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
 // forward 8-point DFT of real samples p[n]: the same steps with the
 // imaginary parts known to be zero (y[8 - m] = conj y[m])
 __device__ __forceinline__ void dft8_real(const float (&p)[8], float2 (&v)[8]) {
@@ -150,6 +173,19 @@ __device__ __forceinline__ void row64(float2 (&v)[8], float2 *row, int k, const 
     dft8<INV>(v);
 }
 
+// the same for a band line: inputs v[2..5] (elements k + 16 .. k + 40)
+__device__ __forceinline__ void row64_mid(float2 (&v)[8], float2 *row, int k, const float2 (&tw)[8]) {
+    dft8_mid(v);
+    twiddle8<false>(v, tw);
+    xchg8(v, row, k);
+    dft8<false>(v);
+}
+
+// UPLINK: passes A to D on the LLT grid give the uplink OTF
+// lotfc = FFT(|FFT(embed)|^2) / nlwvf^2 (maos/wfsints.c:149-199); one
+// workgroup per (WFS, wavelength) writes conj(lotfc) over the half plane
+// (half_slot order) and the full plane, as the MFMA form's uplink mode.
+template <bool UPLINK>
 __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) void k_shwfs_fft(MfmaArgs a) {
     __shared__ float2 G[64 * MX_FFT_GS];
     __shared__ float img[MX_MFMA_NS * MX_MFMA_NS];
@@ -157,22 +193,30 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
     float2 *const Fy = G + 32 * MX_FFT_GS; // pass F's table, in grid rows that pass D no longer needs
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int k = lane & 7, li = lane >> 3; // lane of a line, line of the wave's eight
-    const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
-    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    int iwfs, isa, iwvl0, iwvl1;
+    if constexpr (UPLINK) {
+        iwfs = (int)blockIdx.x / a.nwvl;
+        isa = 0;
+        iwvl0 = (int)blockIdx.x % a.nwvl;
+        iwvl1 = iwvl0 + 1;
+    } else {
+        const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
+        iwfs = unit % a.nwfs;
+        isa = unit / a.nwfs;
+        iwvl0 = 0;
+        iwvl1 = a.nwvl;
+    }
     const DevWfs &W = a.wfs[iwfs];
-    // stage twiddles w64^(k q) of this lane, from FP32 sincospi of k q / 32
+    // stage twiddles w64^(k q) of this lane: cos and sin of 2 pi k q / 64
+    // from the FP64-built table of 2 pi m / 256
     float2 tw[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        float s, c;
-        sincospif((float)(k * q) * (1.f / 32.f), &s, &c);
-        tw[q] = make_float2(c, s);
-    }
+    for (int q = 0; q < 8; q++) tw[q] = make_float2(kTwCos[4 * k * q], kTwSin[4 * k * q]);
     // OPD and amplitude: thread (ix = tid % 32, iy = 4 (tid / 32) + j)
     constexpr int NX = 32, B0 = 16;
     const int ix = tid % 32, iy0 = 4 * (tid / 32);
-    const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
-    const float *ag = a.amp + (long)isa * NX * NX;
+    const float *og = UPLINK ? a.llt_opd + (long)iwfs * NX * NX : a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
+    const float *ag = UPLINK ? a.llt_amp : a.amp + (long)isa * NX * NX;
     float mean;
     {
         float s = 0.f;
@@ -184,7 +228,7 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
         mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
     }
     float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
-    for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
+    for (int iwvl = iwvl0; iwvl < iwvl1; iwvl++) {
         const MfmaWvl &WV = a.wv[iwvl];
         // the field on the band: element (y, x)
 #pragma unroll
@@ -200,11 +244,10 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
         {
             const int y = B0 + 8 * wave + li;
             float2 *row = G + y * MX_FFT_GS;
-            v[0] = v[1] = v[6] = v[7] = make_float2(0.f, 0.f);
 #pragma unroll
             for (int j = 2; j < 6; j++) v[j] = row[k + 9 * j];
             wave_sync();
-            row64<false>(v, row, k, tw);
+            row64_mid(v, row, k, tw);
             __syncthreads(); // every row read before the transposed writes
             float2 *col = G + k * MX_FFT_GS + y + (y >> 3);
 #pragma unroll
@@ -218,11 +261,10 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
         for (int t = 0; t < 2; t++) {
             const int kx = 16 * wave + 8 * t + li;
             float2 *row = G + kx * MX_FFT_GS;
-            v[0] = v[1] = v[6] = v[7] = make_float2(0.f, 0.f);
 #pragma unroll
             for (int j = 2; j < 6; j++) v[j] = row[k + 9 * j];
             wave_sync();
-            row64<false>(v, row, k, tw);
+            row64_mid(v, row, k, tw);
             float P[8];
 #pragma unroll
             for (int p = 0; p < 8; p++) P[p] = v[p].x * v[p].x + v[p].y * v[p].y;
@@ -254,10 +296,30 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
             for (int j = 0; j < 8; j++) v[j] = row[k + 9 * j];
             // pass F's table for this wavelength's sampled rows (grid rows
             // 32 and on are free from here)
-            Fy[tid] = WV.fy[tid];
-            Fy[tid + MX_FFT_T] = WV.fy[tid + MX_FFT_T];
+            if (!UPLINK) {
+                Fy[tid] = WV.fy[tid];
+                Fy[tid + MX_FFT_T] = WV.fy[tid + MX_FFT_T];
+            }
             wave_sync();
             row64<false>(v, row, k, tw);
+            if constexpr (UPLINK) {
+                // conj(lotfc) = conj(O) / nlwvf^2: the half plane uy < 32 in
+                // slot order, and the full plane [ux][uy] with uy > 32 from
+                // conj(lotfc)(-u) = lotfc(u) and the lag-32 row zero (the LLT
+                // field fills 32 of 64 samples)
+                float2 *hp = a.lotf_half + ((long)iwfs * a.nwvl + iwvl) * MX_HALF;
+                float2 *fp = a.lotf_full + ((long)iwfs * a.nwvl + iwvl) * 64 * 64;
+#pragma unroll
+                for (int p = 0; p < 8; p++) {
+                    const int ux = k + 8 * p;
+                    const float re = v[p].x * a.llt_norm, im = v[p].y * a.llt_norm;
+                    hp[half_slot(ux, uy)] = make_float2(re, -im);
+                    fp[ux * 64 + uy] = make_float2(re, -im);
+                    if (uy) fp[((64 - ux) & 63) * 64 + (64 - uy)] = make_float2(re, im);
+                    if (uy == 0) fp[ux * 64 + 32] = make_float2(0.f, 0.f);
+                }
+                continue;
+            }
             // v[p] = O[uy][ux], ux = k + 8p
             if (a.lotf_t && WV.etf_t) {
                 const float2 *lt = a.lotf_t + ((long)iwfs * a.nwvl + iwvl) * MX_HALF;
@@ -292,8 +354,7 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int xs = WV.fxs[k + 8 * h], q = xs & 7, p = xs >> 3;
-                float zs, zc;
-                sincospif((float)p * 0.25f, &zs, &zc);
+                const float zc = kTwCos[32 * p], zs = kTwSin[32 * p]; // exp(i pi p / 4)
                 float2 acc = row[63 + q]; // element 56 + q
 #pragma unroll
                 for (int kk = 6; kk >= 0; kk--) {
@@ -327,5 +388,6 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
         }
         __syncthreads();
     }
+    if constexpr (UPLINK) return;
     if (wave == 0) shwfs_pixels(a, iwfs, isa, lane, pv);
 }

@@ -229,6 +229,11 @@ class WfsGrad:
         """Photon and read noise on every later step's intensities."""
         check(_lib.hip().mx_wfsgrad_set_noise(self.h, int(enable), rne, bkgrnd, bkgrndc, seed))
 
+    def set_dtrat(self, dtrat):
+        """Exposures of dtrat steps: intensities and gradients at the last
+        step of each (mx_wfsgrad_set_dtrat)."""
+        check(_lib.hip().mx_wfsgrad_set_dtrat(self.h, int(dtrat)))
+
     def set_graph(self, enable=True):
         """Launch each step's kernels as one HIP graph."""
         check(_lib.hip().mx_wfsgrad_set_graph(self.h, int(enable)))

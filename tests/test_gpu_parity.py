@@ -890,3 +890,62 @@ def test_errors_reported():
     assert L.mx_wfsgrad_init(C.byref(h), C.byref(cfgp), w, 1) == -1
     assert b"unsupported" in L.mx_last_error()
     assert L.mx_atm2gpu(None, 17, 1.0) == -1
+
+
+@pytest.mark.parametrize("case,variant", [("lgs_small", "mfma_bf16"), ("lgs_small", "fft32"),
+                                          ("lgs_small", "fft32_reg"), ("lf_mixed", "fft32"),
+                                          ("ngs_small", "fft32")])
+def test_dtrat_exposure(case, variant, request, monkeypatch):
+    """dtrat = 3 (maos/wfsgrad.c:164-167, 411-495): an exposure's
+    intensities are the sum of its three frames and its gradients those of
+    the summed image (the oracle's shwfs_grad of the sum); the steps inside
+    an exposure leave intensities and gradients as they were; the FP64 i0
+    sum takes whole exposures and counts them.  Run as one HIP graph per
+    step, so the exposure phase travels in the nodes' arguments."""
+    cfg, p, layers = request.getfixturevalue(case)
+    if variant == "fft32_reg":
+        monkeypatch.setenv("MX_FFT_REG", "1")
+    print("case: dtrat 3, %s, %s" % (case, variant), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    if not eng.variant_ok(VARIANTS[variant]):
+        pytest.skip("%s does not apply to %s" % (variant, case))
+    eng.set_variant(VARIANTS[variant])
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    if p.cfg.phytype == 1:  # matched filter
+        mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+        o.set_ref(mtche, i0sum)
+        eng.update_ref(mtche, i0sum)
+    eng.set_graph(True)
+    eng.set_dtrat(3)
+    eng.i0_control(enable=True, reset=True)
+    iw = len(wfs) - 1
+    frames = {isim: o.step(wfs[iw], isim, dt)[0] for isim in range(6)}
+    for isim in range(3):
+        eng.queue(isim, dt)
+    g0, ints0 = eng.sync(iw)
+    want0 = frames[0] + frames[1] + frames[2]
+    e_int = rel_l2(ints0, want0)
+    e_g = np.abs(g0 - o.grad(want0)).max() * RAD2MAS
+    print("exposure 0-2: ints=%.2e grad=%.3e mas" % (e_int, e_g), flush=True)
+    assert e_int <= INTS_TOL and e_g <= GRAD_TOL_MAS
+    for isim in (3, 4):
+        eng.queue(isim, dt)
+        g, ints = eng.sync(iw)
+        assert np.array_equal(g, g0) and np.array_equal(ints, ints0)  # inside an exposure
+    eng.queue(5, dt)
+    g1, ints1 = eng.sync(iw)
+    want1 = frames[3] + frames[4] + frames[5]
+    e_int = rel_l2(ints1, want1)
+    e_g = np.abs(g1 - o.grad(want1)).max() * RAD2MAS
+    print("exposure 3-5: ints=%.2e grad=%.3e mas" % (e_int, e_g), flush=True)
+    assert e_int <= INTS_TOL and e_g <= GRAD_TOL_MAS
+    i0, nframe = eng.i0_get(iw)
+    assert nframe == 2
+    assert np.abs(i0 - (ints0 + ints1)).max() <= 1e-6 * np.abs(i0).max()
+    eng.set_dtrat(1)
+    eng.queue(6, dt)
+    g, ints = eng.sync(iw)
+    ri, rg, _, _ = o.step(wfs[iw], 6, dt)
+    assert rel_l2(ints, ri) <= INTS_TOL and np.abs(g - rg).max() * RAD2MAS <= GRAD_TOL_MAS
+    eng.close()
