@@ -1563,9 +1563,10 @@ __device__ __forceinline__ void mac3(f32x4 &c, v8_t<E> ah, v8_t<E> al, v8_t<E> b
 // b[1]; b[2] = -b[1]
 template <typename E, bool DREAL>
 __device__ __forceinline__ void cmacd(f32x4 &cr, f32x4 &ci, const E *const (&L)[4], int row, int k,
-                                      const v8_t<E> (&b)[3][2]) {
-    const v8_t<E> dh = *reinterpret_cast<const v8_t<E> *>(L[0] + lds_at(row, k));
-    const v8_t<E> dl = *reinterpret_cast<const v8_t<E> *>(L[1] + lds_at(row, k));
+                                      const v8_t<E> (&b)[3][2], int poff = 0) {
+    // poff: element offset of the real data's hi/lo planes (DREAL)
+    const v8_t<E> dh = *reinterpret_cast<const v8_t<E> *>(L[0] + poff + lds_at(row, k));
+    const v8_t<E> dl = *reinterpret_cast<const v8_t<E> *>(L[1] + poff + lds_at(row, k));
     mac3<E>(cr, dh, dl, b[0][0], b[0][1]);
     mac3<E>(ci, dh, dl, b[1][0], b[1][1]);
     if (!DREAL) {
@@ -1693,7 +1694,12 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
 // UPLINK: the same stages through 2b on the LLT grid give the uplink OTF
 // lotfc = FFT(|FFT(embed)|^2) / nlwvf^2 (maos/wfsints.c:149-199); one
 // workgroup per (WFS, wavelength) writes conj(lotfc) and stops there.
-template <bool UPLINK, typename E>
+// PP: the stages alternate between two halves of the planes, logical
+// columns 16..47 (region A) and 0..15 + 48..63 (region B), each stage
+// reading one and writing the other, so a stage's writes need no barrier
+// after its reads (one barrier per stage instead of two).  Arrays wider
+// than 32 columns fold their second half into rows 32..63.
+template <bool UPLINK, typename E, bool PP = true>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAVES))) void k_shwfs_mfma(MfmaArgs a) {
     using V8 = v8_t<E>;
     // range scales of the FP16 form (powers of two; tables carry 2^13): the
@@ -1708,6 +1714,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
     __shared__ float red[4];
     E *const L[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
     const E *const Lc[4] = {Lb, Lb + MPLANE, Lb + 2 * MPLANE, Lb + 3 * MPLANE};
+    // logical column of region-A / region-B column c < 32
+    auto colA = [](int c) { return 16 + c; };
+    auto colB = [](int c) { return c < 16 ? c : c + 32; };
     const V8 *const F = static_cast<const V8 *>(a.F);
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
     int iwfs, isa, iwvl0, iwvl1;
@@ -1771,10 +1780,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
                 cmacd<E, false>(zr[mt], zi[mt], Lc, B0 + 16 * mt + l16, B0 + 8 * h16, b1);
-            __syncthreads();
+            if (!PP) __syncthreads();
+            else __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
-                const int row = 16 * lw + l16, col = B0 + 16 * mt + 4 * h16;
+                const int row = 16 * lw + l16, col = PP ? colB(16 * mt + 4 * h16) : B0 + 16 * mt + 4 * h16;
                 st4(L[0], L[1], row, col, zr[mt][0] * F1, zr[mt][1] * F1, zr[mt][2] * F1, zr[mt][3] * F1);
                 st4(L[2], L[3], row, col, zi[mt][0] * F1, zi[mt][1] * F1, zi[mt][2] * F1, zi[mt][3] * F1);
             }
@@ -1789,9 +1799,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         {
             f32x4 yr[4] = {}, yi[4] = {};
 #pragma unroll
-            for (int t = 0; t < 4; t++) cmacd<E, false>(yr[t], yi[t], Lc, 16 * t + l16, B0 + 8 * h16, b1);
+            for (int t = 0; t < 4; t++)
+                cmacd<E, false>(yr[t], yi[t], Lc, 16 * t + l16, PP ? colB(8 * h16) : B0 + 8 * h16, b1);
             loadT<E>(F + (MX_F2 + 2 * nb) * MX_TPARTS * 64, lane, b2); // stage 2a's operand
-            __syncthreads();
+            if (!PP) __syncthreads();
+            else __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = 0; t < 4; t++)
 #pragma unroll
@@ -1799,11 +1811,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             const int kx = 16 * lw + l16;
 #pragma unroll
             for (int t = 0; t < 2; t++) {
+                // PP: P+ in planes 0, 1 and P- in planes 2, 3 of region A
                 const int col = 16 * t + 4 * h16;
-                st4(L[0], L[1], kx, col, yr[t][0] + yr[t + 2][0], yr[t][1] + yr[t + 2][1], yr[t][2] + yr[t + 2][2],
-                    yr[t][3] + yr[t + 2][3]);
-                st4(L[0], L[1], kx, 32 + col, yr[t][0] - yr[t + 2][0], yr[t][1] - yr[t + 2][1],
-                    yr[t][2] - yr[t + 2][2], yr[t][3] - yr[t + 2][3]);
+                st4(L[0], L[1], kx, PP ? colA(col) : col, yr[t][0] + yr[t + 2][0], yr[t][1] + yr[t + 2][1],
+                    yr[t][2] + yr[t + 2][2], yr[t][3] + yr[t + 2][3]);
+                st4(L[PP ? 2 : 0], L[PP ? 3 : 1], kx, PP ? colA(col) : 32 + col, yr[t][0] - yr[t + 2][0],
+                    yr[t][1] - yr[t + 2][1], yr[t][2] - yr[t + 2][2], yr[t][3] - yr[t + 2][3]);
             }
         }
         __syncthreads();
@@ -1816,10 +1829,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             f32x4 qr[2] = {}, qi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<E, true>(qr[mt], qi[mt], Lc, 16 * mb + 32 * mt + l16, 32 * e + 8 * h16, b2);
+                cmacd<E, true>(qr[mt], qi[mt], Lc, 16 * mb + 32 * mt + l16, PP ? colA(8 * h16) : 32 * e + 8 * h16,
+                               b2, PP ? 2 * e * MPLANE : 0);
             loadT<E>(F + (MX_F2 + 2 * nb + mb) * MX_TPARTS * 64, lane, b1); // stage 2b's operand
-            __syncthreads();
-            const int uy = 2 * l16 + e, col = 16 * mb + 4 * h16;
+            if (!PP) __syncthreads();
+            else __builtin_amdgcn_sched_barrier(0);
+            const int uy = 2 * l16 + e, col = PP ? colB(16 * mb + 4 * h16) : 16 * mb + 4 * h16;
+            // PP: Q+ (columns < 32) in rows uy, Q- in rows 32 + uy, region B
+            const int rq = PP ? uy + 32 : uy, cq = PP ? col : 32 + col;
             if constexpr (H)
 #pragma unroll
                 for (int mt = 0; mt < 2; mt++)
@@ -1832,9 +1849,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
                 qr[0][3] + qr[1][3]);
             st4(L[2], L[3], uy, col, qi[0][0] + qi[1][0], qi[0][1] + qi[1][1], qi[0][2] + qi[1][2],
                 qi[0][3] + qi[1][3]);
-            st4(L[0], L[1], uy, 32 + col, qr[0][0] - qr[1][0], qr[0][1] - qr[1][1], qr[0][2] - qr[1][2],
+            st4(L[0], L[1], rq, cq, qr[0][0] - qr[1][0], qr[0][1] - qr[1][1], qr[0][2] - qr[1][2],
                 qr[0][3] - qr[1][3]);
-            st4(L[2], L[3], uy, 32 + col, qi[0][0] - qi[1][0], qi[0][1] - qi[1][1], qi[0][2] - qi[1][2],
+            st4(L[2], L[3], rq, cq, qi[0][0] - qi[1][0], qi[0][1] - qi[1][1], qi[0][2] - qi[1][2],
                 qi[0][3] - qi[1][3]);
         }
         __syncthreads();
@@ -1847,7 +1864,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             f32x4 xr[2] = {}, xi[2] = {};
 #pragma unroll
             for (int mt = 0; mt < 2; mt++)
-                cmacd<E, false>(xr[mt], xi[mt], Lc, 16 * mt + l16, 32 * e + 8 * h16, b1);
+                cmacd<E, false>(xr[mt], xi[mt], Lc, PP ? 16 * mt + l16 + 32 * e : 16 * mt + l16,
+                                PP ? colB(8 * h16) : 32 * e + 8 * h16, b1);
             // element (mt, v) of this lane is table slot ((2 lw + mt) 4 + v) 64 + lane
             const long tbase = (long)(2 * lw) * 4 * 64 + lane;
             const int ux = 2 * (16 * hf + l16) + e;
@@ -1904,10 +1922,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
                 }
             }
             loadT<E>(static_cast<const V8 *>(WV.gy), lane, b2); // stage 3a's operand
-            __syncthreads();
+            if (!PP) __syncthreads();
+            else __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int mt = 0; mt < 2; mt++) {
-                const int col = 16 * mt + 4 * h16;
+                const int col = PP ? colA(16 * mt + 4 * h16) : 16 * mt + 4 * h16;
                 st4(L[0], L[1], ux, col, xr[mt][0] * F4, xr[mt][1] * F4, xr[mt][2] * F4, xr[mt][3] * F4);
                 st4(L[2], L[3], ux, col, xi[mt][0] * F4, xi[mt][1] * F4, xi[mt][2] * F4, xi[mt][3] * F4);
             }
@@ -1917,12 +1936,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         // and 2 elsewhere (folded into the table); role r: ux in [16r, 16r+16)
         {
             f32x4 rr = {}, ri = {};
-            cmacd<E, false>(rr, ri, Lc, 16 * lw + l16, 8 * h16, b2);
+            cmacd<E, false>(rr, ri, Lc, 16 * lw + l16, PP ? colA(8 * h16) : 8 * h16, b2);
             if (lw == 3) loadT<E>(static_cast<const V8 *>(WV.gx), lane, b1); // stage 3b's operand, first K step
-            __syncthreads();
+            if (!PP) __syncthreads();
+            else __builtin_amdgcn_sched_barrier(0);
             // element v: ux = 16 lw + 4 (l/16) + v, ys = l%16 -> row ys, columns ux
-            st4(L[0], L[1], l16, 16 * lw + 4 * h16, rr[0] * F5, rr[1] * F5, rr[2] * F5, rr[3] * F5);
-            st4(L[2], L[3], l16, 16 * lw + 4 * h16, ri[0] * F5, ri[1] * F5, ri[2] * F5, ri[3] * F5);
+            // (PP: ux >= 32 in rows 32 + ys, region B)
+            const int rr3 = PP ? l16 + 32 * (lw >> 1) : l16;
+            const int cr3 = PP ? colB(16 * (lw & 1) + 4 * h16) : 16 * lw + 4 * h16;
+            st4(L[0], L[1], rr3, cr3, rr[0] * F5, rr[1] * F5, rr[2] * F5, rr[3] * F5);
+            st4(L[2], L[3], rr3, cr3, ri[0] * F5, ri[1] * F5, ri[2] * F5, ri[3] * F5);
         }
         __syncthreads();
         // 3b: I[ys][xs] = Re sum_ux R[ys][ux] Gx[ux][xs]  (role 3)
@@ -1932,11 +1955,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 #pragma unroll
             for (int ks = 0; ks < 2; ks++) {
                 const V8(&g)[3][2] = ks ? b2 : b1;
-                const int k = 32 * ks + 8 * h16;
-                const V8 dh = *reinterpret_cast<const V8 *>(Lc[0] + lds_at(l16, k));
-                const V8 dl = *reinterpret_cast<const V8 *>(Lc[1] + lds_at(l16, k));
-                const V8 eh = *reinterpret_cast<const V8 *>(Lc[2] + lds_at(l16, k));
-                const V8 el = *reinterpret_cast<const V8 *>(Lc[3] + lds_at(l16, k));
+                const int k = PP ? colB(8 * h16) : 32 * ks + 8 * h16, r3 = PP ? l16 + 32 * ks : l16;
+                const V8 dh = *reinterpret_cast<const V8 *>(Lc[0] + lds_at(r3, k));
+                const V8 dl = *reinterpret_cast<const V8 *>(Lc[1] + lds_at(r3, k));
+                const V8 eh = *reinterpret_cast<const V8 *>(Lc[2] + lds_at(r3, k));
+                const V8 el = *reinterpret_cast<const V8 *>(Lc[3] + lds_at(r3, k));
                 mac3<E>(ir, dh, dl, g[0][0], g[0][1]); // Re R * Re G
                 mac3<E>(ir, eh, el, g[2][0], g[2][1]); // Im R * (-Im G)
             }
@@ -1949,7 +1972,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             const float4 w = WV.si_w[tid];
             pv += W.alpha[iwvl] * (w.x * img[id.x] + w.y * img[id.y] + w.z * img[id.z] + w.w * img[id.w]);
         }
-        __syncthreads();
+        // (PP: the next wavelength writes img only after further barriers)
+        if (!PP) __syncthreads();
     }
     if constexpr (UPLINK) return;
     // outputs and pixel processing, wave 0
@@ -3747,6 +3771,13 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         const char *e = getenv("MX_FFT_REG");
         if (e && atoi(e) != 0) fftform = false;
     }
+    // (MX_MFMA_PP=0 in the environment: the MFMA imaging kernel with in-place
+    // stages, two barriers each, for comparison)
+    bool pp = true;
+    {
+        const char *e = getenv("MX_MFMA_PP");
+        if (e && atoi(e) == 0) pp = false;
+    }
     if (variant == MX_VARIANT_MFMA_BF16 || f16 || fftform) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
@@ -4074,8 +4105,10 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
                        dim3((unsigned)std::min<long>((b.n * h->nwvl + 255) / 256, 8192)), dim3(256), &b};
         }
         const void *kimg = fftform ? reinterpret_cast<const void *>(&k_shwfs_fft<false>)
-                           : f16   ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16>)
-                                   : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16>);
+                           : f16   ? (pp ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16, true>)
+                                         : reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16, false>))
+                                   : (pp ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16, true>)
+                                         : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16, false>));
         L[nl++] = {kimg, dim3((unsigned)(h->nsa * h->nwfs)), dim3(256), &m};
     } else {
         const long spw = 64 / N, ngroup = (h->nsa + spw - 1) / spw;
@@ -4108,7 +4141,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
              (direct ? 8 : 0) | (variant << 4) | (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) |
              (h->lf ? h->lf_threads << 14 : 0) | (h->cfg.sigmatch == 2 ? 1 << 25 : 0) |
-             ((mf || fftform) && h->d_etf2_t[0] ? 1 << 24 : 0) | (fftform ? 1 << 6 : 0);
+             ((mf || fftform) && h->d_etf2_t[0] ? 1 << 24 : 0) | (fftform ? 1 << 6 : 0) |
+             (mf && !pp ? 1 << 14 : 0); // (the large-field bits 14-23 are zero with an MFMA form)
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;
