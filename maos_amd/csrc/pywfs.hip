@@ -18,7 +18,7 @@
 // that each pass reads and writes whole rows.  Layouts are [row][column]
 // with the column contiguous; "transposed" arrays hold [x][y].
 constexpr int MX_PY_MAXN = 4096;
-constexpr int MX_PY_T = 256;
+constexpr int MX_PY_T = 1024;
 constexpr int MX_PY_MAXSIDE = 4;
 
 enum PyMode { PY_EMBED = 0, PY_PSF, PY_MOD, PY_ACC, PY_REAL, PY_DTF, PY_IMG };
@@ -41,11 +41,16 @@ struct PyRowArgs {
     const float *nomt;        // PY_DTF: detector transfer function, transposed [kx][ky]
 };
 
-// L rows of the pass through the LDS line FFT, one block per L rows.
+// L rows of the pass through the LDS line FFT, one block per L rows; the
+// two line buffers (L S complex each) are the block's dynamic LDS.  1024
+// threads take at most one butterfly each per stage at the 2916-point lines
+// of a 30 m pyramid, so the stage's twiddle and line loads overlap across
+// 16 waves rather than queue within a thread.
 template <int MODE>
-__global__ __launch_bounds__(MX_PY_T) void k_py_rows(PyRowArgs a) {
-    __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
+__global__ __launch_bounds__(MX_PY_T) __attribute__((amdgpu_waves_per_eu(8))) void k_py_rows(PyRowArgs a) {
+    extern __shared__ float2 pyl[];
     const int n = a.n, L = a.L, S = a.S, tid = threadIdx.x;
+    float2 *const X = pyl, *const Y = pyl + L * S;
     const int r0 = (int)blockIdx.x * L;
     for (int l = 0; l < L; l++) {
         const int r = r0 + l;
@@ -303,14 +308,15 @@ int py_rows(mx_pywfs_t *h, int mode, PyRowArgs a, int nrows) {
     a.L = lf_lines(a.n);
     a.nrows = nrows;
     const dim3 grid((unsigned)((nrows + a.L - 1) / a.L)), block(MX_PY_T);
+    const size_t lds = sizeof(float2) * 2 * (size_t)a.L * a.S; // two line buffers, <= 2 MX_LF_LDS complex
     switch (mode) {
-    case PY_EMBED: k_py_rows<PY_EMBED><<<grid, block, 0, h->stream>>>(a); break;
-    case PY_PSF: k_py_rows<PY_PSF><<<grid, block, 0, h->stream>>>(a); break;
-    case PY_MOD: k_py_rows<PY_MOD><<<grid, block, 0, h->stream>>>(a); break;
-    case PY_ACC: k_py_rows<PY_ACC><<<grid, block, 0, h->stream>>>(a); break;
-    case PY_REAL: k_py_rows<PY_REAL><<<grid, block, 0, h->stream>>>(a); break;
-    case PY_DTF: k_py_rows<PY_DTF><<<grid, block, 0, h->stream>>>(a); break;
-    default: k_py_rows<PY_IMG><<<grid, block, 0, h->stream>>>(a); break;
+    case PY_EMBED: k_py_rows<PY_EMBED><<<grid, block, lds, h->stream>>>(a); break;
+    case PY_PSF: k_py_rows<PY_PSF><<<grid, block, lds, h->stream>>>(a); break;
+    case PY_MOD: k_py_rows<PY_MOD><<<grid, block, lds, h->stream>>>(a); break;
+    case PY_ACC: k_py_rows<PY_ACC><<<grid, block, lds, h->stream>>>(a); break;
+    case PY_REAL: k_py_rows<PY_REAL><<<grid, block, lds, h->stream>>>(a); break;
+    case PY_DTF: k_py_rows<PY_DTF><<<grid, block, lds, h->stream>>>(a); break;
+    default: k_py_rows<PY_IMG><<<grid, block, lds, h->stream>>>(a); break;
     }
     return MX_OK;
 }
@@ -477,6 +483,18 @@ int mx_pywfs_init(mx_pywfs_t **out, const mx_pywfs_cfg_t *c) {
     for (long i = 0; i < c->nloc; i++) h->ampnorm += c->amp[i] * c->amp[i];
     h->plan = plan;
     PYCHK(hipSetDevice(h->device));
+    {
+        // the row passes' two line buffers may exceed the default dynamic LDS limit
+        const int lds = (int)(sizeof(float2) * 2 * MX_LF_LDS);
+        const void *k[] = {reinterpret_cast<const void *>(&k_py_rows<PY_EMBED>),
+                           reinterpret_cast<const void *>(&k_py_rows<PY_PSF>),
+                           reinterpret_cast<const void *>(&k_py_rows<PY_MOD>),
+                           reinterpret_cast<const void *>(&k_py_rows<PY_ACC>),
+                           reinterpret_cast<const void *>(&k_py_rows<PY_REAL>),
+                           reinterpret_cast<const void *>(&k_py_rows<PY_DTF>),
+                           reinterpret_cast<const void *>(&k_py_rows<PY_IMG>)};
+        for (const void *f : k) PYCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+    }
     PYCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     PYCHK(hipEventCreate(&h->ev[0]));
     PYCHK(hipEventCreate(&h->ev[1]));
