@@ -116,6 +116,17 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h);
 int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal,
                        long si_nx, long si_ny, const long *si_pp,
                        const long *si_pi, const double *si_px);
+/* The same tables per subaperture (multi_nominal, maos/wfsints.c:200-233:
+ * powfs.dtf[iwvl].nominal and .si with nsa entries, which mkdtf makes for
+ * radial-pixel detectors or per-subaperture pixel offsets, lib/mkdtf.c:
+ * 140-161).  Subaperture isa's nominal is nominal + isa*notf*notf (NULL
+ * with an LLT), its si CSC si_pp + isa*(si_ny + 1), si_pi and si_px +
+ * isa*si_nzmax.  The MFMA and FP32 radix-8 forms need the subapertures'
+ * sampled image rows and columns to number at most 16 together; otherwise
+ * the FP32 register form runs. */
+int mx_wfsgrad_set_dtf_sa(mx_wfsgrad_t *h, int iwvl, const double *nominal,
+                          long si_nx, long si_ny, long si_nzmax, const long *si_pp,
+                          const long *si_pi, const double *si_px);
 
 /* Sodium elongation transfer function of one wavelength
  * (gpu_wfsgrad_update_etf, cuda/sim/wfsinit.cu:32; maos/sim_utils.c:388):
@@ -183,6 +194,32 @@ int mx_wfsgrad_i0_control(mx_wfsgrad_t *h, int enable, int reset);
 /* Copy WFS iwfs's accumulated sum (nsa*npix) and the number of frames. */
 int mx_wfsgrad_i0_get(mx_wfsgrad_t *h, int iwfs, double *i0sum_out,
                       long *nframe);
+
+/* Tip/tilt-removed PSF statistics (powfs.pistatout: maos/wfsints.c:262-275,
+ * GPU form cuda/sim/wfsints.cu:458-472, accumulated from pistatstart by
+ * maos/wfsgrad.c:155-156 and saved by gpu_save_pistat, cuda/sim/
+ * wfsgrad.cu:777-800).  With enable=1 every later step adds, per WFS,
+ * wavelength and subaperture, the downlink PSF (no uplink, DTF or ETF)
+ * shifted by the step's geometric gradient of that subaperture (gtype 1:
+ * Zernike tilt, pts_ztilt; 2: G-tilt, GS0 from mx_wfsgrad_set_gs0), peak at
+ * the centre, scaled by norm_psf^2 as MAOS scales it.  The nxsa 32, embfac
+ * 2, notf 64 geometry only; not with geometric gradients.  reset=1 clears
+ * the sums.  The sums are FP32 on the GPU, as the reference GPU code keeps
+ * them. */
+int mx_wfsgrad_pistat_control(mx_wfsgrad_t *h, int enable, int gtype, int reset);
+/* Copy WFS iwfs's sums, [nwvl][nsa][notf*notf] (x fastest; MAOS's dcell
+ * nsa x nwvl of notf x notf), and the number of steps they hold
+ * (save_pistat, maos/save.c:29-58, divides by it). */
+int mx_wfsgrad_pistat_get(mx_wfsgrad_t *h, int iwfs, double *out, long *nstep);
+
+/* The complex field of every subaperture of WFS iwfs at the last queued
+ * step (powfs.psfout: maos/wfsints.c:243-253, GPU form cuda/sim/
+ * wfsints.cu:443-450, pushed to wvfout_*.bin each step by maos/wfsgrad.c:
+ * 403): the subaperture's complex PSF scaled by norm_psf / nwvf^2,
+ * transformed back, centre nout^2 with nout = notf/2 + 2.  out is
+ * [nwvl][nsa][nout*nout] complex interleaved (x fastest; MAOS's ccell nsa x
+ * nwvl).  Needs nxsa*embfac == notf; waits for the queued steps. */
+int mx_wfsgrad_psfout(mx_wfsgrad_t *h, int iwfs, double *out);
 
 /* TT-dither statistics for the matched-filter update (maos/wfsgrad.c:
  * 449-466).  With enable=1 every later step adds, per WFS and pixel, ints,

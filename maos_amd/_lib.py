@@ -96,6 +96,8 @@ HIP_API = {
     "mx_wfsgrad_free": (C.c_int, [C.c_void_p]),
     "mx_wfsgrad_set_dtf": (C.c_int, [C.c_void_p, C.c_int, c_double_p, C.c_long, C.c_long,
                                      c_long_p, c_long_p, c_double_p]),
+    "mx_wfsgrad_set_dtf_sa": (C.c_int, [C.c_void_p, C.c_int, c_double_p, C.c_long, C.c_long, C.c_long,
+                                        c_long_p, c_long_p, c_double_p]),
     "mx_wfsgrad_update_etf": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p, C.c_double]),
     "mx_wfsgrad_update_ref": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_atm2gpu": (C.c_int, [C.POINTER(MxAtmLayer), C.c_int, C.c_double]),
@@ -106,6 +108,9 @@ HIP_API = {
     "mx_wfsgrad_sync": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_double_p]),
     "mx_wfsgrad_i0_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "mx_wfsgrad_i0_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_long_p]),
+    "mx_wfsgrad_pistat_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "mx_wfsgrad_pistat_get": (C.c_int, [C.c_void_p, C.c_int, c_double_p, c_long_p]),
+    "mx_wfsgrad_psfout": (C.c_int, [C.c_void_p, C.c_int, c_double_p]),
     "mx_wfsgrad_set_etf_weight": (C.c_int, [C.c_void_p, C.c_double]),
     "mx_wfsgrad_set_gs0": (C.c_int, [C.c_void_p, c_long_p, c_long_p, c_double_p, C.c_long]),
     "mx_wfsgrad_dither_control": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),

@@ -45,13 +45,14 @@ def ngs8m(nx=2048):
     return dict(name="ngs8m", aper=aper, powfs=pcfg, sim=sim, atm=atm, wfs=wfs, nstep=100)
 
 
-def lgs30m(nx=4096, nwfs=6, order=60, embfac=2):
-    """embfac=4 is the 128^2 padding variant (2x Nyquist OTF)."""
+def lgs30m(nx=4096, nwfs=6, order=60, embfac=2, llt_d=0.4):
+    """embfac=4 is the 128^2 padding variant (2x Nyquist OTF); llt_d above
+    the 0.5 m subaperture makes the LLT grid wider than the subaperture's."""
     aper = AperCfg(d=30.0 * order / 60, din=3.6 * order / 60)
     pcfg = PowfsCfg(order=order, dsa=0.5, dx=1.0 / 64, saat=0.3, hs=90e3, embfac=embfac,
                     wvl=[0.589e-6], wvlwts=[1.0], pixpsa=6, pixtheta=1.0, pixblur=0.3,
                     phytype=PTYPE_MF, sigmatch=1, mtchcr=1, rne=3.0, siglev=960.0,
-                    llt=LltCfg(d=0.4, widthp=0.6, ox=0.0, oy=0.0),
+                    llt=LltCfg(d=llt_d, widthp=0.6, ox=0.0, oy=0.0),
                     sodium=SodiumCfg(center=90e3, fwhm=10e3, hmin=70e3, hmax=110e3, dh=20.0))
     sim = SimCfg(dt=1.0 / 800)
     rs = np.random.RandomState(2024)

@@ -144,6 +144,9 @@ struct DevWvl {
     const float2 *etf, *etf2;
     const int4 *si_idx;
     const float4 *si_w;
+    // per-subaperture tables (multi_nominal): element strides between the
+    // subapertures' si entries and nominals, 0 when one table serves all
+    int si_sa, nom_sa;
 };
 
 // ------------------------------------------------------------ pixel noise
@@ -213,20 +216,28 @@ __device__ __forceinline__ void philox4x32_10(unsigned (&c)[4], unsigned k0, uns
     }
 }
 struct NoiseStream {
-    unsigned k0, k1, pix, unit, step, blk = 0, buf[4] = {};
+    // the four words of the current Philox block are consumed in order;
+    // they are shifted down rather than indexed, so the stream stays in
+    // registers (a run-time array index would place it in private memory)
+    unsigned k0, k1, pix, unit, step, blk = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
     int left = 0;
     // uniform in (0, 1]
     __device__ double uniform() {
         if (left == 0) {
             unsigned c[4] = {blk++, pix, unit, step};
             philox4x32_10(c, k0, k1);
-            buf[0] = c[0];
-            buf[1] = c[1];
-            buf[2] = c[2];
-            buf[3] = c[3];
+            b0 = c[0];
+            b1 = c[1];
+            b2 = c[2];
+            b3 = c[3];
             left = 4;
         }
-        return ((double)buf[4 - left--] + 1.) * 2.3283064365386962890625e-10;
+        const unsigned u = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        left--;
+        return ((double)u + 1.) * 2.3283064365386962890625e-10;
     }
     __device__ double normal() {
         const double u1 = uniform(), u2 = uniform();
@@ -299,11 +310,13 @@ struct OpdArgs {
     long isim;
     double dt;
     float *opd; // [nwfs][nsa][nxsa^2]
-    // LLT uplink grids (nllt = nwfs, or 0 without LLT): origin and launch
-    // offset; the grid is nxsa x nxsa like a subaperture's
-    int nllt;
+    // LLT uplink grids (nllt = nwfs llt_tiles^2 blocks, or 0 without LLT):
+    // origin and launch offset; WFS w's grid is llt_tiles x llt_tiles blocks
+    // of the kernel's NXSA, stored with side llt_tiles NXSA (its first llt_nx
+    // rows and columns are the LLT's points)
+    int nllt, llt_tiles;
     double llt_origx, llt_origy, llt_ox, llt_oy;
-    float *llt_opd; // [nwfs][nxsa^2]
+    float *llt_opd; // [nwfs][side^2]
     int tiles;      // a subaperture grid is tiles x tiles blocks of the kernel's NXSA
     int tworow;     // interpolate every layer from two tile rows a point (verification)
 };
@@ -759,7 +772,7 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
                     for (int k = 0; k < N; k++) v[k] = cmul(v[k], e1[bitrev(k, LG) * N]);
                 }
             } else if (WV.nominal) {
-                const float *nm = WV.nominal + uy;
+                const float *nm = WV.nominal + (long)isa * WV.nom_sa + uy;
 #pragma unroll
                 for (int k = 0; k < N; k++) {
                     const float s = nm[bitrev(k, LG) * N];
@@ -786,8 +799,8 @@ __global__ __launch_bounds__(64) void k_shwfs_step(KArgs a) {
         // bilinear pixel sampler (dspmulcreal with si, math/spmm.c:77-86)
         {
             const float *img = reinterpret_cast<const float *>(T);
-            const int4 *sidx = WV.si_idx;
-            const float4 *sw = WV.si_w;
+            const int4 *sidx = WV.si_idx + (long)isa * WV.si_sa;
+            const float4 *sw = WV.si_w + (long)isa * WV.si_sa;
             const float alpha = W.alpha[iwvl];
 #pragma unroll
             for (int j = 0; j < PPL; j++) {
@@ -895,14 +908,18 @@ __global__ __launch_bounds__(MX_OPD_THREADS) __attribute__((amdgpu_waves_per_eu(
     __shared__ __align__(16) float tile[rt_tile_rows(NXSA) * rt_tile_stride(NXSA)];
     const int b = (int)blockIdx.x;
     if (b < a.nllt) {
-        // LLT uplink grid of WFS b (maos/wfsgrad.c:329-386): theta - o/hs,
-        // shift o - v dt isim.  These blocks come first so that they start
-        // early.
-        const DevWfs &W = a.wfs[b];
+        // LLT uplink grid of WFS b / llt_tiles^2 (maos/wfsgrad.c:329-386):
+        // theta - o/hs, shift o - v dt isim.  These blocks come first so that
+        // they start early.
+        const int lt = a.llt_tiles, lt2 = lt * lt, iw = b / lt2, t = b % lt2, tx = t % lt, ty = t / lt;
+        const int side = NXSA * lt;
+        const DevWfs &W = a.wfs[iw];
         ray_trace<MX_OPD_THREADS, NXSA, DIRECT>(a.layers + W.ireal * MX_MAXLAYER, a.nps, W.hs, W.thetax - a.llt_ox / W.hs,
                                         W.thetay - a.llt_oy / W.hs, 0., 0., a.llt_ox, a.llt_oy, a.isim, a.dt,
-                                        a.atmscale[W.ireal], a.ptsdx, a.llt_origx, a.llt_origy,
-                            threadIdx.x, a.llt_opd + (long)b * NXSA * NXSA, cidx, cfrac, tile, ctab, a.tworow);
+                                        a.atmscale[W.ireal], a.ptsdx, a.llt_origx + a.ptsdx * (double)(NXSA * tx),
+                                        a.llt_origy + a.ptsdx * (double)(NXSA * ty), threadIdx.x,
+                                        a.llt_opd + (long)iw * side * side + (long)ty * NXSA * side + tx * NXSA, cidx,
+                                        cfrac, tile, ctab, a.tworow, side);
         return;
     }
     if constexpr (!TILED) {
@@ -1321,6 +1338,7 @@ struct ZtiltArgs {
     const float *saa;
     float saat;
     float *grad;        // [nwfs][2*nsa]
+    int all;            // every subaperture (else those with saa >= saat)
 };
 __global__ __launch_bounds__(64) void k_ztilt(ZtiltArgs a) {
     const int unit = xcd_unit((int)blockIdx.x, a.nsa * a.nwfs);
@@ -1343,7 +1361,7 @@ __global__ __launch_bounds__(64) void k_ztilt(ZtiltArgs a) {
         a1 += __shfl_xor(a1, o, 64);
         a2 += __shfl_xor(a2, o, 64);
     }
-    if (threadIdx.x == 0 && !(a.saa[isa] < a.saat)) {
+    if (threadIdx.x == 0 && (a.all || !(a.saa[isa] < a.saat))) {
         const double *m = a.imcc + 9 * (long)isa; // column-major 3x3
         a.grad[(long)iwfs * 2 * a.nsa + isa] = (float)(m[1] * a0 + m[4] * a1 + m[7] * a2);
         a.grad[(long)iwfs * 2 * a.nsa + a.nsa + isa] = (float)(m[2] * a0 + m[5] * a1 + m[8] * a2);
@@ -1513,6 +1531,7 @@ struct MfmaWvl {
     const void *gy;          // [6][64] w(uy) conj(F) at the sampled rows, uy < 32
     const int4 *si_c;        // [npix] compact indices (iy*16 + ix) into the 16x16 image
     const float4 *si_w;      // [npix]
+    int si_sa, nom_sa;       // per-subaperture strides of si_c / si_w and nominal_t (0: shared)
     float inv_wvl;
     // FP32 radix-8 form: w(uy) exp(+2 pi i uy ys / 64) [uy < 32][16 sampled
     // rows ys], and the 16 sampled columns xs (the list padded with its first)
@@ -1911,7 +1930,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
                     }
                 }
             } else if (WV.nominal_t) {
-                const float *nm = WV.nominal_t + tbase;
+                const float *nm = WV.nominal_t + (long)isa * WV.nom_sa + tbase;
                 float tn[8];
 #pragma unroll
                 for (int i = 0; i < 8; i++) tn[i] = nm[i * 64];
@@ -1968,8 +1987,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         }
         __syncthreads();
         if (tid < a.npix) {
-            const int4 id = WV.si_c[tid];
-            const float4 w = WV.si_w[tid];
+            const int4 id = WV.si_c[isa * WV.si_sa + tid];
+            const float4 w = WV.si_w[isa * WV.si_sa + tid];
             pv += W.alpha[iwvl] * (w.x * img[id.x] + w.y * img[id.y] + w.z * img[id.z] + w.w * img[id.w]);
         }
         // (PP: the next wavelength writes img only after further barriers)
@@ -1981,6 +2000,92 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
 }
 
 #include "shwfs_fft.hip"
+
+// psfout of wfsints (maos/wfsints.c:243-253; GPU form cuda/sim/wfsints.cu:
+// 443-450): the complex PSF scaled by norm_psf / nwvf^2 and transformed
+// back, centre nout^2 (nout = notf/2 + 2).  With nwvf == notf the two
+// transforms cancel and leave norm_psf amp exp(i 2 pi opd / wvl) at the
+// embedded position, zero around it, which is what this writes: one block
+// per (subaperture, wavelength) of one WFS.
+struct PsfoutArgs {
+    int nsa, nxsa, nout, b; // b: output index of the field's first sample
+    float norm;             // norm_psf
+    float inv_wvl[MX_MAXWVL];
+    const float *opd, *amp; // this WFS's [nsa][nxsa^2], [nsa][nxsa^2]
+    float2 *out;            // [nwvl][nsa][nout^2]
+};
+__global__ __launch_bounds__(256) void k_psfout(PsfoutArgs a) {
+    const int isa = (int)blockIdx.x, iwvl = (int)blockIdx.y, n2 = a.nout * a.nout;
+    float2 *o = a.out + ((long)iwvl * a.nsa + isa) * n2;
+    for (int e = (int)threadIdx.x; e < n2; e += 256) {
+        const int ix = e % a.nout - a.b, iy = e / a.nout - a.b;
+        float2 v = make_float2(0.f, 0.f);
+        if (ix >= 0 && ix < a.nxsa && iy >= 0 && iy < a.nxsa) {
+            const long pt = (long)isa * a.nxsa * a.nxsa + (long)iy * a.nxsa + ix;
+            cis_rev(a.opd[pt] * a.inv_wvl[iwvl], a.amp[pt] * a.norm, v.x, v.y);
+        }
+        o[e] = v;
+    }
+}
+
+// The uplink OTF of an LLT grid wider than the subaperture grid
+// (maos/wfsints.c:160-185: nlwvf > nwvf, the OTF cropped by ccpcorner to
+// nwvf = notf).  FFT(|FFT(E)|^2 / nlwvf^2) is the autocorrelation
+// lotfc(u) = sum_x E(x) conj(E(x + u)) of the LLT field E (its support,
+// nlx <= nlwvf / 2, leaves no wrap-around), and the crop keeps the lags
+// -notf/2 <= u < notf/2, so each is summed directly.  One block per (WFS,
+// wavelength, row uy); conj(lotfc) goes to the full plane [ux][uy] and, for
+// the half-plane forms, to the half plane in half_slot order.
+struct LltAcorrArgs {
+    int nlx, side, nwvl, N; // N = notf = nwvf <= 64, nlx <= 64
+    float inv_wvl[MX_MAXWVL];
+    const float *opd;       // [nwfs][side^2], the LLT points in the first nlx rows and columns
+    const float *amp;       // [nlx^2]
+    float2 *lotf_full;      // [nwfs][nwvl][N*N]
+    float2 *lotf_half;      // [nwfs][nwvl][MX_HALF] or null
+};
+__global__ __launch_bounds__(256) void k_llt_acorr(LltAcorrArgs a) {
+    __shared__ float2 E[64 * 64];
+    __shared__ float2 part[256];
+    __shared__ float red[4];
+    const int N = a.N, nlx = a.nlx, tid = (int)threadIdx.x;
+    const int uy = (int)blockIdx.x % N, r = (int)blockIdx.x / N, iwvl = r % a.nwvl, iwfs = r / a.nwvl;
+    const float *op = a.opd + (long)iwfs * a.side * a.side;
+    // the field, piston removed (it cancels in E(x) conj(E(x + u)))
+    float s = 0.f;
+    for (int e = tid; e < nlx * nlx; e += 256) s += op[(e / nlx) * a.side + e % nlx];
+    seg_sum<64>(s);
+    if ((tid & 63) == 0) red[tid >> 6] = s;
+    __syncthreads();
+    const float mean = (red[0] + red[1] + red[2] + red[3]) / (float)(nlx * nlx);
+    for (int e = tid; e < nlx * nlx; e += 256) {
+        float re, im;
+        cis_rev((op[(e / nlx) * a.side + e % nlx] - mean) * a.inv_wvl[iwvl], a.amp[e], re, im);
+        E[e] = make_float2(re, im);
+    }
+    __syncthreads();
+    const int np = 256 / N, ux = tid % N, pp = tid / N;
+    const int sx = ux < N / 2 ? ux : ux - N, sy = uy < N / 2 ? uy : uy - N;
+    const int x0 = max(0, -sx), x1 = min(nlx, nlx - sx), y0 = max(0, -sy), y1 = min(nlx, nlx - sy);
+    float2 acc = make_float2(0.f, 0.f);
+    for (int y = y0 + pp; y < y1; y += np) {
+        const float2 *e0 = E + y * nlx, *e1 = E + (y + sy) * nlx + sx;
+        for (int x = x0; x < x1; x++) {
+            const float2 p = e0[x], q = e1[x]; // p conj(q)
+            acc.x = fmaf(p.x, q.x, fmaf(p.y, q.y, acc.x));
+            acc.y = fmaf(p.y, q.x, fmaf(-p.x, q.y, acc.y));
+        }
+    }
+    part[tid] = acc;
+    __syncthreads();
+    if (tid < N) {
+        float2 t = make_float2(0.f, 0.f);
+        for (int k = 0; k < np; k++) t = cadd(t, part[k * N + tid]);
+        const long w = (long)iwfs * a.nwvl + iwvl;
+        a.lotf_full[w * N * N + (long)tid * N + uy] = make_float2(t.x, -t.y);
+        if (a.lotf_half && uy < N / 2) a.lotf_half[w * MX_HALF + half_slot(tid, uy)] = make_float2(t.x, -t.y);
+    }
+}
 
 // ------------------------------------------------------------ large fields
 // Fields larger than the one-workgroup subaperture forms hold: nwvf and notf
@@ -2047,6 +2152,7 @@ struct LfWvl {
     const int *rows, *cols;
     const int4 *si_c;     // si in compact indices iy * nx + ix
     const float4 *si_w;
+    int si_sa, nom_sa;    // per-subaperture strides of si_c / si_w and nominal (0: shared)
 };
 struct LfArgs {
     LfPlan pn, pm;        // lengths nwvf and notf
@@ -2379,7 +2485,7 @@ __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
         for (int e = threadIdx.x; e < L * m; e += T) {
             const int l = ldiv<(N > 0)>(e, m, rm), y2 = e - l * m, x2 = x0 + l;
             if (x2 < m) {
-                const float f = W.nominal[(long)x2 * m + y2];
+                const float f = W.nominal[(long)isa * W.nom_sa + (long)x2 * m + y2];
                 float2 &z = r[l * S + lf_pos(y2)];
                 z = make_float2(z.x * f, z.y * f);
             }
@@ -2450,8 +2556,9 @@ __global__ __launch_bounds__(MX_LF_THREADS) void k_lf_sample(LfSampleArgs a) {
         float pv = 0.f;
         for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
             const float *im = a.img + (((long)iwfs * a.nsa + isa) * a.nwvl + iwvl) * a.imgstride;
-            const int4 id = a.wv[iwvl].si_c[p];
-            const float4 w = a.wv[iwvl].si_w[p];
+            const LfWvl &SW = a.wv[iwvl];
+            const int4 id = SW.si_c[isa * SW.si_sa + p];
+            const float4 w = SW.si_w[isa * SW.si_sa + p];
             pv += W.alpha[iwvl] * (w.x * im[id.x] + w.y * im[id.y] + w.z * im[id.z] + w.w * im[id.w]);
         }
         if (!dt_frame(a.dt, ibase + p, pv)) continue; // uniform: not the exposure's last frame
@@ -2648,8 +2755,10 @@ struct mx_wfsgrad_s {
     int4 *d_si_idx[MX_MAXWVL] = {};
     float4 *d_si_w[MX_MAXWVL] = {};
     bool dtf_set[MX_MAXWVL] = {};
+    long dtf_ntab[MX_MAXWVL] = {};  // si / nominal tables: 1, or nsa (one per subaperture)
     float *d_llt_amp = nullptr;
-    float *d_llt_opd = nullptr;        // [nwfs][llt_nx^2] uplink OPD
+    float *d_llt_opd = nullptr;        // [nwfs][side^2] uplink OPD (side: llt_nx in whole ray-trace tiles)
+    LltAcorrArgs arg_lltac{};         // uplink of an LLT grid wider than the subaperture grid
     float2 *d_lotf = nullptr;
     DevWvl *d_wv = nullptr;
     DevWvl h_wv[MX_MAXWVL] = {};
@@ -2657,6 +2766,7 @@ struct mx_wfsgrad_s {
     int variant = MX_VARIANT_AUTO;     // requested form
     bool mfma_size = false;            // N = 64, nxsa = 32
     double amp_max = 0;                // largest subaperture or LLT amplitude (the FP16 form needs <= 1)
+    double llt_sum2 = 0;               // sum of the LLT amplitudes squared, lotf(0) (the FP16 form needs <= 1024)
     bool mfma_wvl_ok[MX_MAXWVL] = {};  // sampled rows/columns fit in 16
     uint16_t *d_F = nullptr, *d_F16 = nullptr; // DFT tables [MX_FENT][6][64] x 8, BF16 and FP16
     uint16_t *d_gx[MX_MAXWVL] = {}, *d_gy[MX_MAXWVL] = {};     // BF16
@@ -2723,6 +2833,16 @@ struct mx_wfsgrad_s {
     int dtrat = 1;
     float *d_dtacc = nullptr;
     bool last_out = true;
+    // tip/tilt-removed PSF statistics (mx_wfsgrad_pistat_control)
+    bool pistat = false;
+    int pistat_gtype = 1;             // shift by 1: Zernike tilt, 2: G-tilt
+    long pistat_nstep = 0;
+    float *d_pistat = nullptr;        // [nwfs][nwvl][nsa][notf^2]
+    float *d_gpis = nullptr;          // [nwfs][2*nsa] the step's geometric gradients
+    ZtiltArgs arg_pis_z{};
+    GtiltArgs arg_pis_g{};
+    PistatArgs arg_pis{};
+    float2 *d_psfout = nullptr;       // [nwvl][nsa][nout^2] of one WFS (mx_wfsgrad_psfout)
     FoldArgs arg_fold{};
     MtchArgs arg_mtch{};
     Sig2Args arg_sig2{};
@@ -3067,8 +3187,13 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
         return set_err(MX_EINVAL, "sigmatch %d unsupported (0, 1 or 2)", p->sigmatch);
     if (p->sigmatch == 2 && !(p->saasum > 0))
         return set_err(MX_EINVAL, "sigmatch 2 needs saasum > 0 (saasum=%g)", p->saasum);
-    if (p->llt_nx > 0 && !(p->llt_nx * p->embfac == p->notf && p->llt_nx == p->nxsa))
-        return set_err(MX_EINVAL, "LLT grid %ld must equal nxsa with nlwvf == notf", p->llt_nx);
+    // an LLT grid wider than the subaperture grid (MAOS makes it max(llt.d,
+    // dsa) wide) takes the direct-autocorrelation uplink: nwvf == notf <= 64
+    // and at most 64 LLT points a side
+    if (p->llt_nx > 0 && p->llt_nx != p->nxsa &&
+        !(!lf && p->llt_nx > p->nxsa && p->llt_nx <= 64 && p->nxsa * p->embfac == p->notf && p->notf <= 64))
+        return set_err(MX_EINVAL, "LLT grid %ld: needs llt_nx == nxsa, or nxsa < llt_nx <= 64 with "
+                       "nxsa*embfac == notf <= 64 (got nxsa=%ld notf=%ld)", p->llt_nx, p->nxsa, p->notf);
     if (p->nsa < 1) return set_err(MX_EINVAL, "nsa=%ld", p->nsa);
     HIPCHK(hipSetDevice(g_device));
     mx_wfsgrad_t *h = new mx_wfsgrad_t();
@@ -3126,7 +3251,10 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
     }
     if (p->llt_nx > 0) {
         ALLOC(h->d_llt_amp, p->llt_nx * p->llt_nx);
-        ALLOC(h->d_llt_opd, nwfs * p->llt_nx * p->llt_nx);
+        // traced in tiles of 16 or 32 points: the grid is stored with its side
+        // rounded up to a multiple of 32
+        const long side = (p->llt_nx + 31) / 32 * 32;
+        ALLOC(h->d_llt_opd, nwfs * side * side);
         ALLOC(h->d_lotf, nwfs * h->nwvl * N * N);
         if (h->mfma_size) ALLOC(h->d_lotf_t, nwfs * h->nwvl * MX_HALF);
     }
@@ -3147,7 +3275,10 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
         tmp[i] = (float)p->amp[i];
         h->amp_max = std::max(h->amp_max, std::fabs(p->amp[i]));
     }
-    for (long i = 0; i < p->llt_nx * p->llt_nx; i++) h->amp_max = std::max(h->amp_max, std::fabs(p->llt_amp[i]));
+    for (long i = 0; i < p->llt_nx * p->llt_nx; i++) {
+        h->amp_max = std::max(h->amp_max, std::fabs(p->llt_amp[i]));
+        h->llt_sum2 += p->llt_amp[i] * p->llt_amp[i];
+    }
     if (hipMemcpy(h->d_amp, tmp.data(), sizeof(float) * nsa * npsa, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->d_ptsx, p->ptsx, sizeof(double) * nsa, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->d_ptsy, p->ptsy, sizeof(double) * nsa, hipMemcpyHostToDevice) != hipSuccess)
@@ -3249,6 +3380,9 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     for (void *q : ptrs)
         if (q) hipFree(q);
     if (h->d_dtacc) hipFree(h->d_dtacc);
+    if (h->d_pistat) hipFree(h->d_pistat);
+    if (h->d_gpis) hipFree(h->d_gpis);
+    if (h->d_psfout) hipFree(h->d_psfout);
     for (int w = 0; w < MX_MAXWVL; w++) {
         if (h->d_nominal[w]) hipFree(h->d_nominal[w]);
         if (h->d_gx[w]) hipFree(h->d_gx[w]);
@@ -3276,65 +3410,81 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     return MX_OK;
 }
 
-int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si_nx, long si_ny,
-                       const long *si_pp, const long *si_pi, const double *si_px) {
-    if (!h || iwvl < 0 || iwvl >= h->nwvl) return set_err(MX_EINVAL, "mx_wfsgrad_set_dtf: bad handle or iwvl");
-    const long N = h->notf;
-    if (si_nx != h->npix || si_ny != N * N || !si_pp || !si_pi || !si_px)
-        return set_err(MX_EINVAL, "si must be %ld x %ld (got %ld x %ld)", h->npix, N * N, si_nx, si_ny);
+namespace {
+// The detector tables of one wavelength: ntab = 1 (one si and nominal for
+// every subaperture) or nsa (multi_nominal, maos/wfsints.c:200-233).  Table
+// t is si_pp + t (si_ny + 1), si_pi / si_px + t si_nzmax and nominal +
+// t notf^2.  Device arrays are [ntab][...] with the per-subaperture stride
+// in the kernels' argument blocks (0 when ntab = 1).
+int set_dtf_tables(mx_wfsgrad_t *h, int iwvl, long ntab, const double *nominal, long si_nx, long si_ny,
+                   long si_nzmax, const long *si_pp, const long *si_pi, const double *si_px) {
+    const long N = h->notf, npix = h->npix;
+    if (si_nx != npix || si_ny != N * N || !si_pp || !si_pi || !si_px)
+        return set_err(MX_EINVAL, "si must be %ld x %ld (got %ld x %ld)", npix, N * N, si_nx, si_ny);
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamSynchronize(h->stream)); // a queued step may still read the tables replaced here
-    // per-pixel list of up to 4 (index, weight) pairs
-    std::vector<int4> idx(h->npix);
-    std::vector<float4> wt(h->npix);
-    std::vector<int> cnt(h->npix, 0);
-    for (long i = 0; i < h->npix; i++) {
-        idx[i] = make_int4(0, 0, 0, 0);
-        wt[i] = make_float4(0, 0, 0, 0);
-    }
-    for (long col = 0; col < N * N; col++) {
-        for (long k = si_pp[col]; k < si_pp[col + 1]; k++) {
-            long r = si_pi[k];
-            if (r < 0 || r >= h->npix) return set_err(MX_EINVAL, "si row index %ld out of range", r);
-            int c = cnt[r]++;
-            if (c >= 4) return set_err(MX_EINVAL, "si pixel %ld has more than 4 entries", r);
-            int *ip = &idx[r].x;
-            float *wp = &wt[r].x;
-            ip[c] = (int)col;
-            wp[c] = (float)si_px[k];
+    // per-pixel list of up to 4 (index, weight) pairs of every table
+    std::vector<int4> idx(ntab * npix, make_int4(0, 0, 0, 0));
+    std::vector<float4> wt(ntab * npix, make_float4(0, 0, 0, 0));
+    for (long t = 0; t < ntab; t++) {
+        const long *pp = si_pp + t * (si_ny + 1), *pi = si_pi + t * si_nzmax;
+        const double *px = si_px + t * si_nzmax;
+        std::vector<int> cnt(npix, 0);
+        if (pp[0] < 0 || pp[N * N] > si_nzmax) return set_err(MX_EINVAL, "si table %ld: bad column pointers", t);
+        for (long col = 0; col < N * N; col++) {
+            for (long k = pp[col]; k < pp[col + 1]; k++) {
+                const long r = pi[k];
+                if (r < 0 || r >= npix) return set_err(MX_EINVAL, "si row index %ld out of range", r);
+                const int c = cnt[r]++;
+                if (c >= 4) return set_err(MX_EINVAL, "si pixel %ld has more than 4 entries", r);
+                (&idx[t * npix + r].x)[c] = (int)col;
+                (&wt[t * npix + r].x)[c] = (float)px[k];
+            }
         }
+    }
+    // a change of table count frees the buffers sized for the old one
+    if (h->dtf_ntab[iwvl] != ntab) {
+        void **bufs[] = {(void **)&h->d_si_idx[iwvl], (void **)&h->d_si_w[iwvl], (void **)&h->d_nominal[iwvl],
+                         (void **)&h->d_sic[iwvl], (void **)&h->d_nominal_t[iwvl], (void **)&h->d_lf_sic[iwvl]};
+        for (void **q : bufs)
+            if (*q) {
+                HIPCHK(hipFree(*q));
+                *q = nullptr;
+            }
+        h->dtf_ntab[iwvl] = ntab;
     }
     int rc;
-    if ((rc = dev_alloc(&h->d_si_idx[iwvl], h->npix)) || (rc = dev_alloc(&h->d_si_w[iwvl], h->npix))) return rc;
-    HIPCHK(hipMemcpy(h->d_si_idx[iwvl], idx.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->d_si_w[iwvl], wt.data(), sizeof(float4) * h->npix, hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(&h->d_si_idx[iwvl], ntab * npix)) || (rc = dev_alloc(&h->d_si_w[iwvl], ntab * npix)))
+        return rc;
+    HIPCHK(hipMemcpy(h->d_si_idx[iwvl], idx.data(), sizeof(int4) * ntab * npix, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_si_w[iwvl], wt.data(), sizeof(float4) * ntab * npix, hipMemcpyHostToDevice));
     if (nominal && h->cfg.llt_nx == 0) {
-        if ((rc = dev_alloc(&h->d_nominal[iwvl], N * N))) return rc;
-        std::vector<float> nt(N * N);
-        for (long uy = 0; uy < N; uy++)
-            for (long ux = 0; ux < N; ux++) nt[ux * N + uy] = (float)nominal[ux + uy * N];
-        HIPCHK(hipMemcpy(h->d_nominal[iwvl], nt.data(), sizeof(float) * N * N, hipMemcpyHostToDevice));
+        if ((rc = dev_alloc(&h->d_nominal[iwvl], ntab * N * N))) return rc;
+        std::vector<float> nt(ntab * N * N);
+        for (long t = 0; t < ntab; t++)
+            for (long uy = 0; uy < N; uy++)
+                for (long ux = 0; ux < N; ux++)
+                    nt[t * N * N + ux * N + uy] = (float)nominal[t * N * N + ux + uy * N];
+        HIPCHK(hipMemcpy(h->d_nominal[iwvl], nt.data(), sizeof(float) * ntab * N * N, hipMemcpyHostToDevice));
+    }
+    // image rows and columns the samplers read: the union over the tables
+    std::vector<int> xs, ys;
+    for (long i = 0; i < ntab * npix; i++)
+        for (int c = 0; c < 4; c++)
+            if ((&wt[i].x)[c] != 0.f) {
+                xs.push_back((&idx[i].x)[c] % (int)N);
+                ys.push_back((&idx[i].x)[c] / (int)N);
+            }
+    for (std::vector<int> *v : {&xs, &ys}) {
+        std::sort(v->begin(), v->end());
+        v->erase(std::unique(v->begin(), v->end()), v->end());
     }
     if (h->mfma_size) {
-        // image columns / rows the sampler reads, and si in compact 16x16 indices
-        std::vector<int> xs, ys;
-        for (long i = 0; i < h->npix; i++) {
-            const int *ip = &idx[i].x;
-            const float *wp = &wt[i].x;
-            for (int c = 0; c < 4; c++)
-                if (wp[c] != 0.f) {
-                    xs.push_back(ip[c] % (int)N);
-                    ys.push_back(ip[c] / (int)N);
-                }
-        }
-        std::sort(xs.begin(), xs.end());
-        xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
-        std::sort(ys.begin(), ys.end());
-        ys.erase(std::unique(ys.begin(), ys.end()), ys.end());
         h->mfma_wvl_ok[iwvl] = xs.size() <= MX_MFMA_NS && ys.size() <= MX_MFMA_NS;
         if (h->mfma_wvl_ok[iwvl]) {
-            std::vector<int4> sic(h->npix);
-            for (long i = 0; i < h->npix; i++) {
+            // si in compact 16x16 indices
+            std::vector<int4> sic(ntab * npix);
+            for (long i = 0; i < ntab * npix; i++) {
                 int *op = &sic[i].x;
                 const int *ip = &idx[i].x;
                 for (int c = 0; c < 4; c++) {
@@ -3352,11 +3502,11 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
             HIPCHK(hipMemcpy(h->d_gx16[iwvl], gx16.data(), gx16.size() * 2, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(h->d_gy16[iwvl], gy16.data(), gy16.size() * 2, hipMemcpyHostToDevice));
             if ((rc = dev_alloc(&h->d_gx[iwvl], gx.size())) || (rc = dev_alloc(&h->d_gy[iwvl], gy.size())) ||
-                (rc = dev_alloc(&h->d_sic[iwvl], h->npix)))
+                (rc = dev_alloc(&h->d_sic[iwvl], ntab * npix)))
                 return rc;
             HIPCHK(hipMemcpy(h->d_gx[iwvl], gx.data(), gx.size() * 2, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(h->d_gy[iwvl], gy.data(), gy.size() * 2, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(h->d_sic[iwvl], sic.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(h->d_sic[iwvl], sic.data(), sizeof(int4) * ntab * npix, hipMemcpyHostToDevice));
             // the FP32 radix-8 form's last pass: w(uy) exp(+2 pi i uy ys / 64)
             // for uy < 32 at the sampled rows (FP64 angles reduced mod 64),
             // and the sampled columns
@@ -3373,52 +3523,44 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
             HIPCHK(hipMemcpy(h->d_fy[iwvl], fy.data(), sizeof(float2) * fy.size(), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(h->d_fxs[iwvl], fxs.data(), sizeof(int) * fxs.size(), hipMemcpyHostToDevice));
             if (nominal && h->cfg.llt_nx == 0) {
-                std::vector<float> nt(MX_HALF);
-                for (long uy = 0; uy < N / 2; uy++)
-                    for (long ux = 0; ux < N; ux++) nt[half_slot((int)ux, (int)uy)] = (float)nominal[ux + uy * N];
-                if ((rc = dev_alloc(&h->d_nominal_t[iwvl], MX_HALF))) return rc;
-                HIPCHK(hipMemcpy(h->d_nominal_t[iwvl], nt.data(), sizeof(float) * MX_HALF, hipMemcpyHostToDevice));
+                std::vector<float> nt(ntab * MX_HALF);
+                for (long t = 0; t < ntab; t++)
+                    for (long uy = 0; uy < N / 2; uy++)
+                        for (long ux = 0; ux < N; ux++)
+                            nt[t * MX_HALF + half_slot((int)ux, (int)uy)] = (float)nominal[t * N * N + ux + uy * N];
+                if ((rc = dev_alloc(&h->d_nominal_t[iwvl], ntab * MX_HALF))) return rc;
+                HIPCHK(hipMemcpy(h->d_nominal_t[iwvl], nt.data(), sizeof(float) * ntab * MX_HALF,
+                                 hipMemcpyHostToDevice));
             }
         }
     }
     if (h->lf) {
-        // image rows / columns the sampler reads; all of them past MX_LF_MAXS
-        std::vector<int> xs, ys;
-        for (long i = 0; i < h->npix; i++)
-            for (int c = 0; c < 4; c++)
-                if ((&wt[i].x)[c] != 0.f) {
-                    xs.push_back((&idx[i].x)[c] % (int)N);
-                    ys.push_back((&idx[i].x)[c] / (int)N);
-                }
-        for (std::vector<int> *v : {&xs, &ys}) {
-            std::sort(v->begin(), v->end());
-            v->erase(std::unique(v->begin(), v->end()), v->end());
+        // all rows / columns past MX_LF_MAXS
+        for (std::vector<int> *v : {&xs, &ys})
             if (v->size() > (size_t)MX_LF_MAXS || v->empty()) {
                 v->resize(N);
                 for (long k = 0; k < N; k++) (*v)[k] = (int)k;
             }
-        }
         const int nx = (int)xs.size(), ny = (int)ys.size();
-        std::vector<int4> sic(h->npix);
-        for (long i = 0; i < h->npix; i++)
+        std::vector<int4> sic(ntab * npix);
+        for (long i = 0; i < ntab * npix; i++)
             for (int c = 0; c < 4; c++) {
                 const int id = (&idx[i].x)[c];
                 const int cx = (int)(std::lower_bound(xs.begin(), xs.end(), id % (int)N) - xs.begin());
                 const int cy = (int)(std::lower_bound(ys.begin(), ys.end(), id / (int)N) - ys.begin());
                 (&sic[i].x)[c] = (&wt[i].x)[c] != 0.f ? cy * nx + cx : 0;
             }
-        HIPCHK(hipStreamSynchronize(h->stream)); // no queued step reads the tables replaced here
         for (int **pp : {&h->d_lf_rows[iwvl], &h->d_lf_cols[iwvl]})
             if (*pp) {
                 HIPCHK(hipFree(*pp));
                 *pp = nullptr;
             }
         if ((rc = dev_alloc(&h->d_lf_rows[iwvl], (size_t)ny)) || (rc = dev_alloc(&h->d_lf_cols[iwvl], (size_t)nx)) ||
-            (rc = dev_alloc(&h->d_lf_sic[iwvl], h->npix)))
+            (rc = dev_alloc(&h->d_lf_sic[iwvl], ntab * npix)))
             return rc;
         HIPCHK(hipMemcpy(h->d_lf_rows[iwvl], ys.data(), sizeof(int) * ny, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(h->d_lf_cols[iwvl], xs.data(), sizeof(int) * nx, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(h->d_lf_sic[iwvl], sic.data(), sizeof(int4) * h->npix, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(h->d_lf_sic[iwvl], sic.data(), sizeof(int4) * ntab * npix, hipMemcpyHostToDevice));
         h->lf_ny[iwvl] = ny;
         h->lf_nx[iwvl] = nx;
         // B3 and the image hold the largest sampled set of any wavelength
@@ -3443,6 +3585,21 @@ int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si
     }
     h->dtf_set[iwvl] = true;
     return MX_OK;
+}
+} // namespace
+
+int mx_wfsgrad_set_dtf(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si_nx, long si_ny,
+                       const long *si_pp, const long *si_pi, const double *si_px) {
+    if (!h || iwvl < 0 || iwvl >= h->nwvl) return set_err(MX_EINVAL, "mx_wfsgrad_set_dtf: bad handle or iwvl");
+    if (!si_pp) return set_err(MX_EINVAL, "mx_wfsgrad_set_dtf: no si");
+    return set_dtf_tables(h, iwvl, 1, nominal, si_nx, si_ny, si_pp[h->notf * h->notf], si_pp, si_pi, si_px);
+}
+
+int mx_wfsgrad_set_dtf_sa(mx_wfsgrad_t *h, int iwvl, const double *nominal, long si_nx, long si_ny,
+                          long si_nzmax, const long *si_pp, const long *si_pi, const double *si_px) {
+    if (!h || iwvl < 0 || iwvl >= h->nwvl) return set_err(MX_EINVAL, "mx_wfsgrad_set_dtf_sa: bad handle or iwvl");
+    if (si_nzmax < 0) return set_err(MX_EINVAL, "mx_wfsgrad_set_dtf_sa: si_nzmax %ld", si_nzmax);
+    return set_dtf_tables(h, iwvl, h->nsa, nominal, si_nx, si_ny, si_nzmax, si_pp, si_pi, si_px);
 }
 
 int mx_wfsgrad_update_etf(mx_wfsgrad_t *h, int iwvl, const double *etf, const double *etf2, double etf2wt) {
@@ -3678,6 +3835,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             hw[w].etf2 = h->d_etf2[w];
             hw[w].si_idx = h->d_si_idx[w];
             hw[w].si_w = h->d_si_w[w];
+            hw[w].si_sa = h->dtf_ntab[w] > 1 ? (int)h->npix : 0;
+            hw[w].nom_sa = h->dtf_ntab[w] > 1 ? (int)(N * N) : 0;
         }
         if (std::memcmp(hw, h->h_wv, sizeof(hw)) != 0) {
             HIPCHK(hipMemcpyAsync(h->d_wv, hw, sizeof(hw), hipMemcpyHostToDevice, h->stream));
@@ -3701,7 +3860,6 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     o.isim = isim;
     o.dt = dt;
     o.opd = h->d_opd;
-    o.nllt = llt ? h->nwfs : 0;
     o.llt_origx = h->cfg.llt_origx;
     o.llt_origy = h->cfg.llt_origy;
     o.llt_ox = h->cfg.llt_ox;
@@ -3717,6 +3875,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     // a side (an LLT grid is one tile of nxsa)
     const int opd_nx = h->opd_tile ? h->opd_tile : ((h->lf && h->nxsa % 32 == 0) || h->nxsa == 32 ? 32 : 16);
     o.tiles = (int)(h->nxsa / opd_nx);
+    o.llt_tiles = (int)((h->cfg.llt_nx + opd_nx - 1) / opd_nx);
+    o.nllt = llt ? h->nwfs * o.llt_tiles * o.llt_tiles : 0;
     KArgs &a = h->arg_step;
     a = KArgs{};
     a.nsa = (int)h->nsa;
@@ -3778,6 +3938,14 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         const char *e = getenv("MX_MFMA_PP");
         if (e && atoi(e) == 0) pp = false;
     }
+    // (MX_UPLINK_DIRECT=1 in the environment: the direct-autocorrelation
+    // uplink of the wide LLT grid also for an LLT grid of the subaperture's
+    // size, for comparison with the transform forms)
+    bool uplink_direct = h->cfg.llt_nx != h->nxsa;
+    {
+        const char *e = getenv("MX_UPLINK_DIRECT");
+        if (e && atoi(e) != 0 && !h->lf && h->nwvf == N && N <= 64) uplink_direct = true;
+    }
     if (variant == MX_VARIANT_MFMA_BF16 || f16 || fftform) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
@@ -3787,6 +3955,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             hw[w].gy = f16 ? h->d_gy16[w] : h->d_gy[w];
             hw[w].si_c = h->d_sic[w];
             hw[w].si_w = h->d_si_w[w];
+            hw[w].si_sa = h->dtf_ntab[w] > 1 ? (int)h->npix : 0;
+            hw[w].nom_sa = h->dtf_ntab[w] > 1 ? MX_HALF : 0;
             hw[w].inv_wvl = (float)(1. / h->wvl[w]);
             hw[w].fy = h->d_fy[w];
             hw[w].fxs = h->d_fxs[w];
@@ -3960,6 +4130,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             hw[w].cols = h->d_lf_cols[w];
             hw[w].si_c = h->d_lf_sic[w];
             hw[w].si_w = h->d_si_w[w];
+            hw[w].si_sa = h->dtf_ntab[w] > 1 ? (int)h->npix : 0;
+            hw[w].nom_sa = h->dtf_ntab[w] > 1 ? (int)(N * N) : 0;
             hw[w].etf = h->d_etf[w];
             hw[w].etf2 = h->d_etf2[w];
         }
@@ -4021,6 +4193,20 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             L[nl++] = {lf_kernel(up[q].q, h->lf_threads, up[q].len), dim3((unsigned)(up[q].ngroup * nch)),
                        dim3((unsigned)h->lf_threads), &u};
         }
+    } else if (llt && uplink_direct) {
+        LltAcorrArgs &u = h->arg_lltac;
+        u = LltAcorrArgs{};
+        u.nlx = (int)h->cfg.llt_nx;
+        u.side = o.llt_tiles * opd_nx;
+        u.nwvl = (int)h->nwvl;
+        u.N = (int)N;
+        for (int w = 0; w < h->nwvl; w++) u.inv_wvl[w] = (float)(1. / h->wvl[w]);
+        u.opd = h->d_llt_opd;
+        u.amp = h->d_llt_amp;
+        u.lotf_full = h->d_lotf;
+        u.lotf_half = h->d_lotf_t;
+        L[nl++] = {reinterpret_cast<const void *>(&k_llt_acorr), dim3((unsigned)(h->nwfs * h->nwvl * N)), dim3(256),
+                   &u};
     } else if (llt) {
         if (mf)
             L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<true, _Float16>)
@@ -4136,13 +4322,64 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         q.phase = a.dt.phase;
         L[nl++] = {reinterpret_cast<const void *>(&k_sigmatch2), dim3((unsigned)h->nwfs), dim3(256), &q};
     }
+    if (h->pistat) {
+        // the step's geometric gradients of every subaperture, then the
+        // tip/tilt-removed PSFs shifted by them
+        if (h->pistat_gtype == 2) {
+            if (h->gs_nnz < 0) return set_err(MX_ESTATE, "pistat with G-tilt needs GS0 (mx_wfsgrad_set_gs0)");
+            GtiltArgs &q = h->arg_pis_g;
+            q = GtiltArgs{};
+            q.nsa = (int)h->nsa;
+            q.nwfs = h->nwfs;
+            q.npts = h->nsa * h->nxsa * h->nxsa;
+            q.rowp = h->d_gs_rowp;
+            q.coli = h->d_gs_coli;
+            q.val = h->d_gs_val;
+            q.opd = h->d_opd;
+            q.grad = h->d_gpis;
+            L[nl++] = {reinterpret_cast<const void *>(&k_gtilt), dim3((unsigned)(2 * h->nsa)), dim3(64), &q};
+        } else {
+            ZtiltArgs &z = h->arg_pis_z;
+            z = ZtiltArgs{};
+            z.nsa = (int)h->nsa;
+            z.nxsa = (int)h->nxsa;
+            z.nwfs = h->nwfs;
+            z.dx = h->cfg.dx;
+            z.ptsx = h->d_ptsx;
+            z.ptsy = h->d_ptsy;
+            z.amp = h->d_amp;
+            z.imcc = h->d_imcc;
+            z.opd = h->d_opd;
+            z.saa = h->d_saa;
+            z.saat = (float)h->cfg.saat;
+            z.grad = h->d_gpis;
+            z.all = 1;
+            L[nl++] = {reinterpret_cast<const void *>(&k_ztilt), dim3((unsigned)(h->nsa * h->nwfs)), dim3(64), &z};
+        }
+        PistatArgs &s = h->arg_pis;
+        s = PistatArgs{};
+        s.nsa = (int)h->nsa;
+        s.nwfs = h->nwfs;
+        s.nwvl = (int)h->nwvl;
+        s.dx = (float)h->cfg.dx;
+        const double norm_psf = std::sqrt(h->cfg.areascale) / ((double)h->nxsa * (double)h->nwvf);
+        s.norm = (float)(norm_psf * norm_psf);
+        for (int w = 0; w < h->nwvl; w++) s.inv_wvl[w] = (float)(1. / h->wvl[w]);
+        s.opd = h->d_opd;
+        s.amp = h->d_amp;
+        s.gref = h->d_gpis;
+        s.acc = h->d_pistat;
+        L[nl++] = {reinterpret_cast<const void *>(&k_pistat), dim3((unsigned)(h->nsa * h->nwfs)), dim3(MX_FFT_T), &s};
+    }
     if (nl > MX_MAXLAUNCH) return set_err(MX_EINVAL, "internal: %d kernels in a step", nl);
     sp.nl = nl;
     sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
              (direct ? 8 : 0) | (variant << 4) | (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8) |
              (h->lf ? h->lf_threads << 14 : 0) | (h->cfg.sigmatch == 2 ? 1 << 25 : 0) |
              ((mf || fftform) && h->d_etf2_t[0] ? 1 << 24 : 0) | (fftform ? 1 << 6 : 0) |
-             (mf && !pp ? 1 << 14 : 0); // (the large-field bits 14-23 are zero with an MFMA form)
+             (mf && !pp ? 1 << 14 : 0) | // (lf_threads << 14 sets bits 21-23 only)
+             (h->pistat ? 1 << 26 : 0) | (h->pistat && h->pistat_gtype == 2 ? 1 << 15 : 0) |
+             (uplink_direct ? 1 << 16 : 0);
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;
@@ -4232,6 +4469,7 @@ void step_queued(mx_wfsgrad_t *h) {
     // the i0 and dither sums take whole exposures
     if (h->i0_enable && h->last_out) h->i0_nframe++;
     if (h->dith_on && h->last_out) h->dith_nframe++;
+    if (h->pistat) h->pistat_nstep++; // every step, as MAOS accumulates it
     h->queued = true;
 }
 } // namespace
@@ -4402,6 +4640,82 @@ int mx_wfsgrad_i0_get(mx_wfsgrad_t *h, int iwfs, double *out, long *nframe) {
     const long n = h->nsa * h->npix;
     if (out) HIPCHK(hipMemcpy(out, h->d_i0acc + iwfs * n, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (nframe) *nframe = h->i0_nframe;
+    return MX_OK;
+}
+
+int mx_wfsgrad_pistat_control(mx_wfsgrad_t *h, int enable, int gtype, int reset) {
+    if (!h) return set_err(MX_EINVAL, "null handle");
+    if (enable && gtype != 1 && gtype != 2)
+        return set_err(MX_EINVAL, "mx_wfsgrad_pistat_control: gtype %d (1: Zernike tilt, 2: G-tilt)", gtype);
+    if (enable && !(h->notf == 64 && h->nxsa == 32 && h->nwvf == 64))
+        return set_err(MX_EINVAL, "pistat needs the 64-point subaperture geometry (nxsa 32, embfac 2, notf 64); "
+                       "got nxsa %ld, notf %ld", h->nxsa, h->notf);
+    if (enable && h->geom) return set_err(MX_ESTATE, "pistat needs the physical-optics step (geometric mode is on)");
+    HIPCHK(hipSetDevice(h->device));
+    const size_t n = (size_t)h->nwfs * h->nwvl * h->nsa * 4096;
+    if (enable && !h->d_pistat) {
+        int rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if ((rc = dev_alloc(&h->d_pistat, n)) || (rc = dev_alloc(&h->d_gpis, (size_t)2 * h->nwfs * h->nsa)))
+            return rc;
+        HIPCHK(hipMemset(h->d_pistat, 0, sizeof(float) * n));
+        h->pistat_nstep = 0;
+    } else if (reset && h->d_pistat) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemset(h->d_pistat, 0, sizeof(float) * n));
+        h->pistat_nstep = 0;
+    }
+    if (enable) h->pistat_gtype = gtype;
+    h->pistat = enable != 0;
+    return MX_OK;
+}
+
+int mx_wfsgrad_pistat_get(mx_wfsgrad_t *h, int iwfs, double *out, long *nstep) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs) return set_err(MX_EINVAL, "mx_wfsgrad_pistat_get: bad handle or iwfs");
+    if (!h->d_pistat) return set_err(MX_ESTATE, "mx_wfsgrad_pistat_get: pistat was never enabled");
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const size_t n = (size_t)h->nwvl * h->nsa * 4096;
+    if (out) {
+        std::vector<float> t(n);
+        HIPCHK(hipMemcpy(t.data(), h->d_pistat + iwfs * n, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) out[i] = t[i];
+    }
+    if (nstep) *nstep = h->pistat_nstep;
+    return MX_OK;
+}
+
+int mx_wfsgrad_psfout(mx_wfsgrad_t *h, int iwfs, double *out) {
+    if (!h || iwfs < 0 || iwfs >= h->nwfs || !out) return set_err(MX_EINVAL, "mx_wfsgrad_psfout: bad argument");
+    if (h->nwvf != h->notf)
+        return set_err(MX_EINVAL, "mx_wfsgrad_psfout needs nxsa*embfac == notf (got %ld and %ld)", h->nwvf, h->notf);
+    if (h->geom) return set_err(MX_ESTATE, "mx_wfsgrad_psfout: geometric mode keeps no physical-optics field");
+    HIPCHK(hipSetDevice(h->device));
+    const int nout = (int)(h->notf / 2 + 2);
+    const size_t n = (size_t)h->nwvl * h->nsa * nout * nout;
+    int rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if ((rc = dev_alloc(&h->d_psfout, n))) return rc;
+    PsfoutArgs a{};
+    a.nsa = (int)h->nsa;
+    a.nxsa = (int)h->nxsa;
+    a.nout = nout;
+    a.b = (int)((h->nwvf - h->nxsa) / 2 - (h->notf / 2 - nout / 2));
+    a.norm = (float)(std::sqrt(h->cfg.areascale) / ((double)h->nxsa * (double)h->nwvf));
+    for (int w = 0; w < h->nwvl; w++) a.inv_wvl[w] = (float)(1. / h->wvl[w]);
+    a.opd = h->d_opd + (long)iwfs * h->nsa * h->nxsa * h->nxsa;
+    a.amp = h->d_amp;
+    a.out = h->d_psfout;
+    void *argv[1] = {&a};
+    HIPCHK(hipLaunchKernel(reinterpret_cast<const void *>(&k_psfout), dim3((unsigned)h->nsa, (unsigned)h->nwvl),
+                           dim3(256), argv, 0, h->stream));
+    std::vector<float2> t(n);
+    HIPCHK(hipMemcpyAsync(t.data(), h->d_psfout, sizeof(float2) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; i++) {
+        out[2 * i] = t[i].x;
+        out[2 * i + 1] = t[i].y;
+    }
     return MX_OK;
 }
 
@@ -4622,6 +4936,9 @@ int mx_wfsgrad_set_variant(mx_wfsgrad_t *h, int variant) {
         return set_err(MX_EINVAL, "the MFMA form needs notf=64 and nxsa=32");
     if (variant == MX_VARIANT_MFMA_FP16 && h->amp_max > 1. + 1e-6)
         return set_err(MX_EINVAL, "the FP16 form's range plan needs amplitudes <= 1 (largest %g)", h->amp_max);
+    if (variant == MX_VARIANT_MFMA_FP16 && h->llt_sum2 > 1024.)
+        return set_err(MX_EINVAL, "the FP16 form's range plan needs lotf(0) = sum(llt_amp^2) <= 1024 (got %g)",
+                       h->llt_sum2);
     h->variant = variant;
     return MX_OK;
 }
@@ -4671,6 +4988,8 @@ int mx_wfsgrad_set_geom(mx_wfsgrad_t *h, int mode) {
     if (!h || mode < 0 || mode > 2) return set_err(MX_EINVAL, "mx_wfsgrad_set_geom: bad handle or mode %d", mode);
     if (mode && h->dtrat > 1)
         return set_err(MX_EINVAL, "mx_wfsgrad_set_geom: geometric gradients are per step (set dtrat 1 first)");
+    if (mode && h->pistat)
+        return set_err(MX_ESTATE, "mx_wfsgrad_set_geom: pistat needs the physical-optics step (turn it off first)");
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->geom = mode;
@@ -4776,7 +5095,7 @@ int mx_wfsgrad_autotune(mx_wfsgrad_t *h, long isim, double dt, int nrep, char *r
         h->variant = MX_VARIANT_MFMA_BF16;
         if (effective_variant(h) == MX_VARIANT_MFMA_BF16) {
             variants.push_back(MX_VARIANT_MFMA_BF16);
-            if (h->amp_max <= 1. + 1e-6) variants.push_back(MX_VARIANT_MFMA_FP16);
+            if (h->amp_max <= 1. + 1e-6 && h->llt_sum2 <= 1024.) variants.push_back(MX_VARIANT_MFMA_FP16);
         }
         h->variant = saved;
         variants.push_back(MX_VARIANT_FFT32);

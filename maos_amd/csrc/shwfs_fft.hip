@@ -336,7 +336,7 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
             } else if (WV.nominal_t) {
                 float tn[8];
 #pragma unroll
-                for (int p = 0; p < 8; p++) tn[p] = WV.nominal_t[half_slot(k + 8 * p, uy)];
+                for (int p = 0; p < 8; p++) tn[p] = WV.nominal_t[isa * WV.nom_sa + half_slot(k + 8 * p, uy)];
 #pragma unroll
                 for (int p = 0; p < 8; p++) v[p] = make_float2(v[p].x * tn[p], v[p].y * tn[p]);
             }
@@ -382,12 +382,118 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
         }
         __syncthreads();
         if (tid < a.npix) {
-            const int4 id = WV.si_c[tid];
-            const float4 w = WV.si_w[tid];
+            const int4 id = WV.si_c[isa * WV.si_sa + tid];
+            const float4 w = WV.si_w[isa * WV.si_sa + tid];
             pv += W.alpha[iwvl] * (w.x * img[id.x] + w.y * img[id.y] + w.z * img[id.z] + w.w * img[id.w]);
         }
         __syncthreads();
     }
     if constexpr (UPLINK) return;
     if (wave == 0) shwfs_pixels(a, iwfs, isa, lane, pv);
+}
+
+// Tip/tilt-removed PSF statistics (pistatout of wfsints, maos/wfsints.c:
+// 262-275; GPU form cuda/sim/wfsints.cu:458-472) for the N = 64, nxsa = 32
+// geometry.  MAOS tilts the downlink OTF by the step's geometric gradient g
+// and transforms back.  The OTF of a 32-sample field vanishes at lag 32 of
+// 64, so that is the PSF sampled at k + g dtheta1 (dtheta1 = 64 dx / wvl),
+// which equals |FFT(amp exp(i 2 pi (opd - g.x) / wvl))|^2, the PSF of the
+// field with the tilt g removed.  One workgroup per (WFS, subaperture) forms
+// it per wavelength with passes A and B of k_shwfs_fft and adds
+// norm_psf^2 |.|^2, the peak moved to the centre (x fastest), to an FP32
+// accumulator.
+struct PistatArgs {
+    int nsa, nwfs, nwvl;
+    float dx;
+    float norm;             // norm_psf^2
+    float inv_wvl[MX_MAXWVL];
+    const float *opd, *amp; // [nwfs][nsa][32*32], [nsa][32*32]
+    const float *gref;      // [nwfs][2*nsa] geometric gradients (radians)
+    float *acc;             // [nwfs][nwvl][nsa][64*64]
+};
+__global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) void k_pistat(PistatArgs a) {
+    __shared__ float2 G[64 * MX_FFT_GS];
+    __shared__ float red[4];
+    float *const S = reinterpret_cast<float *>(G); // |.|^2 at Y 65 + X after pass B
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int k = lane & 7, li = lane >> 3;
+    const int unit = xcd_unit(blockIdx.x, a.nsa * a.nwfs);
+    const int iwfs = unit % a.nwfs, isa = unit / a.nwfs;
+    float2 tw[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) tw[q] = make_float2(kTwCos[4 * k * q], kTwSin[4 * k * q]);
+    constexpr int NX = 32, B0 = 16;
+    const int ix = tid % 32, iy0 = 4 * (tid / 32);
+    const float *og = a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
+    const float *ag = a.amp + (long)isa * NX * NX;
+    const float gx = a.gref[(long)iwfs * 2 * a.nsa + isa], gy = a.gref[(long)iwfs * 2 * a.nsa + a.nsa + isa];
+    // the OPD less the tilt (coordinates from the grid centre) and its mean
+    float o[4];
+    float mean;
+    {
+        const float cx = ((float)ix - 15.5f) * a.dx;
+        float s = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const float cy = ((float)(iy0 + j) - 15.5f) * a.dx;
+            o[j] = og[(iy0 + j) * NX + ix] - fmaf(gx, cx, gy * cy);
+            s += o[j];
+        }
+        seg_sum<64>(s);
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
+    }
+    float am[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) am[j] = ag[(iy0 + j) * NX + ix];
+    for (int iwvl = 0; iwvl < a.nwvl; iwvl++) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            float re, im;
+            cis_rev((o[j] - mean) * a.inv_wvl[iwvl], am[j], re, im);
+            G[gidx(B0 + iy0 + j, B0 + ix)] = make_float2(re, im);
+        }
+        __syncthreads();
+        float2 v[8];
+        // A: band rows y, FFT along x, written transposed: element (kx, y)
+        {
+            const int y = B0 + 8 * wave + li;
+            float2 *row = G + y * MX_FFT_GS;
+#pragma unroll
+            for (int j = 2; j < 6; j++) v[j] = row[k + 9 * j];
+            wave_sync();
+            row64_mid(v, row, k, tw);
+            __syncthreads();
+            float2 *col = G + k * MX_FFT_GS + y + (y >> 3);
+#pragma unroll
+            for (int p = 0; p < 8; p++) col[8 * MX_FFT_GS * p] = v[p];
+        }
+        __syncthreads();
+        // B: rows kx, FFT along y (band input), |.|^2 of Y[kx][ky = k + 8p]
+        float P[2][8];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int kx = 16 * wave + 8 * t + li;
+            float2 *row = G + kx * MX_FFT_GS;
+#pragma unroll
+            for (int j = 2; j < 6; j++) v[j] = row[k + 9 * j];
+            wave_sync();
+            row64_mid(v, row, k, tw);
+#pragma unroll
+            for (int p = 0; p < 8; p++) P[t][p] = v[p].x * v[p].x + v[p].y * v[p].y;
+        }
+        __syncthreads(); // every row read before S overwrites the grid
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int X = (16 * wave + 8 * t + li + 32) & 63;
+#pragma unroll
+            for (int p = 0; p < 8; p++) S[((k + 8 * p + 32) & 63) * 65 + X] = P[t][p];
+        }
+        __syncthreads();
+        float *acc = a.acc + (((long)iwfs * a.nwvl + iwvl) * a.nsa + isa) * 4096;
+#pragma unroll 4
+        for (int e = tid; e < 4096; e += MX_FFT_T) acc[e] += a.norm * S[(e >> 6) * 65 + (e & 63)];
+        __syncthreads(); // S read before the next wavelength's field
+    }
 }

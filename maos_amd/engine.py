@@ -131,6 +131,39 @@ class WfsGrad:
         check(_lib.hip().mx_wfsgrad_i0_get(self.h, iwfs, dptr(out), C.cast(C.byref(n), _lib.c_long_p)))
         return out.reshape(self.nsa, self.npix), n.value
 
+    def pistat_control(self, enable=True, gtype=1, reset=False):
+        """Tip/tilt-removed PSF statistics (powfs.pistatout) from the next
+        step on; gtype 1 shifts by the Zernike tilt, 2 by the G-tilt."""
+        check(_lib.hip().mx_wfsgrad_pistat_control(self.h, int(enable), int(gtype), int(reset)))
+
+    def pistat_get(self, iwfs):
+        """(sums [nwvl, nsa, notf, notf] with the peak at the centre, steps)."""
+        p = self.powfs
+        out = np.zeros(p.nwvl * self.nsa * p.notf * p.notf)
+        n = C.c_long(0)
+        check(_lib.hip().mx_wfsgrad_pistat_get(self.h, iwfs, dptr(out), C.cast(C.byref(n), _lib.c_long_p)))
+        return out.reshape(p.nwvl, self.nsa, p.notf, p.notf), n.value
+
+    def set_dtf_sa(self, iwvl, nominal, si_pp, si_pi, si_px):
+        """Per-subaperture detector tables of one wavelength: nominal
+        [nsa, notf^2] (or None), si CSC [nsa, notf^2 + 1] / [nsa, nzmax]."""
+        p = self.powfs
+        pp = np.ascontiguousarray(si_pp, dtype=np.int64)
+        pi = np.ascontiguousarray(si_pi, dtype=np.int64)
+        px = np.ascontiguousarray(si_px, dtype=np.float64)
+        nom = None if nominal is None else np.ascontiguousarray(nominal, dtype=np.float64)
+        check(_lib.hip().mx_wfsgrad_set_dtf_sa(self.h, iwvl, dptr(nom), p.npix, p.notf * p.notf, pi.shape[-1],
+                                               lptr(pp), lptr(pi), dptr(px)))
+
+    def psfout(self, iwfs):
+        """The complex field output (powfs.psfout) of WFS iwfs at the last
+        queued step: [nwvl, nsa, nout, nout] complex, nout = notf/2 + 2."""
+        p = self.powfs
+        nout = p.notf // 2 + 2
+        out = np.zeros(2 * p.nwvl * self.nsa * nout * nout)
+        check(_lib.hip().mx_wfsgrad_psfout(self.h, iwfs, dptr(out)))
+        return out.view(complex).reshape(p.nwvl, self.nsa, nout, nout)
+
     # TT-dither statistics and the on-GPU matched-filter update
     def dither_control(self, enable=True, reset=False):
         check(_lib.hip().mx_wfsgrad_dither_control(self.h, int(enable), int(reset)))

@@ -264,6 +264,33 @@ class Powfs:
         return i0.reshape(nsa, npix), gx.reshape(nsa, npix), gy.reshape(nsa, npix)
 
     # -------------------------------------------------------------- G-tilt
+    def dtf_sa(self, pixoffx, pixoffy, pixrot=None):
+        """Per-subaperture detector tables (mkdtf's loop over subapertures,
+        lib/mkdtf.c:140-161): pixel offsets pixoffx, pixoffy (pixels) and
+        pixel rotations pixrot (radians) per subaperture.  Returns nominal
+        [nwvl, nsa, N^2] and si CSC pp [nwvl, nsa, N^2+1], pi and px
+        [nwvl, nsa, 4 npix]."""
+        S = _lib.setup()
+        N, nsa = self.notf, self.nsa
+        rot = np.zeros(nsa) if pixrot is None else np.asarray(pixrot, dtype=float)
+        nz = 4 * self.npix
+        nominal = np.zeros((self.nwvl, nsa, N * N))
+        pp = np.zeros((self.nwvl, nsa, N * N + 1), dtype=np.int64)
+        pi = np.zeros((self.nwvl, nsa, nz), dtype=np.int64)
+        px = np.zeros((self.nwvl, nsa, nz))
+        for iw in range(self.nwvl):
+            dth = self.dtheta[iw]
+            for isa in range(nsa):
+                nom = np.zeros(N * N)
+                S.mxs_dtf_nominal(dptr(nom), N, N, self.pixthetax / dth, self.pixthetay / dth, float(rot[isa]),
+                                  self.cfg.pixblur)
+                nominal[iw, isa] = nom
+                a, b, c = (np.zeros(N * N + 1, dtype=np.int64), np.zeros(nz, dtype=np.int64), np.zeros(nz))
+                S.mxs_si(N, N, dth, self.pixpsax, self.pixpsay, self.pixthetax, self.pixthetay,
+                         float(pixoffx[isa]), float(pixoffy[isa]), float(rot[isa]), lptr(a), lptr(b), dptr(c))
+                pp[iw, isa], pi[iw, isa], px[iw, isa] = a, b, c
+        return nominal, pp, pi, px
+
     def gs0(self):
         """Average-gradient operator GS0 (mkg, lib/mkg.c:108-468) as CSR
         (rowp, coli, val): 2*nsa rows (x then y) over the OPD points."""

@@ -27,7 +27,7 @@ class OrcPowfs(C.Structure):
                 ("phytype", C.c_int), ("sigmatch", C.c_int), ("cogthres", C.c_double),
                 ("cogoff", C.c_double), ("pixthetax", C.c_double), ("pixthetay", C.c_double),
                 ("siglev_dtrat", C.c_double), ("saasum", C.c_double), ("mtche", dp),
-                ("i0sum", dp), ("i0sumsum", C.c_double)]
+                ("i0sum", dp), ("i0sumsum", C.c_double), ("si_multi", C.c_long)]
 
 
 class OrcWfs(C.Structure):
@@ -97,6 +97,8 @@ def lib():
                                        C.c_int, dp, dp]
         L.orc_ztilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double]
         L.orc_gtilt.argtypes = [dp, dp, dp, dp, dp, C.c_long, C.c_long, C.c_double, dp, C.c_double]
+        L.orc_pistat.argtypes = [dp, dp, C.POINTER(OrcPowfs), dp, dp, C.c_long, C.c_long]
+        L.orc_psfout.argtypes = [dp, dp, C.POINTER(OrcPowfs), C.c_long, C.c_long]
         L.orc_locfft_psf.argtypes = [dp, dp, dp, C.c_long, C.POINTER(C.c_long), C.c_long, C.c_long,
                                      C.c_double]
         L.orc_loc_calc_ptt.argtypes = [dp, dp, dp, dp, dp, dp, C.c_long]
@@ -237,6 +239,21 @@ class Oracle:
         s.saasum = p.saasum
         self.set_ref(mtche, i0sum)
 
+    def set_dtf_sa(self, nominal, si_pp, si_pi, si_px):
+        """Per-subaperture detector tables (multi_nominal): nominal
+        [nwvl, nsa, notf^2] or None, si CSC [nwvl, nsa, notf^2 + 1] and
+        [nwvl, nsa, nzmax]."""
+        s = self.s
+        if nominal is not None:
+            self._nom = np.ascontiguousarray(nominal, dtype=np.float64)
+            s.nominal = _d(self._nom)
+        self._pp = np.ascontiguousarray(si_pp, dtype=np.int64)
+        self._pi = np.ascontiguousarray(si_pi, dtype=np.int64)
+        self._px = np.ascontiguousarray(si_px, dtype=np.float64)
+        s.si_pp, s.si_pi, s.si_px = _l(self._pp), _l(self._pi), _d(self._px)
+        s.si_nzmax = self._pi.shape[-1]
+        s.si_multi = 1
+
     def set_ref(self, mtche=None, i0sum=None):
         if mtche is not None:
             self._mtche = np.ascontiguousarray(mtche, dtype=np.float64)
@@ -302,6 +319,27 @@ class Oracle:
                         _d(np.ascontiguousarray(p.ptsy)), p.nsa, p.nxsa, p.dx, _d(np.ascontiguousarray(p.saa)),
                         p.cfg.saat)
         return g
+
+    def pistat(self, opd, grad, acc=None):
+        """pistatout of one step: adds the tip/tilt-removed PSFs of an OPD
+        [nsa*nxsa^2], shifted by the geometric gradients grad (2*nsa, radians),
+        to acc [nwvl, nsa, notf, notf] (peak at the centre, x fastest)."""
+        p = self.p
+        if acc is None:
+            acc = np.zeros((p.nwvl, p.nsa, p.notf, p.notf))
+        o = np.ascontiguousarray(opd, dtype=np.float64).reshape(-1)
+        g = np.ascontiguousarray(grad, dtype=np.float64).reshape(-1)
+        lib().orc_pistat(_d(acc), _d(o), C.byref(self.s), _d(g), _d(g[p.nsa:]), 0, p.nsa)
+        return acc
+
+    def psfout(self, opd):
+        """psfout of one step: [nwvl, nsa, nout, nout] complex, nout = notf/2 + 2."""
+        p = self.p
+        nout = p.notf // 2 + 2
+        out = np.zeros((p.nwvl, p.nsa, nout, nout), complex)
+        o = np.ascontiguousarray(opd, dtype=np.float64).reshape(-1)
+        lib().orc_psfout(out.ctypes.data_as(dp), _d(o), C.byref(self.s), 0, p.nsa)
+        return out
 
     def step(self, wfs, isim, dt, atmscale=1.0, nthread=0, want_opd=False, want_lotf=False):
         """One wfsgrad_iwfs step for WfsSpec wfs; returns (ints, grad, opd, lotf)."""

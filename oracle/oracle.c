@@ -507,15 +507,21 @@ void orc_wfsints(double *ints, const double *opd, const double *lotfc_, const or
     for (long iwvl = 0; iwvl < p->nwvl; iwvl++) {
         const double wvl = p->wvl[iwvl];
         const cplx *lotfc = lotfc_ ? (const cplx *)lotfc_ + iwvl * notf2 : NULL;
-        const double *nominal = (!hasllt && p->nominal) ? p->nominal + iwvl * notf2 : NULL;
-        const long *pp = p->si_pp + iwvl * (notf2 + 1);
-        const long *pi = p->si_pi + iwvl * p->si_nzmax;
-        const double *px = p->si_px + iwvl * p->si_nzmax;
+        const long ntab = p->si_multi ? p->nsa : 1;
+        const double *nominal0 = (!hasllt && p->nominal) ? p->nominal + iwvl * ntab * notf2 : NULL;
+        const long *pp0 = p->si_pp + iwvl * ntab * (notf2 + 1);
+        const long *pi0 = p->si_pi + iwvl * ntab * p->si_nzmax;
+        const double *px0 = p->si_px + iwvl * ntab * p->si_nzmax;
         const cplx *etf1 = p->etf ? (const cplx *)p->etf + iwvl * p->nsa * notf2 : NULL;
         const cplx *etf2 = p->etf2 ? (const cplx *)p->etf2 + iwvl * p->nsa * notf2 : NULL;
         double etf2wt = etf2 ? p->etf2wt : 0, etf1wt = etf2 ? 1. - p->etf2wt : 1;
         const double alpha = w->wvlwts[iwvl] * norm_ints;
         for (long isa = isa_start; isa < isa_end; isa++) {
+            /* the subaperture's own tables when multi_nominal */
+            const long it = p->si_multi ? isa : 0;
+            const double *nominal = nominal0 ? nominal0 + it * notf2 : NULL;
+            const long *pp = pp0 + it * (notf2 + 1), *pi = pi0 + it * p->si_nzmax;
+            const double *px = px0 + it * p->si_nzmax;
             orc_embed_wvf((double *)wvf, nwvf, opd + isa * saoffset, p->amp + isa * saoffset, nopd, wvl);
             orc_fft2((double *)wvf, nwvf, nwvf, -1);
             if (psf != wvf) ccpcorner(psf, notf, notf, wvf, nwvf, nwvf);
@@ -543,6 +549,85 @@ void orc_wfsints(double *ints, const double *opd, const double *lotfc_, const or
             for (long icol = 0; icol < notf2; icol++)
                 for (long k = pp[icol]; k < pp[icol + 1]; k++)
                     out[pi[k]] += alpha * px[k] * creal(psf[icol]);
+        }
+    }
+    if (psf != wvf) free(psf);
+    free(wvf);
+}
+
+/* pistatout of wfsints (maos/wfsints.c:262-275): the downlink OTF (no
+ * uplink, DTF or ETF), ctilt by -g dtheta1 pixels with the peak in the corner
+ * (math/matcomp.c:287-336), the inverse transform (+1), cfftshift, |.|,
+ * and norm_pistat = norm_psf^2 / notf^2 times that added to acc. */
+void orc_pistat(double *acc, const double *opd, const orc_powfs_t *p, const double *gx, const double *gy,
+                long isa_start, long isa_end) {
+    const long nopd = p->nxsa, nwvf = nopd * p->embfac, notf = p->notf, notf2 = notf * notf;
+    const long saoffset = nopd * nopd;
+    const double norm_psf = sqrt(p->areascale) / (double)(nopd * nwvf);
+    const double norm_pistat = norm_psf * norm_psf / ((double)notf * notf);
+    cplx *wvf = (cplx *)malloc(sizeof(cplx) * nwvf * nwvf);
+    cplx *psf = nwvf != notf ? (cplx *)malloc(sizeof(cplx) * notf2) : wvf;
+    cplx *ux = (cplx *)malloc(sizeof(cplx) * notf * 2), *uy = ux + notf;
+    for (long iwvl = 0; iwvl < p->nwvl; iwvl++) {
+        const double wvl = p->wvl[iwvl];
+        const double dtheta1 = (double)nwvf * p->dx / wvl;
+        for (long isa = isa_start; isa < isa_end; isa++) {
+            orc_embed_wvf((double *)wvf, nwvf, opd + isa * saoffset, p->amp + isa * saoffset, nopd, wvl);
+            orc_fft2((double *)wvf, nwvf, nwvf, -1);
+            if (psf != wvf) ccpcorner(psf, notf, notf, wvf, nwvf, nwvf);
+            for (long i = 0; i < notf2; i++) {
+                double a = creal(psf[i]), b = cimag(psf[i]);
+                psf[i] = a * a + b * b;
+            }
+            orc_fft2((double *)psf, notf, notf, -1);
+            /* ctilt(psf, sx, sy, 0): exp(-2 pi i u s / n) with u the signed
+             * frequency of each index (u >= n/2 taken as u - n) */
+            const double sx = -gx[isa] * dtheta1, sy = -gy[isa] * dtheta1;
+            for (long i = 0; i < notf; i++) {
+                const double u = (double)(i < notf / 2 ? i : i - notf);
+                ux[i] = cexp(-2. * I * M_PI * u * sx / (double)notf);
+                uy[i] = cexp(-2. * I * M_PI * u * sy / (double)notf);
+            }
+            for (long iy = 0; iy < notf; iy++)
+                for (long ix = 0; ix < notf; ix++) psf[ix + iy * notf] *= ux[ix] * uy[iy];
+            orc_fft2((double *)psf, notf, notf, 1);
+            /* cfftshift, cabstoreal and creal2d: peak moved to the centre */
+            double *out = acc + (iwvl * p->nsa + isa) * notf2;
+            for (long iy = 0; iy < notf; iy++)
+                for (long ix = 0; ix < notf; ix++)
+                    out[(ix + notf / 2) % notf + ((iy + notf / 2) % notf) * notf] +=
+                        norm_pistat * cabs(psf[ix + iy * notf]);
+        }
+    }
+    free(ux);
+    if (psf != wvf) free(psf);
+    free(wvf);
+}
+
+/* psfout of wfsints (maos/wfsints.c:243-253): the complex PSF (the field's
+ * transform, cropped or padded to notf by ccpcorner) scaled by
+ * norm_psf / nwvf^2, transformed back (+1), and its centre (notf/2 + 2)^2
+ * taken as cembed crops (math/matmath.c:1428-1458: out index i holds in
+ * index i + notf/2 - nout/2). */
+void orc_psfout(double *out_, const double *opd, const orc_powfs_t *p, long isa_start, long isa_end) {
+    const long nopd = p->nxsa, nwvf = nopd * p->embfac, notf = p->notf, notf2 = notf * notf;
+    const long nout = notf / 2 + 2, off = notf / 2 - nout / 2;
+    const long saoffset = nopd * nopd;
+    const double norm_psf = sqrt(p->areascale) / (double)(nopd * nwvf);
+    cplx *out = (cplx *)out_;
+    cplx *wvf = (cplx *)malloc(sizeof(cplx) * nwvf * nwvf);
+    cplx *psf = nwvf != notf ? (cplx *)malloc(sizeof(cplx) * notf2) : wvf;
+    for (long iwvl = 0; iwvl < p->nwvl; iwvl++) {
+        const double wvl = p->wvl[iwvl];
+        for (long isa = isa_start; isa < isa_end; isa++) {
+            orc_embed_wvf((double *)wvf, nwvf, opd + isa * saoffset, p->amp + isa * saoffset, nopd, wvl);
+            orc_fft2((double *)wvf, nwvf, nwvf, -1);
+            if (psf != wvf) ccpcorner(psf, notf, notf, wvf, nwvf, nwvf);
+            for (long i = 0; i < notf2; i++) psf[i] *= norm_psf / ((double)nwvf * nwvf);
+            orc_fft2((double *)psf, notf, notf, 1);
+            cplx *o = out + (iwvl * p->nsa + isa) * nout * nout;
+            for (long iy = 0; iy < nout; iy++)
+                for (long ix = 0; ix < nout; ix++) o[ix + iy * nout] = psf[(ix + off) + (iy + off) * notf];
         }
     }
     if (psf != wvf) free(psf);

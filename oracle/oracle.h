@@ -70,6 +70,10 @@ typedef struct orc_powfs_t {
     const double *mtche;       /* [nsa][2*npix] for this WFS */
     const double *i0sum;       /* [nsa] */
     double i0sumsum;
+    /* multi_nominal (maos/wfsints.c:200-233): nonzero when nominal and si
+     * hold one table per subaperture: nominal [nwvl][nsa][notf^2], si_pp
+     * [nwvl][nsa][notf^2+1], si_pi / si_px [nwvl][nsa][si_nzmax] */
+    long si_multi;
 } orc_powfs_t;
 
 typedef struct orc_wfs_t {
@@ -98,6 +102,22 @@ void orc_lotf(double *lotfc, const double *lltopd, const double *lltamp, long nl
 void orc_wfsints(double *ints, const double *opd, const double *lotfc,
                  const orc_powfs_t *p, const orc_wfs_t *w, long isa_start,
                  long isa_end);
+
+/* pistatout of wfsints (maos/wfsints.c:262-275) over [isa_start,
+ * isa_end): the downlink OTF of each subaperture (no uplink, DTF or ETF)
+ * tilted by ctilt (math/matcomp.c:287-336) by -g dtheta1 pixels, g =
+ * (gx[isa], gy[isa]) the step's geometric gradient in radians and dtheta1 =
+ * nwvf dx / wvl, back to a PSF, fftshifted (peak at the centre), |.| times
+ * norm_pistat = norm_psf^2 / notf^2 added to acc [nwvl][nsa][notf*notf]
+ * (x fastest). */
+void orc_pistat(double *acc, const double *opd, const orc_powfs_t *p, const double *gx, const double *gy,
+                long isa_start, long isa_end);
+
+/* psfout of wfsints (maos/wfsints.c:243-253) over [isa_start, isa_end):
+ * the subaperture's complex PSF scaled by norm_psf / nwvf^2, transformed
+ * back, centre (notf/2 + 2)^2 as cembed crops it; out [nwvl][nsa][nout^2]
+ * complex (x fastest). */
+void orc_psfout(double *out, const double *opd, const orc_powfs_t *p, long isa_start, long isa_end);
 
 /* dcog (math/matmath.c:717-761) on an nx*ny image */
 void orc_dcog(double *grad, const double *im, long nx, long ny, double offsetx,

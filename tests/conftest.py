@@ -106,6 +106,18 @@ def ngs_tt_small():
 
 
 @pytest.fixture(scope="session")
+def lgs_wide_llt():
+    """lgs_small with a 0.6 m launch telescope: the LLT grid (38^2 points at
+    1/64 m) is wider than the 0.5 m subaperture's 32^2, so MAOS crops the
+    uplink OTF from 76^2 to 64^2 (maos/wfsints.c:160-185)."""
+    from maos_amd import configs
+    from maos_amd.powfs import Powfs, genatm
+    cfg = configs.lgs30m(nx=1024, order=20, llt_d=0.6)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    return cfg, p, genatm(cfg["atm"])
+
+
+@pytest.fixture(scope="session")
 def lgs_pad128():
     """lgs_small with embfac 4: 32^2 points padded to a 128^2 field and OTF
     (the 2x Nyquist variant), LLT grid 32^2 -> nlwvf 128, ETF at 128^2."""

@@ -949,3 +949,137 @@ def test_dtrat_exposure(case, variant, request, monkeypatch):
     ri, rg, _, _ = o.step(wfs[iw], 6, dt)
     assert rel_l2(ints, ri) <= INTS_TOL and np.abs(g - rg).max() * RAD2MAS <= GRAD_TOL_MAS
     eng.close()
+
+
+@pytest.mark.parametrize("gtype", [1, 2])
+def test_pistat_matches_oracle(lgs_small, gtype):
+    """Tip/tilt-removed PSF statistics (pistatout) over three steps against
+    the oracle's restatement of MAOS's OTF-tilt path, fed the GPU's own
+    traced OPD and the oracle's geometric gradients of it (Zernike tilt or
+    G-tilt); the imaging step itself is unchanged by the extra kernels."""
+    cfg, p, layers = lgs_small
+    print("case: pistat gtype=%d nsa=%d" % (gtype, p.nsa), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    o = _oracle(p, layers)
+    eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    if gtype == 2:
+        eng.set_gs0(*p.gs0())
+    dt = cfg["sim"].dt
+    eng.queue(4, dt)
+    g_off, i_off = eng.sync(0)
+    eng.pistat_control(True, gtype=gtype, reset=True)
+    acc = {iw: None for iw in (0, 3)}
+    for isim in (4, 5, 6):
+        eng.queue(isim, dt)
+        for iw in acc:
+            opd = eng.debug_opd(iw)
+            g = o.ztilt(opd) if gtype == 1 else o.gtilt(opd)
+            acc[iw] = o.pistat(opd, g, acc[iw])
+        if isim == 4:
+            g_on, i_on = eng.sync(0)
+            assert np.array_equal(g_on, g_off) and np.array_equal(i_on, i_off)
+    for iw, ref in acc.items():
+        got, n = eng.pistat_get(iw)
+        assert n == 3
+        e = rel_l2(got, ref)
+        c = np.unravel_index(np.argmax(got[0, np.argmax(p.saa)]), got.shape[2:])
+        print("wfs=%d pistat rel L2 %.2e, peak of a full subaperture at %s" % (iw, e, c))
+        assert e < INTS_TOL
+    eng.pistat_control(False)
+    eng.close()
+
+
+@pytest.mark.parametrize("case", ["lgs_small", "ngs_small", "lgs_pad128"])
+def test_psfout_matches_oracle(case, request):
+    """The complex field output (psfout) of one step against the oracle's
+    restatement of MAOS's transform pair, on the GPU's own traced OPD
+    (FP32 phases: 1e-5 of the largest value)."""
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: psfout %s nsa=%d notf=%d" % (case, p.nsa, p.notf), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    o = _oracle(p, layers)
+    eng.queue(7, cfg["sim"].dt)
+    for iw in sorted({0, len(wfs) - 1}):
+        got = eng.psfout(iw)
+        ref = o.psfout(eng.debug_opd(iw))
+        e = np.abs(got - ref).max() / np.abs(ref).max()
+        print("wfs=%d psfout max error %.2e of the largest value" % (iw, e))
+        assert got.shape == ref.shape and e < 1e-5
+    eng.close()
+
+
+@pytest.mark.parametrize("case,offsets,variant", [
+    ("ngs_small", "random", "fft32"),          # N = 32: the register form
+    ("ngs_two_wvl", "random", "fft32"),        # more than 16 sampled columns: the register form
+    ("ngs_two_wvl", "samples", "mfma_bf16"),   # whole PSF samples: the union fits the MFMA form
+    ("ngs_two_wvl", "samples", "fft32"),       # ... and the FP32 radix-8 form
+    ("lgs_small", "random", "fft32"),          # with an LLT: si per subaperture, nominal in the ETF
+])
+def test_per_subaperture_tables(case, offsets, variant, request):
+    """multi_nominal: an si table (and a nominal without an LLT) per
+    subaperture, from per-subaperture pixel offsets and rotations
+    (mkdtf's loop, lib/mkdtf.c:140-161), against the oracle fed the same
+    tables."""
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: per-subaperture tables %s %s %s" % (case, offsets, variant), flush=True)
+    rs = np.random.RandomState(17)
+    if offsets == "random":
+        ox, oy = rs.uniform(-0.4, 0.4, p.nsa), rs.uniform(-0.4, 0.4, p.nsa)
+        rot = rs.uniform(-0.1, 0.1, p.nsa) if p.llt_nx == 0 else np.zeros(p.nsa)
+    else:
+        step = p.dtheta[0] / p.pixthetax  # one PSF sample, in pixels
+        ox, oy = step * rs.randint(-1, 2, p.nsa), step * rs.randint(-1, 2, p.nsa)
+        rot = np.zeros(p.nsa)
+    tabs = p.dtf_sa(ox, oy, rot)
+    eng, wfs = _engine(cfg, p, layers)
+    eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    for iw in range(p.nwvl):
+        eng.set_dtf_sa(iw, tabs[0][iw], tabs[1][iw], tabs[2][iw], tabs[3][iw])
+    eng.set_variant(VARIANTS[variant])
+    o = _oracle(p, layers, np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    o.set_dtf_sa(*tabs)
+    dt = cfg["sim"].dt
+    eng.queue(9, dt)
+    for iwfs in sorted({0, len(wfs) - 1}):
+        g, ints = eng.sync(iwfs)
+        ri, rg, _, _ = o.step(wfs[iwfs], 9, dt)
+        e_int = rel_l2(ints, ri)
+        e_g = np.abs(g - rg).max() * RAD2MAS
+        print("wfs=%d variant=%d ints=%.2e grad=%.3e mas" % (iwfs, eng.variant(), e_int, e_g))
+        assert e_int <= INTS_TOL
+        assert e_g <= GRAD_TOL_MAS
+    eng.close()
+
+
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16", "fft32_reg"])
+def test_wide_llt_parity(lgs_wide_llt, variant, monkeypatch):
+    """A 0.6 m launch telescope on 0.5 m subapertures: the 38^2 LLT grid is
+    traced in two-by-two tiles, and its uplink OTF (the 76^2 transform
+    cropped to 64^2, maos/wfsints.c:160-185) is summed directly as the
+    field's autocorrelation; every imaging form against the oracle."""
+    cfg, p, layers = lgs_wide_llt
+    if variant == "fft32_reg":
+        monkeypatch.setenv("MX_FFT_REG", "1")
+    print("case: wide LLT nlx=%d nxsa=%d nsa=%d, %s" % (p.llt_nx, p.nxsa, p.nsa, variant), flush=True)
+    eng, wfs = _engine(cfg, p, layers)
+    eng.set_variant(VARIANTS[variant])
+    o = _oracle(p, layers)
+    dt = cfg["sim"].dt
+    mtche, i0sum = _lgs_ref(p, o, wfs, dt)
+    o.set_ref(mtche, i0sum)
+    eng.update_ref(mtche, i0sum)
+    for isim in (0, 77):
+        eng.queue(isim, dt)
+        for iwfs in sorted({0, len(wfs) - 1}):
+            g, ints = eng.sync(iwfs)
+            ri, rg, _, rlotf = o.step(wfs[iwfs], isim, dt, want_lotf=True)
+            lotf = eng.debug_lotf(iwfs)
+            e_lotf = rel_l2(lotf, rlotf)
+            e_int = rel_l2(ints, ri)
+            e_g = np.abs(g - rg).max() * RAD2MAS
+            print("isim=%d wfs=%d lotf=%.2e ints=%.2e grad=%.3e mas" % (isim, iwfs, e_lotf, e_int, e_g))
+            assert e_lotf < 1e-5
+            assert e_int <= INTS_TOL
+            assert e_g <= GRAD_TOL_MAS
+    eng.close()

@@ -424,3 +424,143 @@ def test_prop_nowrap_matches_wrap_inside_and_skips_outside():
         inside = xi < n - 1
         assert miss == nxsa * np.sum(~inside) and np.all(a3[2][:, ~inside] == 0)
         assert np.allclose(a3[2][:, inside], b3[2][:, inside], rtol=0, atol=1e-12)
+
+
+def _field_psf(p, opd, gx, gy, iwvl):
+    """|FFT2|^2 of the embedded field amp exp(i 2 pi (opd - gx x - gy y)/wvl)
+    per subaperture, by numpy (the field-domain form of a tilt removal)."""
+    n, nw = p.nxsa, p.nxsa * p.cfg.embfac
+    wvl = p.wvl[iwvl]
+    c = (np.arange(n) - (n - 1) / 2) * p.dx
+    out = np.zeros((p.nsa, nw, nw))
+    amp = p.amp.reshape(p.nsa, n, n)
+    o = opd.reshape(p.nsa, n, n)
+    for isa in range(p.nsa):
+        ph = (o[isa] - gx[isa] * c[None, :] - gy[isa] * c[:, None]) / wvl
+        w = np.zeros((nw, nw), complex)
+        s = (nw - n) // 2
+        w[s:s + n, s:s + n] = amp[isa] * np.exp(2j * np.pi * ph)
+        out[isa] = np.abs(np.fft.fft2(w)) ** 2
+    return out
+
+
+def test_pistat_is_the_tilt_removed_psf(lgs_small):
+    """orc_pistat shifts the OTF by the gradient and transforms back; the PSF
+    is band-limited (the OTF of a 32-sample field vanishes at lag 32 of 64),
+    so the result is the PSF sampled at k + g dtheta1: |FFT|^2 of the field
+    with the tilt g removed, fftshifted and scaled by norm_psf^2.  A sign
+    error in the ramp, dtheta1 or the shift direction fails this."""
+    cfg, p, layers = lgs_small
+    o = oracle.Oracle(p)
+    rs = np.random.RandomState(11)
+    n = p.nxsa
+    opd = 2e-7 * rs.standard_normal(p.nsa * n * n)
+    g = np.concatenate([rs.uniform(-2e-6, 2e-6, p.nsa), rs.uniform(-2e-6, 2e-6, p.nsa)])
+    acc = o.pistat(opd, g)
+    norm_psf = math.sqrt(p.areascale) / (n * n * p.cfg.embfac)
+    for iwvl in range(p.nwvl):
+        ref = norm_psf ** 2 * np.fft.fftshift(_field_psf(p, opd, g[:p.nsa], g[p.nsa:], iwvl), axes=(1, 2))
+        assert rel_l2(acc[iwvl], ref) < 1e-11
+
+
+def test_pistat_of_a_pure_tilt_is_the_centred_flat_psf(lgs_small):
+    """A wavefront that is only a tilt, shifted by its own Zernike tilt,
+    gives the flat-wavefront PSF with its peak at the centre pixel
+    (notf/2, notf/2); its sum is norm_psf^2 notf^2 sum(amp^2) (Parseval)."""
+    cfg, p, layers = lgs_small
+    o = oracle.Oracle(p)
+    n = p.nxsa
+    gg = np.arange(n) * p.dx
+    x = np.broadcast_to(p.ptsx[:, None, None] + gg[None, None, :], (p.nsa, n, n))
+    y = np.broadcast_to(p.ptsy[:, None, None] + gg[None, :, None], (p.nsa, n, n))
+    opd = (2.3e-6 * x - 1.1e-6 * y).ravel()
+    g = o.ztilt(opd)
+    acc = o.pistat(opd, g)
+    flat = o.pistat(np.zeros_like(opd), np.zeros_like(g))
+    N = p.notf
+    norm_psf = math.sqrt(p.areascale) / (n * n * p.cfg.embfac)
+    amp2 = (p.amp.reshape(p.nsa, -1) ** 2).sum(1)
+    full = np.nonzero(p.saa > 0.99)[0]
+    assert rel_l2(acc, flat) < 1e-9
+    for isa in full[:5]:
+        im = acc[0, isa]
+        assert np.unravel_index(np.argmax(im), im.shape) == (N // 2, N // 2)
+        assert abs(im.sum() - norm_psf ** 2 * N * N * amp2[isa]) < 1e-9 * im.sum()
+
+
+@pytest.mark.parametrize("case", ["lgs_small", "ngs_small"])
+def test_psfout_is_the_scaled_pupil_field(case, request):
+    """orc_psfout transforms the field to the focal plane and back: with
+    nwvf == notf that returns norm_psf amp exp(i 2 pi opd / wvl) at the
+    embedded position, so the (notf/2 + 2)^2 centre holds the field with a
+    one-sample zero border (a wrong scale, sign or crop offset fails)."""
+    cfg, p, layers = request.getfixturevalue(case)
+    o = oracle.Oracle(p)
+    rs = np.random.RandomState(3)
+    n, N = p.nxsa, p.notf
+    assert n * p.cfg.embfac == N
+    opd = 3e-7 * rs.standard_normal(p.nsa * n * n)
+    out = o.psfout(opd)
+    nout = N // 2 + 2
+    b = (N - n) // 2 - (N // 2 - nout // 2)
+    norm_psf = math.sqrt(p.areascale) / (n * N)
+    for iwvl in range(p.nwvl):
+        f = p.amp.reshape(p.nsa, n, n) * np.exp(2j * np.pi * opd.reshape(p.nsa, n, n) / p.wvl[iwvl])
+        ref = np.zeros((p.nsa, nout, nout), complex)
+        ref[:, b:b + n, b:b + n] = norm_psf * f
+        assert np.abs(out[iwvl] - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_per_subaperture_tables(ngs_small):
+    """multi_nominal: per-subaperture si and nominal tables.  The shared
+    tables repeated per subaperture give the shared result exactly; a pixel
+    offset of exactly one pixel in x (pixoffx = 1) moves every image by one
+    pixel: pixel ix then reads what pixel ix - 1 read (interior columns)."""
+    cfg, p, layers = ngs_small
+    from maos_amd import configs
+    o1 = oracle.Oracle(p)
+    o1.atm(layers)
+    wfs = configs.wfs_specs(cfg, p)[0]
+    _, _, opd, _ = o1.step(wfs, 3, cfg["sim"].dt, want_opd=True)
+    ints, _ = o1.image(wfs, opd)
+    z = np.zeros(p.nsa)
+    o2 = oracle.Oracle(p)
+    o2.set_dtf_sa(*p.dtf_sa(z, z))
+    ints2, _ = o2.image(wfs, opd)
+    assert np.array_equal(ints2, ints)
+    o3 = oracle.Oracle(p)
+    o3.set_dtf_sa(*p.dtf_sa(np.ones(p.nsa), z))
+    ints3, _ = o3.image(wfs, opd)
+    a = ints.reshape(p.nsa, p.pixpsay, p.pixpsax)
+    b = ints3.reshape(p.nsa, p.pixpsay, p.pixpsax)
+    assert np.allclose(b[:, :, 1:], a[:, :, :-1], rtol=1e-10, atol=1e-12 * a.max())
+    assert not np.allclose(b, a)
+
+
+def test_lotf_of_a_wide_llt_is_the_cropped_autocorrelation(lgs_wide_llt):
+    """An LLT grid wider than the subaperture's (38 > 32 points): orc_lotf
+    transforms on the 76^2 grid and crops the OTF to 64^2 (ccpcorner).  The
+    OTF of |FFT(E)|^2 / n^2 is the autocorrelation sum_x E(x) conj(E(x + u)),
+    so the crop equals that sum at the lags -32 <= u < 32, computed directly
+    here (a wrong crop, lag sign or normalisation fails)."""
+    cfg, p, layers = lgs_wide_llt
+    nlx, N = p.llt_nx, p.notf
+    assert nlx > p.nxsa and 2 * nlx > N
+    rs = np.random.RandomState(8)
+    opd = 2e-7 * rs.standard_normal(nlx * nlx)
+    lib = oracle.lib()
+    out = np.zeros(2 * N * N)
+    lib.orc_lotf(out.ctypes.data_as(oracle.dp), opd.ctypes.data_as(oracle.dp),
+                 np.ascontiguousarray(p.llt_amp).ctypes.data_as(oracle.dp), nlx, p.cfg.embfac,
+                 p.nxsa * p.cfg.embfac, N, p.wvl[0])
+    lotf = out.view(complex).reshape(N, N)  # [uy][ux]
+    E = np.asarray(p.llt_amp).reshape(nlx, nlx) * np.exp(2j * np.pi * opd.reshape(nlx, nlx) / p.wvl[0])
+    ref = np.zeros((N, N), complex)
+    for iy in range(N):
+        sy = iy if iy < N // 2 else iy - N
+        for ix in range(N):
+            sx = ix if ix < N // 2 else ix - N
+            a = E[max(0, -sy):min(nlx, nlx - sy), max(0, -sx):min(nlx, nlx - sx)]
+            b = E[max(0, sy):min(nlx, nlx + sy), max(0, sx):min(nlx, nlx + sx)]
+            ref[iy, ix] = (a * b.conj()).sum()
+    assert rel_l2(lotf, ref) < 1e-12
