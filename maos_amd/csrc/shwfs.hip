@@ -2713,10 +2713,17 @@ struct StepLaunch {
     const void *f;
     dim3 grid, block;
     void *arg;
+    int dep = -2; // the launch it waits for: -2 the previous one, -1 none (a branch's first)
 };
 struct StepPlan {
     StepLaunch L[MX_MAXLAUNCH];
     int nl = 0;
+    int nmain = 0;     // launches of the step itself; the rest compute the next step's uplink OTF
+    bool pre_set = false; // the step computes the uplink OTF of isim + 1 (pre_* below)
+    long pre_isim = 0;
+    double pre_dt = 0;
+    int pre_form = 0, pre_buf = 0;
+    unsigned pre_gen = 0;
     int key = 0;       // the kernel set: a graph built for one key fits any step of it
     int slot_opd_last = 0; // last kernel of the OPD stage (ray trace, OPD inputs)
     int slot_llt = -1; // index of the first uplink-OTF kernel, or -1 (they run up to slot_img)
@@ -2759,6 +2766,28 @@ struct mx_wfsgrad_s {
     float *d_llt_amp = nullptr;
     float *d_llt_opd = nullptr;        // [nwfs][side^2] uplink OPD (side: llt_nx in whole ray-trace tiles)
     LltAcorrArgs arg_lltac{};         // uplink of an LLT grid wider than the subaperture grid
+    // The uplink OTF of step isim + 1, computed beside step isim's imaging
+    // (it needs only the screens): two lotf buffer pairs, the LLT OPD of the
+    // next step, and what the prefetched buffer holds.
+    bool uplink_pre = false;          // MX_UPLINK_PREFETCH=1 in the environment turns it on
+    float2 *d_lotf2 = nullptr, *d_lotf_t2 = nullptr;
+    float *d_llt_opd2 = nullptr;
+    int lotf_last = 0;                // buffer the last queued step's imaging read
+    bool pre_valid = false;
+    long pre_isim = 0;
+    double pre_dt = 0;
+    int pre_form = 0, pre_buf = 0;
+    unsigned pre_gen = 0;
+    // The branch runs on a second stream (default priority) beside the step
+    // (ev_main: the work queued before the step's chain; ev_branch: the last
+    // branch)
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_main = nullptr, ev_branch = nullptr;
+    bool branch_pending = false;
+    OpdArgs arg_opd_pre{};
+    MfmaArgs arg_m_pre{};
+    LltArgs arg_la_pre{};
+    LltAcorrArgs arg_lltac_pre{};
     float2 *d_lotf = nullptr;
     DevWvl *d_wv = nullptr;
     DevWvl h_wv[MX_MAXWVL] = {};
@@ -2868,6 +2897,7 @@ struct AtmState {
     size_t bytes = 0;
 };
 AtmState g_atm[MX_MAXREAL];
+unsigned g_atm_gen = 0; // counts screen uploads: an uplink OTF computed ahead is valid for one generation
 DevLayer *g_dlayers = nullptr; // device [MX_MAXREAL][MX_MAXLAYER]
 // DM surfaces of one realisation
 struct DmState {
@@ -3257,6 +3287,24 @@ int mx_wfsgrad_init_batch(mx_wfsgrad_t **out, const mx_powfs_t *p, const mx_wfs_
         ALLOC(h->d_llt_opd, nwfs * side * side);
         ALLOC(h->d_lotf, nwfs * h->nwvl * N * N);
         if (h->mfma_size) ALLOC(h->d_lotf_t, nwfs * h->nwvl * MX_HALF);
+        // The next step's uplink computed beside a step's chain, on a second
+        // stream (MX_UPLINK_PREFETCH=1): off by default, since the two event
+        // waits it adds per step cost more than the uplink (0.2494 against
+        // 0.2436 ms per lgs30m step on one box).
+        const char *e = getenv("MX_UPLINK_PREFETCH");
+        h->uplink_pre = e && atoi(e) != 0 && !lf;
+        if (h->uplink_pre) {
+            ALLOC(h->d_llt_opd2, nwfs * side * side);
+            ALLOC(h->d_lotf2, nwfs * h->nwvl * N * N);
+            if (h->mfma_size) ALLOC(h->d_lotf_t2, nwfs * h->nwvl * MX_HALF);
+        }
+    }
+    if (h->d_lotf2 && h->uplink_pre &&
+        (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&h->ev_main, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&h->ev_branch, hipEventDisableTiming) != hipSuccess)) {
+        mx_wfsgrad_free(h);
+        return set_err(MX_EHIP, "second stream or its events could not be created");
     }
     if (h->mfma_size) {
         ALLOC(h->d_F, MX_FENT * MX_TPARTS * 64 * 8);
@@ -3370,6 +3418,7 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (!h) return MX_OK;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->stream2) hipStreamSynchronize(h->stream2);
     drop_graph(h);
     void *ptrs[] = {h->d_ptsx, h->d_ptsy, h->d_amp, h->d_saa, h->d_wfs, h->d_wv, h->d_llt_amp, h->d_llt_opd, h->d_lotf,
                     h->d_lotf_t, h->d_F, h->d_F16, h->d_mwv, h->d_imcc, h->d_opdadd, h->d_ptt, h->d_lf_tw, h->d_b1, h->d_b2, h->d_b3, h->d_psf, h->d_img,
@@ -3383,6 +3432,11 @@ int mx_wfsgrad_free(mx_wfsgrad_t *h) {
     if (h->d_pistat) hipFree(h->d_pistat);
     if (h->d_gpis) hipFree(h->d_gpis);
     if (h->d_psfout) hipFree(h->d_psfout);
+    for (void *q : {(void *)h->d_lotf2, (void *)h->d_lotf_t2, (void *)h->d_llt_opd2})
+        if (q) hipFree(q);
+    for (hipEvent_t e : {h->ev_main, h->ev_branch})
+        if (e) hipEventDestroy(e);
+    if (h->stream2) hipStreamDestroy(h->stream2);
     for (int w = 0; w < MX_MAXWVL; w++) {
         if (h->d_nominal[w]) hipFree(h->d_nominal[w]);
         if (h->d_gx[w]) hipFree(h->d_gx[w]);
@@ -3678,6 +3732,7 @@ int mx_atm2gpu_real(int ireal, const mx_atm_layer_t *layers, int nps, double atm
     if (ireal < 0 || ireal >= MX_MAXREAL) return set_err(MX_EINVAL, "mx_atm2gpu_real: ireal=%d", ireal);
     if (nps < 0 || nps > MX_MAXLAYER || (nps && !layers)) return set_err(MX_EINVAL, "mx_atm2gpu: nps=%d", nps);
     HIPCHK(hipSetDevice(g_device));
+    g_atm_gen++; // uplink OTFs computed ahead from the old screens are stale
     AtmState &A = g_atm[ireal];
     size_t total = 0;
     for (int i = 0; i < nps; i++) {
@@ -3946,6 +4001,20 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         const char *e = getenv("MX_UPLINK_DIRECT");
         if (e && atoi(e) != 0 && !h->lf && h->nwvf == N && N <= 64) uplink_direct = true;
     }
+    // The uplink OTF of this step was computed beside the previous step's
+    // imaging if that step planned this isim with the same screens, dt and
+    // uplink kernel; then the chain skips the LLT trace and the uplink.  The
+    // imaging reads buffer pair cur, the next step's uplink goes to 1 - cur.
+    const int upform = (variant == MX_VARIANT_MFMA_BF16 ? 1 : 0) | (f16 ? 2 : 0) | (fftform ? 4 : 0) |
+                       (uplink_direct ? 8 : 0);
+    const bool pre_on = llt && !h->lf && h->uplink_pre && h->d_lotf2 != nullptr;
+    const bool pre_hit = pre_on && h->pre_valid && h->pre_isim == isim && h->pre_dt == dt &&
+                         h->pre_gen == g_atm_gen && h->pre_form == upform;
+    const int cur = pre_hit ? h->pre_buf : 0;
+    float2 *const lotf_cur = cur ? h->d_lotf2 : h->d_lotf, *const lotf_t_cur = cur ? h->d_lotf_t2 : h->d_lotf_t;
+    float2 *const lotf_nxt = cur ? h->d_lotf : h->d_lotf2, *const lotf_t_nxt = cur ? h->d_lotf_t : h->d_lotf_t2;
+    if (pre_hit) o.nllt = 0;
+    a.lotf = h->cfg.llt_nx > 0 ? lotf_cur : nullptr;
     if (variant == MX_VARIANT_MFMA_BF16 || f16 || fftform) {
         MfmaWvl hw[MX_MAXWVL] = {};
         for (int w = 0; w < h->nwvl; w++) {
@@ -3976,7 +4045,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         m.opd = a.opd;
         m.F = f16 ? h->d_F16 : h->d_F;
         m.wv = h->d_mwv;
-        m.lotf_t = h->cfg.llt_nx > 0 ? h->d_lotf_t : nullptr;
+        m.lotf_t = h->cfg.llt_nx > 0 ? lotf_t_cur : nullptr;
         m.wfs = a.wfs;
         m.phytype = a.phytype;
         m.sigmatch = a.sigmatch;
@@ -3998,8 +4067,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         m.llt_opd = h->d_llt_opd;
         m.llt_amp = h->d_llt_amp;
         m.llt_norm = (float)(1. / ((double)h->cfg.llt_nx * h->embfac * (double)h->cfg.llt_nx * h->embfac));
-        m.lotf_full = h->d_lotf;
-        m.lotf_half = h->d_lotf_t;
+        m.lotf_full = lotf_cur;
+        m.lotf_half = lotf_t_cur;
     }
     LltArgs &la = h->arg_llt;
     la = LltArgs{};
@@ -4010,8 +4079,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     la.wv = h->d_wv;
     const double nlwvf = (double)h->cfg.llt_nx * h->embfac;
     la.norm = (float)(1. / (nlwvf * nlwvf));
-    la.lotf = h->d_lotf;
-    la.lotf_t = h->d_lotf_t;
+    la.lotf = lotf_cur;
+    la.lotf_t = lotf_t_cur;
     // The step in stream order: ray trace (subaperture and LLT grids), uplink
     // OTF (the MFMA kernel's uplink mode with the MFMA form, else the FP32 FFT
     // kernel), imaging.
@@ -4088,7 +4157,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         q.grad = h->d_grad;
         sp.slot_img = nl;
         L[nl++] = {reinterpret_cast<const void *>(&k_gtilt), dim3((unsigned)(2 * h->nsa)), dim3(64), &q};
-        sp.nl = nl;
+        sp.nl = sp.nmain = nl;
         sp.key = (1 << 30) | (1 << 29) | (1 << 26) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (direct ? 8 : 0) |
                  (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8);
         sp.slot_llt = -1;
@@ -4112,7 +4181,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         z.grad = h->d_grad;
         sp.slot_img = nl;
         L[nl++] = {reinterpret_cast<const void *>(&k_ztilt), dim3((unsigned)(h->nsa * h->nwfs)), dim3(64), &z};
-        sp.nl = nl;
+        sp.nl = sp.nmain = nl;
         sp.key = (1 << 30) | (1 << 29) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (direct ? 8 : 0) |
                  (o.tiles > 1 ? 1 << 7 : 0) | (opd_nx << 8);
         sp.slot_llt = -1;
@@ -4167,7 +4236,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         f.etf1wt = 1.f - h->etf2wt;
         f.etf2wt = h->etf2wt;
     }
-    const int slot_llt = llt ? nl : -1;
+    const int slot_llt = llt && !pre_hit ? nl : -1;
     if (llt && h->lf) {
         // uplink OTF of each (WFS, wavelength) by passes A-C and the uplink
         // column pass on the LLT grids (nlwvf == nwvf == notf)
@@ -4193,7 +4262,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
             L[nl++] = {lf_kernel(up[q].q, h->lf_threads, up[q].len), dim3((unsigned)(up[q].ngroup * nch)),
                        dim3((unsigned)h->lf_threads), &u};
         }
-    } else if (llt && uplink_direct) {
+    } else if (llt && !pre_hit && uplink_direct) {
         LltAcorrArgs &u = h->arg_lltac;
         u = LltAcorrArgs{};
         u.nlx = (int)h->cfg.llt_nx;
@@ -4203,11 +4272,11 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         for (int w = 0; w < h->nwvl; w++) u.inv_wvl[w] = (float)(1. / h->wvl[w]);
         u.opd = h->d_llt_opd;
         u.amp = h->d_llt_amp;
-        u.lotf_full = h->d_lotf;
-        u.lotf_half = h->d_lotf_t;
+        u.lotf_full = lotf_cur;
+        u.lotf_half = lotf_t_cur;
         L[nl++] = {reinterpret_cast<const void *>(&k_llt_acorr), dim3((unsigned)(h->nwfs * h->nwvl * N)), dim3(256),
                    &u};
-    } else if (llt) {
+    } else if (llt && !pre_hit) {
         if (mf)
             L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<true, _Float16>)
                            : reinterpret_cast<const void *>(&k_shwfs_mfma<true, __bf16>),
@@ -4371,6 +4440,59 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         s.acc = h->d_pistat;
         L[nl++] = {reinterpret_cast<const void *>(&k_pistat), dim3((unsigned)(h->nsa * h->nwfs)), dim3(MX_FFT_T), &s};
     }
+    sp.nmain = nl;
+    if (pre_on) {
+        // branch: the LLT grids of step isim + 1 (the ray trace's LLT blocks
+        // alone), then their uplink OTF into the other buffer pair
+        OpdArgs &q = h->arg_opd_pre;
+        q = o;
+        q.isim = isim + 1;
+        q.llt_opd = h->d_llt_opd2;
+        q.nllt = h->nwfs * o.llt_tiles * o.llt_tiles;
+        L[nl] = {kopd, dim3((unsigned)q.nllt), dim3(MX_OPD_THREADS), &q};
+        L[nl++].dep = -1;
+        if (uplink_direct) {
+            LltAcorrArgs &u = h->arg_lltac_pre;
+            u = LltAcorrArgs{};
+            u.nlx = (int)h->cfg.llt_nx;
+            u.side = o.llt_tiles * opd_nx;
+            u.nwvl = (int)h->nwvl;
+            u.N = (int)N;
+            for (int w = 0; w < h->nwvl; w++) u.inv_wvl[w] = (float)(1. / h->wvl[w]);
+            u.opd = h->d_llt_opd2;
+            u.amp = h->d_llt_amp;
+            u.lotf_full = lotf_nxt;
+            u.lotf_half = lotf_t_nxt;
+            L[nl++] = {reinterpret_cast<const void *>(&k_llt_acorr), dim3((unsigned)(h->nwfs * h->nwvl * N)),
+                       dim3(256), &u};
+        } else if (mf || fftform) {
+            MfmaArgs &u = h->arg_m_pre;
+            u = m;
+            u.llt_opd = h->d_llt_opd2;
+            u.lotf_full = lotf_nxt;
+            u.lotf_half = lotf_t_nxt;
+            const void *k = fftform ? reinterpret_cast<const void *>(&k_shwfs_fft<true>)
+                            : f16   ? reinterpret_cast<const void *>(&k_shwfs_mfma<true, _Float16>)
+                                    : reinterpret_cast<const void *>(&k_shwfs_mfma<true, __bf16>);
+            L[nl++] = {k, dim3((unsigned)(h->nwfs * h->nwvl)), dim3(256), &u};
+        } else {
+            LltArgs &u = h->arg_la_pre;
+            u = la;
+            u.opd = h->d_llt_opd2;
+            u.lotf = lotf_nxt;
+            u.lotf_t = lotf_t_nxt;
+            L[nl++] = {N == 64 ? reinterpret_cast<const void *>(&k_llt_otf<64, 32>)
+                               : reinterpret_cast<const void *>(&k_llt_otf<32, 16>),
+                       dim3((unsigned)(h->nwfs * h->nwvl)), dim3(64), &u};
+        }
+        sp.pre_set = true;
+        sp.pre_isim = isim + 1;
+        sp.pre_dt = dt;
+        sp.pre_form = upform;
+        sp.pre_buf = 1 - cur;
+        sp.pre_gen = g_atm_gen;
+    }
+    h->lotf_last = cur;
     if (nl > MX_MAXLAUNCH) return set_err(MX_EINVAL, "internal: %d kernels in a step", nl);
     sp.nl = nl;
     sp.key = (1 << 30) | (opdin ? 1 << 28 : 0) | (ndm ? 1 << 27 : 0) | (N == 64 ? 1 : 0) | (llt ? 2 : 0) | (h->lf ? 4 : 0) |
@@ -4379,7 +4501,7 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
              ((mf || fftform) && h->d_etf2_t[0] ? 1 << 24 : 0) | (fftform ? 1 << 6 : 0) |
              (mf && !pp ? 1 << 14 : 0) | // (lf_threads << 14 sets bits 21-23 only)
              (h->pistat ? 1 << 26 : 0) | (h->pistat && h->pistat_gtype == 2 ? 1 << 15 : 0) |
-             (uplink_direct ? 1 << 16 : 0);
+             (uplink_direct ? 1 << 16 : 0) | (pre_hit ? 1 << 17 : 0) | (pre_on ? 1 << 18 : 0);
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;
@@ -4400,9 +4522,43 @@ void node_params(const StepPlan &sp, hipKernelNodeParams *kp, void *(*argv)[1]) 
     }
 }
 
-// launch a planned step on the engine's stream: one graph launch, or the
-// kernels one by one (with the timing events when timing is on)
-int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
+// the kernel nodes of a planned step, each after the launch it waits for
+int add_step_nodes(hipGraph_t graph, hipGraphNode_t *nd, const StepPlan &sp, const hipKernelNodeParams *kp) {
+    for (int i = 0; i < sp.nl; i++) {
+        const int d = sp.L[i].dep == -2 ? i - 1 : sp.L[i].dep;
+        HIPCHK(hipGraphAddKernelNode(&nd[i], graph, d >= 0 ? &nd[d] : nullptr, d >= 0 ? 1 : 0, &kp[i]));
+    }
+    return MX_OK;
+}
+
+int launch_chain(mx_wfsgrad_t *h, const StepPlan &sp);
+
+// launch a planned step; with a second stream the next step's uplink branch
+// runs there, beside the step's chain
+int launch_step(mx_wfsgrad_t *h, const StepPlan &in) {
+    if (!h->stream2 || in.nl == in.nmain) return launch_chain(h, in);
+    StepPlan sp = in;
+    sp.nl = sp.nmain;
+    // the chain waits for the last branch (it reads that branch's buffers);
+    // the branch waits for all work before the chain (the buffer pair it
+    // overwrites was read there) and then runs beside the chain
+    if (h->branch_pending) HIPCHK(hipStreamWaitEvent(h->stream, h->ev_branch, 0));
+    HIPCHK(hipEventRecord(h->ev_main, h->stream));
+    int rc = launch_chain(h, sp);
+    if (rc) return rc;
+    HIPCHK(hipStreamWaitEvent(h->stream2, h->ev_main, 0));
+    for (int i = in.nmain; i < in.nl; i++) {
+        void *argv[1] = {in.L[i].arg};
+        HIPCHK(hipLaunchKernel(in.L[i].f, in.L[i].grid, in.L[i].block, argv, 0, h->stream2));
+    }
+    HIPCHK(hipEventRecord(h->ev_branch, h->stream2));
+    h->branch_pending = true;
+    return MX_OK;
+}
+
+// the step's launches in stream order: one graph launch, or the kernels one
+// by one (with the timing events when timing is on)
+int launch_chain(mx_wfsgrad_t *h, const StepPlan &sp) {
     const StepLaunch *L = sp.L;
     const int nl = sp.nl, slot_llt = sp.slot_llt, slot_img = sp.slot_img;
     const bool llt = sp.llt;
@@ -4416,9 +4572,8 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
         if (!h->gexec || h->gkey != key) {
             drop_graph(h);
             HIPCHK(hipGraphCreate(&h->graph, 0));
-            for (int i = 0; i < nl; i++)
-                HIPCHK(hipGraphAddKernelNode(&h->gnode[i], h->graph, i ? &h->gnode[i - 1] : nullptr, i ? 1 : 0,
-                                             &kp[i]));
+            int rc = add_step_nodes(h->graph, h->gnode, sp, kp);
+            if (rc) return rc;
             HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
             h->gkey = key;
         } else {
@@ -4442,7 +4597,7 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
             hipEvent_t eb = i == 0 ? ev[2] : i == slot_llt ? ev[0] : i == slot_img ? ev[4] : nullptr;
             hipEvent_t ee = i == sp.slot_opd_last                     ? ev[3]
                             : (slot_llt >= 0 && i == slot_img - 1) ? ev[1]
-                            : i == nl - 1                          ? ev[5]
+                            : i == sp.nmain - 1                    ? ev[5]
                                                                    : nullptr;
             if (h->timing && eb) HIPCHK(hipEventRecord(eb, h->stream));
             void *argv[1] = {L[i].arg};
@@ -4452,7 +4607,8 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
         if (h->timing) {
             HIPCHK(hipEventSynchronize(ev[5]));
             float m1 = 0, m2 = 0, m3 = 0;
-            if (llt) HIPCHK(hipEventElapsedTime(&m1, ev[0], ev[1]));
+            // (no uplink in the chain when it was computed beside the last step)
+            if (llt && slot_llt >= 0) HIPCHK(hipEventElapsedTime(&m1, ev[0], ev[1]));
             HIPCHK(hipEventElapsedTime(&m2, ev[2], ev[3]));
             HIPCHK(hipEventElapsedTime(&m3, ev[4], ev[5]));
             h->t_llt += m1;
@@ -4465,7 +4621,17 @@ int launch_step(mx_wfsgrad_t *h, const StepPlan &sp) {
 }
 
 // bookkeeping after a step was queued
-void step_queued(mx_wfsgrad_t *h) {
+void step_queued(mx_wfsgrad_t *h, const StepPlan &sp) {
+    // what the step's branch computes ahead (valid once the step has run,
+    // which every later step on the stream waits for)
+    h->pre_valid = sp.pre_set;
+    if (sp.pre_set) {
+        h->pre_isim = sp.pre_isim;
+        h->pre_dt = sp.pre_dt;
+        h->pre_form = sp.pre_form;
+        h->pre_buf = sp.pre_buf;
+        h->pre_gen = sp.pre_gen;
+    }
     // the i0 and dither sums take whole exposures
     if (h->i0_enable && h->last_out) h->i0_nframe++;
     if (h->dith_on && h->last_out) h->dith_nframe++;
@@ -4481,7 +4647,7 @@ int mx_wfsgrad_queue(mx_wfsgrad_t *h, long isim, double dt) {
     int rc = plan_step(h, isim, dt, sp);
     if (rc) return rc;
     if ((rc = launch_step(h, sp))) return rc;
-    step_queued(h);
+    step_queued(h, sp);
     return MX_OK;
 }
 
@@ -4554,9 +4720,8 @@ int mx_wfsgroup_queue(mx_wfsgroup_t *g, long isim, double dt) {
         HIPCHK(hipGraphCreate(&g->graph, 0));
         for (int e = 0; e < n; e++) {
             hipGraphNode_t *nd = g->node.data() + (size_t)e * MX_MAXLAUNCH;
-            for (int i = 0; i < sp[e].nl; i++)
-                HIPCHK(hipGraphAddKernelNode(&nd[i], g->graph, i ? &nd[i - 1] : nullptr, i ? 1 : 0,
-                                             &kp[(size_t)e * MX_MAXLAUNCH + i]));
+            int rc = add_step_nodes(g->graph, nd, sp[e], kp.data() + (size_t)e * MX_MAXLAUNCH);
+            if (rc) return rc;
             g->key[e] = sp[e].key;
         }
         HIPCHK(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
@@ -4576,7 +4741,7 @@ int mx_wfsgroup_queue(mx_wfsgroup_t *g, long isim, double dt) {
     HIPCHK(hipEventRecord(g->done, g->stream));
     for (int e = 0; e < n; e++) {
         HIPCHK(hipStreamWaitEvent(g->h[e]->stream, g->done, 0));
-        step_queued(g->h[e]);
+        step_queued(g->h[e], sp[e]);
     }
     return MX_OK;
 }
@@ -4599,6 +4764,7 @@ int mx_wfsgrad_wait(mx_wfsgrad_t *h) {
     if (!h) return set_err(MX_EINVAL, "null handle");
     HIPCHK(hipSetDevice(h->device));
     HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->stream2) HIPCHK(hipStreamSynchronize(h->stream2));
     return MX_OK;
 }
 
@@ -4916,7 +5082,8 @@ int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf) {
     HIPCHK(hipStreamSynchronize(h->stream));
     const long N = h->notf;
     std::vector<float2> t(N * N);
-    HIPCHK(hipMemcpy(t.data(), h->d_lotf + ((long)iwfs * h->nwvl + iwvl) * N * N, sizeof(float2) * N * N,
+    const float2 *src = h->lotf_last ? h->d_lotf2 : h->d_lotf; // the pair the last step's imaging read
+    HIPCHK(hipMemcpy(t.data(), src + ((long)iwfs * h->nwvl + iwvl) * N * N, sizeof(float2) * N * N,
                      hipMemcpyDeviceToHost));
     for (long ux = 0; ux < N; ux++)
         for (long uy = 0; uy < N; uy++) {

@@ -267,3 +267,47 @@ def test_trace_other_realisation(lgs_small):
     ev.trace(3, 1 / 800, add_psf=False)
     np.testing.assert_allclose(ev.opd()[0], -got, rtol=0, atol=1e-12)
     ev.close()
+
+
+def test_reupload_waits_for_queued_trace(lgs_small):
+    """Screens are overwritten in place: an upload right after an
+    asynchronous trace (and a queued WFS step) waits for them, so the traced
+    OPD is that of the old screens and the next trace sees the new ones."""
+    import dataclasses
+    from maos_amd import configs, engine
+    from maos_amd.powfs import genatm
+    cfg, p, layers = lgs_small
+    print("case: screen upload after a queued trace", flush=True)
+    layers2 = genatm(dataclasses.replace(cfg["atm"], seeds=[s + 50 for s in cfg["atm"].seeds]))
+    dx = 1 / 64
+    locx, locy = annulus(10.0, 1.2, dx)
+    emb, N = engine.loc_embed(locx, locy, dx, nembed=2048)
+    amp = np.full(len(locx), 1.0 / len(locx))
+    engine.init(0)
+    engine.atm2gpu(layers)
+    dirs = [(0.0, 0.0, 0.0), (2e-5, 1e-5, 0.0)]
+    ev = engine.Evl(amp, [1.65e-6], [N], [256], emb, locx=locx, locy=locy, dirs=dirs)
+    wfs = configs.wfs_specs(cfg, p)
+    eng = engine.WfsGrad(p, wfs)
+    eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    dt = 1 / 800
+    for step, (old, new) in enumerate([(layers, layers2), (layers2, layers)]):
+        isim = 3 + step
+        ev.trace(isim, dt, add_psf=False)
+        eng.queue(isim, dt)
+        engine.atm2gpu(new)  # must wait for the trace and the step above
+        got = ev.opd()
+        want = np.stack([_oracle_opd(old, [], locx, locy, d, isim, dt, 1.0) for d in dirs])
+        err = np.abs(got - want).max() / np.sqrt(np.mean(want ** 2))
+        other = np.abs(got - np.stack([_oracle_opd(new, [], locx, locy, d, isim, dt, 1.0) for d in dirs])).max()
+        print("  step %d: OPD error %.2e of rms (against the new screens %.2e)" % (step, err, other), flush=True)
+        assert err <= 1e-5 and other > 100 * err * np.sqrt(np.mean(want ** 2))
+        wopd = eng.debug_opd(0)
+        import oracle
+        o = oracle.Oracle(p)
+        o.atm(old)
+        _, _, ropd, _ = o.step(wfs[0], isim, dt, want_opd=True)
+        assert np.abs(wopd - ropd).max() <= 1e-5 * np.abs(ropd).max()
+    eng.close()
+    ev.close()
+    engine.atm2gpu(layers)

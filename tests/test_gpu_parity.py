@@ -1083,3 +1083,42 @@ def test_wide_llt_parity(lgs_wide_llt, variant, monkeypatch):
             assert e_int <= INTS_TOL
             assert e_g <= GRAD_TOL_MAS
     eng.close()
+
+
+def test_uplink_computed_ahead(lgs_small, monkeypatch):
+    """With MX_UPLINK_PREFETCH=1 the uplink OTF of step isim + 1 is
+    computed beside step isim's chain on a second stream (it needs only the
+    screens).  Consecutive steps, a jump in isim, a change of imaging form
+    and new screens give bit-identical intensities and uplink OTFs to an
+    engine that computes every uplink in its step's chain (the default)."""
+    from maos_amd import _lib, engine
+    from maos_amd.powfs import genatm
+    import dataclasses
+    cfg, p, layers = lgs_small
+    print("case: uplink computed ahead, lgs order 20", flush=True)
+    monkeypatch.setenv("MX_UPLINK_PREFETCH", "1")
+    eng, wfs = _engine(cfg, p, layers)
+    monkeypatch.delenv("MX_UPLINK_PREFETCH")
+    ref = engine.WfsGrad(p, wfs)
+    for e in (eng, ref):
+        e.set_graph(True)
+        e.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+    dt = cfg["sim"].dt
+    layers2 = genatm(dataclasses.replace(cfg["atm"], seeds=[s + 100 for s in cfg["atm"].seeds]))
+    for isim, what in [(10, ""), (11, ""), (12, ""), (40, "jump"), (41, "fft32"), (42, ""), (43, "screens"),
+                       (44, "")]:
+        if what == "fft32":
+            for e in (eng, ref):
+                e.set_variant(_lib.VARIANT_FFT32)
+        if what == "screens":
+            engine.atm2gpu(layers2)
+        eng.queue(isim, dt)
+        ref.queue(isim, dt)
+        for iw in (0, len(wfs) - 1):
+            _, a = eng.sync(iw)
+            _, b = ref.sync(iw)
+            assert np.array_equal(a, b), (isim, iw)
+            assert np.array_equal(eng.debug_lotf(iw), ref.debug_lotf(iw)), (isim, iw)
+        print("isim=%d %s: equal" % (isim, what))
+    ref.close()
+    eng.close()
