@@ -334,6 +334,8 @@ def hetero_main(args):
     group.close()
     for c in classes:
         c["eng"].close()
+    if args.copy_calib > 0:
+        copy_calibration(args.copy_calib)
 
 
 def evl_main(args):
