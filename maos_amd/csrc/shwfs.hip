@@ -2118,10 +2118,14 @@ constexpr int MX_LF_MAXS = 128;    // most sampled rows / columns kept (else the
 __host__ __device__ constexpr int lf_pos(int e) { return e + (e >> 5); }
 // LDS stride of lines of length n: the padded length, made odd
 __host__ __device__ constexpr int lf_stride(int n) { return lf_pos(n) | 1; }
-// lines of length n per block: a power of two <= 32 that one buffer holds
+// lines of length n per block: a power of two <= 32 that one buffer holds;
+// for lines of 768 and more at most 2048 / n of them, so that a one-chain
+// plane (the 960^2 and 1024^2 tip-tilt sizes: 2 lines, 480-512 blocks)
+// spreads over more blocks than CUs (53 -> 46 us per step for the 1024^2
+// tip-tilt imaging; the 512^2 planes, with many chains, keep 8 lines)
 __host__ __device__ constexpr int lf_lines(int n) {
     int L = 32;
-    while (L > 1 && L * lf_stride(n) > MX_LF_LDS) L /= 2;
+    while (L > 1 && (L * lf_stride(n) > MX_LF_LDS || (n >= 768 && L * n > 2048))) L /= 2;
     return L;
 }
 // radix of stage s in the plan of n (radix 4 first, then 2, 3, 5, 7); 0
