@@ -1562,7 +1562,15 @@ struct MfmaArgs {
     const float *llt_opd, *llt_amp;
     float llt_norm;
     float2 *lotf_full, *lotf_half;
+    // FUSED instances trace their own grid first (the screens and geometry
+    // of k_wfs_opd in tr) into LDS, and store the traced OPD to opd_out
+    // ([nwfs][nsa][32*32]) or llt_opd_out ([nwfs][32*32]) for later readers
+    OpdArgs tr;
+    float *opd_out, *llt_opd_out;
 };
+// per-wave LDS scratch of the fused ray trace (cidx, cfrac, the layer table,
+// the staged 16 x 16 tile), carved from the imaging planes
+constexpr int MX_FUSE_SCR = 3072;
 
 template <typename E>
 __device__ __forceinline__ f32x4 mf16(v8_t<E> a, v8_t<E> b, f32x4 c) {
@@ -1718,7 +1726,7 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
 // reading one and writing the other, so a stage's writes need no barrier
 // after its reads (one barrier per stage instead of two).  Arrays wider
 // than 32 columns fold their second half into rows 32..63.
-template <bool UPLINK, typename E, bool PP = true>
+template <bool UPLINK, typename E, bool PP = true, bool FUSED = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAVES))) void k_shwfs_mfma(MfmaArgs a) {
     using V8 = v8_t<E>;
     // range scales of the FP16 form (powers of two; tables carry 2^13): the
@@ -1762,6 +1770,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
     const int ix = tid % 32, iy0 = 4 * (tid / 32);
     const float *og = UPLINK ? a.llt_opd + (long)iwfs * NX * NX : a.opd + ((long)iwfs * a.nsa + isa) * NX * NX;
     const float *ag = UPLINK ? a.llt_amp : a.amp + (long)isa * NX * NX;
+    __shared__ float opdl[FUSED ? NX * NX : 1];
+    if constexpr (FUSED) {
+        // the grid's OPD (k_wfs_opd's ray trace, maos/wfsgrad.c:175-194 and
+        // the LLT's :329-386): wave w traces the 16 x 16 quadrant (w%2, w/2)
+        // into opdl with its scratch in the planes, which the embed
+        // overwrites only after the barrier below
+        const OpdArgs &t = a.tr;
+        char *scr = reinterpret_cast<char *>(Lb) + wave * MX_FUSE_SCR;
+        int *cidx = reinterpret_cast<int *>(scr);
+        float *cfrac = reinterpret_cast<float *>(scr + 256);
+        double *ctab = reinterpret_cast<double *>(scr + 512);
+        float *tile = reinterpret_cast<float *>(scr + 1024);
+        const int qx = wave & 1, qy = wave >> 1;
+        const double o16 = t.ptsdx * 16.;
+        if constexpr (UPLINK)
+            ray_trace<64, 16, false>(t.layers + W.ireal * MX_MAXLAYER, t.nps, W.hs, W.thetax - t.llt_ox / W.hs,
+                                     W.thetay - t.llt_oy / W.hs, 0., 0., t.llt_ox, t.llt_oy, t.isim, t.dt,
+                                     t.atmscale[W.ireal], t.ptsdx, t.llt_origx + o16 * qx, t.llt_origy + o16 * qy,
+                                     lane, opdl + 16 * NX * qy + 16 * qx, cidx, cfrac, tile, ctab, t.tworow, NX);
+        else
+            ray_trace<64, 16, false>(t.layers + W.ireal * MX_MAXLAYER, t.nps, W.hs, W.thetax, W.thetay, W.misregx,
+                                     W.misregy, 0., 0., t.isim, t.dt, t.atmscale[W.ireal], t.ptsdx,
+                                     t.ptsx[isa] + o16 * qx, t.ptsy[isa] + o16 * qy, lane,
+                                     opdl + 16 * NX * qy + 16 * qx, cidx, cfrac, tile, ctab, t.tworow, NX);
+        __syncthreads();
+        float *go = UPLINK ? a.llt_opd_out + (long)iwfs * NX * NX : a.opd_out + ((long)iwfs * a.nsa + isa) * NX * NX;
+#pragma unroll
+        for (int j = 0; j < 4; j++) go[tid + 256 * j] = opdl[tid + 256 * j];
+        og = opdl;
+    }
     float mean;
     {
         float s = 0.f;
@@ -4101,8 +4139,26 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
          {reinterpret_cast<const void *>(&k_wfs_opd<32, true, false>),
           reinterpret_cast<const void *>(&k_wfs_opd<32, true, true>)}}};
     const void *kopd = kopd_t[opd_nx == 32][direct][o.tiles > 1];
-    L[nl++] = {kopd,
-               dim3((unsigned)(h->nsa * h->nwfs * o.tiles * o.tiles + o.nllt)), dim3(MX_OPD_THREADS), &o};
+    // MX_FUSED=1 in the environment: the MFMA imaging and uplink kernels trace
+    // their own grids (no ray-trace launch, the OPD staged in LDS), for the
+    // MFMA forms on the 32-point grid without DM surfaces or OPD inputs.
+    // Slower than the separate ray trace (0.327 against 0.256 ms per lgs30m
+    // step), so kept for comparison only.
+    bool fused = false;
+    {
+        const char *e = getenv("MX_FUSED");
+        fused = e && atoi(e) != 0 && mf && !h->lf && h->nxsa == 32 && N == 64 && !direct && !h->geom &&
+                g_dm[0].ndm == 0 && !h->opdadd_set && !h->ptt_set && !pre_on &&
+                (!llt || (h->cfg.llt_nx == 32 && !uplink_direct)) && g_atm[0].nps <= MX_MAXLAYER;
+    }
+    if (fused) {
+        m.tr = o;
+        m.opd_out = h->d_opd;
+        m.llt_opd_out = h->d_llt_opd;
+    } else {
+        L[nl++] = {kopd,
+                   dim3((unsigned)(h->nsa * h->nwfs * o.tiles * o.tiles + o.nllt)), dim3(MX_OPD_THREADS), &o};
+    }
     // the DM surfaces (every realisation of the engine has the same count)
     const int ndm = g_dm[0].ndm;
     for (int r = 1; r < h->nreal; r++)
@@ -4281,7 +4337,11 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         L[nl++] = {reinterpret_cast<const void *>(&k_llt_acorr), dim3((unsigned)(h->nwfs * h->nwvl * N)), dim3(256),
                    &u};
     } else if (llt && !pre_hit) {
-        if (mf)
+        if (mf && fused)
+            L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<true, _Float16, true, true>)
+                           : reinterpret_cast<const void *>(&k_shwfs_mfma<true, __bf16, true, true>),
+                       dim3((unsigned)(h->nwfs * h->nwvl)), dim3(256), &m};
+        else if (mf)
             L[nl++] = {f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<true, _Float16>)
                            : reinterpret_cast<const void *>(&k_shwfs_mfma<true, __bf16>),
                        dim3((unsigned)(h->nwfs * h->nwvl)),
@@ -4364,6 +4424,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
                        dim3((unsigned)std::min<long>((b.n * h->nwvl + 255) / 256, 8192)), dim3(256), &b};
         }
         const void *kimg = fftform ? reinterpret_cast<const void *>(&k_shwfs_fft<false>)
+                           : fused ? (f16 ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16, true, true>)
+                                          : reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16, true, true>))
                            : f16   ? (pp ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16, true>)
                                          : reinterpret_cast<const void *>(&k_shwfs_mfma<false, _Float16, false>))
                                    : (pp ? reinterpret_cast<const void *>(&k_shwfs_mfma<false, __bf16, true>)
@@ -4505,7 +4567,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
              ((mf || fftform) && h->d_etf2_t[0] ? 1 << 24 : 0) | (fftform ? 1 << 6 : 0) |
              (mf && !pp ? 1 << 14 : 0) | // (lf_threads << 14 sets bits 21-23 only)
              (h->pistat ? 1 << 26 : 0) | (h->pistat && h->pistat_gtype == 2 ? 1 << 15 : 0) |
-             (uplink_direct ? 1 << 16 : 0) | (pre_hit ? 1 << 17 : 0) | (pre_on ? 1 << 18 : 0);
+             (uplink_direct ? 1 << 16 : 0) | (pre_hit ? 1 << 17 : 0) | (pre_on ? 1 << 18 : 0) |
+             (fused ? 1 << 19 : 0);
     sp.slot_llt = slot_llt;
     sp.slot_img = slot_img;
     sp.llt = llt;
@@ -4597,8 +4660,17 @@ int launch_chain(mx_wfsgrad_t *h, const StepPlan &sp) {
             }
             for (int i = 0; i < 6; i++) ev[i] = h->ev[i];
         }
+        // (no ray-trace launch when the imaging kernels trace their grids:
+        // the ray-trace interval is empty)
+        if (h->timing && sp.slot_opd_last < 0) {
+            HIPCHK(hipEventRecord(ev[2], h->stream));
+            HIPCHK(hipEventRecord(ev[3], h->stream));
+        }
         for (int i = 0; i < nl; i++) {
-            hipEvent_t eb = i == 0 ? ev[2] : i == slot_llt ? ev[0] : i == slot_img ? ev[4] : nullptr;
+            hipEvent_t eb = (i == 0 && sp.slot_opd_last >= 0) ? ev[2]
+                            : i == slot_llt                    ? ev[0]
+                            : i == slot_img                    ? ev[4]
+                                                               : nullptr;
             hipEvent_t ee = i == sp.slot_opd_last                     ? ev[3]
                             : (slot_llt >= 0 && i == slot_img - 1) ? ev[1]
                             : i == sp.nmain - 1                    ? ev[5]

@@ -64,14 +64,21 @@ def _lgs_ref(p, o, wfs, dt, isim=3):
 
 # _lib.VARIANT_*; fft32_reg is the FP32 form with its register-FFT kernel
 # (MX_FFT_REG=1), which it also runs where the half-plane tables do not apply
-VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2, "fft32_reg": 0}
+VARIANTS = {"fft32": 0, "mfma_bf16": 1, "mfma_fp16": 2, "fft32_reg": 0, "mfma_bf16_fused": 1,
+            "mfma_fp16_fused": 2}
 
 
-@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16", "fft32_reg"])
+@pytest.mark.parametrize("variant", ["fft32", "mfma_bf16", "mfma_fp16", "fft32_reg", "mfma_bf16_fused",
+                                     "mfma_fp16_fused"])
 def test_lgs_step_parity(lgs_small, variant, monkeypatch):
+    """Every imaging form against the oracle; the _fused variants
+    (MX_FUSED=1) trace the subaperture and LLT grids inside the imaging and
+    uplink kernels."""
     cfg, p, layers = lgs_small
     if variant == "fft32_reg":
         monkeypatch.setenv("MX_FFT_REG", "1")
+    if variant.endswith("_fused"):
+        monkeypatch.setenv("MX_FUSED", "1")
     print("case: lgs order20 nsa=%d notf=%d nwfs=%d 1024^2 screens, %s"
           % (p.nsa, p.notf, len(cfg["wfs"]), variant), flush=True)
     eng, wfs = _engine(cfg, p, layers)
