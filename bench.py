@@ -220,6 +220,57 @@ def batch_main(args):
     eng.close()
 
 
+def split_main(args):
+    """The six lgs30m WFS as one engine against K engines of 6/K WFS grouped
+    into one graph with a branch per engine (mx_wfsgroup_*): µs per step of
+    each, alternated twice."""
+    from maos_amd import configs, engine
+    from maos_amd.powfs import Powfs, genatm
+
+    cfg = configs.lgs30m(nx=args.nx, order=args.order)
+    p = Powfs(cfg["aper"], cfg["powfs"], cfg["sim"])
+    wfs = configs.wfs_specs(cfg, p)
+    engine.init(0)
+    engine.atm2gpu(genatm(cfg["atm"]))
+    dt = cfg["sim"].dt
+    k = args.split
+    if k not in (2, 3, 6):
+        raise SystemExit("--split takes 2, 3 or 6")
+
+    def mk(ws):
+        e = engine.WfsGrad(p, ws)
+        e.set_graph(True)
+        e.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
+        return e
+
+    def run(queue, wait):
+        isim = 100
+        for _ in range(args.warmup):
+            queue(isim, dt)
+            isim += 1
+        wait()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            queue(isim, dt)
+            isim += 1
+        wait()
+        return (time.perf_counter() - t) / args.steps * 1e6
+
+    res = {"one engine": [], "%d engines" % k: []}
+    for _ in range(2):
+        e = mk(wfs)
+        res["one engine"].append(round(run(e.queue, e.wait), 1))
+        e.close()
+        n = len(wfs) // k
+        engs = [mk(wfs[i * n:(i + 1) * n]) for i in range(k)]
+        g = engine.WfsGroup(engs)
+        res["%d engines" % k].append(round(run(g.queue, g.wait), 1))
+        g.close()
+        for x in engs:
+            x.close()
+    print(json.dumps({"split": k, "us_per_step": res, "steps": args.steps, "warmup": args.warmup}), flush=True)
+
+
 def hetero_main(args):
     """The heterogeneous step: the six lgs30m LGS WFS, three low-order NGS
     WFS (2x2 of 15 m, 480^2 fields in 512^2 OTFs, two wavelengths) and one
@@ -567,6 +618,8 @@ def main():
                     help="with --pywfs: one frame against the FP64 oracle (about a minute of host time)")
     ap.add_argument("--hetero", action="store_true",
                     help="LGS + low-order NGS + TT NGS stepped as one group graph per step")
+    ap.add_argument("--split", type=int, default=0,
+                    help="lgs30m µs per step as one engine and as K grouped engines of 6/K WFS (K = 2, 3, 6)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the WFS path of this build is single-GPU "
@@ -577,6 +630,9 @@ def main():
         return
     if args.hetero:
         hetero_main(args)
+        return
+    if args.split:
+        split_main(args)
         return
     if args.evl:
         evl_main(args)
