@@ -958,14 +958,15 @@ def test_dtrat_exposure(case, variant, request, monkeypatch):
     eng.close()
 
 
-@pytest.mark.parametrize("gtype", [1, 2])
-def test_pistat_matches_oracle(lgs_small, gtype):
+@pytest.mark.parametrize("case,gtype", [("lgs_small", 1), ("lgs_small", 2), ("ngs_two_wvl", 1)])
+def test_pistat_matches_oracle(case, gtype, request):
     """Tip/tilt-removed PSF statistics (pistatout) over three steps against
     the oracle's restatement of MAOS's OTF-tilt path, fed the GPU's own
     traced OPD and the oracle's geometric gradients of it (Zernike tilt or
-    G-tilt); the imaging step itself is unchanged by the extra kernels."""
-    cfg, p, layers = lgs_small
-    print("case: pistat gtype=%d nsa=%d" % (gtype, p.nsa), flush=True)
+    G-tilt); the imaging step itself is unchanged by the extra kernels.
+    ngs_two_wvl has two wavelengths and no LLT."""
+    cfg, p, layers = request.getfixturevalue(case)
+    print("case: pistat %s gtype=%d nsa=%d" % (case, gtype, p.nsa), flush=True)
     eng, wfs = _engine(cfg, p, layers)
     o = _oracle(p, layers)
     eng.update_ref(np.zeros(p.nsa * 2 * p.npix), np.ones(p.nsa))
@@ -975,7 +976,7 @@ def test_pistat_matches_oracle(lgs_small, gtype):
     eng.queue(4, dt)
     g_off, i_off = eng.sync(0)
     eng.pistat_control(True, gtype=gtype, reset=True)
-    acc = {iw: None for iw in (0, 3)}
+    acc = {iw: None for iw in sorted({0, len(wfs) - 1})}
     for isim in (4, 5, 6):
         eng.queue(isim, dt)
         for iw in acc:
@@ -1022,6 +1023,7 @@ def test_psfout_matches_oracle(case, request):
     ("ngs_two_wvl", "samples", "mfma_bf16"),   # whole PSF samples: the union fits the MFMA form
     ("ngs_two_wvl", "samples", "fft32"),       # ... and the FP32 radix-8 form
     ("lgs_small", "random", "fft32"),          # with an LLT: si per subaperture, nominal in the ETF
+    ("lf_mixed", "random", "fft32"),           # the large-field form (96^2 field in a 120^2 OTF)
 ])
 def test_per_subaperture_tables(case, offsets, variant, request):
     """multi_nominal: an si table (and a nominal without an LLT) per
