@@ -13,7 +13,7 @@
 //   sample  bilinear at every subaperture centre of each pupil image;
 //   grads   the quad-cell (or three-facet) combination, flux-normalised.
 // Every 2-D transform is a pass of line FFTs along the contiguous axis
-// (Stockham stages in LDS, radices 4, 2, 3, 5, 7: lds_fft_rt), a 32 x 32
+// (Stockham stages in LDS, radices 8, 4, 2, 3, 5, 7: lds_fft_rt), a 32 x 32
 // tiled transpose, and a second line pass; the orientation alternates so
 // that each pass reads and writes whole rows.  Layouts are [row][column]
 // with the column contiguous; "transposed" arrays hold [x][y].

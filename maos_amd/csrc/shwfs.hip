@@ -2166,12 +2166,12 @@ __host__ __device__ constexpr int lf_lines(int n) {
     while (L > 1 && (L * lf_stride(n) > MX_LF_LDS || (n >= 768 && L * n > 2048))) L /= 2;
     return L;
 }
-// radix of stage s in the plan of n (radix 4 first, then 2, 3, 5, 7); 0
+// radix of stage s in the plan of n (radix 8 first, then 4, 2, 3, 5, 7); 0
 // past the last stage or if n has another prime factor
 __host__ __device__ constexpr int lf_radix(int n, int s) {
-    const int rad[5] = {4, 2, 3, 5, 7};
+    const int rad[6] = {8, 4, 2, 3, 5, 7};
     int r = n, c = 0;
-    for (int k = 0; k < 5; k++)
+    for (int k = 0; k < 6; k++)
         while (r > 1 && r % rad[k] == 0) {
             if (c == s) return rad[k];
             c++;
@@ -2239,6 +2239,22 @@ __device__ __forceinline__ void small_dft(float2 (&v)[R], float t) {
         v[2] = csub(s02, s13);
         v[1] = cadd(d02, d13);
         v[3] = csub(d02, d13);
+    } else if constexpr (R == 8) {
+        // even and odd halves by 4-point transforms, then w8^q = exp(t 2 pi i q / 8)
+        // on the odd half: w8 = (1 + t i) / sqrt 2, w8^2 = t i, w8^3 = (-1 + t i) / sqrt 2
+        const float h = 0.70710678118654752f;
+        float2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        small_dft<4>(e, t);
+        small_dft<4>(o, t);
+        const float2 o1 = mul_ti(o[1], t), o3 = mul_ti(o[3], t);
+        o[1] = make_float2(h * (o[1].x + o1.x), h * (o[1].y + o1.y));
+        o[2] = mul_ti(o[2], t);
+        o[3] = make_float2(h * (o3.x - o[3].x), h * (o3.y - o[3].y));
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            v[q] = cadd(e[q], o[q]);
+            v[q + 4] = csub(e[q], o[q]);
+        }
     } else if constexpr (R == 3) {
         const float h = 0.86602540378443865f; // sqrt(3)/2
         const float2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
@@ -2330,7 +2346,8 @@ __device__ float2 *lds_fft_rt(float2 *src, float2 *dst, const LfPlan &P, int L, 
     for (int s = 0; s < P.nrad; s++) {
         const int R = P.rad[s];
         const float2 *tw = P.tw + P.off[s];
-        if (R == 4) lds_stage<4, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
+        if (R == 8) lds_stage<8, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
+        else if (R == 4) lds_stage<4, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
         else if (R == 2) lds_stage<2, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
         else if (R == 3) lds_stage<3, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
         else if (R == 5) lds_stage<5, INV, false, T>(src, dst, P.n, Ns, L, S, tw);
@@ -3005,7 +3022,7 @@ bool supported(long notf, long nxsa, long embfac) {
 }
 
 // Stockham plan of a line length n <= MX_LF_MAXN whose prime factors are 2,
-// 3, 5 and 7: radix 4 first, then 2, 3, 5, 7.  False if n does not factor so.
+// 3, 5 and 7: radix 8 first, then 4, 2, 3, 5, 7.  False if n does not factor so.
 bool lf_plan(long n, LfPlan &P) {
     P = LfPlan{};
     if (n < 2 || n > MX_LF_MAXN) return false;
