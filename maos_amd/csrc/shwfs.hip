@@ -2138,11 +2138,13 @@ __global__ __launch_bounds__(256) void k_llt_acorr(LltAcorrArgs a) {
 //                                                     -> PSF [nwvf][nwvf]
 //   C  FFT the PSF rows padded to notf by the corner copy (ccpcorner,
 //      math/matcomp.c:176-216: x < nwvf/2 stays, the rest moves up by
-//      notf - nwvf, the middle is zero)              -> B2 [nwvf][notf]
-//   D  FFT the columns of B2 (rows padded the same way), x nominal DTF,
-//      inverse FFT, keep the rows the detector samples -> B3 [ny][notf]
-//   E  inverse FFT of those rows, keep the sampled columns (real part)
-//                                                     -> image [ny][nx]
+//      notf - nwvf, the middle is zero), two real rows per complex FFT,
+//      columns <= notf/2 kept                       -> B2 [nwvf][notf/2+1]
+//   D  FFT those columns of B2 (rows padded the same way), x nominal DTF,
+//      inverse FFT, keep the rows the detector samples
+//                                                     -> B3 [ny][notf/2+1]
+//   E  inverse FFT of those rows (the other columns are the conjugate
+//      mirror), keep the sampled columns (real part) -> image [ny][nx]
 // then k_lf_sample applies si (maos/wfsints.c:286) and the pixel processing.
 constexpr int MX_LF_MAXN = 1024;
 constexpr int MX_LF_MAXRAD = 12;
@@ -2466,46 +2468,58 @@ __global__ __launch_bounds__(T) void k_lf_b(LfArgs a) {
     }
 }
 
-// pass C: FFT the PSF rows, padded to notf by the corner copy
+// pass C: FFT the PSF rows, padded to notf by the corner copy.  The PSF is
+// real, so rows 2p and 2p+1 go through one complex FFT as z = a + i b, and
+// A[k] = (Z[k] + conj Z[m-k]) / 2, B[k] = (Z[k] - conj Z[m-k]) / 2i are kept
+// for k <= m/2 only (the rest is their conjugate mirror): B2 is [nwvf][m/2+1].
 template <int T, int N>
 __global__ __launch_bounds__(T) void k_lf_c(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
     MX_LF_GEOM(a.notf, a.L)
-    const int n = a.nwvf, m = n_;
+    const int n = a.nwvf, m = n_, mh = m / 2 + 1, np = n / 2;
     int g, c;
-    lf_unit(a, (n + L - 1) / L, g, c);
+    lf_unit(a, (np + L - 1) / L, g, c);
     const float *in = a.psf + (long)c * n * n;
-    const float rm = 1.f / (float)m;
+    const float rm = 1.f / (float)m, rmh = 1.f / (float)mh, s = a.psf_scale;
     for (int e = threadIdx.x; e < L * m; e += T) {
-        const int l = ldiv<(N > 0)>(e, m, rm), x2 = e - l * m, y = g * L + l, x = corner_src(x2, m, n);
-        X[l * S + lf_pos(x2)] = make_float2(y < n && x >= 0 ? in[(long)y * n + x] * a.psf_scale : 0.f, 0.f);
+        const int l = ldiv<(N > 0)>(e, m, rm), x2 = e - l * m, p = g * L + l, x = corner_src(x2, m, n);
+        X[l * S + lf_pos(x2)] = p < np && x >= 0
+                                    ? make_float2(in[(long)(2 * p) * n + x] * s, in[(long)(2 * p + 1) * n + x] * s)
+                                    : make_float2(0.f, 0.f);
     }
     __syncthreads();
     const float2 *r = lf_fft<false, T, N>(X, Y, a.pm, L, S);
-    float2 *out = a.b2 + (long)c * n * m;
-    for (int e = threadIdx.x; e < L * m; e += T) {
-        const int l = ldiv<(N > 0)>(e, m, rm), x2 = e - l * m, y = g * L + l;
-        if (y < n) out[(long)y * m + x2] = r[l * S + lf_pos(x2)];
+    float2 *out = a.b2 + (long)c * n * mh;
+    for (int e = threadIdx.x; e < L * mh; e += T) {
+        const int l = ldiv<(N > 0)>(e, mh, rmh), k = e - l * mh, p = g * L + l;
+        if (p < np) {
+            const float2 z = r[l * S + lf_pos(k)], w = r[l * S + lf_pos(k == 0 ? 0 : m - k)];
+            out[(long)(2 * p) * mh + k] = make_float2(0.5f * (z.x + w.x), 0.5f * (z.y - w.y));
+            out[(long)(2 * p + 1) * mh + k] = make_float2(0.5f * (z.y + w.y), 0.5f * (w.x - z.x));
+        }
     }
 }
 
-// pass D: FFT the columns of B2 (rows padded by the corner copy), x conj(lotf)
-// x ETF (LLT) or x nominal, inverse FFT, keep the sampled rows.  UPLINK: the
-// chains are (WFS, wavelength) LLT fields and the column FFT is conj-stored
-// as the uplink OTF lotfc (maos/wfsints.c:149-199).
+// pass D: FFT the columns ux <= m/2 of B2 (rows padded by the corner copy),
+// x conj(lotf) x ETF (LLT) or x nominal, inverse FFT, keep the sampled rows:
+// B3 is [ny][m/2+1].  Every factor is the transform of a real function, so
+// the product at -u is the conjugate of that at u and the other columns are
+// not needed.  UPLINK: the chains are (WFS, wavelength) LLT fields and the
+// column FFT is conj-stored as the uplink OTF lotfc over ux <= m/2
+// (maos/wfsints.c:149-199).
 template <int T, int N, bool UPLINK>
 __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
     MX_LF_GEOM(a.notf, a.L)
-    const int n = a.nwvf, m = n_, lgL = N > 0 ? ilog2c(LC) : a.lgL;
+    const int n = a.nwvf, m = n_, mh = m / 2 + 1, lgL = N > 0 ? ilog2c(LC) : a.lgL;
     int g, c, iwfs, isa, iwvl;
-    lf_unit(a, (m + L - 1) / L, g, c);
+    lf_unit(a, (mh + L - 1) / L, g, c);
     lf_chain(a, c, iwfs, isa, iwvl);
     const int x0 = g * L;
-    const float2 *in = a.b2 + (long)c * n * m;
+    const float2 *in = a.b2 + (long)c * n * mh;
     for (int e = threadIdx.x; e < L * m; e += T) {
         const int l = e & (L - 1), y2 = e >> lgL, x2 = x0 + l, y = corner_src(y2, m, n);
-        X[l * S + lf_pos(y2)] = x2 < m && y >= 0 ? in[(long)y * m + x2] : make_float2(0.f, 0.f);
+        X[l * S + lf_pos(y2)] = x2 < mh && y >= 0 ? in[(long)y * mh + x2] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     float2 *r = lf_fft<false, T, N>(X, Y, a.pm, L, S);
@@ -2513,7 +2527,7 @@ __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
         float2 *lo = a.lotf + (long)c * m * m;
         for (int e = threadIdx.x; e < L * m; e += T) {
             const int l = e & (L - 1), y2 = e >> lgL, x2 = x0 + l;
-            if (x2 < m) {
+            if (x2 < mh) {
                 const float2 z = r[l * S + lf_pos(y2)];
                 lo[(long)x2 * m + y2] = make_float2(z.x, -z.y);
             }
@@ -2527,7 +2541,7 @@ __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
         const float rm = 1.f / (float)m;
         for (int e = threadIdx.x; e < L * m; e += T) {
             const int l = ldiv<(N > 0)>(e, m, rm), y2 = e - l * m, x2 = x0 + l;
-            if (x2 < m) {
+            if (x2 < mh) {
                 const long i = (long)x2 * m + y2;
                 float2 et = e1[i];
                 if (e2) {
@@ -2543,7 +2557,7 @@ __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
         const float rm = 1.f / (float)m;
         for (int e = threadIdx.x; e < L * m; e += T) {
             const int l = ldiv<(N > 0)>(e, m, rm), y2 = e - l * m, x2 = x0 + l;
-            if (x2 < m) {
+            if (x2 < mh) {
                 const float f = W.nominal[(long)isa * W.nom_sa + (long)x2 * m + y2];
                 float2 &z = r[l * S + lf_pos(y2)];
                 z = make_float2(z.x * f, z.y * f);
@@ -2552,14 +2566,15 @@ __global__ __launch_bounds__(T) void k_lf_d(LfArgs a) {
         __syncthreads();
     }
     const float2 *q = lf_fft<true, T, N>(r, r == X ? Y : X, a.pm, L, S);
-    float2 *out = a.b3 + (long)c * a.maxny * m;
+    float2 *out = a.b3 + (long)c * a.maxny * mh;
     for (int e = threadIdx.x; e < L * W.ny; e += T) {
         const int l = e & (L - 1), iyy = e >> lgL, x2 = x0 + l;
-        if (x2 < m) out[(long)iyy * m + x2] = q[l * S + lf_pos(W.rows[iyy])];
+        if (x2 < mh) out[(long)iyy * mh + x2] = q[l * S + lf_pos(W.rows[iyy])];
     }
 }
 
-// pass E: inverse FFT of the sampled rows, real part at the sampled columns
+// pass E: inverse FFT of the sampled rows, real part at the sampled columns;
+// a row's columns past m/2 are the conjugates of columns m - ux
 template <int T, int N>
 __global__ __launch_bounds__(T) void k_lf_e(LfArgs a) {
     __shared__ float2 X[MX_LF_LDS], Y[MX_LF_LDS];
@@ -2570,11 +2585,21 @@ __global__ __launch_bounds__(T) void k_lf_e(LfArgs a) {
     lf_chain(a, c, iwfs, isa, iwvl);
     const LfWvl &W = a.wv[iwvl];
     if (g * L >= W.ny) return; // uniform: no sampled row in this group
-    const float2 *in = a.b3 + (long)c * a.maxny * m;
+    const int mh = m / 2 + 1;
+    const float2 *in = a.b3 + (long)c * a.maxny * mh;
     const float rm = 1.f / (float)m;
     for (int e = threadIdx.x; e < L * m; e += T) {
         const int l = ldiv<(N > 0)>(e, m, rm), x = e - l * m, iyy = g * L + l;
-        X[l * S + lf_pos(x)] = iyy < W.ny ? in[(long)iyy * m + x] : make_float2(0.f, 0.f);
+        float2 v = make_float2(0.f, 0.f);
+        if (iyy < W.ny) {
+            if (x < mh) {
+                v = in[(long)iyy * mh + x];
+            } else {
+                v = in[(long)iyy * mh + m - x];
+                v.y = -v.y;
+            }
+        }
+        X[l * S + lf_pos(x)] = v;
     }
     __syncthreads();
     const float2 *r = lf_fft<true, T, N>(X, Y, a.pm, L, S);
@@ -3573,11 +3598,20 @@ int set_dtf_tables(mx_wfsgrad_t *h, int iwvl, long ntab, const double *nominal, 
     HIPCHK(hipMemcpy(h->d_si_w[iwvl], wt.data(), sizeof(float4) * ntab * npix, hipMemcpyHostToDevice));
     if (nominal && h->cfg.llt_nx == 0) {
         if ((rc = dev_alloc(&h->d_nominal[iwvl], ntab * N * N))) return rc;
+        // The large-field form transforms only the columns ux <= N/2 and takes
+        // the rest as conjugates, which gives MAOS's real part of the inverse
+        // transform exactly when the nominal is even: (n(u) + n(-u)) / 2 is
+        // stored for it (the same real part for any nominal; rotated pixels
+        // make the nominal uneven on the Nyquist row and column).
         std::vector<float> nt(ntab * N * N);
         for (long t = 0; t < ntab; t++)
             for (long uy = 0; uy < N; uy++)
-                for (long ux = 0; ux < N; ux++)
-                    nt[t * N * N + ux * N + uy] = (float)nominal[t * N * N + ux + uy * N];
+                for (long ux = 0; ux < N; ux++) {
+                    const double *nm = nominal + t * N * N;
+                    const double v = nm[ux + uy * N];
+                    nt[t * N * N + ux * N + uy] =
+                        h->lf ? (float)(0.5 * (v + nm[(N - ux) % N + (N - uy) % N * N])) : (float)v;
+                }
         HIPCHK(hipMemcpy(h->d_nominal[iwvl], nt.data(), sizeof(float) * ntab * N * N, hipMemcpyHostToDevice));
     }
     // image rows and columns the samplers read: the union over the tables
@@ -4316,15 +4350,17 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     const int slot_llt = llt && !pre_hit ? nl : -1;
     if (llt && h->lf) {
         // uplink OTF of each (WFS, wavelength) by passes A-C and the uplink
-        // column pass on the LLT grids (nlwvf == nwvf == notf)
+        // column pass on the LLT grids (nlwvf == nwvf == notf); pass C takes
+        // the PSF rows two at a time, the column passes the columns <= N/2
         const int Ln = lf_lines((int)h->nwvf), Lm = lf_lines((int)N), nch = (int)(h->nwfs * h->nwvl);
+        const int nph = (int)(h->nwvf / 2), mh = (int)(N / 2 + 1);
         const int nlx = (int)h->cfg.llt_nx;
         const struct {
             int q, len, L, ngroup;
         } up[4] = {{0, (int)h->nwvf, Ln, (nlx + Ln - 1) / Ln},
                    {1, (int)h->nwvf, Ln, (int)((h->nwvf + Ln - 1) / Ln)},
-                   {2, (int)N, Lm, (int)((h->nwvf + Lm - 1) / Lm)},
-                   {5, (int)N, Lm, (int)((N + Lm - 1) / Lm)}};
+                   {2, (int)N, Lm, (nph + Lm - 1) / Lm},
+                   {5, (int)N, Lm, (mh + Lm - 1) / Lm}};
         for (int q = 0; q < 4; q++) {
             LfArgs &u = h->arg_lf_up[q];
             u = h->arg_lf_base;
@@ -4375,14 +4411,16 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
     if (h->lf) {
         const LfArgs &f = h->arg_lf_base;
         const int Ln = lf_lines((int)h->nwvf), Lm = lf_lines((int)N);
-        // pass: kernel, lines per block, line groups per chain
+        const int nph = (int)(h->nwvf / 2), mh = (int)(N / 2 + 1);
+        // pass: kernel, lines per block, line groups per chain (pass C: pairs
+        // of PSF rows; pass D: the columns <= N/2)
         const struct {
             const void *k;
             int L, ngroup;
         } pass[5] = {{lf_kernel(0, h->lf_threads, (int)h->nwvf), Ln, (int)((h->nxsa + Ln - 1) / Ln)},
                      {lf_kernel(1, h->lf_threads, (int)h->nwvf), Ln, (int)((h->nwvf + Ln - 1) / Ln)},
-                     {lf_kernel(2, h->lf_threads, (int)N), Lm, (int)((h->nwvf + Lm - 1) / Lm)},
-                     {lf_kernel(3, h->lf_threads, (int)N), Lm, (int)((N + Lm - 1) / Lm)},
+                     {lf_kernel(2, h->lf_threads, (int)N), Lm, (nph + Lm - 1) / Lm},
+                     {lf_kernel(3, h->lf_threads, (int)N), Lm, (mh + Lm - 1) / Lm},
                      {lf_kernel(4, h->lf_threads, (int)N), Lm, (h->lf_maxny + Lm - 1) / Lm}};
         const dim3 blk((unsigned)h->lf_threads);
         for (int q = 0; q < 5; q++) {
@@ -5180,9 +5218,11 @@ int mx_wfsgrad_debug_lotf(mx_wfsgrad_t *h, int iwfs, int iwvl, double *lotf) {
                      hipMemcpyDeviceToHost));
     for (long ux = 0; ux < N; ux++)
         for (long uy = 0; uy < N; uy++) {
-            const float2 v = t[ux * N + uy];
+            // the large-field form stores ux <= N/2; lotf(-u) = conj lotf(u)
+            const bool mirror = h->lf && ux > N / 2;
+            const float2 v = mirror ? t[(N - ux) * N + (N - uy) % N] : t[ux * N + uy];
             lotf[2 * (ux + uy * N)] = v.x;
-            lotf[2 * (ux + uy * N) + 1] = -v.y; // stored conjugated
+            lotf[2 * (ux + uy * N) + 1] = mirror ? v.y : -v.y; // stored conjugated
         }
     return MX_OK;
 }
