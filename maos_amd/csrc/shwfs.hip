@@ -352,6 +352,22 @@ __device__ __forceinline__ void seg_sum(float &v) {
 #pragma unroll
     for (int o = N / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
 }
+// sum over the 64 lanes of a wave (every lane gets it): DPP steps within
+// rows of 16 lanes (lane xor 1, xor 2, mirror in 8, mirror in 16), then the
+// four row sums by readlane; no LDS round trips
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float wave_sum64(float v) {
+    v += dpp_f<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+    v += dpp_f<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+    v += dpp_f<0x141>(v); // row_half_mirror
+    v += dpp_f<0x140>(v); // row_mirror
+    const int b = __float_as_int(v);
+    return (__int_as_float(__builtin_amdgcn_readlane(b, 0)) + __int_as_float(__builtin_amdgcn_readlane(b, 16))) +
+           (__int_as_float(__builtin_amdgcn_readlane(b, 32)) + __int_as_float(__builtin_amdgcn_readlane(b, 48)));
+}
 
 // Ray trace every layer onto one NXSA x NXSA point grid whose first point is
 // at (origx, origy), writing the sum over layers to opd (LDS, x fastest).
@@ -1805,7 +1821,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         float s = 0.f;
 #pragma unroll
         for (int j = 0; j < 4; j++) s += og[(iy0 + j) * NX + ix];
-        seg_sum<64>(s);
+        s = wave_sum64(s);
         if (lane == 0) red[wave] = s;
         __syncthreads();
         mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
@@ -1813,11 +1829,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
     float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
     for (int iwvl = iwvl0; iwvl < iwvl1; iwvl++) {
         const MfmaWvl &WV = a.wv[iwvl];
-        // B operand of stages 1a and 1b: F over the band [16, 48) at the 16
-        // columns 16 lw + l%16; fetched before the embed so that it arrives
-        // meanwhile
-        V8 b1[3][2];
-        loadT<E>(F + (MX_F1A + lw) * MX_TPARTS * 64, lane, b1);
         // embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf) as W[x][y], y contiguous;
         // OPD and amplitude are read again per wavelength (cache hits) rather
         // than held in registers
@@ -1829,6 +1840,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
             st4(L[0], L[1], B0 + ix, B0 + iy0, re[0], re[1], re[2], re[3]);
             st4(L[2], L[3], B0 + ix, B0 + iy0, im[0], im[1], im[2], im[3]);
         }
+        // B operand of stages 1a and 1b: F over the band [16, 48) at the 16
+        // columns 16 lw + l%16, fetched after the embed (which then waits for
+        // the OPD and amplitude only; the scheduling barrier keeps the loads
+        // from being hoisted above it) and in flight over the barrier
+        __builtin_amdgcn_sched_barrier(0);
+        V8 b1[3][2];
+        loadT<E>(F + (MX_F1A + lw) * MX_TPARTS * 64, lane, b1);
         __syncthreads();
         // 1a: C[x][ky] = sum_{y in band} W[x][y] F[y][ky]; wave: ky in [16 lw, 16 lw + 16),
         // both 16-row blocks of x; stored as Z row ky, columns x
