@@ -1700,9 +1700,9 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
             }
         }
     }
-    seg_sum<64>(s0);
-    seg_sum<64>(s1);
-    seg_sum<64>(s2);
+    s0 = wave_sum64(s0);
+    s1 = wave_sum64(s1);
+    s2 = wave_sum64(s2);
     if (lane == 0 && !(a.saa[isa] < a.saat)) {
         float gx = 0.f, gy = 0.f;
         if (a.sigmatch == 2) { // raw moments; k_sigmatch2 forms the gradients
