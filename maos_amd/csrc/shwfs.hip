@@ -1672,7 +1672,24 @@ __device__ __forceinline__ void cis_rev(float x, float am, float &re, float &im)
 // intensity pv (npix <= 64).  Noise, the intensities, the FP64 i0 and dither
 // sums, then the matched-filter or CoG gradient (raw moments for sigmatch 2,
 // finished by k_sigmatch2).
-__device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int isa, int lane, float pv) {
+// Inputs of the pixel epilogue that do not depend on the image, loaded at
+// the start of a subaperture so that their latency is not on the tail of
+// the workgroup: the pixel's matched-filter row and the subaperture's area
+// and model flux.
+struct PixPre {
+    float m0, m1, saa, i0s;
+};
+__device__ __forceinline__ PixPre pix_pre(const MfmaArgs &a, int iwfs, int isa, int p) {
+    PixPre q{0.f, 0.f, a.saa[isa], a.i0sum ? a.i0sum[(long)iwfs * a.nsa + isa] : 0.f};
+    if (a.phytype == MX_PTYPE_MF && p < a.npix) {
+        const float2 m = *reinterpret_cast<const float2 *>(a.mtche + (((long)iwfs * a.nsa + isa) * a.npix + p) * 2);
+        q.m0 = m.x;
+        q.m1 = m.y;
+    }
+    return q;
+}
+__device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int isa, int lane, float pv,
+                                             const PixPre &q) {
     const long ibase = ((long)iwfs * a.nsa + isa) * a.npix;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
     const int p = lane;
@@ -1687,9 +1704,8 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
         if (a.i0acc) a.i0acc[ibase + p] += (double)pv;
         dither_add(a.dith, iwfs, ibase + p, pv);
         if (a.phytype == MX_PTYPE_MF) {
-            const float *m = a.mtche + ibase * 2 + 2 * p;
-            s0 = m[0] * pv;
-            s1 = m[1] * pv;
+            s0 = q.m0 * pv;
+            s1 = q.m1 * pv;
             s2 = pv;
         } else {
             const float iI = pv - a.cogoff;
@@ -1703,7 +1719,7 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
     s0 = wave_sum64(s0);
     s1 = wave_sum64(s1);
     s2 = wave_sum64(s2);
-    if (lane == 0 && !(a.saa[isa] < a.saat)) {
+    if (lane == 0 && !(q.saa < a.saat)) {
         float gx = 0.f, gy = 0.f;
         if (a.sigmatch == 2) { // raw moments; k_sigmatch2 forms the gradients
             gx = s0;
@@ -1711,7 +1727,7 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
         } else if (a.phytype == MX_PTYPE_MF) {
             float scale = 1.f;
             if (a.sigmatch == 1) {
-                scale = a.i0sum[(long)iwfs * a.nsa + isa] / s2;
+                scale = q.i0s / s2;
                 if (scale <= 0.f || scale > 10.f || !isfinite(scale)) scale = 1.f;
             }
             gx = s0 * scale;
@@ -1719,7 +1735,7 @@ __device__ __forceinline__ void shwfs_pixels(const MfmaArgs &a, int iwfs, int is
         } else {
             float sum = s2;
             if (a.sigmatch == 0)
-                sum = a.i0sum ? a.i0sum[(long)iwfs * a.nsa + isa] : a.sigtot * a.saa[isa];
+                sum = a.i0sum ? q.i0s : a.sigtot * q.saa;
             if (sum > a.cogthres) {
                 gx = (s0 / sum - (float)(a.pixpsax - 1) * 0.5f) * a.pixthetax;
                 gy = (s1 / sum - (float)(a.pixpsay - 1) * 0.5f) * a.pixthetay;
@@ -1827,6 +1843,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
         mean = (red[0] + red[1] + red[2] + red[3]) * (1.f / (NX * NX));
     }
     float pv = 0.f; // pixel tid (< npix <= 64, wave 0)
+    PixPre pq{};
+    if (!UPLINK && wave == 0) pq = pix_pre(a, iwfs, isa, lane);
     for (int iwvl = iwvl0; iwvl < iwvl1; iwvl++) {
         const MfmaWvl &WV = a.wv[iwvl];
         // embed amp*exp(+i 2 pi opd/wvl) (cembed_wvf) as W[x][y], y contiguous;
@@ -2052,7 +2070,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MX_MFMA_WAV
     }
     if constexpr (UPLINK) return;
     // outputs and pixel processing, wave 0
-    if (wave == 0) shwfs_pixels(a, iwfs, isa, lane, pv);
+    if (wave == 0) shwfs_pixels(a, iwfs, isa, lane, pv, pq);
 }
 
 #include "shwfs_fft.hip"

@@ -389,7 +389,7 @@ __global__ __launch_bounds__(MX_FFT_T) __attribute__((amdgpu_waves_per_eu(4))) v
         __syncthreads();
     }
     if constexpr (UPLINK) return;
-    if (wave == 0) shwfs_pixels(a, iwfs, isa, lane, pv);
+    if (wave == 0) shwfs_pixels(a, iwfs, isa, lane, pv, pix_pre(a, iwfs, isa, lane));
 }
 
 // Tip/tilt-removed PSF statistics (pistatout of wfsints, maos/wfsints.c:
