@@ -3274,9 +3274,10 @@ std::vector<uint16_t> build_Gh(const std::vector<int> &rows) {
 // whenever its size and sampling limits hold; -1 if the request cannot run
 // the half-plane tables (N = 64, nxsa = 32, <= 16 sampled rows and columns
 // per wavelength, a second ETF in half-plane form) that the MFMA forms and
-// the FP32 radix-8 form read
+// the FP32 radix-8 form read; their pixel epilogue is one lane per pixel of
+// one wave, so at most 64 pixels (more run on the register-FFT form)
 bool half_tables_ok(const mx_wfsgrad_t *h) {
-    bool ok = h->mfma_size;
+    bool ok = h->mfma_size && h->npix <= 64;
     for (int w = 0; w < h->nwvl; w++) ok = ok && h->mfma_wvl_ok[w] && (!h->d_etf2[w] || h->d_etf2_t[w]);
     return ok;
 }
@@ -4006,8 +4007,8 @@ int plan_step(mx_wfsgrad_t *h, long isim, double dt, StepPlan &sp) {
         return set_err(MX_ESTATE, "i0sum not set (matched filter with sigmatch=%d)", h->cfg.sigmatch);
     const int variant = effective_variant(h);
     if (variant < 0 && !h->geom)
-        return set_err(MX_EINVAL, "the MFMA form needs notf=64, nxsa=32, <= %d sampled image rows and "
-                       "columns and no second ETF", MX_MFMA_NS);
+        return set_err(MX_EINVAL, "the MFMA form needs notf=64, nxsa=32, <= 64 pixels, <= %d sampled image "
+                       "rows and columns and no second ETF", MX_MFMA_NS);
     HIPCHK(hipSetDevice(h->device));
     const long N = h->notf;
     {
